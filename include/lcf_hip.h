@@ -1,0 +1,127 @@
+/*
+ * lcf_hip.h -- C ABI of the MI355X (gfx950) batched light-curve fitting
+ * library.  This is the boundary a `hip` backend of the reference crate binds
+ * (a src/gpu_hip.rs next to src/gpu_types.rs, selected like the `cuda`
+ * backend in src/gpu.rs:12-18 and src/lib.rs:5-10).
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a negative LCF_E* code; the message
+ *     of the last failure on a context is lcf_last_error(ctx).  This replaces
+ *     the reference's `Err(format!("CUDA error {code}: {msg}"))`
+ *     (src/gpu_cuda.rs:170-186).
+ *   - Plain host pointers and sizes only.  Inputs are read, outputs written,
+ *     before the call returns (each call synchronises its context stream).
+ *   - Model ids are GpuModelName::model_id (src/gpu_types.rs:51-62):
+ *     0 Bazin, 1 Villar, 2 Tde, 3 Arnett, 4 Magnetar, 5 ShockCooling,
+ *     6 Afterglow, 7 MetzgerKN.  Per-source parameter rows are padded to
+ *     LCF_MAX_PARAMS doubles.
+ *   - One context per host thread (the reference GpuContext is tied to the
+ *     calling thread's current device, src/gpu_cuda.rs:297-312).
+ */
+#ifndef LCF_HIP_H
+#define LCF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LCF_MAX_PARAMS 7
+#define LCF_N_MODELS 8
+
+/* status codes */
+#define LCF_OK 0
+#define LCF_E_INVALID (-1)   /* bad argument (size, model id, null pointer) */
+#define LCF_E_HIP (-2)       /* HIP runtime error (message has the HIP text) */
+#define LCF_E_NOMEM (-3)     /* device allocation failed */
+
+/* precision of model evaluation / swarm state in the fitting kernels */
+#define LCF_FP32 0           /* FP32 + Kahan-compensated chi-square (default) */
+#define LCF_FP64 1           /* FP64 throughout */
+
+/* work distribution for batched fits */
+#define LCF_SCHED_PERSISTENT 0  /* one persistent launch, size-sorted device queue (default) */
+#define LCF_SCHED_PER_CLASS 1   /* one launch per power-of-two size class, synchronised */
+
+/* PSO flags */
+#define LCF_PSO_STALL_FROM_FIRST_EVAL 1u /* opt-in: stall baseline = first global best
+                                            (the reference's GPU path starts it at +inf) */
+
+typedef struct lcf_ctx lcf_ctx;
+typedef struct lcf_batch lcf_batch;
+
+/* ---- context: GpuContext::new (src/gpu_cuda.rs:307-312) ---------------- */
+int lcf_ctx_create(int device, lcf_ctx** out);
+void lcf_ctx_destroy(lcf_ctx* ctx);
+const char* lcf_last_error(const lcf_ctx* ctx);
+/* Precision (LCF_FP32 / LCF_FP64) and schedule (LCF_SCHED_*) used by the
+ * batched fits of this context.  No counterpart in the reference (FP64 only,
+ * one launch per PSO iteration). */
+int lcf_ctx_set_precision(lcf_ctx* ctx, int precision);
+int lcf_ctx_set_schedule(lcf_ctx* ctx, int schedule);
+/* Number of compute units of the context's device. */
+int lcf_ctx_num_cus(const lcf_ctx* ctx);
+
+/* ---- resident source data: GpuBatchData (src/gpu_cuda.rs:236-292) ------ */
+/* offsets[n_sources+1] index the concatenated per-observation arrays
+ * (BatchSource fields, src/gpu_types.rs:122-130); is_upper is 0/1. */
+int lcf_batch_create(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets,
+                     const double* times, const double* flux, const double* obs_var,
+                     const int32_t* is_upper, const double* upper_flux,
+                     lcf_batch** out);
+void lcf_batch_destroy(lcf_batch* batch);
+int64_t lcf_batch_n_sources(const lcf_batch* batch);
+/* GpuBatchData::n_obs_per_source: observation count, 1 when out of range. */
+int64_t lcf_batch_n_obs(const lcf_batch* batch, int64_t source_idx);
+
+/* ---- forward evaluation: GpuContext::eval_batch (src/gpu_cuda.rs:317-360) */
+/* out[d*n_times + i] = model(params[d*n_params(model) ..], times[i]), FP64. */
+int lcf_eval_batch(lcf_ctx* ctx, int model_id, const double* params, int64_t n_draws,
+                   const double* times, int64_t n_times, double* out);
+
+/* ---- batched PSO --------------------------------------------------------- */
+/* GpuContext::batch_pso (src/gpu_cuda.rs:368-567).
+ * out_params[n_sources*LCF_MAX_PARAMS], out_cost[n_sources]. */
+int lcf_batch_pso(lcf_ctx* ctx, int model_id, const lcf_batch* data, int n_particles,
+                  int max_iters, int stall_iters, uint64_t seed,
+                  double* out_params, double* out_cost);
+
+/* GpuContext::batch_model_select (src/gpu_cuda.rs:573-620): Bazin first,
+ * the other models of ALL_GPU_MODELS only where Bazin's cost >= gate.
+ * out_model[n_sources] holds model ids. */
+int lcf_batch_model_select(lcf_ctx* ctx, const lcf_batch* data, int n_particles,
+                           int max_iters, int stall_iters, double bazin_good_enough,
+                           int32_t* out_model, double* out_params, double* out_cost);
+
+/* GpuContext::batch_all_models (src/gpu_cuda.rs:624-637): every model of
+ * ALL_GPU_MODELS on every source, in that order.
+ * out_params[8][n_sources][LCF_MAX_PARAMS], out_cost[8][n_sources]; row m is
+ * ALL_GPU_MODELS[m] (ids 0,3,2,6,1,4,5,7). */
+int lcf_batch_all_models(lcf_ctx* ctx, const lcf_batch* data, int n_particles,
+                         int max_iters, int stall_iters,
+                         double* out_params, double* out_cost);
+
+/* General form behind the three calls above: fits models[0..n_models) in
+ * order, seed `seed`; when use_gate != 0, models[1..] run only for sources
+ * whose models[0] cost is >= gate.  Any output pointer may be NULL.
+ *   model_params[n_models][n_sources][7], model_cost[n_models][n_sources],
+ *   model_iters[n_models][n_sources] (iterations executed),
+ *   best_model/best_cost[n_sources], best_params[n_sources][7]. */
+int lcf_batch_fit_models(lcf_ctx* ctx, const lcf_batch* data, const int32_t* models,
+                         int n_models, int use_gate, double gate, int n_particles,
+                         int max_iters, int stall_iters, uint64_t seed, uint32_t flags,
+                         double* model_params, double* model_cost, int32_t* model_iters,
+                         int32_t* best_model, double* best_params, double* best_cost);
+
+/* Timing of the last lcf_batch_fit_models-family call on this context:
+ * milliseconds between HIP events around the kernel launches (inputs already
+ * resident, outputs still on the device). */
+double lcf_last_kernel_ms(const lcf_ctx* ctx);
+/* Milliseconds of the launch(es) of model slot `slot` in that call. */
+double lcf_last_model_ms(const lcf_ctx* ctx, int slot);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,25 @@
+"""MI355X-native batched light-curve fitting (HIP / gfx950).
+
+Python mirror of the reference's GPU batch interface (src/gpu_types.rs,
+src/gpu_cuda.rs) over the C ABI of ``liblcf_hip.so`` (include/lcf_hip.h).
+The directory name contains a hyphen, so it is imported by path; see
+``load_package()`` in tests/conftest.py and bench.py.
+"""
+from .gpu import (  # noqa: F401
+    ALL_GPU_MODELS,
+    FP32,
+    FP64,
+    MAX_PARAMS,
+    PSO_STALL_FROM_FIRST_EVAL,
+    SCHED_PER_CLASS,
+    SCHED_PERSISTENT,
+    BatchPsoResult,
+    BatchSource,
+    GpuBatchData,
+    GpuContext,
+    GpuError,
+    GpuModelName,
+    sources_to_csr,
+)
+from . import synthetic  # noqa: F401
+from ._lib import LIB_PATH  # noqa: F401

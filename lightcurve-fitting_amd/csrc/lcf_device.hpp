@@ -1,0 +1,168 @@
+// lcf_device.hpp -- device-side building blocks shared by the gfx950 kernels:
+// precision-dispatched math, Philox4x32-10, wave64 reductions, Kahan sums.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lcf {
+
+constexpr int WAVE = 64;
+
+// Largest argument for which exp() is finite in IEEE binary64.  The reference
+// evaluates every model in FP64, so a term whose FP64 exponential overflows is
+// non-finite there; the kernels test arguments against this bound in both
+// precisions to reproduce exactly those cases.
+constexpr double EXP_OVF64 = 709.782712893384;
+
+// ---------------------------------------------------------------------------
+// Math per precision.  FP32 uses the hardware transcendental unit directly
+// (v_exp_f32 / v_log_f32 / v_rcp_f32 / v_sqrt_f32: base-2, ~1 ulp);
+// FP64 uses the device math library.
+// ---------------------------------------------------------------------------
+template <typename R> struct Math;
+
+template <> struct Math<float> {
+    static __device__ __forceinline__ float exp(float x) {
+        return __builtin_amdgcn_exp2f(x * 1.44269504088896341f);
+    }
+    static __device__ __forceinline__ float log(float x) {
+        return __builtin_amdgcn_logf(x) * 0.693147180559945309f;
+    }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+    static __device__ __forceinline__ float rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+    static __device__ __forceinline__ bool finite(float x) { return __builtin_isfinite(x); }
+    // Sentinel standing in for the reference's 1e99 "invalid" cost.
+    static __device__ __forceinline__ float big() { return 1e30f; }
+};
+
+template <> struct Math<double> {
+    static __device__ __forceinline__ double exp(double x) { return ::exp(x); }
+    static __device__ __forceinline__ double log(double x) { return ::log(x); }
+    static __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double sqrt(double x) { return ::sqrt(x); }
+    static __device__ __forceinline__ double rsqrt(double x) { return 1.0 / ::sqrt(x); }
+    static __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
+    static __device__ __forceinline__ double big() { return 1e99; }
+};
+
+// log(1 + e) for e in [0, 1], accurate also for tiny e.
+template <typename R>
+__device__ __forceinline__ R log1p_unit(R e) {
+    using M = Math<R>;
+    // Below 2^-7 use the series e - e^2/2 + e^3/3 - e^4/4 (rel. err < 2^-30).
+    R s = e * (R(1) - e * (R(0.5) - e * (R(1) / R(3) - e * R(0.25))));
+    R l = M::log(R(1) + e);
+    return e < R(0.0078125) ? s : l;
+}
+
+template <>
+__device__ __forceinline__ double log1p_unit<double>(double e) { return ::log1p(e); }
+
+// softplus(x) = log(1 + exp(x)) evaluated without overflow.
+template <typename R>
+__device__ __forceinline__ R softplus(R x) {
+    using M = Math<R>;
+    R ax = x < R(0) ? -x : x;
+    R l = log1p_unit<R>(M::exp(-ax));
+    return (x > R(0) ? x : R(0)) + l;
+}
+
+// 1 - exp(-y) for y >= 0, accurate for small y.
+template <typename R>
+__device__ __forceinline__ R one_minus_exp_neg(R y) {
+    using M = Math<R>;
+    R s = y * (R(1) - y * (R(0.5) - y * (R(1) / R(6) - y * (R(1) / R(24)))));
+    R d = R(1) - M::exp(-y);
+    return y < R(0.03125) ? s : d;
+}
+
+template <>
+__device__ __forceinline__ double one_minus_exp_neg<double>(double y) { return -::expm1(-y); }
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11).  Counter (c0..c3), key (k0, k1).
+// ---------------------------------------------------------------------------
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ __forceinline__ U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2,
+                                                     uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+        uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+#else
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+#endif
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return U4{c0, c1, c2, c3};
+}
+
+// 24-bit uniform in [0,1): exactly representable in both precisions, so the
+// FP32 and FP64 swarms (and the oracle) consume identical random numbers.
+template <typename R>
+__device__ __forceinline__ R u24(uint32_t x) {
+    return R(x >> 8) * R(1.0 / 16777216.0);
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 helpers.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int bcast_int(int v, int src_lane) {
+    return __builtin_amdgcn_readlane(v, src_lane);
+}
+__device__ __forceinline__ float bcast(float v, int src_lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src_lane));
+}
+__device__ __forceinline__ double bcast(double v, int src_lane) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src_lane);
+    int hi = __builtin_amdgcn_readlane((int)(b >> 32), src_lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Lexicographic (value, index) minimum across the wave.  Callers map NaN to
+// +inf beforehand.  Returns the winning index, wave-uniform.
+template <typename R>
+__device__ __forceinline__ int wave_argmin(R v, int idx) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        R ov = __shfl_xor(v, off, WAVE);
+        int oi = __shfl_xor(idx, off, WAVE);
+        bool take = (ov < v) || (ov == v && oi < idx);
+        v = take ? ov : v;
+        idx = take ? oi : idx;
+    }
+    return __builtin_amdgcn_readfirstlane(idx);
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+// Kahan-compensated accumulator.
+template <typename R>
+struct Kahan {
+    R s, c;
+    __device__ __forceinline__ void init() { s = R(0); c = R(0); }
+    __device__ __forceinline__ void add(R x) {
+        R y = x - c;
+        R t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+}  // namespace lcf

@@ -1,0 +1,37 @@
+// pso_kernels.hpp -- host-visible launch interface of the batched PSO kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace lcf {
+
+struct PsoArgs {
+    const void* obs;            // packed records: float4 (FP32) or double4 (FP64) {t, y, var, flag}
+    const int64_t* offsets;     // [n_bands + 1] CSR offsets into obs
+    const int32_t* order;       // [n_tasks] band indices, decreasing size
+    int n_tasks;
+    int task_base;              // first task of a one-shot launch
+    int n_bands;
+    int* queue;                 // persistent-launch work counter (zeroed before launch)
+    int slot;                   // output slot of this model
+    const double* gate_cost;    // slot-0 costs when gated (skip band if < gate), else null
+    double gate;
+    int n_particles, max_iters, stall_iters;
+    uint64_t seed;
+    uint32_t flags;             // bit 0: stall baseline from the first evaluation
+    int tile;                   // LDS observation tile (records per wave)
+    double* model_params;       // [n_slots][n_bands][7]
+    double* model_cost;         // [n_slots][n_bands]
+    int32_t* model_iters;       // [n_slots][n_bands]
+};
+
+size_t pso_lds_bytes(bool fp64, int tile);
+hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool persistent, int grid,
+                      hipStream_t st);
+hipError_t pso_occupancy(int model, bool fp64, int ppl, int tile, int* blocks_per_cu);
+hipError_t launch_select_best(const double* mp, const double* mc, const int32_t* models, int n_models,
+                              int n_bands, int32_t* best_model, double* best_params,
+                              double* best_cost, hipStream_t st);
+
+}  // namespace lcf

@@ -1,0 +1,408 @@
+// runtime.hip -- host runtime behind the C ABI in include/lcf_hip.h:
+// contexts, resident batches, and the launch logic of the fitting kernels.
+#include "../../include/lcf_hip.h"
+#include "pso_kernels.hpp"
+#include "eval_kernels.hpp"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+struct lcf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int precision = LCF_FP32;
+    int schedule = LCF_SCHED_PERSISTENT;
+    int n_cu = 0;
+    int* d_queue = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_slot[LCF_N_MODELS + 1] = {};  // boundaries of the per-model launches
+    double last_ms = 0.0;
+    double slot_ms[LCF_N_MODELS] = {};
+};
+
+struct lcf_batch {
+    lcf_ctx* ctx = nullptr;
+    int64_t n_sources = 0;
+    int64_t n_obs = 0;
+    std::vector<int64_t> h_off;
+    int64_t* d_off = nullptr;
+    float4* d_obs32 = nullptr;    // packed {t, y, var, flag}
+    double4* d_obs64 = nullptr;
+    int32_t* d_order = nullptr;   // source indices by decreasing observation count
+    std::vector<int32_t> h_order;
+    int max_n = 0;
+};
+
+namespace {
+
+int fail(lcf_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? LCF_E_NOMEM : LCF_E_HIP,           \
+                        "HIP error %d: %s (%s)", (int)e_, hipGetErrorString(e_), #expr);    \
+    } while (0)
+
+// Device buffer freed on scope exit.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t n) { return hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)); }
+};
+
+const int N_PARAMS[LCF_N_MODELS] = {6, 7, 7, 5, 5, 5, 6, 5};
+// ALL_GPU_MODELS (src/gpu_types.rs:107-116)
+const int32_t ALL_MODELS[LCF_N_MODELS] = {0, 3, 2, 6, 1, 4, 5, 7};
+
+__global__ void pack_obs(const double* __restrict__ t, const double* __restrict__ f,
+                         const double* __restrict__ v, const int32_t* __restrict__ up,
+                         const double* __restrict__ uf, int64_t n, float4* o32, double4* o64) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool u = up[i] != 0;
+    double y = u ? uf[i] : f[i];
+    o64[i] = make_double4(t[i], y, v[i], u ? 1.0 : 0.0);
+    o32[i] = make_float4((float)t[i], (float)y, (float)v[i], u ? 1.0f : 0.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int lcf_ctx_create(int device, lcf_ctx** out) {
+    if (!out) return LCF_E_INVALID;
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || device < 0 || device >= n) return LCF_E_INVALID;
+    lcf_ctx* c = new lcf_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+        hipMalloc((void**)&c->d_queue, sizeof(int) * 64) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        lcf_ctx_destroy(c);
+        return LCF_E_HIP;
+    }
+    for (auto& e : c->ev_slot)
+        if (hipEventCreate(&e) != hipSuccess) {
+        lcf_ctx_destroy(c);
+        return LCF_E_HIP;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+    if (c->n_cu <= 0) c->n_cu = 256;
+    *out = c;
+    return LCF_OK;
+}
+
+void lcf_ctx_destroy(lcf_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->d_queue) (void)hipFree(c->d_queue);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (auto& e : c->ev_slot)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* lcf_last_error(const lcf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int lcf_ctx_set_precision(lcf_ctx* c, int p) {
+    if (!c) return LCF_E_INVALID;
+    if (p != LCF_FP32 && p != LCF_FP64) return fail(c, LCF_E_INVALID, "unknown precision %d", p);
+    c->precision = p;
+    return LCF_OK;
+}
+
+int lcf_ctx_set_schedule(lcf_ctx* c, int s) {
+    if (!c) return LCF_E_INVALID;
+    if (s != LCF_SCHED_PERSISTENT && s != LCF_SCHED_PER_CLASS)
+        return fail(c, LCF_E_INVALID, "unknown schedule %d", s);
+    c->schedule = s;
+    return LCF_OK;
+}
+
+int lcf_ctx_num_cus(const lcf_ctx* c) { return c ? c->n_cu : 0; }
+
+double lcf_last_kernel_ms(const lcf_ctx* c) { return c ? c->last_ms : 0.0; }
+
+double lcf_last_model_ms(const lcf_ctx* c, int slot) {
+    return (c && slot >= 0 && slot < LCF_N_MODELS) ? c->slot_ms[slot] : 0.0;
+}
+
+int lcf_batch_create(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t,
+                     const double* f, const double* v, const int32_t* up, const double* uf,
+                     lcf_batch** out) {
+    if (!c || !out) return LCF_E_INVALID;
+    *out = nullptr;
+    if (n_src < 0 || n_src > INT32_MAX - 1 || !off)
+        return fail(c, LCF_E_INVALID, "invalid source count %lld", (long long)n_src);
+    if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
+    for (int64_t s = 0; s < n_src; s++)
+        if (off[s + 1] < off[s]) return fail(c, LCF_E_INVALID, "offsets not non-decreasing at %lld", (long long)s);
+    const int64_t n = off[n_src];
+    if (n > 0 && (!t || !f || !v || !up || !uf)) return fail(c, LCF_E_INVALID, "null observation array");
+    HIPCHK(c, hipSetDevice(c->device));
+    lcf_batch* b = new lcf_batch();
+    b->ctx = c;
+    b->n_sources = n_src;
+    b->n_obs = n;
+    b->h_off.assign(off, off + n_src + 1);
+    b->h_order.resize(n_src);
+    std::iota(b->h_order.begin(), b->h_order.end(), 0);
+    std::stable_sort(b->h_order.begin(), b->h_order.end(), [&](int32_t x, int32_t y) {
+        return off[x + 1] - off[x] > off[y + 1] - off[y];
+    });
+    for (int64_t s = 0; s < n_src; s++)
+        b->max_n = (int)std::max<int64_t>(b->max_n, off[s + 1] - off[s]);
+    auto cleanup = [&](int code) { lcf_batch_destroy(b); return code; };
+    int rc = LCF_OK;
+    do {
+        size_t nn = (size_t)std::max<int64_t>(n, 1);
+        if (hipMalloc((void**)&b->d_off, sizeof(int64_t) * (n_src + 1)) != hipSuccess ||
+            hipMalloc((void**)&b->d_obs32, sizeof(float4) * nn) != hipSuccess ||
+            hipMalloc((void**)&b->d_obs64, sizeof(double4) * nn) != hipSuccess ||
+            hipMalloc((void**)&b->d_order, sizeof(int32_t) * std::max<int64_t>(n_src, 1)) != hipSuccess) {
+            rc = fail(c, LCF_E_NOMEM, "device allocation failed for %lld observations", (long long)n);
+            break;
+        }
+        if (hipMemcpyAsync(b->d_off, off, sizeof(int64_t) * (n_src + 1), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            (n_src > 0 && hipMemcpyAsync(b->d_order, b->h_order.data(), sizeof(int32_t) * n_src,
+                                         hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+            rc = fail(c, LCF_E_HIP, "upload failed");
+            break;
+        }
+        if (n > 0) {
+            DevBuf<double> dt, df, dv, duf;
+            DevBuf<int32_t> dup;
+            if (dt.alloc(n) || df.alloc(n) || dv.alloc(n) || duf.alloc(n) || dup.alloc(n)) {
+                rc = fail(c, LCF_E_NOMEM, "staging allocation failed");
+                break;
+            }
+            hipError_t e = hipSuccess;
+            e = e ? e : hipMemcpyAsync(dt.p, t, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+            e = e ? e : hipMemcpyAsync(df.p, f, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+            e = e ? e : hipMemcpyAsync(dv.p, v, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+            e = e ? e : hipMemcpyAsync(duf.p, uf, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
+            e = e ? e : hipMemcpyAsync(dup.p, up, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) {
+                int64_t blocks = (n + 255) / 256;
+                hipLaunchKernelGGL(pack_obs, dim3((unsigned)blocks), dim3(256), 0, c->stream, dt.p, df.p,
+                                   dv.p, dup.p, duf.p, n, b->d_obs32, b->d_obs64);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) {
+                rc = fail(c, LCF_E_HIP, "HIP error %d: %s (batch upload)", (int)e, hipGetErrorString(e));
+                break;
+            }
+        } else if (hipStreamSynchronize(c->stream) != hipSuccess) {
+            rc = fail(c, LCF_E_HIP, "upload sync failed");
+            break;
+        }
+    } while (0);
+    if (rc != LCF_OK) return cleanup(rc);
+    *out = b;
+    return LCF_OK;
+}
+
+void lcf_batch_destroy(lcf_batch* b) {
+    if (!b) return;
+    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    if (b->d_off) (void)hipFree(b->d_off);
+    if (b->d_obs32) (void)hipFree(b->d_obs32);
+    if (b->d_obs64) (void)hipFree(b->d_obs64);
+    if (b->d_order) (void)hipFree(b->d_order);
+    delete b;
+}
+
+int64_t lcf_batch_n_sources(const lcf_batch* b) { return b ? b->n_sources : 0; }
+
+int64_t lcf_batch_n_obs(const lcf_batch* b, int64_t s) {
+    if (!b || s < 0 || s >= b->n_sources) return 1;
+    return b->h_off[s + 1] - b->h_off[s];
+}
+
+int lcf_eval_batch(lcf_ctx* c, int model, const double* params, int64_t n_draws, const double* times,
+                   int64_t n_times, double* out) {
+    if (!c) return LCF_E_INVALID;
+    if (model < 0 || model >= LCF_N_MODELS) return fail(c, LCF_E_INVALID, "unknown model id %d", model);
+    if (n_draws < 0 || n_times < 0) return fail(c, LCF_E_INVALID, "negative size");
+    const int np = N_PARAMS[model];
+    const int64_t total = n_draws * n_times;
+    if (total == 0) return LCF_OK;
+    if (!params || !times || !out) return fail(c, LCF_E_INVALID, "null pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf<double> dp, dt, dout;
+    HIPCHK(c, dp.alloc(n_draws * np));
+    HIPCHK(c, dt.alloc(n_times));
+    HIPCHK(c, dout.alloc(total));
+    HIPCHK(c, hipMemcpyAsync(dp.p, params, sizeof(double) * n_draws * np, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(dt.p, times, sizeof(double) * n_times, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, lcf::launch_eval(model, dp.p, dt.p, dout.p, n_draws, n_times, np, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out, dout.p, sizeof(double) * total, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+int lcf_batch_fit_models(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models,
+                         int use_gate, double gate, int P, int max_iters, int stall_iters,
+                         uint64_t seed, uint32_t flags, double* model_params, double* model_cost,
+                         int32_t* model_iters, int32_t* best_model, double* best_params,
+                         double* best_cost) {
+    if (!c) return LCF_E_INVALID;
+    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!models || n_models <= 0 || n_models > LCF_N_MODELS)
+        return fail(c, LCF_E_INVALID, "n_models must be in [1, %d]", LCF_N_MODELS);
+    for (int i = 0; i < n_models; i++)
+        if (models[i] < 0 || models[i] >= LCF_N_MODELS)
+            return fail(c, LCF_E_INVALID, "unknown model id %d", models[i]);
+    if (P <= 0 || P > 4 * 64) return fail(c, LCF_E_INVALID, "n_particles must be in [1, 256], got %d", P);
+    if (max_iters <= 0) return fail(c, LCF_E_INVALID, "max_iters must be positive");
+    if (stall_iters < 0) return fail(c, LCF_E_INVALID, "stall_iters must be non-negative");
+    const int64_t ns = b->n_sources;
+    if (ns == 0) return LCF_OK;
+    HIPCHK(c, hipSetDevice(c->device));
+
+    const bool fp64 = c->precision == LCF_FP64;
+    const int ppl = P <= 64 ? 1 : (P <= 128 ? 2 : 4);
+    // LDS tile: the whole band when it fits 16 KiB per wave, else 16 KiB tiles
+    // re-staged per iteration.
+    const int rec = fp64 ? (int)sizeof(double4) : (int)sizeof(float4);
+    const int tile_cap = 16384 / rec;
+    const int tile = std::max(1, std::min(tile_cap, std::max(b->max_n, 1)));
+
+    const int64_t nm = n_models;
+    DevBuf<double> d_mp, d_mc, d_bp, d_bc;
+    DevBuf<int32_t> d_mi, d_bm, d_models;
+    HIPCHK(c, d_mp.alloc(nm * ns * 7));
+    HIPCHK(c, d_mc.alloc(nm * ns));
+    HIPCHK(c, d_mi.alloc(nm * ns));
+    HIPCHK(c, d_models.alloc(nm));
+    if (best_params) HIPCHK(c, d_bp.alloc(ns * 7));
+    if (best_cost) HIPCHK(c, d_bc.alloc(ns));
+    if (best_model) HIPCHK(c, d_bm.alloc(ns));
+    HIPCHK(c, hipMemcpyAsync(d_models.p, models, sizeof(int32_t) * nm, hipMemcpyHostToDevice, c->stream));
+
+    lcf::PsoArgs a{};
+    a.obs = fp64 ? (const void*)b->d_obs64 : (const void*)b->d_obs32;
+    a.offsets = b->d_off;
+    a.order = b->d_order;
+    a.n_bands = (int)ns;
+    a.queue = c->d_queue;
+    a.gate = gate;
+    a.n_particles = P;
+    a.max_iters = max_iters;
+    a.stall_iters = stall_iters;
+    a.seed = seed;
+    a.flags = flags;
+    a.tile = tile;
+    a.model_params = d_mp.p;
+    a.model_cost = d_mc.p;
+    a.model_iters = d_mi.p;
+
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int mi = 0; mi < n_models; mi++) {
+        const int model = models[mi];
+        a.slot = mi;
+        HIPCHK(c, hipEventRecord(c->ev_slot[mi], c->stream));
+        a.gate_cost = (use_gate && mi > 0) ? d_mc.p : nullptr;
+        if (c->schedule == LCF_SCHED_PERSISTENT) {
+            int per_cu = 0;
+            HIPCHK(c, lcf::pso_occupancy(model, fp64, ppl, tile, &per_cu));
+            per_cu = std::max(per_cu, 1);
+            a.task_base = 0;
+            a.n_tasks = (int)ns;
+            int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, ns);
+            HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+            HIPCHK(c, lcf::launch_pso(model, a, fp64, ppl, true, grid, c->stream));
+        } else {
+            // One synchronised launch per power-of-two size class, in
+            // decreasing size (the band order is already size-sorted).
+            int64_t i = 0;
+            while (i < ns) {
+                int64_t s0 = b->h_order[i];
+                int64_t n0 = b->h_off[s0 + 1] - b->h_off[s0];
+                int64_t cls = 1;
+                while (cls < n0) cls <<= 1;
+                int64_t j = i;
+                while (j < ns) {
+                    int64_t sj = b->h_order[j];
+                    int64_t nj = b->h_off[sj + 1] - b->h_off[sj];
+                    if (cls > 1 && nj * 2 <= cls) break;
+                    j++;
+                }
+                a.task_base = (int)i;
+                a.n_tasks = (int)j;
+                HIPCHK(c, lcf::launch_pso(model, a, fp64, ppl, false, (int)(j - i), c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                i = j;
+            }
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev_slot[n_models], c->stream));
+    if (best_model || best_params || best_cost)
+        HIPCHK(c, lcf::launch_select_best(d_mp.p, d_mc.p, d_models.p, n_models, (int)ns, d_bm.p, d_bp.p,
+                                          d_bc.p, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    for (int mi = 0; mi < LCF_N_MODELS; mi++) c->slot_ms[mi] = 0.0;
+    for (int mi = 0; mi < n_models; mi++) {
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_slot[mi], c->ev_slot[mi + 1]));
+        c->slot_ms[mi] = ms;
+    }
+
+    if (model_params) HIPCHK(c, hipMemcpyAsync(model_params, d_mp.p, sizeof(double) * nm * ns * 7, hipMemcpyDeviceToHost, c->stream));
+    if (model_cost) HIPCHK(c, hipMemcpyAsync(model_cost, d_mc.p, sizeof(double) * nm * ns, hipMemcpyDeviceToHost, c->stream));
+    if (model_iters) HIPCHK(c, hipMemcpyAsync(model_iters, d_mi.p, sizeof(int32_t) * nm * ns, hipMemcpyDeviceToHost, c->stream));
+    if (best_params) HIPCHK(c, hipMemcpyAsync(best_params, d_bp.p, sizeof(double) * ns * 7, hipMemcpyDeviceToHost, c->stream));
+    if (best_cost) HIPCHK(c, hipMemcpyAsync(best_cost, d_bc.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    if (best_model) HIPCHK(c, hipMemcpyAsync(best_model, d_bm.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+int lcf_batch_pso(lcf_ctx* c, int model, const lcf_batch* b, int P, int max_iters, int stall_iters,
+                  uint64_t seed, double* out_params, double* out_cost) {
+    if (!c) return LCF_E_INVALID;
+    if (model < 0 || model >= LCF_N_MODELS) return fail(c, LCF_E_INVALID, "unknown model id %d", model);
+    int32_t m = model;
+    return lcf_batch_fit_models(c, b, &m, 1, 0, 0.0, P, max_iters, stall_iters, seed, 0u, out_params,
+                                out_cost, nullptr, nullptr, nullptr, nullptr);
+}
+
+int lcf_batch_model_select(lcf_ctx* c, const lcf_batch* b, int P, int max_iters, int stall_iters,
+                           double gate, int32_t* out_model, double* out_params, double* out_cost) {
+    return lcf_batch_fit_models(c, b, ALL_MODELS, LCF_N_MODELS, 1, gate, P, max_iters, stall_iters, 42,
+                                0u, nullptr, nullptr, nullptr, out_model, out_params, out_cost);
+}
+
+int lcf_batch_all_models(lcf_ctx* c, const lcf_batch* b, int P, int max_iters, int stall_iters,
+                         double* out_params, double* out_cost) {
+    return lcf_batch_fit_models(c, b, ALL_MODELS, LCF_N_MODELS, 0, 0.0, P, max_iters, stall_iters, 42,
+                                0u, out_params, out_cost, nullptr, nullptr, nullptr, nullptr);
+}
+
+}  // extern "C"

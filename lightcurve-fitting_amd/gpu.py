@@ -1,0 +1,296 @@
+"""Host-side mirror of the reference's GPU batch interface.
+
+Same names, argument meaning and error behaviour as the reference's
+``src/gpu_types.rs`` (types) and ``src/gpu_cuda.rs`` (``GpuContext``,
+``GpuBatchData``); every call goes through the C ABI of liblcf_hip.so.
+Errors surface as ``GpuError`` carrying the library's message, the analogue
+of the reference's ``Err(String)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass, field
+from typing import Sequence
+
+import numpy as np
+
+from . import _lib
+
+MAX_PARAMS = 7
+FP32, FP64 = 0, 1
+SCHED_PERSISTENT, SCHED_PER_CLASS = 0, 1
+PSO_STALL_FROM_FIRST_EVAL = 1
+
+
+class GpuError(RuntimeError):
+    """Failure reported by the native library (reference: ``Err(String)``)."""
+
+
+class GpuModelName(enum.IntEnum):
+    """Model ids of ``GpuModelName::model_id`` (src/gpu_types.rs:51-62)."""
+
+    Bazin = 0
+    Villar = 1
+    Tde = 2
+    Arnett = 3
+    Magnetar = 4
+    ShockCooling = 5
+    Afterglow = 6
+    MetzgerKN = 7
+
+    def n_params(self) -> int:
+        """``GpuModelName::n_params`` (src/gpu_types.rs:23-34)."""
+        return (6, 7, 7, 5, 5, 5, 6, 5)[int(self)]
+
+    def model_id(self) -> int:
+        return int(self)
+
+    def pso_bounds(self) -> tuple[list[float], list[float]]:
+        """``GpuModelName::pso_bounds`` (src/gpu_types.rs:64-103)."""
+        return [list(x) for x in _PSO_BOUNDS[int(self)]]
+
+
+_PSO_BOUNDS = (
+    ((-3.0, -1.0, -100.0, -2.0, -2.0, -5.0), (3.0, 1.0, 100.0, 5.0, 6.0, 0.0)),
+    ((-1.5, -0.02, 1.0, -60.0, -1.0, 1.0, -6.0), (1.5, 0.05, 5.0, 40.0, 3.0, 5.5, 0.0)),
+    ((-3.0, -1.0, -100.0, -2.0, -1.0, 0.5, -5.0), (3.0, 1.0, 100.0, 5.0, 6.0, 4.0, 0.0)),
+    ((-3.0, -100.0, 0.5, -3.0, -5.0), (3.0, 100.0, 4.5, 3.0, 0.0)),
+    ((-3.0, -100.0, 0.0, 0.5, -5.0), (3.0, 100.0, 6.0, 4.5, 0.0)),
+    ((-3.0, -100.0, 0.1, -1.0, -5.0), (3.0, 100.0, 3.0, 4.0, 0.0)),
+    ((-3.0, -100.0, -2.0, -2.0, 0.5, -5.0), (3.0, 100.0, 6.0, 3.0, 5.0, 0.0)),
+    ((-3.0, -2.0, -1.0, -2.0, -5.0), (-0.5, -0.5, 2.0, 1.0, 0.0)),
+)
+
+# ALL_GPU_MODELS (src/gpu_types.rs:107-116)
+ALL_GPU_MODELS = (
+    GpuModelName.Bazin, GpuModelName.Arnett, GpuModelName.Tde, GpuModelName.Afterglow,
+    GpuModelName.Villar, GpuModelName.Magnetar, GpuModelName.ShockCooling, GpuModelName.MetzgerKN,
+)
+
+
+@dataclass
+class BatchSource:
+    """Pre-normalised observations of one source/band (src/gpu_types.rs:122-130)."""
+
+    times: Sequence[float]
+    flux: Sequence[float]
+    obs_var: Sequence[float]
+    is_upper: Sequence[bool]
+    upper_flux: Sequence[float]
+
+
+@dataclass
+class BatchPsoResult:
+    """Best position and cost of one swarm (src/gpu_types.rs:133-137)."""
+
+    params: list = field(default_factory=list)
+    cost: float = float("inf")
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class GpuContext:
+    """``GpuContext`` (src/gpu_cuda.rs:297-1423) on one HIP device."""
+
+    def __init__(self, device: int = 0, precision: int = FP32, schedule: int = SCHED_PERSISTENT):
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        rc = self._lib.lcf_ctx_create(int(device), C.byref(h))
+        if rc != 0 or not h.value:
+            raise GpuError(f"HIP device {device} unavailable (status {rc})")
+        self._h = h
+        self.device = device
+        self.set_precision(precision)
+        self.set_schedule(schedule)
+
+    @classmethod
+    def new(cls, device: int = 0) -> "GpuContext":
+        return cls(device)
+
+    # -- plumbing -----------------------------------------------------------
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self._lib.lcf_last_error(self._h)
+            raise GpuError((msg or b"").decode() or f"status {rc}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.lcf_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_precision(self, precision: int) -> None:
+        self._check(self._lib.lcf_ctx_set_precision(self._h, int(precision)))
+        self.precision = int(precision)
+
+    def set_schedule(self, schedule: int) -> None:
+        self._check(self._lib.lcf_ctx_set_schedule(self._h, int(schedule)))
+        self.schedule = int(schedule)
+
+    @property
+    def num_cus(self) -> int:
+        return int(self._lib.lcf_ctx_num_cus(self._h))
+
+    @property
+    def last_kernel_ms(self) -> float:
+        return float(self._lib.lcf_last_kernel_ms(self._h))
+
+    def last_model_ms(self, slot: int) -> float:
+        """Device time of model slot `slot` in the last batched fit."""
+        return float(self._lib.lcf_last_model_ms(self._h, int(slot)))
+
+    # -- GpuContext::eval_batch (src/gpu_cuda.rs:317-360) --------------------
+    def eval_batch(self, model: GpuModelName, params, times, n_draws: int) -> np.ndarray:
+        model = GpuModelName(model)
+        p, t = _f64(params).ravel(), _f64(times).ravel()
+        npar = model.n_params()
+        if p.size != n_draws * npar:
+            raise GpuError(f"params length {p.size} != n_draws({n_draws}) x n_params({npar})")
+        out = np.empty(n_draws * t.size, dtype=np.float64)
+        self._check(self._lib.lcf_eval_batch(self._h, int(model), _lib.ptr(p, C.c_double), n_draws,
+                                             _lib.ptr(t, C.c_double), t.size, _lib.ptr(out, C.c_double)))
+        return out
+
+    # -- batched PSO ----------------------------------------------------------
+    def fit_models(self, data: "GpuBatchData", models: Sequence[int], n_particles: int,
+                   max_iters: int, stall_iters: int, *, seed: int = 42, gate: float | None = None,
+                   flags: int = 0, per_model: bool = True) -> dict:
+        """Outputs of ``lcf_batch_fit_models`` as numpy arrays (per-model
+        arrays are None when ``per_model`` is false)."""
+        ms = np.ascontiguousarray(np.asarray([int(m) for m in models], dtype=np.int32))
+        ns, nm = data.n_sources, ms.size
+        out = {
+            "model_params": np.zeros((nm, ns, MAX_PARAMS)) if per_model else None,
+            "model_cost": np.zeros((nm, ns)) if per_model else None,
+            "model_iters": np.zeros((nm, ns), dtype=np.int32) if per_model else None,
+            "best_model": np.zeros(ns, dtype=np.int32),
+            "best_params": np.zeros((ns, MAX_PARAMS)),
+            "best_cost": np.zeros(ns),
+        }
+        rc = self._lib.lcf_batch_fit_models(
+            self._h, data._h, _lib.ptr(ms, C.c_int32), nm, 0 if gate is None else 1,
+            0.0 if gate is None else float(gate), n_particles, max_iters, stall_iters,
+            C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), flags,
+            _lib.ptr(out["model_params"], C.c_double), _lib.ptr(out["model_cost"], C.c_double),
+            _lib.ptr(out["model_iters"], C.c_int32), _lib.ptr(out["best_model"], C.c_int32),
+            _lib.ptr(out["best_params"], C.c_double), _lib.ptr(out["best_cost"], C.c_double))
+        self._check(rc)
+        return out
+
+    def batch_pso(self, model: GpuModelName, data: "GpuBatchData", n_particles: int,
+                  max_iters: int, stall_iters: int, seed: int) -> list[BatchPsoResult]:
+        """``GpuContext::batch_pso`` (src/gpu_cuda.rs:368-567)."""
+        model = GpuModelName(model)
+        ns = data.n_sources
+        prm = np.zeros((ns, MAX_PARAMS))
+        cost = np.zeros(ns)
+        self._check(self._lib.lcf_batch_pso(self._h, int(model), data._h, n_particles, max_iters,
+                                            stall_iters, C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF),
+                                            _lib.ptr(prm, C.c_double), _lib.ptr(cost, C.c_double)))
+        npar = model.n_params()
+        return [BatchPsoResult(list(prm[s, :npar]), float(cost[s])) for s in range(ns)]
+
+    def batch_model_select(self, data: "GpuBatchData", n_particles: int, max_iters: int,
+                           stall_iters: int, bazin_good_enough: float):
+        """``GpuContext::batch_model_select`` (src/gpu_cuda.rs:573-620)."""
+        ns = data.n_sources
+        mdl = np.zeros(ns, dtype=np.int32)
+        prm = np.zeros((ns, MAX_PARAMS))
+        cost = np.zeros(ns)
+        self._check(self._lib.lcf_batch_model_select(
+            self._h, data._h, n_particles, max_iters, stall_iters, float(bazin_good_enough),
+            _lib.ptr(mdl, C.c_int32), _lib.ptr(prm, C.c_double), _lib.ptr(cost, C.c_double)))
+        res = []
+        for s in range(ns):
+            m = GpuModelName(int(mdl[s]))
+            res.append((m, BatchPsoResult(list(prm[s, :m.n_params()]), float(cost[s]))))
+        return res
+
+    def batch_all_models(self, data: "GpuBatchData", n_particles: int, max_iters: int,
+                         stall_iters: int):
+        """``GpuContext::batch_all_models`` (src/gpu_cuda.rs:624-637)."""
+        ns = data.n_sources
+        prm = np.zeros((len(ALL_GPU_MODELS), ns, MAX_PARAMS))
+        cost = np.zeros((len(ALL_GPU_MODELS), ns))
+        self._check(self._lib.lcf_batch_all_models(self._h, data._h, n_particles, max_iters,
+                                                   stall_iters, _lib.ptr(prm, C.c_double),
+                                                   _lib.ptr(cost, C.c_double)))
+        out = []
+        for k, m in enumerate(ALL_GPU_MODELS):
+            out.append((m, [BatchPsoResult(list(prm[k, s, :m.n_params()]), float(cost[k, s]))
+                            for s in range(ns)]))
+        return out
+
+
+class GpuBatchData:
+    """Resident CSR observation data (``GpuBatchData``, src/gpu_cuda.rs:236-292)."""
+
+    def __init__(self, ctx: GpuContext, sources: Sequence[BatchSource] | None = None, *,
+                 csr: dict | None = None):
+        self._ctx = ctx
+        self._lib = ctx._lib
+        if csr is None:
+            csr = sources_to_csr(sources or [])
+        off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+        t = _f64(csr["times"])
+        f = _f64(csr["flux"])
+        v = _f64(csr["obs_var"])
+        up = np.ascontiguousarray(np.asarray(csr["is_upper"], dtype=np.int32))
+        uf = _f64(csr["upper_flux"])
+        n = int(off[-1]) if off.size else 0
+        for name, a in (("times", t), ("flux", f), ("obs_var", v), ("is_upper", up), ("upper_flux", uf)):
+            if a.size != n:
+                raise GpuError(f"{name} has {a.size} entries, offsets say {n}")
+        h = C.c_void_p()
+        rc = self._lib.lcf_batch_create(ctx._h, off.size - 1, _lib.ptr(off, C.c_int64),
+                                        _lib.ptr(t, C.c_double), _lib.ptr(f, C.c_double),
+                                        _lib.ptr(v, C.c_double), _lib.ptr(up, C.c_int32),
+                                        _lib.ptr(uf, C.c_double), C.byref(h))
+        ctx._check(rc)
+        self._h = h
+        self.n_sources = off.size - 1
+        self._n_obs = np.diff(off)
+
+    @classmethod
+    def new(cls, ctx: GpuContext, sources: Sequence[BatchSource]) -> "GpuBatchData":
+        return cls(ctx, sources)
+
+    def n_obs_per_source(self, source_idx: int) -> int:
+        return int(self._lib.lcf_batch_n_obs(self._h, int(source_idx)))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.lcf_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sources_to_csr(sources: Sequence[BatchSource]) -> dict:
+    """Concatenate ``BatchSource``s into the CSR arrays of GpuBatchData::new."""
+    off = [0]
+    cols = {k: [] for k in ("times", "flux", "obs_var", "is_upper", "upper_flux")}
+    for s in sources:
+        n = len(s.times)
+        for k in cols:
+            a = getattr(s, k)
+            if len(a) != n:
+                raise GpuError(f"BatchSource.{k} has {len(a)} entries, times has {n}")
+            cols[k].append(np.asarray(a, dtype=np.int32 if k == "is_upper" else np.float64))
+        off.append(off[-1] + n)
+    out = {k: (np.concatenate(v) if v else np.zeros(0, dtype=np.int32 if k == "is_upper" else np.float64))
+           for k, v in cols.items()}
+    out["offsets"] = np.asarray(off, dtype=np.int64)
+    return out

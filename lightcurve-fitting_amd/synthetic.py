@@ -1,0 +1,158 @@
+"""Seeded synthetic light-curve batches for the benchmark and parity tests.
+
+Each generator returns per-band ``BatchSource`` data in the CSR layout of
+``GpuBatchData`` (offsets, times, flux, obs_var, is_upper, upper_flux) plus
+bookkeeping (source id of each band, true model).  Band normalisation is the
+reference's (src/bin/extract_features.rs:548-576): flux and errors divided by
+the band's peak observed flux, ``obs_var = err^2 + 1e-10``, an observation is
+an upper limit when ``flux/err < 3`` and then ``upper_flux = 3 err / peak``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# Models of the ragged model-selection workload: Bazin, Villar, Tde, Arnett,
+# Magnetar (model ids of GpuModelName).
+SELECTION_MODELS = (0, 1, 2, 3, 4)
+
+# Sampling boxes (internal parameterisation, last entry ln sigma_x unused)
+# chosen so that every curve peaks inside a [0, 150] d window.
+_BOXES = {
+    0: ((-0.5, -0.05, 20.0, 0.0, 2.3, -5.0), (0.5, 0.05, 80.0, 2.3, 4.1, -5.0)),
+    1: ((-0.5, 0.0, 1.5, 10.0, 0.0, 2.0, -5.0), (0.5, 0.02, 3.5, 60.0, 2.0, 4.0, -5.0)),
+    2: ((-0.5, -0.05, 10.0, 0.5, 1.0, 1.0, -5.0), (0.5, 0.05, 60.0, 2.5, 3.0, 2.5, -5.0)),
+    3: ((-0.5, 5.0, 2.0, -2.0, -5.0), (0.5, 50.0, 3.5, 2.0, -5.0)),
+    4: ((-0.5, 5.0, 2.0, 1.5, -5.0), (0.5, 50.0, 4.0, 3.0, -5.0)),
+}
+
+
+def _sig(x):
+    return 1.0 / (1.0 + np.exp(-x))
+
+
+def _sp(x):
+    return np.logaddexp(0.0, x) + 1e-6
+
+
+def model_flux(model: int, p: np.ndarray, t: np.ndarray) -> np.ndarray:
+    """Closed-form model flux; p is [n_params] or broadcastable rows [.., n_params]."""
+    p = np.asarray(p, dtype=np.float64)
+    c = [p[..., j] for j in range(p.shape[-1])]
+    with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+        if model == 0:
+            dt = t - c[2]
+            return np.exp(c[0]) * np.exp(-dt / np.exp(c[4])) * _sig(dt / np.exp(c[3])) + c[1]
+        if model == 1:
+            a, beta, g, ph = np.exp(c[0]), c[1], np.exp(c[2]), t - c[3]
+            w = _sig(10.0 * (ph - g))
+            return a * _sig(ph / np.exp(c[4])) * ((1 - w) * (1 - beta * ph)
+                                                  + w * (1 - beta * g) * np.exp((g - ph) / np.exp(c[5])))
+        if model == 2:
+            ph = t - c[2]
+            return (np.exp(c[0]) * _sig(ph / np.exp(c[3]))
+                    * (1.0 + _sp(ph) / np.exp(c[4])) ** (-c[5]) + c[1])
+        if model == 3:
+            ps = _sp(t - c[1])
+            f = _sig(c[3])
+            heat = f * np.exp(-ps / 8.8) + (1 - f) * np.exp(-ps / 111.3)
+            return np.exp(c[0]) * heat * (1 - np.exp(-(ps / np.exp(c[2])) ** 2))
+        if model == 4:
+            ps = _sp(t - c[1])
+            w = 1 + ps / np.exp(c[2])
+            return np.exp(c[0]) / (w * w) * (1 - np.exp(-(ps / np.exp(c[3])) ** 2))
+    raise ValueError(f"no generator for model {model}")
+
+
+def ragged_selection_batch(n_sources: int, seed: int = 0, n_min: int = 10, n_max: int = 500,
+                           t_max: float = 150.0) -> dict:
+    """Ragged model-selection workload.
+
+    ``n_sources`` sources with U{1,2,3} bands each and log-uniform[n_min, n_max]
+    points per band; the true model is drawn uniformly from SELECTION_MODELS
+    with parameters uniform in a per-model box; Gaussian noise with peak
+    SNR ~ U(5, 50); times uniform on [0, t_max], sorted.
+    """
+    rng = np.random.Generator(np.random.Philox(seed))
+    nb = rng.integers(1, 4, size=n_sources)
+    band_src = np.repeat(np.arange(n_sources), nb)
+    n_bands = band_src.size
+    src_model = rng.choice(np.asarray(SELECTION_MODELS), size=n_sources)
+    band_model = src_model[band_src]
+    npts = np.floor(np.exp(rng.uniform(np.log(n_min), np.log(n_max + 1), size=n_bands))).astype(np.int64)
+    npts = np.clip(npts, n_min, n_max)
+    off = np.zeros(n_bands + 1, dtype=np.int64)
+    np.cumsum(npts, out=off[1:])
+    n = int(off[-1])
+    band_of = np.repeat(np.arange(n_bands), npts)
+    t = rng.uniform(0.0, t_max, size=n)
+    # sort times within each band
+    order = np.lexsort((t, band_of))
+    t = t[order]
+    # true parameters per source (shared shape across its bands)
+    params = np.zeros((n_sources, 7))
+    for m in SELECTION_MODELS:
+        sel = src_model == m
+        lo, hi = (np.asarray(x) for x in _BOXES[m])
+        params[sel, :lo.size] = rng.uniform(lo, hi, size=(int(sel.sum()), lo.size))
+    clean = np.empty(n)
+    for m in SELECTION_MODELS:
+        bm = band_model[band_of] == m
+        npar = len(_BOXES[m][0])
+        clean[bm] = model_flux(m, params[band_src[band_of[bm]], :npar], t[bm])
+    clean = np.nan_to_num(clean, nan=0.0, posinf=0.0, neginf=0.0)
+    peak_true = np.maximum.reduceat(np.abs(clean), off[:-1]) if n else np.zeros(0)
+    peak_true = np.where(peak_true > 0, peak_true, 1.0)
+    snr = rng.uniform(5.0, 50.0, size=n_bands)
+    sigma_b = peak_true / snr
+    sig = sigma_b[band_of]
+    fobs = clean + rng.normal(size=n) * sig
+    csr = normalise_bands(off, t, fobs, sig)
+    kept = csr["kept_bands"]
+    csr["band_source"] = band_src[kept]
+    csr["true_model"] = band_model[kept]
+    csr["true_params"] = params
+    csr["n_sources"] = n_sources
+    return csr
+
+
+def normalise_bands(off: np.ndarray, t: np.ndarray, flux: np.ndarray, err: np.ndarray) -> dict:
+    """Reference band normalisation (src/bin/extract_features.rs:556-566)."""
+    nb = off.size - 1
+    npts = np.diff(off)
+    band_of = np.repeat(np.arange(nb), npts)
+    peak = np.full(nb, -np.inf)
+    if flux.size:
+        np.maximum.at(peak, band_of, flux)
+    good = peak > 0
+    # bands whose peak is not positive are dropped, as the reference does
+    keep = good[band_of]
+    new_np = npts[good]
+    new_off = np.zeros(new_np.size + 1, dtype=np.int64)
+    np.cumsum(new_np, out=new_off[1:])
+    pk = peak[band_of][keep]
+    f = flux[keep] / pk
+    e = err[keep] / pk
+    is_up = (err[keep] > 0) & (flux[keep] / err[keep] < 3.0)
+    return {
+        "offsets": new_off,
+        "times": t[keep].copy(),
+        "flux": f,
+        "obs_var": e * e + 1e-10,
+        "is_upper": is_up.astype(np.int32),
+        "upper_flux": 3.0 * e,
+        "kept_bands": np.nonzero(good)[0],
+    }
+
+
+def subset(csr: dict, bands: np.ndarray) -> dict:
+    """CSR restricted to the given band indices (in that order)."""
+    off = csr["offsets"]
+    bands = np.asarray(bands, dtype=np.int64)
+    npts = off[bands + 1] - off[bands]
+    new_off = np.zeros(bands.size + 1, dtype=np.int64)
+    np.cumsum(npts, out=new_off[1:])
+    idx = np.concatenate([np.arange(off[b], off[b + 1]) for b in bands]) if bands.size else np.zeros(0, np.int64)
+    out = {"offsets": new_off}
+    for k in ("times", "flux", "obs_var", "is_upper", "upper_flux"):
+        out[k] = csr[k][idx]
+    return out

@@ -1,0 +1,106 @@
+/*
+ * lcf_oracle.h -- CPU FP64 restatement of the lightcurve-fitting batched
+ * fitting path.  TEST INFRASTRUCTURE ONLY: this library is the parity checker
+ * for the HIP product (tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg load it).  Nothing in the product links or calls it.
+ *
+ * Every function cites the reference file:line whose semantics it restates
+ * (paths relative to the reference repository root).
+ *
+ * Conventions
+ *   - model ids follow GpuModelName::model_id (src/gpu_types.rs:51-62):
+ *     0 Bazin, 1 Villar, 2 Tde, 3 Arnett, 4 Magnetar, 5 ShockCooling,
+ *     6 Afterglow, 7 MetzgerKN.
+ *   - a "batch" is the CSR layout of GpuBatchData::new (src/gpu_cuda.rs:251-285):
+ *     offsets[n_src+1] (int64 here), times/flux/obs_var/upper_flux (f64),
+ *     is_upper (i32).
+ *   - return value 0 = ok, negative = invalid argument.
+ */
+#ifndef LCF_ORACLE_H
+#define LCF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- model metadata ---------------------------------------------------- */
+int orc_n_params(int model_id);
+/* GPU PSO box bounds, GpuModelName::pso_bounds (src/gpu_types.rs:64-103). */
+int orc_pso_bounds(int model_id, double* lower, double* upper);
+/* CPU PSO box bounds, pso_bounds (src/parametric.rs:1541-1579). */
+int orc_pso_bounds_cpu(int model_id, double* lower, double* upper);
+/* population_priors (src/parametric.rs:1588-1621): centers/widths[np]. */
+int orc_population_priors(int model_id, double* centers, double* widths);
+/* prior_params (src/parametric.rs:2072-2115). */
+int orc_prior_params(int model_id, double* centers, double* widths);
+
+/* ---- model evaluation ---------------------------------------------------- */
+/* eval_model / MetzgerKN batch (src/parametric.rs:512-538, 544-648); this is
+ * also GpuContext::eval_batch semantics (cuda/models.cu:146-282, 387-407).
+ * out[d*n_t+i] = f(params[d*np..], times[i]). */
+int orc_eval_batch(int model_id, const double* params, int64_t n_draws,
+                   const double* times, int64_t n_times, double* out);
+/* Per-observation MetzgerKN used by the GPU cost kernel
+ * (cuda/models.cu:429-514): 100-point grid re-solved per time. */
+double orc_metzger_kn_single(const double* p, double t);
+/* eval_model_grad (src/parametric.rs:527-538): out[np] = d f / d p.
+ * Returns -1 for MetzgerKN (no analytic gradient). */
+int orc_eval_grad(int model_id, const double* p, double t, double* out);
+
+/* ---- likelihood helpers ------------------------------------------------- */
+double orc_log_normal_cdf(double x);   /* src/parametric.rs:882-901 */
+double orc_erf_approx(double x);       /* src/parametric.rs:867-880 */
+
+/* ---- PSO cost -------------------------------------------------------------- */
+/* GPU-path cost: kernel batch_pso_cost (cuda/models.cu:516-584) plus the
+ * host-side population prior / n^2 (src/gpu_cuda.rs:461-480). */
+double orc_pso_cost_gpu(int model_id, const double* p, const double* t,
+                        const double* f, const double* v, const int32_t* up,
+                        const double* uf, int64_t n);
+/* CPU-path cost: PsoCost::cost_from_slice (src/parametric.rs:1303-1370),
+ * with the Villar constraint and MetzgerKN rescale. */
+double orc_pso_cost_cpu(int model_id, const double* p, const double* t,
+                        const double* f, const double* v, const int32_t* up,
+                        const double* uf, int64_t n);
+/* PsoCost::gradient (src/parametric.rs:1383-1446). */
+int orc_pso_cost_grad_cpu(int model_id, const double* p, const double* t,
+                          const double* f, const double* v, const int32_t* up,
+                          const double* uf, int64_t n, double* grad);
+
+/* ---- counter-based RNG ---------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11). */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- batched PSO (GPU semantics) ------------------------------------------ */
+/* flags for orc_batch_pso / orc_batch_model_select */
+#define ORC_PSO_STALL_FROM_FIRST_EVAL 1u  /* opt-in: stall baseline = first gbest */
+
+/* GpuContext::batch_pso (src/gpu_cuda.rs:368-567): synchronous swarm update,
+ * Philox stream keyed by (seed, source index) in place of SmallRng.
+ * out_params[n_src*np], out_cost[n_src], out_iters[n_src] (may be NULL). */
+int orc_batch_pso(int model_id, int64_t n_src, const int64_t* offsets,
+                  const double* t, const double* f, const double* v,
+                  const int32_t* up, const double* uf,
+                  int n_particles, int max_iters, int stall_iters,
+                  uint64_t seed, uint32_t flags,
+                  double* out_params, double* out_cost, int32_t* out_iters,
+                  int n_threads);
+
+/* GpuContext::batch_model_select (src/gpu_cuda.rs:573-620) over an explicit
+ * model list (models[0] is the gate model); seed 42 as the reference.
+ * out_model[n_src], out_params[n_src*7] (padded with 0), out_cost[n_src]. */
+int orc_batch_model_select(int64_t n_src, const int64_t* offsets,
+                           const double* t, const double* f, const double* v,
+                           const int32_t* up, const double* uf,
+                           const int32_t* models, int n_models,
+                           int n_particles, int max_iters, int stall_iters,
+                           double gate, uint32_t flags,
+                           int32_t* out_model, double* out_params,
+                           double* out_cost, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""ctypes binding of the oracle library (liblcf_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, always as the checker / baseline, never as the
+product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+LIB_PATH = ORACLE_DIR / "liblcf_oracle.so"
+
+_pd = C.POINTER(C.c_double)
+_pi32 = C.POINTER(C.c_int32)
+_pi64 = C.POINTER(C.c_int64)
+_pu32 = C.POINTER(C.c_uint32)
+_i32, _i64, _u32, _u64, _f64 = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double
+
+SIGNATURES = {
+    "orc_n_params": (_i32, [_i32]),
+    "orc_pso_bounds": (_i32, [_i32, _pd, _pd]),
+    "orc_pso_bounds_cpu": (_i32, [_i32, _pd, _pd]),
+    "orc_population_priors": (_i32, [_i32, _pd, _pd]),
+    "orc_prior_params": (_i32, [_i32, _pd, _pd]),
+    "orc_eval_batch": (_i32, [_i32, _pd, _i64, _pd, _i64, _pd]),
+    "orc_metzger_kn_single": (_f64, [_pd, _f64]),
+    "orc_eval_grad": (_i32, [_i32, _pd, _f64, _pd]),
+    "orc_log_normal_cdf": (_f64, [_f64]),
+    "orc_erf_approx": (_f64, [_f64]),
+    "orc_pso_cost_gpu": (_f64, [_i32, _pd, _pd, _pd, _pd, _pi32, _pd, _i64]),
+    "orc_pso_cost_cpu": (_f64, [_i32, _pd, _pd, _pd, _pd, _pi32, _pd, _i64]),
+    "orc_pso_cost_grad_cpu": (_i32, [_i32, _pd, _pd, _pd, _pd, _pi32, _pd, _i64, _pd]),
+    "orc_philox4x32_10": (None, [_pu32, _pu32, _pu32]),
+    "orc_batch_pso": (_i32, [_i32, _i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _i32, _i32, _i32,
+                             _u64, _u32, _pd, _pd, _pi32, _i32]),
+    "orc_batch_model_select": (_i32, [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _pi32, _i32, _i32,
+                                      _i32, _i32, _f64, _u32, _pi32, _pd, _pd, _i32]),
+}
+
+_LIB = None
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (g++)."""
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build()
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def n_params(m: int) -> int:
+    return lib().orc_n_params(m)
+
+
+def bounds(m: int, cpu: bool = False):
+    n = n_params(m)
+    lo, hi = np.zeros(n), np.zeros(n)
+    (lib().orc_pso_bounds_cpu if cpu else lib().orc_pso_bounds)(m, _p(lo, C.c_double), _p(hi, C.c_double))
+    return lo, hi
+
+
+def population_priors(m: int):
+    n = n_params(m)
+    c, w = np.zeros(n), np.zeros(n)
+    lib().orc_population_priors(m, _p(c, C.c_double), _p(w, C.c_double))
+    return c, w
+
+
+def prior_params(m: int):
+    n = n_params(m)
+    c, w = np.zeros(n), np.zeros(n)
+    lib().orc_prior_params(m, _p(c, C.c_double), _p(w, C.c_double))
+    return c, w
+
+
+def eval_batch(m: int, params, times) -> np.ndarray:
+    p, t = f64(params).ravel(), f64(times).ravel()
+    nd = p.size // n_params(m)
+    out = np.empty(nd * t.size)
+    rc = lib().orc_eval_batch(m, _p(p, C.c_double), nd, _p(t, C.c_double), t.size, _p(out, C.c_double))
+    assert rc == 0
+    return out
+
+
+def eval_grad(m: int, params, t: float) -> np.ndarray:
+    p = f64(params)
+    out = np.zeros(n_params(m))
+    rc = lib().orc_eval_grad(m, _p(p, C.c_double), float(t), _p(out, C.c_double))
+    if rc != 0:
+        raise ValueError("no analytic gradient")
+    return out
+
+
+def metzger_single(params, t: float) -> float:
+    p = f64(params)
+    return lib().orc_metzger_kn_single(_p(p, C.c_double), float(t))
+
+
+def log_normal_cdf(x: float) -> float:
+    return lib().orc_log_normal_cdf(float(x))
+
+
+def erf_approx(x: float) -> float:
+    return lib().orc_erf_approx(float(x))
+
+
+def _obs(src: dict, a: int = 0, b: int | None = None):
+    b = src["offsets"][-1] if b is None else b
+    return (f64(src["times"][a:b]), f64(src["flux"][a:b]), f64(src["obs_var"][a:b]),
+            i32(src["is_upper"][a:b]), f64(src["upper_flux"][a:b]))
+
+
+def pso_cost(m: int, params, t, f, v, up, uf, cpu: bool = False) -> float:
+    p = f64(params)
+    t, f, v, uf = f64(t), f64(f), f64(v), f64(uf)
+    up = i32(up)
+    fn = lib().orc_pso_cost_cpu if cpu else lib().orc_pso_cost_gpu
+    return fn(m, _p(p, C.c_double), _p(t, C.c_double), _p(f, C.c_double), _p(v, C.c_double),
+              _p(up, C.c_int32), _p(uf, C.c_double), t.size)
+
+
+def pso_cost_grad(m: int, params, t, f, v, up, uf) -> np.ndarray:
+    p = f64(params)
+    t, f, v, uf = f64(t), f64(f), f64(v), f64(uf)
+    up = i32(up)
+    g = np.zeros(p.size)
+    rc = lib().orc_pso_cost_grad_cpu(m, _p(p, C.c_double), _p(t, C.c_double), _p(f, C.c_double),
+                                     _p(v, C.c_double), _p(up, C.c_int32), _p(uf, C.c_double),
+                                     t.size, _p(g, C.c_double))
+    if rc != 0:
+        raise ValueError("no analytic gradient")
+    return g
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(np.asarray(ctr, dtype=np.uint32))
+    k = np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+    o = np.zeros(4, dtype=np.uint32)
+    lib().orc_philox4x32_10(_p(c, C.c_uint32), _p(k, C.c_uint32), _p(o, C.c_uint32))
+    return o
+
+
+def default_threads() -> int:
+    return max(1, min(os.cpu_count() or 1, 16))
+
+
+def batch_pso(m: int, csr: dict, n_particles: int, max_iters: int, stall_iters: int, seed: int,
+              flags: int = 0, n_threads: int | None = None):
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, f, v, up, uf = _obs(csr)
+    npar = n_params(m)
+    prm = np.zeros((ns, npar))
+    cost = np.zeros(ns)
+    iters = np.zeros(ns, dtype=np.int32)
+    rc = lib().orc_batch_pso(m, ns, _p(off, C.c_int64), _p(t, C.c_double), _p(f, C.c_double),
+                             _p(v, C.c_double), _p(up, C.c_int32), _p(uf, C.c_double), n_particles,
+                             max_iters, stall_iters, C.c_uint64(seed), flags, _p(prm, C.c_double),
+                             _p(cost, C.c_double), _p(iters, C.c_int32),
+                             n_threads or default_threads())
+    assert rc == 0
+    return prm, cost, iters
+
+
+def batch_model_select(csr: dict, models, n_particles: int, max_iters: int, stall_iters: int,
+                       gate: float, flags: int = 0, n_threads: int | None = None):
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, f, v, up, uf = _obs(csr)
+    ms = i32(models)
+    bm = np.zeros(ns, dtype=np.int32)
+    prm = np.zeros((ns, 7))
+    cost = np.zeros(ns)
+    rc = lib().orc_batch_model_select(ns, _p(off, C.c_int64), _p(t, C.c_double), _p(f, C.c_double),
+                                      _p(v, C.c_double), _p(up, C.c_int32), _p(uf, C.c_double),
+                                      _p(ms, C.c_int32), ms.size, n_particles, max_iters, stall_iters,
+                                      float(gate), flags, _p(bm, C.c_int32), _p(prm, C.c_double),
+                                      _p(cost, C.c_double), n_threads or default_threads())
+    assert rc == 0
+    return bm, prm, cost

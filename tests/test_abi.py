@@ -1,0 +1,76 @@
+"""C-ABI boundary checks that need no GPU: the library loads, exports every
+symbol include/lcf_hip.h declares, and the host-side argument checks fail
+cleanly (no device work is started)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "lcf_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lcf_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_reference_entry_points():
+    syms = declared_symbols()
+    for s in ("lcf_ctx_create", "lcf_eval_batch", "lcf_batch_create", "lcf_batch_pso",
+              "lcf_batch_model_select", "lcf_batch_all_models", "lcf_batch_n_obs"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    lib = C.CDLL(str(pkg.LIB_PATH))
+    missing = [s for s in declared_symbols() if not hasattr(lib, s)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header(pkg):
+    from lightcurve_fitting_amd import _lib
+
+    assert set(declared_symbols()) <= set(_lib.SIGNATURES)
+
+
+def test_null_and_bad_arguments_rejected_without_device(pkg):
+    from lightcurve_fitting_amd import _lib
+
+    lib = _lib.load()
+    assert lib.lcf_ctx_create(0, None) == -1
+    assert lib.lcf_ctx_set_precision(None, 0) == -1
+    assert lib.lcf_batch_n_obs(None, 0) == 1          # reference: 1 when out of range
+    assert lib.lcf_batch_n_sources(None) == 0
+    assert lib.lcf_last_error(None) == b"null context"
+    h = C.c_void_p()
+    # device index beyond the device count (there is none here)
+    assert lib.lcf_ctx_create(10_000, C.byref(h)) == -1 and not h.value
+
+
+def test_model_metadata_mirrors_reference(pkg):
+    M = pkg.GpuModelName
+    assert [m.n_params() for m in M] == [6, 7, 7, 5, 5, 5, 6, 5]
+    assert [int(m) for m in pkg.ALL_GPU_MODELS] == [0, 3, 2, 6, 1, 4, 5, 7]
+    lo, hi = M.Villar.pso_bounds()
+    assert lo == [-1.5, -0.02, 1.0, -60.0, -1.0, 1.0, -6.0] and hi[-1] == 0.0
+
+
+def test_kernel_bounds_tables_match_oracle(orc):
+    """The __constant__ tables in pso_kernels.hip carry the same numbers as the oracle."""
+    src = (ROOT / "lightcurve-fitting_amd" / "csrc" / "pso_kernels.hip").read_text()
+
+    def table(name):
+        m = re.search(name + r"\[8\]\[7\] = \{(.*?)\};", src, re.S)
+        rows = re.findall(r"\{([^{}]*)\}", m.group(1))
+        return [[eval(x.replace("LN10", "2.30258509299404568402")) for x in r.split(",")] for r in rows]
+
+    lo, hi = table("c_lo"), table("c_hi")
+    pc, pw = table("c_pc"), table("c_pw")
+    for m in range(8):
+        n = orc.n_params(m)
+        olo, ohi = orc.bounds(m)
+        assert lo[m][:n] == pytest.approx(list(olo)) and hi[m][:n] == pytest.approx(list(ohi))
+        c, w = orc.population_priors(m)
+        assert pc[m][:n] == pytest.approx(list(c)) and pw[m][:n] == pytest.approx(list(w))
