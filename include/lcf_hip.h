@@ -114,6 +114,41 @@ int lcf_batch_fit_models(lcf_ctx* ctx, const lcf_batch* data, const int32_t* mod
                          double* model_params, double* model_cost, int32_t* model_iters,
                          int32_t* best_model, double* best_params, double* best_cost);
 
+/* ---- batched SVI ----------------------------------------------------------- */
+/* SVI flags */
+#define LCF_SVI_CPU_STOPPING 1u  /* early stop after 200 steps / stall 50 (svi_fit constants,
+                                    src/parametric.rs:2262-2263) instead of 50 / 20 */
+#define LCF_SVI_INFLATE_SIGMA 2u /* add ln 4 to the returned log sigma (src/parametric.rs:2441-2446) */
+#define LCF_SVI_FD_GRADIENT 4u   /* numerical (forward-difference) model gradients instead of the
+                                    analytic adjoint: the plain comparison path */
+
+/* GpuContext::batch_svi_fit (src/gpu_cuda.rs:1315-1422): one job per source
+ * of `data` (the SviBatchInput list is parallel to the batch).  Rows of
+ * pso_params / prior_centers / prior_widths / out_mu / out_log_sigma are
+ * padded to LCF_MAX_PARAMS; se_idx[j] must be n_params(model)-1 as produced by
+ * svi_model_meta (src/parametric.rs:3277-3290).  Seed 42. */
+int lcf_batch_svi_fit(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_ids,
+                      const double* pso_params, const int32_t* se_idx,
+                      const double* prior_centers, const double* prior_widths, int n_steps,
+                      int n_samples, double lr, double* out_mu, double* out_log_sigma,
+                      double* out_elbo);
+/* General form: explicit seed, LCF_SVI_* flags, steps executed per job. */
+int lcf_batch_svi_fit_ex(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_ids,
+                         const double* pso_params, const double* prior_centers,
+                         const double* prior_widths, int n_steps, int n_samples, double lr,
+                         uint64_t seed, uint32_t flags, double* out_mu, double* out_log_sigma,
+                         double* out_elbo, int32_t* out_steps);
+/* One Monte-Carlo ELBO estimate per job at (mu, log_sigma) with the noise of
+ * step `step`, and its exact gradient (the adjoint kernel; LCF_SVI_FD_GRADIENT
+ * selects forward differences).  No counterpart entry point in the reference:
+ * it exposes one step of batch_svi_fit (cuda/svi.cu:140-244) for gradient
+ * checks and autograd bindings. */
+int lcf_svi_elbo_grad(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_ids,
+                      const double* mu, const double* log_sigma, const double* prior_centers,
+                      const double* prior_widths, int n_samples, uint64_t seed, int step,
+                      uint32_t flags, double* out_elbo, double* out_grad_mu,
+                      double* out_grad_log_sigma);
+
 /* Timing of the last lcf_batch_fit_models-family call on this context:
  * milliseconds between HIP events around the kernel launches (inputs already
  * resident, outputs still on the device). */

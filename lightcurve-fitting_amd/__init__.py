@@ -19,7 +19,11 @@ from .gpu import (  # noqa: F401
     GpuContext,
     GpuError,
     GpuModelName,
+    SviBatchInput,
+    SviBatchOutput,
     sources_to_csr,
+    svi_model_meta,
+    svi_prior_for_model,
 )
 from . import synthetic  # noqa: F401
 from ._lib import LIB_PATH  # noqa: F401

@@ -1,0 +1,240 @@
+// models_grad.hpp -- value and parameter gradient of every model in one pass.
+//
+// pred_grad(t, f, g) returns the flux f and g[j] = df/dp_j for the internal
+// parameters (g[sigma_x] = 0), i.e. the reference's analytic gradients
+// (src/parametric.rs:213-510, cuda/models_device.h:152-347), in the same
+// overflow-free form as models_fit.hpp: every product of an exponential with
+// a logistic weight is evaluated as one exponential of a sum, and logistic
+// pairs s(x), 1-s(x), s(x)(1-s(x)) come from a single e^{-|x|}.  The
+// likelihood adjoint multiplies these by dLL/df, so the backward pass needs
+// no per-observation storage: it is recomputed with the forward value.
+// Returns false where the reference's FP64 value is non-finite.  In the
+// regimes where the reference's own gradient entries are NaN (softplus of a
+// phase above ~709.78 overflowing), the reference skips those entries, so
+// they are returned as 0 here.
+#pragma once
+#include "models_fit.hpp"
+
+namespace lcf {
+
+// logistic(x) with its complement and product from one exponential.
+template <typename R>
+struct Logi {
+    R s, oms, s1;  // s(x), 1 - s(x), s(x)(1 - s(x))
+    R e, q;        // e^{-|x|}, 1/(1 + e^{-|x|})
+    __device__ __forceinline__ void set(R x) {
+        logistic_parts(x, q, e);
+        R eq = e * q;
+        s = x >= R(0) ? q : eq;
+        oms = x >= R(0) ? eq : q;
+        s1 = eq * q;
+    }
+};
+
+// softplus(ph) + 1e-6 and its derivative s(ph), sharing one exponential.
+template <typename R>
+__device__ __forceinline__ void softplus_d(R ph, R& ps, R& sig_p) {
+    Logi<R> l;
+    l.set(ph);
+    ps = (ph > R(0) ? ph : R(0)) + log1p_unit<R>(l.e) + R(1e-6);
+    sig_p = l.s;
+}
+
+template <int MODEL, typename R> struct ModelGrad;
+
+template <typename R> struct ModelGrad<0, R> : Model<0, R> {
+    using B = Model<0, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        R dt = t - B::t0;
+        R x1 = -dt * B::itf, x2 = -dt * B::itr;
+        if (x1 > R(EXP_OVF64) || B::la + x1 > R(EXP_OVF64)) return false;
+        g[1] = R(1); g[5] = R(0);
+        if (x2 > R(EXP_OVF64)) { f = B::b; g[0] = g[2] = g[3] = g[4] = R(0); return true; }
+        Logi<R> l;
+        l.set(-x2);  // s(dt/tr)
+        R sp = (x2 > R(0) ? x2 : R(0)) + log1p_unit<R>(l.e);
+        R base = Math<R>::exp(B::la + x1 - sp);
+        f = base + B::b;
+        g[0] = base;
+        g[2] = base * (B::itf - l.oms * B::itr);
+        g[3] = -base * l.oms * dt * B::itr;
+        g[4] = base * dt * B::itf;
+        return true;
+    }
+};
+
+template <typename R> struct ModelGrad<1, R> : Model<1, R> {
+    using B = Model<1, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        R ph = t - B::t0;
+        R y = (B::gamma - ph) * B::itf;
+        if (y > R(EXP_OVF64)) return false;
+        Logi<R> lr, lz;
+        lr.set(ph * B::itr);
+        R z = R(10) * (ph - B::gamma);
+        lz.set(z);
+        R wexp = lz.q * Math<R>::exp(y + (z < R(0) ? z : R(0)));  // w e^{y}
+        R pl = R(1) - B::beta * ph;
+        R piece = lz.oms * pl + B::one_m_bg * wexp;
+        R as = B::a * lr.s;
+        f = as * piece;
+        R kterm = R(10) * (lz.s1 * pl - lz.oms * B::one_m_bg * wexp);  // k[w(1-w) pl - (1-w)(1-bg) w e^y]
+        g[0] = f;
+        g[1] = as * (-lz.oms * ph - B::gamma * wexp);
+        g[2] = as * B::gamma * (kterm + wexp * (-B::beta + B::one_m_bg * B::itf));
+        g[3] = B::a * (-lr.s1 * B::itr * piece + lr.s * (kterm + lz.oms * B::beta + B::one_m_bg * wexp * B::itf));
+        g[4] = -B::a * piece * lr.s1 * ph * B::itr;
+        g[5] = as * B::one_m_bg * wexp * (ph - B::gamma) * B::itf;
+        g[6] = R(0);
+        return true;
+    }
+};
+
+template <typename R> struct ModelGrad<2, R> : Model<2, R> {
+    using B = Model<2, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        using M = Math<R>;
+        R ph = t - B::t0;
+        g[1] = R(1); g[6] = R(0);
+        if (ph > R(EXP_OVF64)) { f = B::b; g[0] = g[2] = g[3] = g[4] = g[5] = R(0); return true; }
+        Logi<R> l;
+        l.set(ph * B::itr);
+        R ps, sp;
+        softplus_d(ph, ps, sp);
+        R w = R(1) + ps * B::itf;
+        R lw = M::log(w), iw = M::rcp(w);
+        R D = M::exp(B::la - B::alpha * lw);
+        R Ds = D * l.s;
+        f = Ds + B::b;
+        g[0] = Ds;
+        g[2] = D * (-l.s1 * B::itr + l.s * B::alpha * sp * B::itf * iw);
+        g[3] = -D * l.s1 * ph * B::itr;
+        g[4] = Ds * B::alpha * ps * B::itf * iw;
+        g[5] = -Ds * lw;
+        return true;
+    }
+};
+
+template <typename R> struct ModelGrad<3, R> : Model<3, R> {
+    using B = Model<3, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        using M = Math<R>;
+        R ph = t - B::t0;
+        g[4] = R(0);
+        if (ph > R(EXP_OVF64)) { f = R(0); g[0] = g[1] = g[2] = g[3] = R(0); return true; }
+        R ps, sp;
+        softplus_d(ph, ps, sp);
+        R eni = M::exp(-ps * R(1.0 / 8.8)), eco = M::exp(-ps * R(1.0 / 111.3));
+        R heat = B::fr * eni + (R(1) - B::fr) * eco;
+        R x = ps * B::itm, x2 = x * x;
+        R ex2 = M::exp(-x2);
+        R trap = one_minus_exp_neg(x2);
+        f = B::a * heat * trap;
+        R dheat = -B::fr * eni * R(1.0 / 8.8) - (R(1) - B::fr) * eco * R(1.0 / 111.3);
+        R dtrap = R(2) * ps * ex2 * B::itm * B::itm;
+        g[0] = f;
+        g[1] = -B::a * sp * (dheat * trap + heat * dtrap);
+        g[2] = R(-2) * B::a * heat * ex2 * x2;
+        g[3] = B::a * trap * (eni - eco) * B::fr * (R(1) - B::fr);
+        return true;
+    }
+};
+
+template <typename R> struct ModelGrad<4, R> : Model<4, R> {
+    using B = Model<4, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        using M = Math<R>;
+        R ph = t - B::t0;
+        g[4] = R(0);
+        if (ph > R(EXP_OVF64)) { f = R(0); g[0] = g[1] = g[2] = g[3] = R(0); return true; }
+        R ps, sp;
+        softplus_d(ph, ps, sp);
+        R w = R(1) + ps * B::itsd, iw = M::rcp(w);
+        R sd = iw * iw, iw3 = sd * iw;
+        R x = ps * B::itd, x2 = x * x;
+        R ex2 = M::exp(-x2);
+        R trap = one_minus_exp_neg(x2);
+        f = B::a * sd * trap;
+        g[0] = f;
+        g[1] = -B::a * sp * (R(-2) * iw3 * B::itsd * trap + sd * R(2) * ps * ex2 * B::itd * B::itd);
+        g[2] = B::a * trap * R(2) * ps * iw3 * B::itsd;
+        g[3] = R(-2) * B::a * sd * ex2 * x2;
+        return true;
+    }
+};
+
+template <typename R> struct ModelGrad<5, R> : Model<5, R> {
+    using B = Model<5, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        using M = Math<R>;
+        R ph = t - B::t0;
+        g[4] = R(0);
+        if (ph > R(EXP_OVF64)) { f = R(0); g[0] = g[1] = g[2] = g[3] = R(0); return true; }
+        Logi<R> l5;
+        l5.set(R(5) * ph);
+        R ps, sp;
+        softplus_d(ph, ps, sp);
+        R lps = M::log(ps), r = ps * B::ittr;
+        R E = M::exp(B::la - B::n * lps - r * r);
+        f = l5.s * E;
+        g[0] = f;
+        g[1] = E * (R(-5) * l5.s1 + l5.s * sp * (B::n * M::rcp(ps) + R(2) * ps * B::ittr * B::ittr));
+        g[2] = -f * lps;
+        g[3] = R(2) * f * r * r;
+        return true;
+    }
+};
+
+template <typename R> struct ModelGrad<6, R> : Model<6, R> {
+    using B = Model<6, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        using M = Math<R>;
+        R ph = t - B::t0;
+        g[5] = R(0);
+        if (ph > R(EXP_OVF64)) {
+            g[0] = g[1] = g[2] = g[3] = g[4] = R(0);
+            return B::pred(t, f);
+        }
+        R ps, sp;
+        softplus_d(ph, ps, sp);
+        R lr = M::log(ps) - B::ltb;
+        R e1 = R(2) * B::a1 * lr, e2 = R(2) * B::a2 * lr;
+        R mx = e1 > e2 ? e1 : e2;
+        if (mx < R(-745.1332191019412)) return false;
+        R ed = M::exp(e1 > e2 ? e2 - e1 : e1 - e2);
+        R ql = M::rcp(R(1) + ed);
+        R wb = ql, ws = ed * ql;  // weights of the larger / smaller term
+        R w1 = e1 >= e2 ? wb : ws, w2 = e1 >= e2 ? ws : wb;
+        f = M::exp(B::la - R(0.5) * (mx + log1p_unit<R>(ed)));
+        R S = B::a1 * w1 + B::a2 * w2;
+        g[0] = f;
+        g[1] = f * S * sp * M::rcp(ps);
+        g[2] = f * S;
+        g[3] = -f * lr * w1;
+        g[4] = -f * lr * w2;
+        return true;
+    }
+};
+
+// MetzgerKN: forward differences of the per-observation ODE in FP64 with the
+// reference's step h = max(1e-5, 1e-5 |theta_j|) (cuda/svi.cu:180-192).
+template <typename R> struct ModelGrad<7, R> : Model<7, R> {
+    using B = Model<7, R>;
+    __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
+        double p[4] = {B::p[0], B::p[1], B::p[2], B::p[3]};
+        double f0 = metzger_kn_single(p, (double)t);
+        f = (R)f0;
+        for (int j = 0; j < 4; j++) {
+            double h = fmax(1e-5, 1e-5 * fabs(p[j]));
+            double keep = p[j];
+            p[j] = keep + h;
+            double fp = metzger_kn_single(p, (double)t);
+            p[j] = keep;
+            g[j] = __builtin_isfinite(fp) ? (R)((fp - f0) / h) : R(0);
+        }
+        g[4] = R(0);
+        return true;
+    }
+};
+
+}  // namespace lcf

@@ -3,6 +3,7 @@
 #include "../../include/lcf_hip.h"
 #include "pso_kernels.hpp"
 #include "eval_kernels.hpp"
+#include "svi_kernels.hpp"
 
 #include <algorithm>
 #include <cstdarg>
@@ -403,6 +404,166 @@ int lcf_batch_all_models(lcf_ctx* c, const lcf_batch* b, int P, int max_iters, i
                          double* out_params, double* out_cost) {
     return lcf_batch_fit_models(c, b, ALL_MODELS, LCF_N_MODELS, 0, 0.0, P, max_iters, stall_iters, 42,
                                 0u, out_params, out_cost, nullptr, nullptr, nullptr, nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shared driver of the SVI entry points: one launch per model present,
+// each over that model's jobs in decreasing size order.
+int svi_run(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* mu,
+            const double* ls_in, const double* pc, const double* pw, int n_steps, int S, double lr,
+            uint64_t seed, uint32_t flags, int grad_only, int step0, double* out_mu, double* out_ls,
+            double* out_elbo, int32_t* out_steps) {
+    if (!c) return LCF_E_INVALID;
+    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    const int64_t nj = b->n_sources;
+    if (nj == 0) return LCF_OK;
+    if (!model_ids || !mu || !pc || !pw || !out_mu || !out_ls || !out_elbo)
+        return fail(c, LCF_E_INVALID, "null pointer");
+    if (S <= 0 || S > 64) return fail(c, LCF_E_INVALID, "n_samples must be in [1, 64], got %d", S);
+    if (n_steps < 0) return fail(c, LCF_E_INVALID, "n_steps must be non-negative");
+    for (int64_t j = 0; j < nj; j++)
+        if (model_ids[j] < 0 || model_ids[j] >= LCF_N_MODELS)
+            return fail(c, LCF_E_INVALID, "job %lld: unknown model id %d", (long long)j, model_ids[j]);
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool fp64 = c->precision == LCF_FP64;
+    const bool fd = (flags & LCF_SVI_FD_GRADIENT) != 0;
+    const int rec = fp64 ? (int)sizeof(double4) : (int)sizeof(float4);
+    const int tile = std::max(1, std::min(16384 / rec, std::max(b->max_n, 1)));
+
+    // per-model job lists, each in the batch's decreasing-size order
+    std::vector<int32_t> order;
+    order.reserve(nj);
+    int64_t starts[LCF_N_MODELS + 1] = {0};
+    for (int m = 0; m < LCF_N_MODELS; m++) {
+        starts[m] = (int64_t)order.size();
+        for (int32_t j : b->h_order)
+            if (model_ids[j] == m) order.push_back(j);
+    }
+    starts[LCF_N_MODELS] = (int64_t)order.size();
+
+    DevBuf<double> d_mu, d_ls, d_pc, d_pw, d_omu, d_ols, d_oe;
+    DevBuf<int32_t> d_order, d_steps;
+    HIPCHK(c, d_mu.alloc(nj * 7));
+    HIPCHK(c, d_pc.alloc(nj * 7));
+    HIPCHK(c, d_pw.alloc(nj * 7));
+    HIPCHK(c, d_omu.alloc(nj * 7));
+    HIPCHK(c, d_ols.alloc(nj * 7));
+    HIPCHK(c, d_oe.alloc(nj));
+    HIPCHK(c, d_order.alloc(nj));
+    HIPCHK(c, d_steps.alloc(nj));
+    if (ls_in) HIPCHK(c, d_ls.alloc(nj * 7));
+    HIPCHK(c, hipMemcpyAsync(d_mu.p, mu, sizeof(double) * nj * 7, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_pc.p, pc, sizeof(double) * nj * 7, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_pw.p, pw, sizeof(double) * nj * 7, hipMemcpyHostToDevice, c->stream));
+    if (ls_in) HIPCHK(c, hipMemcpyAsync(d_ls.p, ls_in, sizeof(double) * nj * 7, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_order.p, order.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice, c->stream));
+
+    lcf::SviArgs a{};
+    a.obs = fp64 ? (const void*)b->d_obs64 : (const void*)b->d_obs32;
+    a.offsets = b->d_off;
+    a.queue = c->d_queue;
+    a.mu_in = d_mu.p;
+    a.ls_in = ls_in ? d_ls.p : nullptr;
+    a.prior_c = d_pc.p;
+    a.prior_w = d_pw.p;
+    a.n_steps = n_steps;
+    a.n_samples = S;
+    a.lr = lr;
+    a.seed = seed;
+    a.flags = flags;
+    a.tile = tile;
+    a.grad_only = grad_only;
+    a.step0 = step0;
+    a.out_mu = d_omu.p;
+    a.out_ls = d_ols.p;
+    a.out_elbo = d_oe.p;
+    a.out_steps = d_steps.p;
+
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int m = 0; m < LCF_N_MODELS; m++) {
+        const int64_t cnt = starts[m + 1] - starts[m];
+        if (cnt == 0) continue;
+        a.order = d_order.p + starts[m];
+        if (c->schedule == LCF_SCHED_PERSISTENT) {
+            int per_cu = 0;
+            HIPCHK(c, lcf::svi_occupancy(m, fp64, fd, tile, &per_cu));
+            per_cu = std::max(per_cu, 1);
+            a.task_base = 0;
+            a.n_tasks = (int)cnt;
+            HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+            int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, cnt);
+            HIPCHK(c, lcf::launch_svi(m, a, fp64, fd, true, grid, c->stream));
+        } else {
+            // one synchronised launch per power-of-two size class
+            int64_t i = 0;
+            while (i < cnt) {
+                int64_t n0 = b->h_off[order[starts[m] + i] + 1] - b->h_off[order[starts[m] + i]];
+                int64_t cls = 1;
+                while (cls < n0) cls <<= 1;
+                int64_t j = i;
+                while (j < cnt) {
+                    int64_t nj2 = b->h_off[order[starts[m] + j] + 1] - b->h_off[order[starts[m] + j]];
+                    if (cls > 1 && nj2 * 2 <= cls) break;
+                    j++;
+                }
+                a.task_base = (int)i;
+                a.n_tasks = (int)j;
+                HIPCHK(c, lcf::launch_svi(m, a, fp64, fd, false, (int)(j - i), c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                i = j;
+            }
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    HIPCHK(c, hipMemcpyAsync(out_mu, d_omu.p, sizeof(double) * nj * 7, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_ls, d_ols.p, sizeof(double) * nj * 7, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_elbo, d_oe.p, sizeof(double) * nj, hipMemcpyDeviceToHost, c->stream));
+    if (out_steps) HIPCHK(c, hipMemcpyAsync(out_steps, d_steps.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lcf_batch_svi_fit(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* pso_params,
+                      const int32_t* se_idx, const double* prior_centers, const double* prior_widths,
+                      int n_steps, int n_samples, double lr, double* out_mu, double* out_log_sigma,
+                      double* out_elbo) {
+    if (!c) return LCF_E_INVALID;
+    if (b && se_idx && model_ids)
+        for (int64_t j = 0; j < b->n_sources; j++)
+            if (model_ids[j] >= 0 && model_ids[j] < LCF_N_MODELS && se_idx[j] != N_PARAMS[model_ids[j]] - 1)
+                return fail(c, LCF_E_INVALID, "job %lld: se_idx %d is not the last parameter", (long long)j, se_idx[j]);
+    return svi_run(c, b, model_ids, pso_params, nullptr, prior_centers, prior_widths, n_steps, n_samples, lr,
+                   42, 0u, 0, 0, out_mu, out_log_sigma, out_elbo, nullptr);
+}
+
+int lcf_batch_svi_fit_ex(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* pso_params,
+                         const double* prior_centers, const double* prior_widths, int n_steps, int n_samples,
+                         double lr, uint64_t seed, uint32_t flags, double* out_mu, double* out_log_sigma,
+                         double* out_elbo, int32_t* out_steps) {
+    return svi_run(c, b, model_ids, pso_params, nullptr, prior_centers, prior_widths, n_steps, n_samples, lr,
+                   seed, flags, 0, 0, out_mu, out_log_sigma, out_elbo, out_steps);
+}
+
+int lcf_svi_elbo_grad(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* mu,
+                      const double* log_sigma, const double* prior_centers, const double* prior_widths,
+                      int n_samples, uint64_t seed, int step, uint32_t flags, double* out_elbo,
+                      double* out_grad_mu, double* out_grad_log_sigma) {
+    if (!c) return LCF_E_INVALID;
+    if (!log_sigma) return fail(c, LCF_E_INVALID, "null log_sigma");
+    if (step < 0) return fail(c, LCF_E_INVALID, "negative step");
+    return svi_run(c, b, model_ids, mu, log_sigma, prior_centers, prior_widths, 1, n_samples, 0.0, seed,
+                   flags, 1, step, out_grad_mu, out_grad_log_sigma, out_elbo, nullptr);
 }
 
 }  // extern "C"

@@ -229,6 +229,125 @@ class GpuContext:
                             for s in range(ns)]))
         return out
 
+    # -- batched SVI -------------------------------------------------------------
+    def batch_svi_fit(self, data: "GpuBatchData", inputs: Sequence["SviBatchInput"], n_steps: int,
+                      n_samples: int, lr: float) -> list["SviBatchOutput"]:
+        """``GpuContext::batch_svi_fit`` (src/gpu_cuda.rs:1315-1422)."""
+        nj = len(inputs)
+        if nj == 0:
+            return []
+        if nj != data.n_sources:
+            raise GpuError(f"SVI inputs ({nj}) must match batch data size ({data.n_sources})")
+        mid = np.zeros(nj, dtype=np.int32)
+        se = np.zeros(nj, dtype=np.int32)
+        pso, pc, pw = (np.zeros((nj, MAX_PARAMS)) for _ in range(3))
+        for i, inp in enumerate(inputs):
+            k = min(len(inp.pso_params), MAX_PARAMS)
+            pso[i, :k] = inp.pso_params[:k]
+            pc[i, :k] = inp.prior_centers[:k]
+            pw[i, :k] = inp.prior_widths[:k]
+            mid[i] = inp.model_id
+            se[i] = inp.se_idx
+        mu, ls, elbo = np.zeros((nj, MAX_PARAMS)), np.zeros((nj, MAX_PARAMS)), np.zeros(nj)
+        self._check(self._lib.lcf_batch_svi_fit(
+            self._h, data._h, _lib.ptr(mid, C.c_int32), _lib.ptr(pso, C.c_double), _lib.ptr(se, C.c_int32),
+            _lib.ptr(pc, C.c_double), _lib.ptr(pw, C.c_double), n_steps, n_samples, float(lr),
+            _lib.ptr(mu, C.c_double), _lib.ptr(ls, C.c_double), _lib.ptr(elbo, C.c_double)))
+        out = []
+        for i, inp in enumerate(inputs):
+            k = min(len(inp.pso_params), MAX_PARAMS)
+            out.append(SviBatchOutput(list(mu[i, :k]), list(ls[i, :k]), float(elbo[i])))
+        return out
+
+    def svi_fit(self, data: "GpuBatchData", model_ids, pso, prior_c, prior_w, n_steps: int,
+                n_samples: int, lr: float, *, seed: int = 42, flags: int = 0) -> dict:
+        """Array form of the batched SVI (``lcf_batch_svi_fit_ex``)."""
+        nj = data.n_sources
+        mid = np.ascontiguousarray(np.asarray(model_ids, dtype=np.int32))
+        pso, pc, pw = (_rows7(x, nj) for x in (pso, prior_c, prior_w))
+        mu, ls, elbo = np.zeros((nj, MAX_PARAMS)), np.zeros((nj, MAX_PARAMS)), np.zeros(nj)
+        steps = np.zeros(nj, dtype=np.int32)
+        self._check(self._lib.lcf_batch_svi_fit_ex(
+            self._h, data._h, _lib.ptr(mid, C.c_int32), _lib.ptr(pso, C.c_double), _lib.ptr(pc, C.c_double),
+            _lib.ptr(pw, C.c_double), n_steps, n_samples, float(lr), C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF),
+            flags, _lib.ptr(mu, C.c_double), _lib.ptr(ls, C.c_double), _lib.ptr(elbo, C.c_double),
+            _lib.ptr(steps, C.c_int32)))
+        return {"mu": mu, "log_sigma": ls, "elbo": elbo, "steps": steps}
+
+    def svi_elbo_grad(self, data: "GpuBatchData", model_ids, mu, log_sigma, prior_c, prior_w,
+                      n_samples: int, *, seed: int = 42, step: int = 0, flags: int = 0):
+        """One Monte-Carlo ELBO estimate per job and its gradient w.r.t.
+        (mu, log_sigma), from the adjoint kernel (``lcf_svi_elbo_grad``)."""
+        nj = data.n_sources
+        mid = np.ascontiguousarray(np.asarray(model_ids, dtype=np.int32))
+        mu, ls, pc, pw = (_rows7(x, nj) for x in (mu, log_sigma, prior_c, prior_w))
+        elbo, gmu, gls = np.zeros(nj), np.zeros((nj, MAX_PARAMS)), np.zeros((nj, MAX_PARAMS))
+        self._check(self._lib.lcf_svi_elbo_grad(
+            self._h, data._h, _lib.ptr(mid, C.c_int32), _lib.ptr(mu, C.c_double), _lib.ptr(ls, C.c_double),
+            _lib.ptr(pc, C.c_double), _lib.ptr(pw, C.c_double), n_samples,
+            C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), step, flags, _lib.ptr(elbo, C.c_double),
+            _lib.ptr(gmu, C.c_double), _lib.ptr(gls, C.c_double)))
+        return elbo, gmu, gls
+
+
+def _rows7(x, n: int) -> np.ndarray:
+    """[n, MAX_PARAMS] contiguous float64 rows, zero padded."""
+    a = np.asarray(x, dtype=np.float64)
+    if a.ndim == 1:
+        a = a.reshape(n, -1)
+    out = np.zeros((n, MAX_PARAMS))
+    out[:, :a.shape[1]] = a[:, :MAX_PARAMS]
+    return out
+
+
+@dataclass
+class SviBatchInput:
+    """Input of one SVI job (src/gpu_types.rs:190-197)."""
+
+    model_id: int
+    pso_params: Sequence[float]
+    se_idx: int
+    prior_centers: Sequence[float]
+    prior_widths: Sequence[float]
+
+
+@dataclass
+class SviBatchOutput:
+    """Output of one SVI job (src/gpu_types.rs:199-205)."""
+
+    mu: list
+    log_sigma: list
+    elbo: float
+
+
+# prior_params (src/parametric.rs:2072-2115): (center, width) per parameter.
+_PRIOR_PARAMS = {
+    0: [(0.0, 2.0)] * 6,
+    1: [(0.0, 2.0)] * 7,
+    2: [(0.0, 2.0)] * 5 + [(1.67, 1.0), (0.0, 2.0)],
+    3: [(0.0, 2.0), (0.0, 2.0), (2.3, 1.0), (0.0, 2.0), (0.0, 2.0)],
+    4: [(0.0, 2.0), (0.0, 2.0), (3.0, 1.5), (2.3, 1.0), (0.0, 2.0)],
+    5: [(0.0, 2.0), (0.0, 2.0), (0.5, 1.0), (1.0, 2.0), (0.0, 2.0)],
+    6: [(0.0, 2.0), (0.0, 2.0), (1.0, 2.0), (0.8, 1.0), (2.0, 1.0), (0.0, 2.0)],
+    7: [(-1.75, 1.25), (-1.25, 0.75), (0.5, 1.5), (-0.5, 1.5), (0.0, 2.0)],
+}
+
+
+def svi_model_meta(model: int) -> tuple[int, int, int]:
+    """``svi_model_meta`` (src/parametric.rs:3277-3290): (model_id, n_params, se_idx)."""
+    m = GpuModelName(model)
+    return int(m), m.n_params(), m.n_params() - 1
+
+
+def svi_prior_for_model(model: int, pso_params: Sequence[float]) -> tuple[list, list]:
+    """``svi_prior_for_model`` (src/parametric.rs:3259-3273): prior widths of
+    prior_params, centres moved to the PSO point except sigma_x's."""
+    pri = _PRIOR_PARAMS[int(model)]
+    se = len(pri) - 1
+    centers = [pri[j][0] if j == se else pso_params[min(j, len(pso_params) - 1)] for j in range(len(pri))]
+    widths = [w for _, w in pri]
+    return centers, widths
+
 
 class GpuBatchData:
     """Resident CSR observation data (``GpuBatchData``, src/gpu_cuda.rs:236-292)."""

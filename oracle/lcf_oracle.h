@@ -100,6 +100,34 @@ int orc_batch_model_select(int64_t n_src, const int64_t* offsets,
                            int32_t* out_model, double* out_params,
                            double* out_cost, int n_threads);
 
+/* ---- SVI (GPU semantics, cuda/svi.cu:48-330) ------------------------------ */
+#define ORC_SVI_RNG_PHILOX 0     /* stream of the HIP kernels */
+#define ORC_SVI_RNG_REFERENCE 1  /* reference xorshift64, seeded 42 + job*123456789 */
+#define ORC_SVI_CPU_STOPPING 1u  /* flag: svi_fit constants, min 200 / stall 50 */
+#define ORC_SVI_INFLATE_SIGMA 2u /* flag: add ln 4 to log sigma (src/parametric.rs:2441-2446) */
+
+/* Philox normals of (job, step, sample): eps[np]. */
+int orc_svi_normals(uint64_t seed, uint32_t job, uint32_t step, uint32_t sample, int np, double* eps);
+/* One step's Monte-Carlo ELBO for noise eps[S][np] and its full gradient,
+ * entropy term included (grad_mode 0 analytic model gradient, 1 forward
+ * differences). */
+double orc_svi_elbo_grad(int model_id, const double* mu, const double* log_sigma,
+                         const double* prior_c, const double* prior_w, const double* eps, int S,
+                         const double* t, const double* f, const double* v, const int32_t* up,
+                         const double* uf, int64_t n, int grad_mode, double* grad_mu,
+                         double* grad_log_sigma);
+/* GpuContext::batch_svi_fit (src/gpu_cuda.rs:1315-1422).  Per job: model id,
+ * pso[7], prior centers/widths [7] (padded rows).  Outputs mu/log_sigma
+ * [n_jobs*7], elbo[n_jobs], steps executed. */
+int orc_batch_svi(int64_t n_jobs, const int64_t* offsets, const double* t, const double* f,
+                  const double* v, const int32_t* up, const double* uf, const int32_t* model_ids,
+                  const double* pso, const double* prior_c, const double* prior_w, int n_steps,
+                  int n_samples, double lr, int rng_mode, uint64_t seed, int grad_mode,
+                  uint32_t flags, double* out_mu, double* out_log_sigma, double* out_elbo,
+                  int32_t* out_steps, int n_threads);
+/* svi_prior_for_model (src/parametric.rs:3259-3273). */
+int orc_svi_prior_for_model(int model_id, const double* pso, double* centers, double* widths);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,7 +41,16 @@ SIGNATURES = {
                              _u64, _u32, _pd, _pd, _pi32, _i32]),
     "orc_batch_model_select": (_i32, [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _pi32, _i32, _i32,
                                       _i32, _i32, _f64, _u32, _pi32, _pd, _pd, _i32]),
+    "orc_svi_normals": (_i32, [_u64, _u32, _u32, _u32, _i32, _pd]),
+    "orc_svi_elbo_grad": (_f64, [_i32, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _pd, _pd, _pi32, _pd,
+                                 _i64, _i32, _pd, _pd]),
+    "orc_batch_svi": (_i32, [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _pi32, _pd, _pd, _pd, _i32, _i32,
+                             _f64, _i32, _u64, _i32, _u32, _pd, _pd, _pd, _pi32, _i32]),
+    "orc_svi_prior_for_model": (_i32, [_i32, _pd, _pd, _pd]),
 }
+
+SVI_RNG_PHILOX, SVI_RNG_REFERENCE = 0, 1
+SVI_CPU_STOPPING, SVI_INFLATE_SIGMA = 1, 2
 
 _LIB = None
 
@@ -208,3 +217,58 @@ def batch_model_select(csr: dict, models, n_particles: int, max_iters: int, stal
                                       _p(cost, C.c_double), n_threads or default_threads())
     assert rc == 0
     return bm, prm, cost
+
+
+def svi_normals(seed: int, job: int, step: int, sample: int, np_: int) -> np.ndarray:
+    e = np.zeros(np_)
+    assert lib().orc_svi_normals(C.c_uint64(seed), job, step, sample, np_, _p(e, C.c_double)) == 0
+    return e
+
+
+def svi_prior_for_model(m: int, pso):
+    p = f64(pso)
+    n = n_params(m)
+    c, w = np.zeros(n), np.zeros(n)
+    lib().orc_svi_prior_for_model(m, _p(p, C.c_double), _p(c, C.c_double), _p(w, C.c_double))
+    return c, w
+
+
+def svi_elbo_grad(m: int, mu, ls, pc, pw, eps, t, f, v, up, uf, grad_mode: int = 0):
+    """One-step MC ELBO for noise eps[S, np] and its gradient."""
+    npar = n_params(m)
+    mu, ls, pc, pw = (f64(x)[:npar].copy() for x in (mu, ls, pc, pw))
+    eps = f64(eps).reshape(-1, npar)
+    t, f, v, uf = f64(t), f64(f), f64(v), f64(uf)
+    up = i32(up)
+    gmu, gls = np.zeros(npar), np.zeros(npar)
+    e = lib().orc_svi_elbo_grad(m, _p(mu, C.c_double), _p(ls, C.c_double), _p(pc, C.c_double),
+                                _p(pw, C.c_double), _p(eps, C.c_double), eps.shape[0], _p(t, C.c_double),
+                                _p(f, C.c_double), _p(v, C.c_double), _p(up, C.c_int32), _p(uf, C.c_double),
+                                t.size, grad_mode, _p(gmu, C.c_double), _p(gls, C.c_double))
+    return e, gmu, gls
+
+
+def batch_svi(csr: dict, model_ids, pso, pc, pw, n_steps: int, n_samples: int, lr: float,
+              rng_mode: int = SVI_RNG_PHILOX, seed: int = 42, grad_mode: int = 0, flags: int = 0,
+              n_threads: int | None = None):
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    nj = off.size - 1
+    t, f, v, up, uf = _obs(csr)
+    mid = i32(model_ids)
+
+    def rows(x):
+        a = np.zeros((nj, 7))
+        x = np.asarray(x, dtype=np.float64).reshape(nj, -1)
+        a[:, :x.shape[1]] = x[:, :7]
+        return a
+
+    pso, pc, pw = rows(pso), rows(pc), rows(pw)
+    mu, ls, elbo = np.zeros((nj, 7)), np.zeros((nj, 7)), np.zeros(nj)
+    steps = np.zeros(nj, dtype=np.int32)
+    rc = lib().orc_batch_svi(nj, _p(off, C.c_int64), _p(t, C.c_double), _p(f, C.c_double), _p(v, C.c_double),
+                             _p(up, C.c_int32), _p(uf, C.c_double), _p(mid, C.c_int32), _p(pso, C.c_double),
+                             _p(pc, C.c_double), _p(pw, C.c_double), n_steps, n_samples, float(lr), rng_mode,
+                             C.c_uint64(seed), grad_mode, flags, _p(mu, C.c_double), _p(ls, C.c_double),
+                             _p(elbo, C.c_double), _p(steps, C.c_int32), n_threads or default_threads())
+    assert rc == 0
+    return mu, ls, elbo, steps
