@@ -15,6 +15,9 @@ from .gpu import (  # noqa: F401
     SCHED_PERSISTENT,
     BatchPsoResult,
     BatchSource,
+    DenseGpState,
+    GpBandInput,
+    GpBandOutput,
     GpuBatchData,
     GpuContext,
     GpuError,

@@ -41,6 +41,10 @@ SIGNATURES = {
                                     _pd, _pd, _pd, _pi32]),
     "lcf_svi_elbo_grad": (_i32, [_vp, _vp, _pi32, _pd, _pd, _pd, _pd, _i32, _u64, _i32, _u32,
                                  _pd, _pd, _pd]),
+    "lcf_batch_gp_fit": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _i32, _pd, _i32, _pd, _i32, _i32,
+                                _pd, _pd, _pd, _pd]),
+    "lcf_batch_gp_fit_ex": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _i32, _i32, _pd, _i32, _pd, _i32,
+                                   _i32, _i32, _i32, _pd, _pd, _pd, _pd, _pd]),
     "lcf_last_kernel_ms": (_f64, [_vp]),
     "lcf_last_model_ms": (_f64, [_vp, _i32]),
 }

@@ -1,0 +1,41 @@
+// gp_kernels.hpp -- launch interface of the batched dense GP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace lcf {
+
+constexpr int GP_LDS_CAP = 128;  // largest training set factorised in LDS
+
+struct GpArgs {
+    const double* t;           // [n_obs] observation times
+    const double* y;           // [n_obs] values (magnitudes)
+    const double* nv;          // [n_obs] noise variances
+    const int64_t* offsets;    // [n_bands + 1]
+    const int32_t* order;      // band indices, decreasing size
+    int n_tasks;
+    int* queue;
+    const double* query;       // [n_query] or [n_bands][n_query]
+    int n_query;
+    int query_per_band;
+    const double* amps;
+    int n_amp;
+    const double* lss;
+    int n_ls;
+    int cap;                   // training-set cap (subsample size)
+    int subsample_rule;        // 0: GPU rule, 1: CPU rule
+    int criterion;             // 0: NLML, 1: train RMS
+    double* state;             // [n_bands][state_stride]
+    int state_stride;          // >= 2*cap + 4
+    double* pred;              // [n_bands][n_query]
+    double* stdv;              // [n_bands][n_query]
+    double* dpred;             // [n_bands][n_query] or null
+    double* pred_obs;          // [n_obs]
+};
+
+size_t gp_lds_bytes(int cap);
+hipError_t launch_gp_fit(const GpArgs& a, int grid, hipStream_t st);
+hipError_t gp_occupancy(int cap, int* blocks_per_cu);
+
+}  // namespace lcf

@@ -4,6 +4,7 @@
 #include "pso_kernels.hpp"
 #include "eval_kernels.hpp"
 #include "svi_kernels.hpp"
+#include "gp_kernels.hpp"
 
 #include <algorithm>
 #include <cstdarg>
@@ -564,6 +565,134 @@ int lcf_svi_elbo_grad(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, 
     if (step < 0) return fail(c, LCF_E_INVALID, "negative step");
     return svi_run(c, b, model_ids, mu, log_sigma, prior_centers, prior_widths, 1, n_samples, 0.0, seed,
                    flags, 1, step, out_grad_mu, out_grad_log_sigma, out_elbo, nullptr);
+}
+
+int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* y,
+                        const double* nv, const double* query, int n_query, int query_per_band,
+                        const double* amps, int n_amp, const double* lss, int n_ls, int cap,
+                        int subsample_rule, int criterion, double* out_state, double* out_pred,
+                        double* out_std, double* out_dpred, double* out_pred_obs) {
+    if (!c) return LCF_E_INVALID;
+    if (n_bands < 0 || n_bands > INT32_MAX - 1 || !off) return fail(c, LCF_E_INVALID, "invalid band count");
+    if (n_bands == 0) return LCF_OK;
+    if (cap < 3 || cap > lcf::GP_LDS_CAP)
+        return fail(c, LCF_E_INVALID, "training-set cap must be in [3, %d], got %d", lcf::GP_LDS_CAP, cap);
+    if (n_amp <= 0 || n_ls <= 0 || !amps || !lss) return fail(c, LCF_E_INVALID, "empty hyperparameter grid");
+    if (n_query < 0 || (n_query > 0 && !query)) return fail(c, LCF_E_INVALID, "invalid query grid");
+    if (subsample_rule < 0 || subsample_rule > 1 || criterion < 0 || criterion > 1)
+        return fail(c, LCF_E_INVALID, "unknown subsample rule or criterion");
+    if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
+    for (int64_t b = 0; b < n_bands; b++)
+        if (off[b + 1] < off[b]) return fail(c, LCF_E_INVALID, "offsets not non-decreasing at %lld", (long long)b);
+    const int64_t n = off[n_bands];
+    if (n > 0 && (!t || !y || !nv || !out_pred_obs)) return fail(c, LCF_E_INVALID, "null observation array");
+    if (!out_state || (n_query > 0 && (!out_pred || !out_std))) return fail(c, LCF_E_INVALID, "null output");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int stride = 2 * cap + 4;
+    std::vector<int32_t> order(n_bands);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return off[a + 1] - off[a] > off[b + 1] - off[b];
+    });
+    const int64_t nq_tot = (int64_t)n_query * (query_per_band ? n_bands : 1);
+    DevBuf<double> dt, dy, dnv, dq, da, dl, dst, dp, ds, dd, dpo;
+    DevBuf<int64_t> doff;
+    DevBuf<int32_t> dord;
+    HIPCHK(c, dt.alloc(n));
+    HIPCHK(c, dy.alloc(n));
+    HIPCHK(c, dnv.alloc(n));
+    HIPCHK(c, dpo.alloc(n));
+    HIPCHK(c, dq.alloc(nq_tot));
+    HIPCHK(c, da.alloc(n_amp));
+    HIPCHK(c, dl.alloc(n_ls));
+    HIPCHK(c, dst.alloc(n_bands * stride));
+    HIPCHK(c, dp.alloc(n_bands * (int64_t)n_query));
+    HIPCHK(c, ds.alloc(n_bands * (int64_t)n_query));
+    if (out_dpred) HIPCHK(c, dd.alloc(n_bands * (int64_t)n_query));
+    HIPCHK(c, doff.alloc(n_bands + 1));
+    HIPCHK(c, dord.alloc(n_bands));
+    auto up = [&](void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, up(dt.p, t, sizeof(double) * n));
+    HIPCHK(c, up(dy.p, y, sizeof(double) * n));
+    HIPCHK(c, up(dnv.p, nv, sizeof(double) * n));
+    HIPCHK(c, up(dq.p, query, sizeof(double) * nq_tot));
+    HIPCHK(c, up(da.p, amps, sizeof(double) * n_amp));
+    HIPCHK(c, up(dl.p, lss, sizeof(double) * n_ls));
+    HIPCHK(c, up(doff.p, off, sizeof(int64_t) * (n_bands + 1)));
+    HIPCHK(c, up(dord.p, order.data(), sizeof(int32_t) * n_bands));
+    lcf::GpArgs a{};
+    a.t = dt.p;
+    a.y = dy.p;
+    a.nv = dnv.p;
+    a.offsets = doff.p;
+    a.order = dord.p;
+    a.n_tasks = (int)n_bands;
+    a.queue = c->d_queue;
+    a.query = dq.p;
+    a.n_query = n_query;
+    a.query_per_band = query_per_band;
+    a.amps = da.p;
+    a.n_amp = n_amp;
+    a.lss = dl.p;
+    a.n_ls = n_ls;
+    a.cap = cap;
+    a.subsample_rule = subsample_rule;
+    a.criterion = criterion;
+    a.state = dst.p;
+    a.state_stride = stride;
+    a.pred = dp.p;
+    a.stdv = ds.p;
+    a.dpred = out_dpred ? dd.p : nullptr;
+    a.pred_obs = dpo.p;
+    int per_cu = 0;
+    HIPCHK(c, lcf::gp_occupancy(cap, &per_cu));
+    per_cu = std::max(per_cu, 1);
+    const int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, n_bands);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+    HIPCHK(c, lcf::launch_gp_fit(a, grid, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    auto down = [&](void* h, const void* d, size_t bytes) {
+        return bytes ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, down(out_state, dst.p, sizeof(double) * n_bands * stride));
+    if (n_query > 0) {
+        HIPCHK(c, down(out_pred, dp.p, sizeof(double) * n_bands * n_query));
+        HIPCHK(c, down(out_std, ds.p, sizeof(double) * n_bands * n_query));
+        if (out_dpred) HIPCHK(c, down(out_dpred, dd.p, sizeof(double) * n_bands * n_query));
+    }
+    HIPCHK(c, down(out_pred_obs, dpo.p, sizeof(double) * n));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+int lcf_batch_gp_fit(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* mags,
+                     const double* noise_var, const double* query, int n_query, const double* amps, int n_amp,
+                     const double* lss, int n_ls, int max_subsample, double* out_state, double* out_pred,
+                     double* out_std, double* out_pred_obs) {
+    if (!c) return LCF_E_INVALID;
+    // the reference caps the training set at GP_MAX_M = 25 (cuda/gp.cu:11, 176-182)
+    const int cap = std::max(3, std::min(max_subsample, 25));
+    if (max_subsample < 3) return fail(c, LCF_E_INVALID, "max_subsample must be >= 3");
+    std::vector<double> st(n_bands > 0 ? (size_t)n_bands * (2 * cap + 4) : 1);
+    int rc = lcf_batch_gp_fit_ex(c, n_bands, off, t, mags, noise_var, query, n_query, 0, amps, n_amp, lss, n_ls,
+                                 cap, 0, 0, st.data(), out_pred, out_std, nullptr, out_pred_obs);
+    if (rc != LCF_OK) return rc;
+    // repack into the reference's 54-double layout: alpha[25], x[25], amp, 1/(2l^2), ybar, m
+    for (int64_t b = 0; b < n_bands; b++) {
+        const double* s = st.data() + b * (2 * cap + 4);
+        double* o = out_state + b * 54;
+        for (int i = 0; i < 54; i++) o[i] = 0.0;
+        for (int i = 0; i < cap; i++) { o[i] = s[i]; o[25 + i] = s[cap + i]; }
+        for (int i = 0; i < 4; i++) o[50 + i] = s[2 * cap + i];
+    }
+    return LCF_OK;
 }
 
 }  // extern "C"

@@ -290,6 +290,115 @@ class GpuContext:
         return elbo, gmu, gls
 
 
+    # -- batched dense GP -----------------------------------------------------
+    def batch_gp_fit(self, bands: Sequence["GpBandInput"], query_times, amp_candidates, ls_candidates,
+                     max_subsample: int) -> list["GpBandOutput"]:
+        """``GpuContext::batch_gp_fit`` (src/gpu_cuda.rs:976-1143).
+
+        The reference rebuilds a CPU ``DenseGP`` per band from the returned
+        state; here ``dense_gp`` is a ``DenseGpState`` carrying the same fit
+        (training set, alpha, amplitude, 1/(2 l^2), mean) from the device."""
+        csr = _gp_csr(bands)
+        q = _f64(query_times)
+        a, l = _f64(amp_candidates), _f64(ls_candidates)
+        nb = len(bands)
+        if nb == 0:
+            return []
+        state = np.zeros((nb, 54))
+        pred, std = np.zeros((nb, q.size)), np.zeros((nb, q.size))
+        pobs = np.zeros(csr["offsets"][-1])
+        self._check(self._lib.lcf_batch_gp_fit(
+            self._h, nb, _lib.ptr(csr["offsets"], C.c_int64), _lib.ptr(csr["times"], C.c_double),
+            _lib.ptr(csr["mags"], C.c_double), _lib.ptr(csr["noise_var"], C.c_double), _lib.ptr(q, C.c_double),
+            q.size, _lib.ptr(a, C.c_double), a.size, _lib.ptr(l, C.c_double), l.size, int(max_subsample),
+            _lib.ptr(state, C.c_double), _lib.ptr(pred, C.c_double), _lib.ptr(std, C.c_double),
+            _lib.ptr(pobs, C.c_double)))
+        off = csr["offsets"]
+        out = []
+        for b in range(nb):
+            m = int(state[b, 53])
+            gp = DenseGpState(state[b, 25:25 + m].copy(), state[b, :m].copy(), float(state[b, 50]),
+                              float(state[b, 51]), float(state[b, 52])) if 0 < m <= 25 else None
+            out.append(GpBandOutput(gp, list(pred[b]), list(std[b]), list(pobs[off[b]:off[b + 1]])))
+        return out
+
+    def gp_fit(self, csr: dict, query, amp_candidates, ls_candidates, *, cap: int = 128,
+               subsample_rule: int = 0, criterion: int = 0, query_per_band: bool = False) -> dict:
+        """Array form (``lcf_batch_gp_fit_ex``): training-set cap up to 128,
+        subsample rule (0 GPU, 1 CPU), criterion (0 NLML, 1 train RMS),
+        shared or per-band query grid; returns state, pred, std, dpred, pred_obs."""
+        off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+        nb = off.size - 1
+        t, y, nv = _f64(csr["times"]), _f64(csr["mags"]), _f64(csr["noise_var"])
+        q = _f64(query)
+        nq = q.shape[-1] if q.ndim else 0
+        a, l = _f64(amp_candidates), _f64(ls_candidates)
+        stride = 2 * cap + 4
+        out = {"state": np.zeros((nb, stride)), "pred": np.zeros((nb, nq)), "std": np.zeros((nb, nq)),
+               "dpred": np.zeros((nb, nq)), "pred_obs": np.zeros(t.size)}
+        self._check(self._lib.lcf_batch_gp_fit_ex(
+            self._h, nb, _lib.ptr(off, C.c_int64), _lib.ptr(t, C.c_double), _lib.ptr(y, C.c_double),
+            _lib.ptr(nv, C.c_double), _lib.ptr(q, C.c_double), nq, int(query_per_band), _lib.ptr(a, C.c_double),
+            a.size, _lib.ptr(l, C.c_double), l.size, cap, subsample_rule, criterion,
+            _lib.ptr(out["state"], C.c_double), _lib.ptr(out["pred"], C.c_double),
+            _lib.ptr(out["std"], C.c_double), _lib.ptr(out["dpred"], C.c_double),
+            _lib.ptr(out["pred_obs"], C.c_double)))
+        return out
+
+
+@dataclass
+class GpBandInput:
+    """One band of a batch GP fit (src/gpu_types.rs:143-148)."""
+
+    times: Sequence[float]
+    mags: Sequence[float]
+    noise_var: Sequence[float]
+
+
+@dataclass
+class DenseGpState:
+    """Fitted dense GP: training times, alpha, amplitude, 1/(2 l^2), mean."""
+
+    x: np.ndarray
+    alpha: np.ndarray
+    amp: float
+    inv_2ls2: float
+    y_mean: float
+
+    def kernel_amp(self) -> float:
+        return self.amp
+
+    def kernel_lengthscale(self) -> float:
+        return float(np.sqrt(0.5 / self.inv_2ls2))
+
+    def predict(self, query) -> np.ndarray:
+        q = np.asarray(query, dtype=np.float64)
+        k = self.amp * np.exp(-(q[:, None] - self.x[None, :]) ** 2 * self.inv_2ls2)
+        return k @ self.alpha + self.y_mean
+
+
+@dataclass
+class GpBandOutput:
+    """Output of one band (src/gpu_types.rs:150-160)."""
+
+    dense_gp: DenseGpState | None
+    pred_grid: list
+    std_grid: list
+    pred_at_obs: list
+
+
+def _gp_csr(bands) -> dict:
+    off = [0]
+    for b in bands:
+        if not (len(b.times) == len(b.mags) == len(b.noise_var)):
+            raise GpuError("GpBandInput arrays differ in length")
+        off.append(off[-1] + len(b.times))
+    cat = lambda k: (np.concatenate([np.asarray(getattr(b, k), dtype=np.float64) for b in bands])
+                     if bands else np.zeros(0))
+    return {"offsets": np.asarray(off, dtype=np.int64), "times": cat("times"), "mags": cat("mags"),
+            "noise_var": cat("noise_var")}
+
+
 def _rows7(x, n: int) -> np.ndarray:
     """[n, MAX_PARAMS] contiguous float64 rows, zero padded."""
     a = np.asarray(x, dtype=np.float64)

@@ -128,6 +128,27 @@ int orc_batch_svi(int64_t n_jobs, const int64_t* offsets, const double* t, const
 /* svi_prior_for_model (src/parametric.rs:3259-3273). */
 int orc_svi_prior_for_model(int model_id, const double* pso, double* centers, double* widths);
 
+/* ---- dense 1D GP (src/sparse_gp.rs:72-176, cuda/gp.cu:124-367) ----------- */
+#define ORC_GP_SUBSAMPLE_GPU 0   /* idx = floor(i (n-1)/(m-1) + 1/2) (cuda/gp.cu:189-191) */
+#define ORC_GP_SUBSAMPLE_CPU 1   /* idx = floor((i + 1/2) n / m)     (src/gp.rs:15-19) */
+#define ORC_GP_SELECT_NLML 0     /* exact NLML (GPU path) */
+#define ORC_GP_SELECT_TRAIN_RMS 1 /* train RMS (CPU dense path) */
+
+/* DenseGP::fit + predict_with_std + d(mean)/dt + NLML + train_rms.
+ * Returns 1 when the Cholesky fails. Output pointers may be NULL. */
+int orc_dense_gp(const double* t, const double* y, const double* nv, int n, double amp, double ls,
+                 const double* q, int nq, double* mean, double* stdv, double* dmean, double* nlml,
+                 double* train_rms);
+/* GpuContext::batch_gp_fit (src/gpu_cuda.rs:976-1143) with the subsample cap
+ * max_sub (the reference's is min(max_subsample, 25)).  state rows (stride
+ * >= 2*max_sub+4): alpha[max_sub], x[max_sub], amp, 1/(2 l^2), ybar, m.
+ * pred/std/dpred [n_bands*nq], pred_obs [n_obs] (NaN where no fit). */
+int orc_batch_gp_fit(int64_t n_bands, const int64_t* offsets, const double* t, const double* y,
+                     const double* noise_var, const double* q, int nq, const double* amps, int n_amp,
+                     const double* lss, int n_ls, int max_sub, int subsample_rule, int criterion,
+                     double* state, int state_stride, double* pred, double* stdv, double* dpred,
+                     double* pred_obs, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

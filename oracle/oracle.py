@@ -272,3 +272,47 @@ def batch_svi(csr: dict, model_ids, pso, pc, pw, n_steps: int, n_samples: int, l
                              _p(elbo, C.c_double), _p(steps, C.c_int32), n_threads or default_threads())
     assert rc == 0
     return mu, ls, elbo, steps
+
+
+GP_SUBSAMPLE_GPU, GP_SUBSAMPLE_CPU = 0, 1
+GP_SELECT_NLML, GP_SELECT_TRAIN_RMS = 0, 1
+
+
+def dense_gp(t, y, nv, amp: float, ls: float, query):
+    """DenseGP fit + (mean, std, dmean) at query, nlml, train_rms; None if not PD."""
+    t, y, nv, q = f64(t), f64(y), f64(nv), f64(query)
+    mean, std, dm = np.zeros(q.size), np.zeros(q.size), np.zeros(q.size)
+    nl, tr = C.c_double(), C.c_double()
+    L = lib()
+    L.orc_dense_gp.restype = _i32
+    L.orc_dense_gp.argtypes = [_pd, _pd, _pd, _i32, _f64, _f64, _pd, _i32, _pd, _pd, _pd,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    rc = L.orc_dense_gp(_p(t, C.c_double), _p(y, C.c_double), _p(nv, C.c_double), t.size, amp, ls,
+                        _p(q, C.c_double), q.size, _p(mean, C.c_double), _p(std, C.c_double),
+                        _p(dm, C.c_double), C.byref(nl), C.byref(tr))
+    if rc != 0:
+        return None
+    return mean, std, dm, nl.value, tr.value
+
+
+def batch_gp_fit(csr: dict, query, amps, lss, cap: int, subsample_rule: int = GP_SUBSAMPLE_GPU,
+                 criterion: int = GP_SELECT_NLML, n_threads: int | None = None):
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    nb = off.size - 1
+    t, y, nv = f64(csr["times"]), f64(csr["mags"]), f64(csr["noise_var"])
+    q, a, l = f64(query), f64(amps), f64(lss)
+    stride = 2 * cap + 4
+    state = np.zeros((nb, stride))
+    pred, std, dpred = np.zeros((nb, q.size)), np.zeros((nb, q.size)), np.zeros((nb, q.size))
+    pobs = np.zeros(t.size)
+    L = lib()
+    L.orc_batch_gp_fit.restype = _i32
+    L.orc_batch_gp_fit.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pd, _i32, _pd, _i32, _pd, _i32, _i32, _i32,
+                                   _i32, _pd, _i32, _pd, _pd, _pd, _pd, _i32]
+    rc = L.orc_batch_gp_fit(nb, _p(off, C.c_int64), _p(t, C.c_double), _p(y, C.c_double), _p(nv, C.c_double),
+                            _p(q, C.c_double), q.size, _p(a, C.c_double), a.size, _p(l, C.c_double), l.size,
+                            cap, subsample_rule, criterion, _p(state, C.c_double), stride,
+                            _p(pred, C.c_double), _p(std, C.c_double), _p(dpred, C.c_double),
+                            _p(pobs, C.c_double), n_threads or default_threads())
+    assert rc == 0
+    return {"state": state, "pred": pred, "std": std, "dpred": dpred, "pred_obs": pobs}
