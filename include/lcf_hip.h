@@ -149,6 +149,45 @@ int lcf_svi_elbo_grad(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_
                       uint32_t flags, double* out_elbo, double* out_grad_mu,
                       double* out_grad_log_sigma);
 
+/* ---- batched Laplace approximation (FP64) ---------------------------------- */
+#define LCF_LAPLACE_FD4 0      /* reference: 4-point central differences of the objective */
+#define LCF_LAPLACE_GRAD_FD 1  /* forward over adjoint: central differences of the analytic gradient */
+
+/* laplace_fit (src/parametric.rs:2623-2693) for every job of `data` (one
+ * band per job, model_ids[j], MAP point map_params[j][7]): log_sigma[j][7]
+ * (shape parameters from the Hessian of the negative log posterior with
+ * sigma_x fixed; sigma_x's slot is ln of its prior width), elbo[j] = -f(MAP),
+ * hessian[j][36] (row-major (np-1)^2 block, may be NULL).  The reference runs
+ * this per band on the CPU inside fit_parametric; batched here. */
+int lcf_batch_laplace(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_ids,
+                      const double* map_params, int mode, double* out_log_sigma, double* out_elbo,
+                      double* out_hessian);
+
+/* ---- batched dense GP (FP64) -------------------------------------------------- */
+/* GpuContext::batch_gp_fit (src/gpu_cuda.rs:976-1143): bands given as CSR
+ * (times, mags, noise_var); training set = at most min(max_subsample, 25)
+ * points, every (amp, ls) candidate with ls >= 0.1 scored by the exact
+ * negative log marginal likelihood; outputs the reference's 54-double state
+ * per band (alpha[25], x[25], amp, 1/(2 l^2), y mean, m), mean and std at the
+ * n_query shared query times, and the mean at every observation. */
+int lcf_batch_gp_fit(lcf_ctx* ctx, int64_t n_bands, const int64_t* offsets, const double* times,
+                     const double* mags, const double* noise_var, const double* query_times,
+                     int n_query, const double* amp_candidates, int n_amp,
+                     const double* ls_candidates, int n_ls, int max_subsample, double* out_state,
+                     double* out_pred, double* out_std, double* out_pred_obs);
+/* General form: training-set cap up to 128 (LDS-resident factorisation),
+ * subsample rule (0: cuda/gp.cu:189-191, 1: src/gp.rs:15-19), criterion
+ * (0: NLML, 1: train RMS, src/nonparametric.rs:1318-1335), query grid shared
+ * (query_per_band = 0, query[n_query]) or per band (query[n_bands][n_query]),
+ * state rows of 2*cap+4 doubles (alpha[cap], x[cap], amp, 1/(2 l^2), mean, m)
+ * and the time derivative of the predictive mean (out_dpred may be NULL). */
+int lcf_batch_gp_fit_ex(lcf_ctx* ctx, int64_t n_bands, const int64_t* offsets, const double* times,
+                        const double* mags, const double* noise_var, const double* query,
+                        int n_query, int query_per_band, const double* amp_candidates, int n_amp,
+                        const double* ls_candidates, int n_ls, int cap, int subsample_rule,
+                        int criterion, double* out_state, double* out_pred, double* out_std,
+                        double* out_dpred, double* out_pred_obs);
+
 /* Timing of the last lcf_batch_fit_models-family call on this context:
  * milliseconds between HIP events around the kernel launches (inputs already
  * resident, outputs still on the device). */

@@ -5,6 +5,7 @@
 #include "eval_kernels.hpp"
 #include "svi_kernels.hpp"
 #include "gp_kernels.hpp"
+#include "laplace_kernels.hpp"
 
 #include <algorithm>
 #include <cstdarg>
@@ -668,6 +669,77 @@ int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const d
         if (out_dpred) HIPCHK(c, down(out_dpred, dd.p, sizeof(double) * n_bands * n_query));
     }
     HIPCHK(c, down(out_pred_obs, dpo.p, sizeof(double) * n));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+int lcf_batch_laplace(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* map_params,
+                      int mode, double* out_log_sigma, double* out_elbo, double* out_hessian) {
+    if (!c) return LCF_E_INVALID;
+    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    const int64_t nj = b->n_sources;
+    if (nj == 0) return LCF_OK;
+    if (!model_ids || !map_params || !out_log_sigma || !out_elbo) return fail(c, LCF_E_INVALID, "null pointer");
+    if (mode != LCF_LAPLACE_FD4 && mode != LCF_LAPLACE_GRAD_FD) return fail(c, LCF_E_INVALID, "unknown mode %d", mode);
+    if (b->max_n > lcf::LAPLACE_MAX_OBS)
+        return fail(c, LCF_E_INVALID, "band of %d observations exceeds the Laplace LDS limit %d", b->max_n,
+                    lcf::LAPLACE_MAX_OBS);
+    for (int64_t j = 0; j < nj; j++) {
+        if (model_ids[j] < 0 || model_ids[j] >= LCF_N_MODELS)
+            return fail(c, LCF_E_INVALID, "job %lld: unknown model id %d", (long long)j, model_ids[j]);
+        if (model_ids[j] == 7 && mode == LCF_LAPLACE_GRAD_FD)
+            return fail(c, LCF_E_INVALID, "MetzgerKN has no analytic gradient; use LCF_LAPLACE_FD4");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int32_t> order;
+    order.reserve(nj);
+    int64_t starts[LCF_N_MODELS + 1] = {0};
+    for (int m = 0; m < LCF_N_MODELS; m++) {
+        starts[m] = (int64_t)order.size();
+        for (int32_t j : b->h_order)
+            if (model_ids[j] == m) order.push_back(j);
+    }
+    starts[LCF_N_MODELS] = (int64_t)order.size();
+    DevBuf<double> d_map, d_ls, d_e, d_h;
+    DevBuf<int32_t> d_order;
+    HIPCHK(c, d_map.alloc(nj * 7));
+    HIPCHK(c, d_ls.alloc(nj * 7));
+    HIPCHK(c, d_e.alloc(nj));
+    if (out_hessian) HIPCHK(c, d_h.alloc(nj * 36));
+    HIPCHK(c, d_order.alloc(nj));
+    HIPCHK(c, hipMemcpyAsync(d_map.p, map_params, sizeof(double) * nj * 7, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_order.p, order.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice, c->stream));
+    if (out_hessian) HIPCHK(c, hipMemsetAsync(d_h.p, 0, sizeof(double) * nj * 36, c->stream));
+    lcf::LaplaceArgs a{};
+    a.obs = b->d_obs64;
+    a.offsets = b->d_off;
+    a.queue = c->d_queue;
+    a.max_n = b->max_n;
+    a.mode = mode;
+    a.map = d_map.p;
+    a.out_log_sigma = d_ls.p;
+    a.out_elbo = d_e.p;
+    a.out_hessian = out_hessian ? d_h.p : nullptr;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int m = 0; m < LCF_N_MODELS; m++) {
+        const int64_t cnt = starts[m + 1] - starts[m];
+        if (cnt == 0) continue;
+        a.order = d_order.p + starts[m];
+        a.n_tasks = (int)cnt;
+        int per_cu = 0;
+        HIPCHK(c, lcf::laplace_occupancy(m, b->max_n, &per_cu));
+        per_cu = std::max(per_cu, 1);
+        HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+        HIPCHK(c, lcf::launch_laplace(m, a, (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, cnt), c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    HIPCHK(c, hipMemcpyAsync(out_log_sigma, d_ls.p, sizeof(double) * nj * 7, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_elbo, d_e.p, sizeof(double) * nj, hipMemcpyDeviceToHost, c->stream));
+    if (out_hessian) HIPCHK(c, hipMemcpyAsync(out_hessian, d_h.p, sizeof(double) * nj * 36, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LCF_OK;
 }

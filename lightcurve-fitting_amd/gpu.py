@@ -290,6 +290,20 @@ class GpuContext:
         return elbo, gmu, gls
 
 
+    # -- batched Laplace ------------------------------------------------------
+    def laplace_fit(self, data: "GpuBatchData", model_ids, map_params, *, mode: int = 0) -> dict:
+        """Batched ``laplace_fit`` (src/parametric.rs:2623-2693), FP64: one job
+        per source of ``data``.  mode 0 = the reference's 4-point difference
+        Hessian, 1 = central differences of the analytic gradient."""
+        nj = data.n_sources
+        mid = np.ascontiguousarray(np.asarray(model_ids, dtype=np.int32))
+        mp = _rows7(map_params, nj)
+        ls, elbo, hess = np.zeros((nj, MAX_PARAMS)), np.zeros(nj), np.zeros((nj, 36))
+        self._check(self._lib.lcf_batch_laplace(self._h, data._h, _lib.ptr(mid, C.c_int32),
+                                                _lib.ptr(mp, C.c_double), int(mode), _lib.ptr(ls, C.c_double),
+                                                _lib.ptr(elbo, C.c_double), _lib.ptr(hess, C.c_double)))
+        return {"mu": mp, "log_sigma": ls, "elbo": elbo, "hessian": hess}
+
     # -- batched dense GP -----------------------------------------------------
     def batch_gp_fit(self, bands: Sequence["GpBandInput"], query_times, amp_candidates, ls_candidates,
                      max_subsample: int) -> list["GpBandOutput"]:

@@ -149,6 +149,18 @@ int orc_batch_gp_fit(int64_t n_bands, const int64_t* offsets, const double* t, c
                      double* state, int state_stride, double* pred, double* stdv, double* dpred,
                      double* pred_obs, int n_threads);
 
+/* ---- Laplace (src/parametric.rs:2464-2693) ---------------------------------- */
+#define ORC_LAPLACE_FD4 0      /* reference: 4-point central differences of f */
+#define ORC_LAPLACE_GRAD_FD 1  /* central differences of the analytic gradient */
+double orc_neg_log_posterior(int model_id, const double* p, const double* t, const double* f,
+                             const double* v, const int32_t* up, const double* uf, int64_t n);
+int orc_symmetric_eigen(const double* mat, int n, double* evals, double* evecs);
+/* log_sigma[np] (shape params from the Hessian, sigma_x = ln prior width),
+ * elbo = -f(MAP), hessian[(np-1)^2] (may be NULL). */
+int orc_laplace_fit(int model_id, const double* map_params, const double* t, const double* f,
+                    const double* v, const int32_t* up, const double* uf, int64_t n, int mode,
+                    double* log_sigma, double* elbo, double* hessian);
+
 #ifdef __cplusplus
 }
 #endif

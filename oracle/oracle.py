@@ -316,3 +316,44 @@ def batch_gp_fit(csr: dict, query, amps, lss, cap: int, subsample_rule: int = GP
                             _p(pobs, C.c_double), n_threads or default_threads())
     assert rc == 0
     return {"state": state, "pred": pred, "std": std, "dpred": dpred, "pred_obs": pobs}
+
+
+LAPLACE_FD4, LAPLACE_GRAD_FD = 0, 1
+
+
+def neg_log_posterior(m: int, p, t, f, v, up, uf) -> float:
+    L = lib()
+    L.orc_neg_log_posterior.restype = _f64
+    L.orc_neg_log_posterior.argtypes = [_i32, _pd, _pd, _pd, _pd, _pi32, _pd, _i64]
+    p, t, f, v, uf = f64(p), f64(t), f64(f), f64(v), f64(uf)
+    up = i32(up)
+    return L.orc_neg_log_posterior(m, _p(p, C.c_double), _p(t, C.c_double), _p(f, C.c_double),
+                                   _p(v, C.c_double), _p(up, C.c_int32), _p(uf, C.c_double), t.size)
+
+
+def symmetric_eigen(mat):
+    L = lib()
+    L.orc_symmetric_eigen.restype = _i32
+    L.orc_symmetric_eigen.argtypes = [_pd, _i32, _pd, _pd]
+    a = f64(mat)
+    n = a.shape[0]
+    ev, vec = np.zeros(n), np.zeros((n, n))
+    assert L.orc_symmetric_eigen(_p(a, C.c_double), n, _p(ev, C.c_double), _p(vec, C.c_double)) == 0
+    return ev, vec
+
+
+def laplace_fit(m: int, map_params, t, f, v, up, uf, mode: int = LAPLACE_FD4):
+    L = lib()
+    L.orc_laplace_fit.restype = _i32
+    L.orc_laplace_fit.argtypes = [_i32, _pd, _pd, _pd, _pd, _pi32, _pd, _i64, _i32, _pd,
+                                  C.POINTER(C.c_double), _pd]
+    p, t, f, v, uf = f64(map_params), f64(t), f64(f), f64(v), f64(uf)
+    up = i32(up)
+    np_ = n_params(m)
+    ls, hess = np.zeros(np_), np.zeros((np_ - 1) ** 2)
+    e = C.c_double()
+    rc = L.orc_laplace_fit(m, _p(p, C.c_double), _p(t, C.c_double), _p(f, C.c_double), _p(v, C.c_double),
+                           _p(up, C.c_int32), _p(uf, C.c_double), t.size, mode, _p(ls, C.c_double),
+                           C.byref(e), _p(hess, C.c_double))
+    assert rc == 0
+    return ls, e.value, hess.reshape(np_ - 1, np_ - 1)

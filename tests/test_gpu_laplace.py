@@ -1,0 +1,61 @@
+"""GPU parity of the batched Laplace approximation (FP64) against the oracle
+(needs an MI355X): Hessian, log sigma and elbo per (band, model) job, for
+both Hessian modes, every model (MetzgerKN in the 4-point mode), ragged bands
+with upper limits and an empty band."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+OBS = ("times", "flux", "obs_var", "is_upper", "upper_flux")
+
+
+@pytest.fixture(scope="module")
+def lap_jobs(pkg, orc):
+    csr = pkg.synthetic.ragged_selection_batch(6, seed=17, n_min=8, n_max=120)
+    nb = csr["offsets"].size - 1
+    band_idx, mids = [], []
+    for b in range(nb):
+        for m in range(8):
+            band_idx.append(b)
+            mids.append(m)
+    jcsr = pkg.synthetic.subset(csr, np.asarray(band_idx))
+    mids = np.asarray(mids, dtype=np.int32)
+    maps = np.zeros((mids.size, 7))
+    for m in range(8):
+        sel = np.nonzero(mids == m)[0]
+        sub = pkg.synthetic.subset(jcsr, sel)
+        prm, _, _ = orc.batch_pso(m, sub, 24, 20 if m != 7 else 3, 20, 42)
+        maps[sel, :prm.shape[1]] = prm
+    return jcsr, mids, maps
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_laplace_matches_oracle(pkg, orc, lap_jobs, mode):
+    jcsr, mids, maps = lap_jobs
+    ctx = pkg.GpuContext(0, precision=pkg.FP64)
+    sel = np.arange(mids.size) if mode == 0 else np.nonzero(mids != 7)[0]
+    sub = pkg.synthetic.subset(jcsr, sel)
+    data = pkg.GpuBatchData(ctx, csr=sub)
+    out = ctx.laplace_fit(data, mids[sel], maps[sel], mode=mode)
+    off = sub["offsets"]
+    for k, j in enumerate(sel):
+        m = int(mids[j])
+        np_ = orc.n_params(m)
+        a, b = off[k], off[k + 1]
+        ls, elbo, H = orc.laplace_fit(m, maps[j, :np_], *(sub[x][a:b] for x in OBS), mode=mode)
+        Hg = out["hessian"][k, :(np_ - 1) ** 2].reshape(np_ - 1, np_ - 1)
+        scale = np.sqrt(np.outer(np.abs(np.diag(H)), np.abs(np.diag(H)))) + 1e-12
+        assert np.all(np.abs(Hg - H) <= 1e-6 * scale + 1e-9), (j, m)
+        assert out["elbo"][k] == pytest.approx(elbo, rel=1e-12, abs=1e-12), (j, m)
+        assert np.allclose(out["log_sigma"][k, :np_], ls, atol=1e-6), (j, m)
+
+
+def test_laplace_empty_band_and_errors(pkg, ctx):
+    e = {"offsets": np.array([0, 0], dtype=np.int64), "times": np.zeros(0), "flux": np.zeros(0),
+         "obs_var": np.zeros(0), "is_upper": np.zeros(0, np.int32), "upper_flux": np.zeros(0)}
+    data = pkg.GpuBatchData(ctx, csr=e)
+    out = ctx.laplace_fit(data, [0], [[0.0, 0.0, 10.0, 1.0, 2.0, -3.0]])
+    assert np.all(np.isfinite(out["log_sigma"][0, :6]))
+    with pytest.raises(pkg.GpuError):
+        ctx.laplace_fit(data, [7], [[-2.0, -1.0, 0.5, 0.0, -3.0]], mode=1)
