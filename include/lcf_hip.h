@@ -188,6 +188,30 @@ int lcf_batch_gp_fit_ex(lcf_ctx* ctx, int64_t n_bands, const int64_t* offsets, c
                         int criterion, double* out_state, double* out_pred, double* out_std,
                         double* out_dpred, double* out_pred_obs);
 
+/* GpuContext::batch_gp_2d (src/gpu_cuda.rs:1152-1306): per source, an
+ * anisotropic squared-exponential GP over (time, log10 wavelength); training
+ * set = at most min(max_subsample, 40) points; every (amp, lst, lsw)
+ * candidate with lst >= 0.1 and lsw >= 0.001 scored by the exact NLML;
+ * outputs the reference's 125-double state (alpha[40], t[40], w[40], amp,
+ * 1/(2 lt^2), 1/(2 lw^2), y mean, m), mean / std at the (qt[k], qw[k]) query
+ * points, and the train RMS over at most 100 observations (NaN on failure). */
+int lcf_batch_gp2d_fit(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,
+                       const double* log_waves, const double* mags, const double* noise_var,
+                       const double* query_times, const double* query_waves, int n_query,
+                       const double* amp_candidates, int n_amp, const double* lst_candidates,
+                       int n_lst, const double* lsw_candidates, int n_lsw, int max_subsample,
+                       double* out_state, double* out_pred, double* out_std, double* out_train_rms);
+/* General 2D form, as lcf_batch_gp_fit_ex: cap up to 128, state rows of
+ * 3*cap+5 doubles (alpha[cap], t[cap], w[cap], amp, 1/(2 lt^2), 1/(2 lw^2),
+ * mean, m), per-source query grids, d(mean)/dt, mean at every observation. */
+int lcf_batch_gp2d_fit_ex(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,
+                          const double* log_waves, const double* mags, const double* noise_var,
+                          const double* query_times, const double* query_waves, int n_query,
+                          int query_per_source, const double* amp_candidates, int n_amp,
+                          const double* lst_candidates, int n_lst, const double* lsw_candidates,
+                          int n_lsw, int cap, int subsample_rule, int criterion, double* out_state,
+                          double* out_pred, double* out_std, double* out_dpred, double* out_pred_obs);
+
 /* Timing of the last lcf_batch_fit_models-family call on this context:
  * milliseconds between HIP events around the kernel launches (inputs already
  * resident, outputs still on the device). */

@@ -16,6 +16,8 @@ from .gpu import (  # noqa: F401
     BatchPsoResult,
     BatchSource,
     DenseGpState,
+    Gp2dInput,
+    Gp2dOutput,
     GpBandInput,
     GpBandOutput,
     GpuBatchData,

@@ -1,20 +1,22 @@
-// gp_kernels.hip -- batched dense 1D Gaussian-process fits in FP64 for gfx950.
+// gp_kernels.hip -- batched dense Gaussian-process fits in FP64 for gfx950,
+// 1D (time) and 2D (time x log10 wavelength, anisotropic squared exponential).
 //
-// One 256-thread workgroup owns one band at a time (persistent, bands pulled
-// from a device queue in decreasing training-set size).  For every (amplitude,
-// length-scale) candidate it builds the kernel matrix K + diag(noise) as a
-// packed lower triangle in LDS (m <= 128: 66 KB), factorises it in place with
-// a right-looking Cholesky whose column scaling and trailing updates are
-// spread over all 256 threads, and solves for alpha on wavefront 0 with the
-// right-hand side held in registers (lane l owns rows l and l+64), so the two
-// triangular solves need no barriers.  The candidate is scored by the exact
-// negative log marginal likelihood or by the train RMS; the winner is refitted
-// and predicted (mean, derivative, standard deviation) at the query grid and
-// at every observation of the band.
+// One 256-thread workgroup owns one band (1D) or source (2D) at a time
+// (persistent, pulled from a device queue in decreasing training-set size).
+// For every hyper-parameter candidate it builds K + diag(noise) as a packed
+// lower triangle in LDS (m <= 128: 66 KB), factorises it in place with a
+// right-looking Cholesky whose column scaling and trailing updates are spread
+// over all 256 threads, and solves for alpha on wavefront 0 with the right-hand
+// side held in registers (lane l owns rows l and l+64), so the triangular
+// solves need no barriers.  Candidates are scored by the exact negative log
+// marginal likelihood or by the train RMS; the winner is refitted and
+// predicted (mean, time derivative of the mean, standard deviation) at the
+// query grid and at every observation.
 //
-// Arithmetic follows DenseGP (src/sparse_gp.rs:16-176) and batch_gp_fit
-// (cuda/gp.cu:23-367): the right-looking updates subtract l_ik l_jk in the
-// same k order as the reference's Crout loops, so factors agree to rounding.
+// Arithmetic follows DenseGP (src/sparse_gp.rs:16-176), batch_gp_fit
+// (cuda/gp.cu:23-367) and batch_gp2d_fit_predict (cuda/gp2d.cu:21-359): the
+// right-looking updates subtract l_ik l_jk in the same k order as the
+// reference's Crout loops, so factors agree to rounding.
 #include "gp_kernels.hpp"
 #include "lcf_device.hpp"
 
@@ -27,6 +29,7 @@ __device__ __forceinline__ int tri(int i, int j) { return (i * (i + 1) >> 1) + j
 struct GpShared {
     double* L;      // packed lower triangle, cap*(cap+1)/2
     double* x;      // training times [cap]
+    double* w;      // training log wavelengths [cap] (2D)
     double* y;      // training values [cap]
     double* nv;     // training noise variances [cap]
     double* alpha;  // [cap]
@@ -37,23 +40,31 @@ __device__ __forceinline__ GpShared gp_carve(double* base, int cap) {
     GpShared s;
     s.L = base;
     s.x = s.L + cap * (cap + 1) / 2;
-    s.y = s.x + cap;
+    s.w = s.x + cap;
+    s.y = s.w + cap;
     s.nv = s.y + cap;
     s.alpha = s.nv + cap;
     s.misc = s.alpha + cap;
     return s;
 }
 
+struct Hyp {
+    double amp, it, iw;  // amplitude, 1/(2 l_t^2), 1/(2 l_w^2) (0 in 1D)
+    __device__ __forceinline__ double k(double t1, double w1, double t2, double w2) const {
+        double dt = t1 - t2, dw = w1 - w2;
+        return amp * ::exp(-dt * dt * it - dw * dw * iw);
+    }
+};
+
 // Build K + diag(max(nv, 1e-10)) and factorise in place.  Returns false
 // (uniformly) when a pivot is not positive.
-__device__ bool gp_factor(const GpShared& s, int m, double amp, double i2) {
+__device__ bool gp_factor(const GpShared& s, int m, const Hyp& h) {
     const int tid = threadIdx.x;
     const int mm = m * m;
     for (int e = tid; e < mm; e += GP_THREADS) {
         int i = e / m, j = e - i * m;
         if (j <= i) {
-            double d = s.x[i] - s.x[j];
-            double v = amp * ::exp(-d * d * i2);
+            double v = h.k(s.x[i], s.w[i], s.x[j], s.w[j]);
             if (i == j) v += fmax(s.nv[i], 1e-10);
             s.L[tri(i, j)] = v;
         }
@@ -88,16 +99,14 @@ __device__ double gp_solve_alpha(const GpShared& s, int m, double ybar) {
     double b0 = lane < m ? s.y[lane] - ybar : 0.0;
     double b1 = lane + 64 < m ? s.y[lane + 64] - ybar : 0.0;
     const double yc0 = b0, yc1 = b1;
-    // forward: L z = yc
-    for (int j = 0; j < m; j++) {
+    for (int j = 0; j < m; j++) {  // forward: L z = yc
         double bj = j < 64 ? bcast(b0, j) : bcast(b1, j - 64);
         double zj = bj / s.L[tri(j, j)];
         if (lane == (j & 63)) { if (j < 64) b0 = zj; else b1 = zj; }
         if (lane > j && lane < m) b0 -= s.L[tri(lane, j)] * zj;
         if (lane + 64 > j && lane + 64 < m) b1 -= s.L[tri(lane + 64, j)] * zj;
     }
-    // backward: L^T a = z
-    for (int j = m - 1; j >= 0; j--) {
+    for (int j = m - 1; j >= 0; j--) {  // backward: L^T a = z
         double bj = j < 64 ? bcast(b0, j) : bcast(b1, j - 64);
         double aj = bj / s.L[tri(j, j)];
         if (lane == (j & 63)) { if (j < 64) b0 = aj; else b1 = aj; }
@@ -112,29 +121,45 @@ __device__ double gp_solve_alpha(const GpShared& s, int m, double ybar) {
 }
 
 // Wavefront 0: RMS of the predictive mean at the training points.
-__device__ double gp_train_rms(const GpShared& s, int m, double amp, double i2, double ybar) {
+__device__ double gp_train_rms(const GpShared& s, int m, const Hyp& h, double ybar) {
     const int lane = lane_id();
     double rss = 0.0;
     for (int i = lane; i < m; i += 64) {
         double mu = 0.0;
-        for (int k = 0; k < m; k++) {
-            double d = s.x[i] - s.x[k];
-            mu += amp * ::exp(-d * d * i2) * s.alpha[k];
-        }
+        for (int k = 0; k < m; k++) mu += h.k(s.x[i], s.w[i], s.x[k], s.w[k]) * s.alpha[k];
         double r = mu + ybar - s.y[i];
         rss += r * r;
     }
     return ::sqrt(wave_sum(rss) / (double)m);
 }
 
+__device__ __forceinline__ Hyp candidate(const GpArgs& a, int c, bool& valid) {
+    const int nw = a.n_lsw > 0 ? a.n_lsw : 1;
+    const int ia = c / (a.n_ls * nw), il = (c / nw) % a.n_ls, iw = c % nw;
+    Hyp h;
+    h.amp = a.amps[ia];
+    const double ls = a.lss[il];
+    valid = ls >= 0.1;
+    h.it = 0.5 / (ls * ls);
+    h.iw = 0.0;
+    if (a.w) {
+        const double lw = a.lsws[iw];
+        valid = valid && lw >= 0.001;
+        h.iw = 0.5 / (lw * lw);
+    }
+    return h;
+}
+
 __device__ void gp_band(const GpArgs& a, double* lds, int band) {
     const int tid = threadIdx.x;
     const int cap = a.cap;
+    const bool two_d = a.w != nullptr;
     GpShared s = gp_carve(lds, cap);
     const int64_t o0 = a.offsets[band];
     const int n = (int)(a.offsets[band + 1] - o0);
     double* st = a.state + (int64_t)band * a.state_stride;
     const double* q = a.query + (a.query_per_band ? (int64_t)band * a.n_query : 0);
+    const double* qw = a.query_w ? a.query_w + (a.query_per_band ? (int64_t)band * a.n_query : 0) : nullptr;
     for (int i = tid; i < a.state_stride; i += GP_THREADS) st[i] = 0.0;
     for (int k = tid; k < a.n_query; k += GP_THREADS) {
         a.pred[(int64_t)band * a.n_query + k] = NAN;
@@ -155,30 +180,30 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
         }
         idx = idx < n - 1 ? idx : n - 1;
         s.x[i] = a.t[o0 + idx];
+        s.w[i] = two_d ? a.w[o0 + idx] : 0.0;
         s.y[i] = a.y[o0 + idx];
         s.nv[i] = a.nv[o0 + idx];
     }
     __syncthreads();
-    // y mean over the training set (all threads, same summation order)
-    double ybar = 0.0;
+    double ybar = 0.0;  // all threads, same summation order
     for (int i = 0; i < m; i++) ybar += s.y[i];
     ybar /= (double)m;
 
     double best = 1e99;
     int best_c = 0;
     bool any = false;
-    const int nc = a.n_amp * a.n_ls;
+    const int nc = a.n_amp * a.n_ls * (a.n_lsw > 0 ? a.n_lsw : 1);
     for (int c = 0; c < nc; c++) {
-        const double amp = a.amps[c / a.n_ls], ls = a.lss[c % a.n_ls];
+        bool valid;
+        const Hyp h = candidate(a, c, valid);
         double score = 1e99;
-        if (ls >= 0.1) {
-            const double i2 = 0.5 / (ls * ls);
-            if (gp_factor(s, m, amp, i2)) {
+        if (valid) {
+            if (gp_factor(s, m, h)) {
                 if (tid < 64) {
                     double half_fit_logdet = gp_solve_alpha(s, m, ybar);
                     double sc;
                     if (a.criterion == 0) sc = half_fit_logdet + 0.5 * (double)m * 1.83787706640934548356;
-                    else sc = gp_train_rms(s, m, amp, i2, ybar);
+                    else sc = gp_train_rms(s, m, h, ybar);
                     if (tid == 0) s.misc[0] = __builtin_isfinite(sc) ? sc : 1e99;
                 }
                 __syncthreads();
@@ -186,43 +211,43 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
             }
             __syncthreads();
         }
-        if (c == 0) any = ls >= 0.1;
+        if (c == 0) any = valid;
         if (score < best) { best = score; best_c = c; any = true; }
     }
     if (!any) return;
-    const double amp = a.amps[best_c / a.n_ls], ls = a.lss[best_c % a.n_ls];
-    const double i2 = 0.5 / (ls * ls);
-    bool ok = gp_factor(s, m, amp, i2);
-    if (!ok) return;
+    bool valid;
+    const Hyp h = candidate(a, best_c, valid);
+    if (!gp_factor(s, m, h)) return;
     if (tid < 64) gp_solve_alpha(s, m, ybar);
     __syncthreads();
-    // state: alpha[cap], x[cap], amp, 1/(2 l^2), ybar, m
+    // state 1D: alpha[cap], x[cap], amp, 1/(2 lt^2), ybar, m
+    //       2D: alpha[cap], x[cap], w[cap], amp, 1/(2 lt^2), 1/(2 lw^2), ybar, m
     for (int i = tid; i < m; i += GP_THREADS) {
         st[i] = s.alpha[i];
         st[cap + i] = s.x[i];
+        if (two_d) st[2 * cap + i] = s.w[i];
     }
     if (tid == 0) {
-        st[2 * cap] = amp;
-        st[2 * cap + 1] = i2;
-        st[2 * cap + 2] = ybar;
-        st[2 * cap + 3] = (double)m;
+        int o = two_d ? 3 * cap : 2 * cap;
+        st[o++] = h.amp;
+        st[o++] = h.it;
+        if (two_d) st[o++] = h.iw;
+        st[o++] = ybar;
+        st[o] = (double)m;
     }
-    // predictive mean, derivative and std at the query grid (one query per thread)
+    // predictive mean, time derivative and std at the query grid
     for (int k = tid; k < a.n_query; k += GP_THREADS) {
-        const double t = q[k];
+        const double t = q[k], wq = qw ? qw[k] : 0.0;
         double mu = 0.0, dmu = 0.0;
         for (int i = 0; i < m; i++) {
-            double d = t - s.x[i];
-            double kv = amp * ::exp(-d * d * i2);
+            double kv = h.k(t, wq, s.x[i], s.w[i]);
             mu += kv * s.alpha[i];
-            dmu += -2.0 * i2 * d * kv * s.alpha[i];
+            dmu += -2.0 * h.it * (t - s.x[i]) * kv * s.alpha[i];
         }
-        // v = L^-1 k*, var = amp - |v|^2 (floored at 1e-10)
-        double vtv = 0.0;
-        double vloc[128];
+        double vtv = 0.0;  // v = L^-1 k*, var = amp - |v|^2 (floored at 1e-10)
+        double vloc[GP_LDS_CAP];
         for (int i = 0; i < m; i++) {
-            double d = t - s.x[i];
-            double si = amp * ::exp(-d * d * i2);
+            double si = h.k(t, wq, s.x[i], s.w[i]);
             for (int j = 0; j < i; j++) si -= s.L[tri(i, j)] * vloc[j];
             double vi = si / s.L[tri(i, i)];
             vloc[i] = vi;
@@ -230,17 +255,13 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
         }
         const int64_t o = (int64_t)band * a.n_query + k;
         a.pred[o] = mu + ybar;
-        a.stdv[o] = ::sqrt(fmax(amp - vtv, 1e-10));
+        a.stdv[o] = ::sqrt(fmax(h.amp - vtv, 1e-10));
         if (a.dpred) a.dpred[o] = dmu;
     }
-    // predictive mean at every observation of the band
-    for (int i = tid; i < n; i += GP_THREADS) {
-        const double t = a.t[o0 + i];
+    for (int i = tid; i < n; i += GP_THREADS) {  // predictive mean at every observation
+        const double t = a.t[o0 + i], wi = two_d ? a.w[o0 + i] : 0.0;
         double mu = 0.0;
-        for (int k = 0; k < m; k++) {
-            double d = t - s.x[k];
-            mu += amp * ::exp(-d * d * i2) * s.alpha[k];
-        }
+        for (int k = 0; k < m; k++) mu += h.k(t, wi, s.x[k], s.w[k]) * s.alpha[k];
         a.pred_obs[o0 + i] = mu + ybar;
     }
 }
@@ -260,7 +281,7 @@ __global__ __launch_bounds__(GP_THREADS) void gp_fit_kernel(GpArgs a) {
 }
 
 size_t gp_lds_bytes(int cap) {
-    return sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 4 * (size_t)cap + 8);
+    return sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 5 * (size_t)cap + 8);
 }
 
 hipError_t launch_gp_fit(const GpArgs& a, int grid, hipStream_t st) {

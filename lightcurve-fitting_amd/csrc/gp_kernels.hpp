@@ -10,6 +10,10 @@ constexpr int GP_LDS_CAP = 128;  // largest training set factorised in LDS
 
 struct GpArgs {
     const double* t;           // [n_obs] observation times
+    const double* w;           // [n_obs] log10 wavelengths (2D fit) or null (1D fit)
+    const double* query_w;     // query wavelengths, same shape as query (2D) or null
+    const double* lsws;        // wavelength length-scale candidates (2D)
+    int n_lsw;                 // 0 in 1D
     const double* y;           // [n_obs] values (magnitudes)
     const double* nv;          // [n_obs] noise variances
     const int64_t* offsets;    // [n_bands + 1]

@@ -568,12 +568,19 @@ int lcf_svi_elbo_grad(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, 
                    flags, 1, step, out_grad_mu, out_grad_log_sigma, out_elbo, nullptr);
 }
 
-int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* y,
-                        const double* nv, const double* query, int n_query, int query_per_band,
-                        const double* amps, int n_amp, const double* lss, int n_ls, int cap,
-                        int subsample_rule, int criterion, double* out_state, double* out_pred,
-                        double* out_std, double* out_dpred, double* out_pred_obs) {
+}  // extern "C"
+
+namespace {
+
+// Shared driver of the 1D (w == null) and 2D GP entry points.
+int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* w,
+           const double* y, const double* nv, const double* query, const double* query_w, int n_query,
+           int query_per_band, const double* amps, int n_amp, const double* lss, int n_ls,
+           const double* lsws, int n_lsw, int cap, int subsample_rule, int criterion, double* out_state,
+           double* out_pred, double* out_std, double* out_dpred, double* out_pred_obs) {
     if (!c) return LCF_E_INVALID;
+    if (w && (n_lsw <= 0 || !lsws || (n_query > 0 && !query_w)))
+        return fail(c, LCF_E_INVALID, "2D fit needs wavelengths, wavelength length scales and query wavelengths");
     if (n_bands < 0 || n_bands > INT32_MAX - 1 || !off) return fail(c, LCF_E_INVALID, "invalid band count");
     if (n_bands == 0) return LCF_OK;
     if (cap < 3 || cap > lcf::GP_LDS_CAP)
@@ -589,7 +596,7 @@ int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const d
     if (n > 0 && (!t || !y || !nv || !out_pred_obs)) return fail(c, LCF_E_INVALID, "null observation array");
     if (!out_state || (n_query > 0 && (!out_pred || !out_std))) return fail(c, LCF_E_INVALID, "null output");
     HIPCHK(c, hipSetDevice(c->device));
-    const int stride = 2 * cap + 4;
+    const int stride = w ? 3 * cap + 5 : 2 * cap + 4;
     std::vector<int32_t> order(n_bands);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
@@ -604,6 +611,12 @@ int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const d
     HIPCHK(c, dnv.alloc(n));
     HIPCHK(c, dpo.alloc(n));
     HIPCHK(c, dq.alloc(nq_tot));
+    DevBuf<double> dw, dqw, dlw;
+    if (w) {
+        HIPCHK(c, dw.alloc(n));
+        HIPCHK(c, dqw.alloc(nq_tot));
+        HIPCHK(c, dlw.alloc(n_lsw));
+    }
     HIPCHK(c, da.alloc(n_amp));
     HIPCHK(c, dl.alloc(n_ls));
     HIPCHK(c, dst.alloc(n_bands * stride));
@@ -619,12 +632,21 @@ int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const d
     HIPCHK(c, up(dy.p, y, sizeof(double) * n));
     HIPCHK(c, up(dnv.p, nv, sizeof(double) * n));
     HIPCHK(c, up(dq.p, query, sizeof(double) * nq_tot));
+    if (w) {
+        HIPCHK(c, up(dw.p, w, sizeof(double) * n));
+        HIPCHK(c, up(dqw.p, query_w, sizeof(double) * nq_tot));
+        HIPCHK(c, up(dlw.p, lsws, sizeof(double) * n_lsw));
+    }
     HIPCHK(c, up(da.p, amps, sizeof(double) * n_amp));
     HIPCHK(c, up(dl.p, lss, sizeof(double) * n_ls));
     HIPCHK(c, up(doff.p, off, sizeof(int64_t) * (n_bands + 1)));
     HIPCHK(c, up(dord.p, order.data(), sizeof(int32_t) * n_bands));
     lcf::GpArgs a{};
     a.t = dt.p;
+    a.w = w ? dw.p : nullptr;
+    a.query_w = w ? dqw.p : nullptr;
+    a.lsws = w ? dlw.p : nullptr;
+    a.n_lsw = w ? n_lsw : 0;
     a.y = dy.p;
     a.nv = dnv.p;
     a.offsets = doff.p;
@@ -670,6 +692,73 @@ int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const d
     }
     HIPCHK(c, down(out_pred_obs, dpo.p, sizeof(double) * n));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lcf_batch_gp_fit_ex(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* y,
+                        const double* nv, const double* query, int n_query, int query_per_band,
+                        const double* amps, int n_amp, const double* lss, int n_ls, int cap,
+                        int subsample_rule, int criterion, double* out_state, double* out_pred,
+                        double* out_std, double* out_dpred, double* out_pred_obs) {
+    return gp_run(c, n_bands, off, t, nullptr, y, nv, query, nullptr, n_query, query_per_band, amps, n_amp, lss,
+                  n_ls, nullptr, 0, cap, subsample_rule, criterion, out_state, out_pred, out_std, out_dpred,
+                  out_pred_obs);
+}
+
+int lcf_batch_gp2d_fit_ex(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t, const double* w,
+                          const double* y, const double* nv, const double* qt, const double* qw, int n_query,
+                          int query_per_source, const double* amps, int n_amp, const double* lst, int n_lst,
+                          const double* lsw, int n_lsw, int cap, int subsample_rule, int criterion,
+                          double* out_state, double* out_pred, double* out_std, double* out_dpred,
+                          double* out_pred_obs) {
+    if (!c) return LCF_E_INVALID;
+    if (!w) return fail(c, LCF_E_INVALID, "null wavelength array");
+    return gp_run(c, n_src, off, t, w, y, nv, qt, qw, n_query, query_per_source, amps, n_amp, lst, n_lst, lsw,
+                  n_lsw, cap, subsample_rule, criterion, out_state, out_pred, out_std, out_dpred, out_pred_obs);
+}
+
+int lcf_batch_gp2d_fit(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t, const double* w,
+                       const double* y, const double* nv, const double* qt, const double* qw, int n_query,
+                       const double* amps, int n_amp, const double* lst, int n_lst, const double* lsw,
+                       int n_lsw, int max_subsample, double* out_state, double* out_pred, double* out_std,
+                       double* out_train_rms) {
+    if (!c) return LCF_E_INVALID;
+    if (max_subsample < 3) return fail(c, LCF_E_INVALID, "max_subsample must be >= 3");
+    // the reference caps the training set at GP2D_MAX_M = 40 (cuda/gp2d.cu:9, 190-195)
+    const int cap = std::min(max_subsample, 40);
+    const int64_t n = n_src > 0 && off ? off[n_src] : 0;
+    std::vector<double> st(n_src > 0 ? (size_t)n_src * (3 * cap + 5) : 1), po(std::max<int64_t>(n, 1));
+    int rc = lcf_batch_gp2d_fit_ex(c, n_src, off, t, w, y, nv, qt, qw, n_query, 0, amps, n_amp, lst, n_lst, lsw,
+                                   n_lsw, cap, 0, 0, st.data(), out_pred, out_std, nullptr, po.data());
+    if (rc != LCF_OK) return rc;
+    // repack into the reference's 125-double layout (cuda/gp2d.cu:313-327) and
+    // the host-side train RMS over at most 100 observations (src/gpu_cuda.rs:1256-1279)
+    for (int64_t s = 0; s < n_src; s++) {
+        const double* in = st.data() + s * (3 * cap + 5);
+        double* o = out_state + s * 125;
+        for (int i = 0; i < 125; i++) o[i] = 0.0;
+        for (int i = 0; i < cap; i++) { o[i] = in[i]; o[40 + i] = in[cap + i]; o[80 + i] = in[2 * cap + i]; }
+        for (int i = 0; i < 5; i++) o[120 + i] = in[3 * cap + i];
+        const int m = (int)o[124];
+        const int64_t nt = off[s + 1] - off[s];
+        if (m >= 3 && out_train_rms) {
+            const int64_t ne = std::min<int64_t>(nt, 100);
+            const double step = ne < nt ? (double)nt / (double)ne : 1.0;
+            double rss = 0.0;
+            for (int64_t i = 0; i < ne; i++) {
+                int64_t idx = (int64_t)((double)i * step);
+                double d = po[off[s] + idx] - y[off[s] + idx];
+                rss += d * d;
+            }
+            out_train_rms[s] = std::sqrt(rss / (double)ne);
+        } else if (out_train_rms) {
+            out_train_rms[s] = NAN;
+        }
+    }
     return LCF_OK;
 }
 
@@ -753,8 +842,8 @@ int lcf_batch_gp_fit(lcf_ctx* c, int64_t n_bands, const int64_t* off, const doub
     const int cap = std::max(3, std::min(max_subsample, 25));
     if (max_subsample < 3) return fail(c, LCF_E_INVALID, "max_subsample must be >= 3");
     std::vector<double> st(n_bands > 0 ? (size_t)n_bands * (2 * cap + 4) : 1);
-    int rc = lcf_batch_gp_fit_ex(c, n_bands, off, t, mags, noise_var, query, n_query, 0, amps, n_amp, lss, n_ls,
-                                 cap, 0, 0, st.data(), out_pred, out_std, nullptr, out_pred_obs);
+    int rc = gp_run(c, n_bands, off, t, nullptr, mags, noise_var, query, nullptr, n_query, 0, amps, n_amp, lss, n_ls,
+                    nullptr, 0, cap, 0, 0, st.data(), out_pred, out_std, nullptr, out_pred_obs);
     if (rc != LCF_OK) return rc;
     // repack into the reference's 54-double layout: alpha[25], x[25], amp, 1/(2l^2), ybar, m
     for (int64_t b = 0; b < n_bands; b++) {

@@ -360,6 +360,99 @@ class GpuContext:
         return out
 
 
+def _gp2d_csr(sources) -> dict:
+    off = [0]
+    for s in sources:
+        if not (len(s.times) == len(s.waves) == len(s.mags) == len(s.noise_var)):
+            raise GpuError("Gp2dInput arrays differ in length")
+        off.append(off[-1] + len(s.times))
+    cat = lambda k: (np.concatenate([np.asarray(getattr(s, k), dtype=np.float64) for s in sources])
+                     if sources else np.zeros(0))
+    return {"offsets": np.asarray(off, dtype=np.int64), "times": cat("times"), "waves": cat("waves"),
+            "mags": cat("mags"), "noise_var": cat("noise_var")}
+
+
+def _batch_gp_2d(self, sources, query_times, query_waves, amp_candidates, lst_candidates, lsw_candidates,
+                 max_subsample: int):
+    """``GpuContext::batch_gp_2d`` (src/gpu_cuda.rs:1152-1306)."""
+    ns = len(sources)
+    if ns == 0:
+        return []
+    csr = _gp2d_csr(sources)
+    qt, qw = _f64(query_times), _f64(query_waves)
+    if qt.size != qw.size:
+        raise GpuError("query_times and query_waves differ in length")
+    a, lt, lw = _f64(amp_candidates), _f64(lst_candidates), _f64(lsw_candidates)
+    state, pred, std, rms = np.zeros((ns, 125)), np.zeros((ns, qt.size)), np.zeros((ns, qt.size)), np.zeros(ns)
+    self._check(self._lib.lcf_batch_gp2d_fit(
+        self._h, ns, _lib.ptr(csr["offsets"], C.c_int64), _lib.ptr(csr["times"], C.c_double),
+        _lib.ptr(csr["waves"], C.c_double), _lib.ptr(csr["mags"], C.c_double), _lib.ptr(csr["noise_var"], C.c_double),
+        _lib.ptr(qt, C.c_double), _lib.ptr(qw, C.c_double), qt.size, _lib.ptr(a, C.c_double), a.size,
+        _lib.ptr(lt, C.c_double), lt.size, _lib.ptr(lw, C.c_double), lw.size, int(max_subsample),
+        _lib.ptr(state, C.c_double), _lib.ptr(pred, C.c_double), _lib.ptr(std, C.c_double),
+        _lib.ptr(rms, C.c_double)))
+    out = []
+    for s in range(ns):
+        m = int(state[s, 124])
+        n_train = len(sources[s].times)
+        if m >= 3:
+            out.append(Gp2dOutput(list(pred[s]), list(std[s]), float(state[s, 120]),
+                                  float(np.sqrt(0.5 / state[s, 121])), float(np.sqrt(0.5 / state[s, 122])),
+                                  float(rms[s]), n_train, True))
+        else:
+            out.append(Gp2dOutput(list(pred[s]), list(std[s]), 0.0, 0.0, 0.0, float("nan"), n_train, False))
+    return out
+
+
+def _gp2d_fit(self, csr: dict, query_t, query_w, amp_candidates, lst_candidates, lsw_candidates, *,
+              cap: int = 128, subsample_rule: int = 0, criterion: int = 0, query_per_source: bool = False) -> dict:
+    """Array form of the 2D fit (``lcf_batch_gp2d_fit_ex``)."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, w, y, nv = (_f64(csr[k]) for k in ("times", "waves", "mags", "noise_var"))
+    qt, qw = _f64(query_t), _f64(query_w)
+    nq = qt.shape[-1] if qt.ndim else 0
+    a, lt, lw = _f64(amp_candidates), _f64(lst_candidates), _f64(lsw_candidates)
+    out = {"state": np.zeros((ns, 3 * cap + 5)), "pred": np.zeros((ns, nq)), "std": np.zeros((ns, nq)),
+           "dpred": np.zeros((ns, nq)), "pred_obs": np.zeros(t.size)}
+    self._check(self._lib.lcf_batch_gp2d_fit_ex(
+        self._h, ns, _lib.ptr(off, C.c_int64), _lib.ptr(t, C.c_double), _lib.ptr(w, C.c_double),
+        _lib.ptr(y, C.c_double), _lib.ptr(nv, C.c_double), _lib.ptr(qt, C.c_double), _lib.ptr(qw, C.c_double),
+        nq, int(query_per_source), _lib.ptr(a, C.c_double), a.size, _lib.ptr(lt, C.c_double), lt.size,
+        _lib.ptr(lw, C.c_double), lw.size, cap, subsample_rule, criterion, _lib.ptr(out["state"], C.c_double),
+        _lib.ptr(out["pred"], C.c_double), _lib.ptr(out["std"], C.c_double), _lib.ptr(out["dpred"], C.c_double),
+        _lib.ptr(out["pred_obs"], C.c_double)))
+    return out
+
+
+GpuContext.batch_gp_2d = _batch_gp_2d
+GpuContext.gp2d_fit = _gp2d_fit
+
+
+@dataclass
+class Gp2dInput:
+    """One source of a batch 2D GP fit (src/gpu_types.rs:166-172)."""
+
+    times: Sequence[float]
+    waves: Sequence[float]  # log10(wavelength / Angstrom)
+    mags: Sequence[float]
+    noise_var: Sequence[float]
+
+
+@dataclass
+class Gp2dOutput:
+    """Output of one source (src/gpu_types.rs:174-184)."""
+
+    pred_grid: list
+    std_grid: list
+    amp: float
+    ls_time: float
+    ls_wave: float
+    train_rms: float
+    n_train: int
+    success: bool
+
+
 @dataclass
 class GpBandInput:
     """One band of a batch GP fit (src/gpu_types.rs:143-148)."""

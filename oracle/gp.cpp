@@ -1,4 +1,4 @@
-// gp.cpp -- oracle restatement of the dense 1D Gaussian-process fits.
+// gp.cpp -- oracle restatement of the dense Gaussian-process fits.
 //
 // TEST INFRASTRUCTURE ONLY (parity checker for the HIP product).
 //
@@ -14,6 +14,9 @@
 //   the Cholesky fails); the first strict minimum in amp-major order wins;
 //   refit, predict mean / std on the query grid and the mean at every
 //   observation of the band.
+//   batch_gp_2d (cuda/gp2d.cu:123-359): the same with the anisotropic kernel
+//   a exp(-dt^2/(2 lt^2) - dw^2/(2 lw^2)) over (time, log10 wavelength),
+//   candidates amp-major then lt then lw, valid when lt >= 0.1 and lw >= 0.001.
 // Extensions used by the HIP kernels (same arithmetic): a train-RMS selection
 // criterion (the CPU path's, src/nonparametric.rs:1318-1335), the CPU
 // subsample rule of src/gp.rs:5-27, and the analytic time derivative of the
@@ -29,10 +32,13 @@ namespace orc {
 
 static const double PI = 3.14159265358979323846;
 
-static inline double rbf(double a, double b, double amp, double i2) {
-    double d = a - b;
-    return amp * std::exp(-d * d * i2);
-}
+struct Kern {
+    double amp, it, iw;
+    double operator()(double t1, double w1, double t2, double w2) const {
+        double dt = t1 - t2, dw = w1 - w2;
+        return amp * std::exp(-dt * dt * it - dw * dw * iw);
+    }
+};
 
 static bool cholesky(std::vector<double>& a, int n) {
     for (int j = 0; j < n; j++) {
@@ -68,67 +74,169 @@ static void solve_lt(const std::vector<double>& l, const double* b, double* x, i
 }
 
 struct Dense {
-    std::vector<double> x, alpha, l;
+    std::vector<double> x, w, alpha, l;
     int n = 0;
-    double amp = 0, i2 = 0, ymean = 0, nlml = 1e99;
+    Kern k{0, 0, 0};
+    double ymean = 0, nlml = 1e99;
     bool ok = false;
 };
 
-// DenseGP::fit (src/sparse_gp.rs:88-119) plus the NLML of cuda/gp.cu:91-99.
-static Dense dense_fit(const double* t, const double* y, const double* nv, int n, double amp, double i2) {
+static Dense dense_fit(const double* t, const double* w, const double* y, const double* nv, int n, Kern k) {
     Dense g;
     g.n = n;
-    g.amp = amp;
-    g.i2 = i2;
+    g.k = k;
     if (n == 0) return g;
     g.x.assign(t, t + n);
+    g.w.assign(n, 0.0);
+    if (w) g.w.assign(w, w + n);
     double ym = 0.0;
     for (int i = 0; i < n; i++) ym += y[i];
     ym /= n;
     g.ymean = ym;
-    std::vector<double> k(n * n);
+    std::vector<double> K(n * n);
     for (int i = 0; i < n; i++) {
         for (int j = 0; j <= i; j++) {
-            double v = rbf(t[i], t[j], amp, i2);
-            k[i * n + j] = v;
-            k[j * n + i] = v;
+            double v = k(g.x[i], g.w[i], g.x[j], g.w[j]);
+            K[i * n + j] = v;
+            K[j * n + i] = v;
         }
-        k[i * n + i] += std::max(nv[i], 1e-10);
+        K[i * n + i] += std::max(nv[i], 1e-10);
     }
-    if (!cholesky(k, n)) return g;
+    if (!cholesky(K, n)) return g;
     std::vector<double> yc(n), tmp(n);
     for (int i = 0; i < n; i++) yc[i] = y[i] - ym;
     g.alpha.resize(n);
-    solve_l(k, yc.data(), tmp.data(), n);
-    solve_lt(k, tmp.data(), g.alpha.data(), n);
+    solve_l(K, yc.data(), tmp.data(), n);
+    solve_lt(K, tmp.data(), g.alpha.data(), n);
     double fit = 0.0, ld = 0.0;
-    for (int i = 0; i < n; i++) { fit += yc[i] * g.alpha[i]; ld += std::log(k[i * n + i]); }
+    for (int i = 0; i < n; i++) { fit += yc[i] * g.alpha[i]; ld += std::log(K[i * n + i]); }
     double nl = 0.5 * fit + ld + 0.5 * n * std::log(2.0 * PI);
     g.nlml = std::isfinite(nl) ? nl : 1e99;
-    g.l = std::move(k);
+    g.l = std::move(K);
     g.ok = true;
     return g;
 }
 
-static double dense_mean(const Dense& g, double t) {
+static double dense_mean(const Dense& g, double t, double w) {
     double s = 0.0;
-    for (int i = 0; i < g.n; i++) s += rbf(t, g.x[i], g.amp, g.i2) * g.alpha[i];
+    for (int i = 0; i < g.n; i++) s += g.k(t, w, g.x[i], g.w[i]) * g.alpha[i];
     return s + g.ymean;
 }
 
-static double dense_dmean(const Dense& g, double t) {
+static double dense_dmean(const Dense& g, double t, double w) {
     double s = 0.0;
-    for (int i = 0; i < g.n; i++) s += -2.0 * g.i2 * (t - g.x[i]) * rbf(t, g.x[i], g.amp, g.i2) * g.alpha[i];
+    for (int i = 0; i < g.n; i++) s += -2.0 * g.k.it * (t - g.x[i]) * g.k(t, w, g.x[i], g.w[i]) * g.alpha[i];
     return s;
 }
 
-static double dense_std(const Dense& g, double t) {
+static double dense_std(const Dense& g, double t, double w) {
     std::vector<double> ks(g.n), v(g.n);
-    for (int i = 0; i < g.n; i++) ks[i] = rbf(t, g.x[i], g.amp, g.i2);
+    for (int i = 0; i < g.n; i++) ks[i] = g.k(t, w, g.x[i], g.w[i]);
     solve_l(g.l, ks.data(), v.data(), g.n);
     double vtv = 0.0;
     for (double z : v) vtv += z * z;
-    return std::sqrt(std::max(g.amp - vtv, 1e-10));
+    return std::sqrt(std::max(g.k.amp - vtv, 1e-10));
+}
+
+// Shared batch driver for the 1D (w == null) and 2D fits.
+static int batch_fit(int64_t n_bands, const int64_t* off, const double* t, const double* w, const double* y,
+                     const double* nv, const double* q, const double* qw, int nq, const double* amps, int n_amp,
+                     const double* lss, int n_ls, const double* lws, int n_lw, int max_sub, int rule,
+                     int criterion, double* state, int stride, double* pred, double* stdv, double* dpred,
+                     double* pred_obs, int n_threads) {
+    const bool two_d = w != nullptr;
+    const int nw = two_d ? n_lw : 1;
+    if (n_bands < 0 || nq < 0 || n_amp <= 0 || n_ls <= 0 || nw <= 0 || max_sub < 3 ||
+        stride < (two_d ? 3 * max_sub + 5 : 2 * max_sub + 4))
+        return -1;
+    parallel_for(n_bands, n_threads, [&](int64_t b) {
+        const int64_t a = off[b];
+        const int n = (int)(off[b + 1] - a);
+        double* st = state + b * stride;
+        for (int i = 0; i < stride; i++) st[i] = 0.0;
+        for (int k = 0; k < nq; k++) {
+            pred[b * nq + k] = NAN;
+            stdv[b * nq + k] = NAN;
+            if (dpred) dpred[b * nq + k] = NAN;
+        }
+        for (int i = 0; i < n; i++) pred_obs[a + i] = NAN;
+        if (n < 3) return;
+        int m = std::min(n, max_sub);
+        std::vector<double> st_t(m), st_w(m, 0.0), st_y(m), st_v(m);
+        for (int i = 0; i < m; i++) {
+            int idx;
+            if (n <= m) idx = i;
+            else if (rule == ORC_GP_SUBSAMPLE_GPU) {
+                double step = (double)(n - 1) / (double)(m - 1);
+                idx = std::min((int)(i * step + 0.5), n - 1);
+            } else {
+                idx = std::min((int)(((double)i + 0.5) * n / m), n - 1);  // src/gp.rs:15-19
+            }
+            st_t[i] = t[a + idx];
+            if (two_d) st_w[i] = w[a + idx];
+            st_y[i] = y[a + idx];
+            st_v[i] = nv[a + idx];
+        }
+        auto cand = [&](int c, bool& valid) {
+            int ia = c / (n_ls * nw), il = (c / nw) % n_ls, iw = c % nw;
+            Kern k{amps[ia], 0.5 / (lss[il] * lss[il]), 0.0};
+            valid = lss[il] >= 0.1;
+            if (two_d) {
+                k.iw = 0.5 / (lws[iw] * lws[iw]);
+                valid = valid && lws[iw] >= 0.001;
+            }
+            return k;
+        };
+        double best = 1e99;
+        int best_c = 0;
+        bool any = false;
+        for (int c = 0; c < n_amp * n_ls * nw; c++) {
+            bool valid;
+            Kern k = cand(c, valid);
+            double score = 1e99;
+            if (valid) {
+                Dense g = dense_fit(st_t.data(), st_w.data(), st_y.data(), st_v.data(), m, k);
+                if (g.ok) {
+                    if (criterion == ORC_GP_SELECT_NLML) score = g.nlml;
+                    else {
+                        double rss = 0.0;
+                        for (int i = 0; i < m; i++) {
+                            double r = dense_mean(g, st_t[i], st_w[i]) - st_y[i];
+                            rss += r * r;
+                        }
+                        score = std::sqrt(rss / m);
+                        if (!std::isfinite(score)) score = 1e99;
+                    }
+                }
+            }
+            if (c == 0) any = valid;
+            if (score < best) { best = score; best_c = c; any = true; }
+        }
+        if (!any) return;
+        bool valid;
+        Kern k = cand(best_c, valid);
+        Dense g = dense_fit(st_t.data(), st_w.data(), st_y.data(), st_v.data(), m, k);
+        if (!g.ok) return;
+        for (int i = 0; i < m; i++) {
+            st[i] = g.alpha[i];
+            st[max_sub + i] = st_t[i];
+            if (two_d) st[2 * max_sub + i] = st_w[i];
+        }
+        int o = two_d ? 3 * max_sub : 2 * max_sub;
+        st[o++] = k.amp;
+        st[o++] = k.it;
+        if (two_d) st[o++] = k.iw;
+        st[o++] = g.ymean;
+        st[o] = (double)m;
+        for (int kq = 0; kq < nq; kq++) {
+            double wq = two_d ? qw[kq] : 0.0;
+            pred[b * nq + kq] = dense_mean(g, q[kq], wq);
+            stdv[b * nq + kq] = dense_std(g, q[kq], wq);
+            if (dpred) dpred[b * nq + kq] = dense_dmean(g, q[kq], wq);
+        }
+        for (int i = 0; i < n; i++) pred_obs[a + i] = dense_mean(g, t[a + i], two_d ? w[a + i] : 0.0);
+    });
+    return 0;
 }
 
 }  // namespace orc
@@ -139,18 +247,18 @@ int orc_dense_gp(const double* t, const double* y, const double* nv, int n, doub
                  const double* q, int nq, double* mean, double* stdv, double* dmean, double* nlml,
                  double* train_rms) {
     if (n <= 0) return -1;
-    orc::Dense g = orc::dense_fit(t, y, nv, n, amp, 0.5 / (ls * ls));
+    orc::Dense g = orc::dense_fit(t, nullptr, y, nv, n, orc::Kern{amp, 0.5 / (ls * ls), 0.0});
     if (!g.ok) return 1;
     for (int k = 0; k < nq; k++) {
-        if (mean) mean[k] = orc::dense_mean(g, q[k]);
-        if (stdv) stdv[k] = orc::dense_std(g, q[k]);
-        if (dmean) dmean[k] = orc::dense_dmean(g, q[k]);
+        if (mean) mean[k] = orc::dense_mean(g, q[k], 0.0);
+        if (stdv) stdv[k] = orc::dense_std(g, q[k], 0.0);
+        if (dmean) dmean[k] = orc::dense_dmean(g, q[k], 0.0);
     }
     if (nlml) *nlml = g.nlml;
     if (train_rms) {
         double rss = 0.0;
         for (int i = 0; i < n; i++) {
-            double r = orc::dense_mean(g, t[i]) - y[i];
+            double r = orc::dense_mean(g, t[i], 0.0) - y[i];
             rss += r * r;
         }
         *train_rms = std::sqrt(rss / n);
@@ -163,76 +271,19 @@ int orc_batch_gp_fit(int64_t n_bands, const int64_t* off, const double* t, const
                      const double* lss, int n_ls, int max_sub, int subsample_rule, int criterion,
                      double* state, int state_stride, double* pred, double* stdv, double* dpred,
                      double* pred_obs, int n_threads) {
-    if (n_bands < 0 || nq < 0 || n_amp <= 0 || n_ls <= 0 || max_sub < 3 || state_stride < 2 * max_sub + 4)
-        return -1;
-    orc::parallel_for(n_bands, n_threads, [&](int64_t b) {
-        const int64_t a = off[b];
-        const int n = (int)(off[b + 1] - a);
-        double* st = state + b * state_stride;
-        for (int i = 0; i < state_stride; i++) st[i] = 0.0;
-        for (int k = 0; k < nq; k++) {
-            pred[b * nq + k] = NAN;
-            stdv[b * nq + k] = NAN;
-            if (dpred) dpred[b * nq + k] = NAN;
-        }
-        for (int i = 0; i < n; i++) pred_obs[a + i] = NAN;
-        if (n < 3) return;
-        int m = std::min(n, max_sub);
-        std::vector<double> st_t(m), st_y(m), st_v(m);
-        for (int i = 0; i < m; i++) {
-            int idx;
-            if (n <= m) idx = i;
-            else if (subsample_rule == ORC_GP_SUBSAMPLE_GPU) {
-                double step = (double)(n - 1) / (double)(m - 1);
-                idx = std::min((int)(i * step + 0.5), n - 1);
-            } else {
-                idx = std::min((int)(((double)i + 0.5) * n / m), n - 1);  // src/gp.rs:15-19
-            }
-            st_t[i] = t[a + idx];
-            st_y[i] = y[a + idx];
-            st_v[i] = nv[a + idx];
-        }
-        double best = 1e99;
-        int best_c = 0;
-        bool any = false;
-        for (int c = 0; c < n_amp * n_ls; c++) {
-            double amp = amps[c / n_ls], ls = lss[c % n_ls];
-            double score = 1e99;
-            if (ls >= 0.1) {
-                orc::Dense g = orc::dense_fit(st_t.data(), st_y.data(), st_v.data(), m, amp, 0.5 / (ls * ls));
-                if (g.ok) {
-                    if (criterion == ORC_GP_SELECT_NLML) score = g.nlml;
-                    else {
-                        double rss = 0.0;
-                        for (int i = 0; i < m; i++) {
-                            double r = orc::dense_mean(g, st_t[i]) - st_y[i];
-                            rss += r * r;
-                        }
-                        score = std::sqrt(rss / m);
-                        if (!std::isfinite(score)) score = 1e99;
-                    }
-                }
-            }
-            if (c == 0) any = ls >= 0.1;
-            if (score < best) { best = score; best_c = c; any = true; }
-        }
-        if (!any) return;
-        double amp = amps[best_c / n_ls], ls = lss[best_c % n_ls];
-        orc::Dense g = orc::dense_fit(st_t.data(), st_y.data(), st_v.data(), m, amp, 0.5 / (ls * ls));
-        if (!g.ok) return;
-        for (int i = 0; i < m; i++) { st[i] = g.alpha[i]; st[max_sub + i] = st_t[i]; }
-        st[2 * max_sub] = amp;
-        st[2 * max_sub + 1] = g.i2;
-        st[2 * max_sub + 2] = g.ymean;
-        st[2 * max_sub + 3] = (double)m;
-        for (int k = 0; k < nq; k++) {
-            pred[b * nq + k] = orc::dense_mean(g, q[k]);
-            stdv[b * nq + k] = orc::dense_std(g, q[k]);
-            if (dpred) dpred[b * nq + k] = orc::dense_dmean(g, q[k]);
-        }
-        for (int i = 0; i < n; i++) pred_obs[a + i] = orc::dense_mean(g, t[a + i]);
-    });
-    return 0;
+    return orc::batch_fit(n_bands, off, t, nullptr, y, nv, q, nullptr, nq, amps, n_amp, lss, n_ls, nullptr, 0,
+                          max_sub, subsample_rule, criterion, state, state_stride, pred, stdv, dpred, pred_obs,
+                          n_threads);
+}
+
+int orc_batch_gp2d_fit(int64_t n_src, const int64_t* off, const double* t, const double* w, const double* y,
+                       const double* nv, const double* qt, const double* qw, int nq, const double* amps,
+                       int n_amp, const double* lst, int n_lst, const double* lsw, int n_lsw, int max_sub,
+                       int subsample_rule, int criterion, double* state, int state_stride, double* pred,
+                       double* stdv, double* dpred, double* pred_obs, int n_threads) {
+    if (!w || !qw) return -1;
+    return orc::batch_fit(n_src, off, t, w, y, nv, qt, qw, nq, amps, n_amp, lst, n_lst, lsw, n_lsw, max_sub,
+                          subsample_rule, criterion, state, state_stride, pred, stdv, dpred, pred_obs, n_threads);
 }
 
 }  // extern "C"

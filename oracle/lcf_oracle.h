@@ -149,6 +149,16 @@ int orc_batch_gp_fit(int64_t n_bands, const int64_t* offsets, const double* t, c
                      double* state, int state_stride, double* pred, double* stdv, double* dpred,
                      double* pred_obs, int n_threads);
 
+/* GpuContext::batch_gp_2d (src/gpu_cuda.rs:1152-1306) with cap max_sub
+ * (the reference's is min(max_subsample, 40)); state rows (stride >=
+ * 3*max_sub+5): alpha, t, w [max_sub each], amp, 1/(2lt^2), 1/(2lw^2), ybar, m. */
+int orc_batch_gp2d_fit(int64_t n_src, const int64_t* offsets, const double* t, const double* w,
+                       const double* y, const double* noise_var, const double* qt, const double* qw,
+                       int nq, const double* amps, int n_amp, const double* lst, int n_lst,
+                       const double* lsw, int n_lsw, int max_sub, int subsample_rule, int criterion,
+                       double* state, int state_stride, double* pred, double* stdv, double* dpred,
+                       double* pred_obs, int n_threads);
+
 /* ---- Laplace (src/parametric.rs:2464-2693) ---------------------------------- */
 #define ORC_LAPLACE_FD4 0      /* reference: 4-point central differences of f */
 #define ORC_LAPLACE_GRAD_FD 1  /* central differences of the analytic gradient */

@@ -357,3 +357,27 @@ def laplace_fit(m: int, map_params, t, f, v, up, uf, mode: int = LAPLACE_FD4):
                            C.byref(e), _p(hess, C.c_double))
     assert rc == 0
     return ls, e.value, hess.reshape(np_ - 1, np_ - 1)
+
+
+def batch_gp2d_fit(csr: dict, qt, qw, amps, lst, lsw, cap: int, subsample_rule: int = GP_SUBSAMPLE_GPU,
+                   criterion: int = GP_SELECT_NLML, n_threads: int | None = None):
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, w, y, nv = (f64(csr[k]) for k in ("times", "waves", "mags", "noise_var"))
+    qt, qw, a, lt, lw = f64(qt), f64(qw), f64(amps), f64(lst), f64(lsw)
+    stride = 3 * cap + 5
+    state = np.zeros((ns, stride))
+    pred, std, dpred = np.zeros((ns, qt.size)), np.zeros((ns, qt.size)), np.zeros((ns, qt.size))
+    pobs = np.zeros(t.size)
+    L = lib()
+    L.orc_batch_gp2d_fit.restype = _i32
+    L.orc_batch_gp2d_fit.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _i32, _pd, _i32,
+                                     _pd, _i32, _i32, _i32, _i32, _pd, _i32, _pd, _pd, _pd, _pd, _i32]
+    rc = L.orc_batch_gp2d_fit(ns, _p(off, C.c_int64), _p(t, C.c_double), _p(w, C.c_double), _p(y, C.c_double),
+                              _p(nv, C.c_double), _p(qt, C.c_double), _p(qw, C.c_double), qt.size,
+                              _p(a, C.c_double), a.size, _p(lt, C.c_double), lt.size, _p(lw, C.c_double), lw.size,
+                              cap, subsample_rule, criterion, _p(state, C.c_double), stride, _p(pred, C.c_double),
+                              _p(std, C.c_double), _p(dpred, C.c_double), _p(pobs, C.c_double),
+                              n_threads or default_threads())
+    assert rc == 0
+    return {"state": state, "pred": pred, "std": std, "dpred": dpred, "pred_obs": pobs}

@@ -110,3 +110,100 @@ def test_batch_selection_semantics(orc, criterion, rule):
         assert np.allclose(out["pred"][b], want[0], rtol=1e-12) and np.allclose(out["std"][b], want[1], rtol=1e-12)
         po = orc.dense_gp(tt, yy, vv, A, ls, csr["times"][a:e])[0]
         assert np.allclose(out["pred_obs"][a:e], po, rtol=1e-12)
+
+
+def np_gp2d(t, w, y, nv, amp, lt, lw, qt, qw):
+    """Anisotropic RBF GP over (t, w), computed with numpy.linalg."""
+    def k(t1, w1, t2, w2):
+        return amp * np.exp(-(t1[:, None] - t2[None, :]) ** 2 / (2 * lt * lt)
+                            - (w1[:, None] - w2[None, :]) ** 2 / (2 * lw * lw))
+    K = k(t, w, t, w) + np.diag(np.maximum(nv, 1e-10))
+    L = np.linalg.cholesky(K)
+    ym = y.mean()
+    alpha = np.linalg.solve(K, y - ym)
+    ks = k(qt, qw, t, w)
+    v = np.linalg.solve(L, ks.T)
+    nlml = 0.5 * (y - ym) @ alpha + np.log(np.diag(L)).sum() + 0.5 * t.size * math.log(2 * math.pi)
+    return ks @ alpha + ym, np.sqrt(np.maximum(amp - (v * v).sum(axis=0), 1e-10)), nlml
+
+
+def make_sources_2d(rng, sizes):
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    cols = {k: [] for k in ("times", "waves", "mags", "noise_var")}
+    for n in sizes:
+        tt = rng.uniform(0, 80, n)
+        ww = rng.choice(np.log10([4770.0, 6231.0, 7625.0]), n)
+        cols["times"].append(tt)
+        cols["waves"].append(ww)
+        cols["mags"].append(19 - 1.2 * np.exp(-((tt - 30) / 12) ** 2) + 0.8 * (ww - 3.75)
+                            + 0.03 * rng.normal(size=n))
+        cols["noise_var"].append(rng.uniform(1e-4, 4e-3, n))
+    return {"offsets": off, **{k: np.concatenate(v) if v else np.zeros(0) for k, v in cols.items()}}
+
+
+AMPS2 = np.array([0.1, 0.5, 2.0])
+LST2 = np.array([0.05, 5.0, 15.0, 40.0])
+LSW2 = np.array([0.0005, 0.05, 0.2, 1.0])
+
+
+@pytest.mark.parametrize("criterion", [0, 1])
+def test_batch_gp2d_selection_against_numpy(orc, criterion):
+    """Every candidate rescored with numpy; the first strict minimum in
+    amp-major, then l_t, then l_w order must be the one the oracle kept."""
+    rng = np.random.default_rng(21)
+    csr = make_sources_2d(rng, [1, 12, 40, 55])
+    qt, qw = np.linspace(0, 80, 30), np.full(30, math.log10(6231.0))
+    cap = 40
+    out = orc.batch_gp2d_fit(csr, qt, qw, AMPS2, LST2, LSW2, cap, 0, criterion)
+    off = csr["offsets"]
+    assert np.all(np.isnan(out["pred"][0])) and out["state"][0, 3 * cap + 4] == 0
+    for s in range(1, 4):
+        a, e = off[s], off[s + 1]
+        n = e - a
+        m = min(n, cap)
+        idx = np.arange(n) if n <= m else np.minimum((np.arange(m) * (n - 1) / (m - 1) + 0.5).astype(int), n - 1)
+        tt, ww, yy, vv = (csr[k][a:e][idx] for k in ("times", "waves", "mags", "noise_var"))
+        scores = []
+        for A in AMPS2:
+            for lt in LST2:
+                for lw in LSW2:
+                    if lt < 0.1 or lw < 0.001:
+                        scores.append(1e99)
+                        continue
+                    if criterion == 0:
+                        scores.append(np_gp2d(tt, ww, yy, vv, A, lt, lw, tt, ww)[2])
+                    else:
+                        mt = np_gp2d(tt, ww, yy, vv, A, lt, lw, tt, ww)[0]
+                        scores.append(math.sqrt(np.mean((mt - yy) ** 2)))
+        c = int(np.argmin(scores))
+        A, lt, lw = AMPS2[c // 16], LST2[(c // 4) % 4], LSW2[c % 4]
+        st = out["state"][s]
+        assert st[3 * cap] == A and st[3 * cap + 1] == pytest.approx(0.5 / lt ** 2)
+        assert st[3 * cap + 2] == pytest.approx(0.5 / lw ** 2) and st[3 * cap + 4] == m
+        assert np.array_equal(st[cap:cap + m], tt) and np.array_equal(st[2 * cap:2 * cap + m], ww)
+        mean, std, _ = np_gp2d(tt, ww, yy, vv, A, lt, lw, qt, qw)
+        assert np.allclose(out["pred"][s], mean, rtol=1e-9, atol=1e-9)
+        assert np.allclose(out["std"][s], std, rtol=1e-6, atol=1e-9)
+        po = np_gp2d(tt, ww, yy, vv, A, lt, lw, csr["times"][a:e], csr["waves"][a:e])[0]
+        assert np.allclose(out["pred_obs"][a:e], po, rtol=1e-9, atol=1e-9)
+        h = 1e-5
+        fd = (np_gp2d(tt, ww, yy, vv, A, lt, lw, qt + h, qw)[0] - np_gp2d(tt, ww, yy, vv, A, lt, lw, qt - h, qw)[0]) / (2 * h)
+        assert np.allclose(out["dpred"][s], fd, rtol=1e-5, atol=1e-7)
+
+
+def test_batch_gp2d_reduces_to_1d(orc):
+    """With a single wavelength the 2D kernel is the 1D one: same selection
+    (one l_w candidate), same alpha and predictions."""
+    rng = np.random.default_rng(5)
+    csr = make_sources_2d(rng, [3, 30, 70])
+    csr["waves"][:] = 3.7
+    q = np.linspace(0, 80, 25)
+    amps, lss = np.array([0.3, 1.0, 3.0]), np.array([5.0, 10.0, 20.0])
+    two = orc.batch_gp2d_fit(csr, q, np.full(25, 3.7), amps, lss, np.array([0.3]), 40)
+    one = orc.batch_gp_fit(csr, q, amps, lss, 40)
+    assert np.array_equal(two["state"][:, :80], one["state"][:, :80])
+    assert np.array_equal(two["state"][:, 120:122], one["state"][:, 80:82])
+    assert np.array_equal(two["state"][:, 123:125], one["state"][:, 82:84])
+    for k in ("pred", "std", "dpred", "pred_obs"):
+        assert np.array_equal(two[k], one[k], equal_nan=True)
