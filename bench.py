@@ -104,8 +104,11 @@ def main() -> int:
 
     import torch  # device-wide synchronisation around the timed region
 
+    log("first pass (untimed)")
+    t_first = time.perf_counter()
     first = step(per_model=True)
     iters_exec = first["model_iters"]
+    log(f"first pass done in {time.perf_counter() - t_first:.2f} s")
     for w in range(max(args.warmup - 1, 0)):
         step()
         log(f"warmup {w + 2}/{args.warmup} done")
