@@ -158,10 +158,7 @@ int lcf_svi_elbo_grad(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_
  * (shape parameters from the Hessian of the negative log posterior with
  * sigma_x fixed; sigma_x's slot is ln of its prior width), elbo[j] = -f(MAP),
  * hessian[j][36] (row-major (np-1)^2 block, may be NULL).  The reference runs
- * this per band on the CPU inside fit_parametric; batched here.
- * Currently switched off: after validating its arguments it returns
- * LCF_E_HIP with a message saying so (its kernel's first device run did not
- * complete and the cause has not been found). */
+ * this per band on the CPU inside fit_parametric; batched here. */
 int lcf_batch_laplace(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_ids,
                       const double* map_params, int mode, double* out_log_sigma, double* out_elbo,
                       double* out_hessian);

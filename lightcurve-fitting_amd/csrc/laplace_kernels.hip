@@ -237,11 +237,12 @@ struct LaplaceJob {
                         H[j * NS + i] = hij;
                     }
             } else {
+                // gradient slot 2i holds theta + h_i e_i, slot 2i+1 theta - h_i e_i
                 const double* G = vals + LAP_MAX_PTS;
                 for (int i = 0; i < NS; i++)
                     for (int j = 0; j < NS; j++) {
-                        double gij = (G[(2 * i + 1) * NS + j] - G[(2 * i) * NS + j]) / (2.0 * h[i]);
-                        double gji = (G[(2 * j + 1) * NS + i] - G[(2 * j) * NS + i]) / (2.0 * h[j]);
+                        double gij = (G[(2 * i) * NS + j] - G[(2 * i + 1) * NS + j]) / (2.0 * h[i]);
+                        double gji = (G[(2 * j) * NS + i] - G[(2 * j + 1) * NS + i]) / (2.0 * h[j]);
                         H[i * NS + j] = 0.5 * (gij + gji);
                     }
             }
@@ -300,11 +301,10 @@ __global__ __launch_bounds__(64) void laplace_kernel(LaplaceArgs a) {
     __shared__ double vals[LAP_MAX_PTS + 12 * 6];
     LaplaceJob<MODEL> job;
     for (;;) {
-        int t = 0;
-        if (lane_id() == 0) t = atomicAdd(a.queue, 1);
-        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, WAVE));
+        const int t = wave_next_task(a.queue);
         if (t >= a.n_tasks) break;
         job.run(a, lap_lds, vals, a.order[t]);
+        wave_reconverge();
     }
 }
 

@@ -152,6 +152,25 @@ __device__ __forceinline__ R wave_sum(R v) {
     return v;
 }
 
+// Next task of a persistent single-wave workgroup from a device work queue.
+// Every lane takes part in the atomic (lane 0 adds 1, the others 0) and the
+// value lane 0 received is read back with readlane, so the fetch has no
+// branch on the lane index.  The obvious form `if (lane == 0) t =
+// atomicAdd(q, 1); t = shfl(t, 0)` is unsafe at -O3: jump threading merges
+// that branch with a lane-0 branch at the end of the loop body, lanes 1..63
+// then re-enter the loop without lane 0 fetching a ticket and spin forever
+// on a stale index (the loop nest of the compiled kernel shows it).
+__device__ __forceinline__ int wave_next_task(int* queue) {
+    int mine = atomicAdd(queue, lane_id() == 0 ? 1 : 0);
+    return __builtin_amdgcn_readlane(mine, 0);
+}
+
+// Convergence point at the end of a loop body whose lanes must stay together
+// (cross-lane operations inside, lane-dependent branches around them): a
+// block holding a convergent operation is never duplicated, so no lane
+// condition known at the end of one iteration can be threaded into the next.
+__device__ __forceinline__ void wave_reconverge() { __builtin_amdgcn_wave_barrier(); }
+
 // Kahan-compensated accumulator.
 template <typename R>
 struct Kahan {

@@ -289,11 +289,10 @@ __global__ __launch_bounds__(64) void pso_model_kernel(PsoArgs a) {
     O* lds = reinterpret_cast<O*>(lds_raw);
     if (PERSISTENT) {
         for (;;) {
-            int t = 0;
-            if (lane_id() == 0) t = atomicAdd(a.queue, 1);
-            t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, WAVE));
+            const int t = wave_next_task(a.queue);
             if (t >= a.n_tasks) break;
             fit_band<MODEL, R, PPL>(a, lds, a.order[t]);
+            wave_reconverge();
         }
     } else {
         int t = a.task_base + (int)blockIdx.x;

@@ -70,9 +70,6 @@ struct DevBuf {
     hipError_t alloc(size_t n) { return hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)); }
 };
 
-// Whether lcf_batch_laplace may launch its kernel (switched off, see there).
-constexpr bool laplace_enabled() { return false; }
-
 const int N_PARAMS[LCF_N_MODELS] = {6, 7, 7, 5, 5, 5, 6, 5};
 // ALL_GPU_MODELS (src/gpu_types.rs:107-116)
 const int32_t ALL_MODELS[LCF_N_MODELS] = {0, 3, 2, 6, 1, 4, 5, 7};
@@ -782,13 +779,6 @@ int lcf_batch_laplace(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, 
         if (model_ids[j] == 7 && mode == LCF_LAPLACE_GRAD_FD)
             return fail(c, LCF_E_INVALID, "MetzgerKN has no analytic gradient; use LCF_LAPLACE_FD4");
     }
-    // Switched off: the first device run of laplace_kernel did not complete,
-    // and neither reading the code nor running it on the CPU has shown why.
-    // The launch path below stays compiled but unreachable until a cause is
-    // found and fixed.
-    if (!laplace_enabled())
-        return fail(c, LCF_E_HIP, "the batched Laplace kernel is switched off (a device run of it did not "
-                                  "complete; cause not yet found)");
     HIPCHK(c, hipSetDevice(c->device));
     std::vector<int32_t> order;
     order.reserve(nj);

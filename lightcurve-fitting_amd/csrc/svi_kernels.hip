@@ -329,11 +329,10 @@ __global__ __launch_bounds__(64) void svi_kernel(SviArgs a) {
     SviJob<MODEL, R, FD> job;
     if (PERSISTENT) {
         for (;;) {
-            int t = 0;
-            if (lane_id() == 0) t = atomicAdd(a.queue, 1);
-            t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, WAVE));
+            const int t = wave_next_task(a.queue);
             if (t >= a.n_tasks) break;
             job.run(a, lds, a.order[t]);
+            wave_reconverge();
         }
     } else {
         int t = a.task_base + (int)blockIdx.x;
