@@ -80,7 +80,7 @@ def main() -> int:
     ap.add_argument("--sources", type=int, default=100_000)
     ap.add_argument("--particles", type=int, default=64)
     ap.add_argument("--iters", type=int, default=200)
-    ap.add_argument("--cpu-sources", type=int, default=24)
+    ap.add_argument("--cpu-sources", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -102,8 +102,6 @@ def main() -> int:
         return ctx.fit_models(data, MODELS, args.particles, args.iters, args.iters, seed=42,
                               per_model=per_model)
 
-    import torch  # device-wide synchronisation around the timed region
-
     log("first pass (untimed)")
     t_first = time.perf_counter()
     first = step(per_model=True)
@@ -112,14 +110,14 @@ def main() -> int:
     for w in range(max(args.warmup - 1, 0)):
         step()
         log(f"warmup {w + 2}/{args.warmup} done")
-    torch.cuda.synchronize()
+    ctx.synchronize()
     t0 = time.perf_counter()
     slot_ms = np.zeros(len(MODELS))
     for k in range(args.steps):
         step()
         slot_ms += [ctx.last_model_ms(i) for i in range(len(MODELS))]
         log(f"step {k + 1}/{args.steps}: {ctx.last_kernel_ms:.1f} ms kernels")
-    torch.cuda.synchronize()
+    ctx.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = dt * 1e3 / args.steps
     value = args.sources / (dt / args.steps)
@@ -143,7 +141,7 @@ def main() -> int:
         ctx.set_schedule(pkg.SCHED_PER_CLASS)
         t1 = time.perf_counter()
         step()
-        torch.cuda.synchronize()
+        ctx.synchronize()
         pdt = time.perf_counter() - t1
         ctx.set_schedule(pkg.SCHED_PERSISTENT)
         plain = {"value": args.sources / pdt, "unit": "sources/s", "ms_per_step": pdt * 1e3,

@@ -62,6 +62,11 @@ int lcf_ctx_set_precision(lcf_ctx* ctx, int precision);
 int lcf_ctx_set_schedule(lcf_ctx* ctx, int schedule);
 /* Number of compute units of the context's device. */
 int lcf_ctx_num_cus(const lcf_ctx* ctx);
+/* Waits for all work of the context's stream and device.  Every batch call
+ * already returns synchronised (as the reference's end with
+ * cudaDeviceSynchronize, src/gpu_cuda.rs:353-354); this is for callers that
+ * time or order work around them. */
+int lcf_ctx_synchronize(lcf_ctx* ctx);
 
 /* ---- resident source data: GpuBatchData (src/gpu_cuda.rs:236-292) ------ */
 /* offsets[n_sources+1] index the concatenated per-observation arrays

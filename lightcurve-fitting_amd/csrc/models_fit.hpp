@@ -46,11 +46,11 @@ __device__ __forceinline__ R logistic(R x) {
     return x >= R(0) ? q : e * q;
 }
 
-template <int MODEL, typename R> struct Model;
+template <int MODEL, typename R> struct ModelStable;
 
 // Bazin: [ln A, B, t0, ln tau_rise, ln tau_fall, ln sigma_x]
 // f = A e^{-dt/tf} / (1 + e^{-dt/tr}) + B      (src/parametric.rs:200-211)
-template <typename R> struct Model<0, R> {
+template <typename R> struct ModelStable<0, R> {
     R la, b, t0, itr, itf;
     __device__ __forceinline__ void setup(const R* p) {
         la = p[0]; b = p[1]; t0 = p[2];
@@ -72,7 +72,7 @@ template <typename R> struct Model<0, R> {
 // Villar: [ln A, beta, ln gamma, t0, ln tau_rise, ln tau_fall, ln sigma_x]
 // f = A s(ph/tr) [(1-w)(1-beta ph) + w (1-beta gamma) e^{(gamma-ph)/tf}],
 // w = s(10 (ph - gamma))                        (src/parametric.rs:231-247)
-template <typename R> struct Model<1, R> {
+template <typename R> struct ModelStable<1, R> {
     R a, beta, gamma, t0, itr, itf, one_m_bg;
     __device__ __forceinline__ void setup(const R* p) {
         using M = Math<R>;
@@ -101,7 +101,7 @@ template <typename R> struct Model<1, R> {
 
 // TDE: [ln A, B, t0, ln tau_rise, ln tau_fall, alpha, ln sigma_x]
 // f = A s(ph/tr) (1 + softplus(ph)/tf)^{-alpha} + B   (src/parametric.rs:297-311)
-template <typename R> struct Model<2, R> {
+template <typename R> struct ModelStable<2, R> {
     R la, b, t0, itr, itf, alpha;
     __device__ __forceinline__ void setup(const R* p) {
         la = p[0]; b = p[1]; t0 = p[2];
@@ -124,7 +124,7 @@ template <typename R> struct Model<2, R> {
 // Arnett: [ln A, t0, ln tau_m, logit f, ln sigma_x]
 // f = A [f e^{-ps/8.8} + (1-f) e^{-ps/111.3}] (1 - e^{-(ps/tm)^2})
 //                                               (src/parametric.rs:339-356)
-template <typename R> struct Model<3, R> {
+template <typename R> struct ModelStable<3, R> {
     R a, t0, itm, fr;
     __device__ __forceinline__ void setup(const R* p) {
         using M = Math<R>;
@@ -146,7 +146,7 @@ template <typename R> struct Model<3, R> {
 
 // Magnetar: [ln A, t0, ln tau_sd, ln tau_diff, ln sigma_x]
 // f = A (1 + ps/tsd)^{-2} (1 - e^{-(ps/td)^2})  (src/parametric.rs:387-400)
-template <typename R> struct Model<4, R> {
+template <typename R> struct ModelStable<4, R> {
     R a, t0, itsd, itd;
     __device__ __forceinline__ void setup(const R* p) {
         using M = Math<R>;
@@ -168,7 +168,7 @@ template <typename R> struct Model<4, R> {
 
 // ShockCooling: [ln A, t0, n, ln tau_tr, ln sigma_x]
 // f = A s(5 ph) ps^{-n} e^{-(ps/ttr)^2}         (src/parametric.rs:427-440)
-template <typename R> struct Model<5, R> {
+template <typename R> struct ModelStable<5, R> {
     R la, t0, n, ittr;
     __device__ __forceinline__ void setup(const R* p) {
         la = p[0]; t0 = p[1]; n = p[2];
@@ -188,7 +188,7 @@ template <typename R> struct Model<5, R> {
 
 // Afterglow: [ln A, t0, ln t_b, alpha1, alpha2, ln sigma_x]
 // f = A (r^{2 a1} + r^{2 a2})^{-1/2}, r = ps/t_b  (src/parametric.rs:468-483)
-template <typename R> struct Model<6, R> {
+template <typename R> struct ModelStable<6, R> {
     R la, t0, ltb, a1, a2;
     __device__ __forceinline__ void setup(const R* p) {
         la = p[0]; t0 = p[1]; ltb = p[2]; a1 = p[3]; a2 = p[4];
@@ -211,6 +211,123 @@ template <typename R> struct Model<6, R> {
         R d = e1 > e2 ? e2 - e1 : e1 - e2;             // -|e1 - e2|
         f = M::exp(la - R(0.5) * (mx + log1p_unit<R>(M::exp(d))));
         return true;
+    }
+};
+
+// The model used by the fitting kernels: the overflow-free form, except in
+// FP64 (specializations below).
+template <int MODEL, typename R> struct Model : ModelStable<MODEL, R> {};
+
+// ---------------------------------------------------------------------------
+// FP64: the reference's expressions evaluated as written (src/parametric.rs:
+// 199-510; the same formulas as cuda/models.cu:46-131), with only the
+// parameter-only exponentials hoisted into setup().  The FP64 kernels then
+// round as the reference does (up to the device math library), and every
+// non-finite prediction is the reference's by construction.  These full
+// specializations take precedence over the overflow-free forms above, which
+// serve FP32.
+// ---------------------------------------------------------------------------
+template <> struct Model<0, double> {
+    double a, b, t0, tr, tf;
+    __device__ __forceinline__ void setup(const double* p) {
+        a = ::exp(p[0]); b = p[1]; t0 = p[2]; tr = ::exp(p[3]); tf = ::exp(p[4]);
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double dt = t - t0;
+        double e_fall = ::exp(-dt / tf);
+        double sig = 1.0 / (1.0 + ::exp(-dt / tr));
+        f = a * e_fall * sig + b;
+        return __builtin_isfinite(f);
+    }
+};
+
+template <> struct Model<1, double> {
+    double a, beta, gamma, t0, tr, tf;
+    __device__ __forceinline__ void setup(const double* p) {
+        a = ::exp(p[0]); beta = p[1]; gamma = ::exp(p[2]); t0 = p[3]; tr = ::exp(p[4]); tf = ::exp(p[5]);
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double ph = t - t0;
+        double sr = 1.0 / (1.0 + ::exp(-ph / tr));
+        double w = 1.0 / (1.0 + ::exp(-10.0 * (ph - gamma)));
+        double pl = 1.0 - beta * ph;
+        double pr = (1.0 - beta * gamma) * ::exp((gamma - ph) / tf);
+        f = a * sr * ((1.0 - w) * pl + w * pr);
+        return __builtin_isfinite(f);
+    }
+};
+
+template <> struct Model<2, double> {
+    double a, b, t0, tr, tf, alpha;
+    __device__ __forceinline__ void setup(const double* p) {
+        a = ::exp(p[0]); b = p[1]; t0 = p[2]; tr = ::exp(p[3]); tf = ::exp(p[4]); alpha = p[5];
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double ph = t - t0;
+        double sig = 1.0 / (1.0 + ::exp(-ph / tr));
+        double ps = ::log(1.0 + ::exp(ph)) + 1e-6;
+        double w = 1.0 + ps / tf;
+        f = a * sig * ::pow(w, -alpha) + b;
+        return __builtin_isfinite(f);
+    }
+};
+
+template <> struct Model<3, double> {
+    double a, t0, tm, fr;
+    __device__ __forceinline__ void setup(const double* p) {
+        a = ::exp(p[0]); t0 = p[1]; tm = ::exp(p[2]); fr = 1.0 / (1.0 + ::exp(-p[3]));
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double ps = ::log(1.0 + ::exp(t - t0)) + 1e-6;
+        double e_ni = ::exp(-ps / 8.8), e_co = ::exp(-ps / 111.3);
+        double heat = fr * e_ni + (1.0 - fr) * e_co;
+        double x = ps / tm;
+        f = a * heat * (1.0 - ::exp(-x * x));
+        return __builtin_isfinite(f);
+    }
+};
+
+template <> struct Model<4, double> {
+    double a, t0, tsd, tdf;
+    __device__ __forceinline__ void setup(const double* p) {
+        a = ::exp(p[0]); t0 = p[1]; tsd = ::exp(p[2]); tdf = ::exp(p[3]);
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double ps = ::log(1.0 + ::exp(t - t0)) + 1e-6;
+        double w = 1.0 + ps / tsd;
+        double spindown = 1.0 / (w * w);
+        double x = ps / tdf;
+        f = a * spindown * (1.0 - ::exp(-x * x));
+        return __builtin_isfinite(f);
+    }
+};
+
+template <> struct Model<5, double> {
+    double a, t0, n, ttr;
+    __device__ __forceinline__ void setup(const double* p) {
+        a = ::exp(p[0]); t0 = p[1]; n = p[2]; ttr = ::exp(p[3]);
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double ph = t - t0;
+        double sig5 = 1.0 / (1.0 + ::exp(-ph * 5.0));
+        double ps = ::log(1.0 + ::exp(ph)) + 1e-6;
+        double ratio = ps / ttr;
+        f = a * sig5 * ::pow(ps, -n) * ::exp(-ratio * ratio);
+        return __builtin_isfinite(f);
+    }
+};
+
+template <> struct Model<6, double> {
+    double a, t0, tb, a1, a2;
+    __device__ __forceinline__ void setup(const double* p) {
+        a = ::exp(p[0]); t0 = p[1]; tb = ::exp(p[2]); a1 = p[3]; a2 = p[4];
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double ps = ::log(1.0 + ::exp(t - t0)) + 1e-6;
+        double ln_r = ::log(ps / tb);
+        double u1 = ::exp(2.0 * a1 * ln_r), u2 = ::exp(2.0 * a2 * ln_r);
+        f = a * ::pow(u1 + u2, -0.5);
+        return __builtin_isfinite(f);
     }
 };
 
@@ -272,7 +389,7 @@ __device__ inline double metzger_kn_single(const double* p, double obs_time) {
 }
 
 // MetzgerKN: [log10 M_ej, log10 v_ej, log10 kappa, t0, ln sigma_x]
-template <typename R> struct Model<7, R> {
+template <typename R> struct ModelStable<7, R> {
     double p[4];
     __device__ __forceinline__ void setup(const R* q) {
         for (int i = 0; i < 4; i++) p[i] = (double)q[i];

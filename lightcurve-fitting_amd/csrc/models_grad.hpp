@@ -42,8 +42,8 @@ __device__ __forceinline__ void softplus_d(R ph, R& ps, R& sig_p) {
 
 template <int MODEL, typename R> struct ModelGrad;
 
-template <typename R> struct ModelGrad<0, R> : Model<0, R> {
-    using B = Model<0, R>;
+template <typename R> struct ModelGrad<0, R> : ModelStable<0, R> {
+    using B = ModelStable<0, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         R dt = t - B::t0;
         R x1 = -dt * B::itf, x2 = -dt * B::itr;
@@ -63,8 +63,8 @@ template <typename R> struct ModelGrad<0, R> : Model<0, R> {
     }
 };
 
-template <typename R> struct ModelGrad<1, R> : Model<1, R> {
-    using B = Model<1, R>;
+template <typename R> struct ModelGrad<1, R> : ModelStable<1, R> {
+    using B = ModelStable<1, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         R ph = t - B::t0;
         R y = (B::gamma - ph) * B::itf;
@@ -90,8 +90,8 @@ template <typename R> struct ModelGrad<1, R> : Model<1, R> {
     }
 };
 
-template <typename R> struct ModelGrad<2, R> : Model<2, R> {
-    using B = Model<2, R>;
+template <typename R> struct ModelGrad<2, R> : ModelStable<2, R> {
+    using B = ModelStable<2, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         using M = Math<R>;
         R ph = t - B::t0;
@@ -115,8 +115,8 @@ template <typename R> struct ModelGrad<2, R> : Model<2, R> {
     }
 };
 
-template <typename R> struct ModelGrad<3, R> : Model<3, R> {
-    using B = Model<3, R>;
+template <typename R> struct ModelGrad<3, R> : ModelStable<3, R> {
+    using B = ModelStable<3, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         using M = Math<R>;
         R ph = t - B::t0;
@@ -140,8 +140,8 @@ template <typename R> struct ModelGrad<3, R> : Model<3, R> {
     }
 };
 
-template <typename R> struct ModelGrad<4, R> : Model<4, R> {
-    using B = Model<4, R>;
+template <typename R> struct ModelGrad<4, R> : ModelStable<4, R> {
+    using B = ModelStable<4, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         using M = Math<R>;
         R ph = t - B::t0;
@@ -163,8 +163,8 @@ template <typename R> struct ModelGrad<4, R> : Model<4, R> {
     }
 };
 
-template <typename R> struct ModelGrad<5, R> : Model<5, R> {
-    using B = Model<5, R>;
+template <typename R> struct ModelGrad<5, R> : ModelStable<5, R> {
+    using B = ModelStable<5, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         using M = Math<R>;
         R ph = t - B::t0;
@@ -185,8 +185,8 @@ template <typename R> struct ModelGrad<5, R> : Model<5, R> {
     }
 };
 
-template <typename R> struct ModelGrad<6, R> : Model<6, R> {
-    using B = Model<6, R>;
+template <typename R> struct ModelGrad<6, R> : ModelStable<6, R> {
+    using B = ModelStable<6, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         using M = Math<R>;
         R ph = t - B::t0;
@@ -216,10 +216,155 @@ template <typename R> struct ModelGrad<6, R> : Model<6, R> {
     }
 };
 
+// ---------------------------------------------------------------------------
+// FP64: the reference's value and gradient expressions as written
+// (src/parametric.rs:213-510, identical to cuda/models_device.h:152-347), on
+// top of the reference-form Model<M, double>.  Entries the reference computes
+// as inf/NaN come out inf/NaN here too, and the likelihood adjoint skips them
+// one by one exactly as the reference does (cuda/svi.cu:209-227).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double ref_softplus(double x) { return ::log(1.0 + ::exp(x)) + 1e-6; }
+__device__ __forceinline__ double ref_softplus_grad(double x) { return ::exp(x) / (1.0 + ::exp(x)); }
+
+template <> struct ModelGrad<0, double> : Model<0, double> {
+    __device__ __forceinline__ bool pred_grad(double t, double& f, double* g) const {
+        if (!pred(t, f)) return false;
+        double dt = t - t0;
+        double ef = ::exp(-dt / tf);
+        double sig = 1.0 / (1.0 + ::exp(-dt / tr));
+        double base = a * ef * sig;
+        g[0] = base; g[1] = 1.0;
+        g[2] = base * (1.0 / tf - (1.0 - sig) / tr);
+        g[3] = -base * (1.0 - sig) * dt / tr;
+        g[4] = base * dt / tf;
+        g[5] = 0.0;
+        return true;
+    }
+};
+
+template <> struct ModelGrad<1, double> : Model<1, double> {
+    __device__ __forceinline__ bool pred_grad(double t, double& f, double* g) const {
+        if (!pred(t, f)) return false;
+        const double k = 10.0;
+        double ph = t - t0;
+        double sr = 1.0 / (1.0 + ::exp(-ph / tr));
+        double w = 1.0 / (1.0 + ::exp(-k * (ph - gamma)));
+        double pl = 1.0 - beta * ph;
+        double ed = ::exp((gamma - ph) / tf);
+        double pr = (1.0 - beta * gamma) * ed;
+        double piece = (1.0 - w) * pl + w * pr;
+        g[0] = a * sr * piece;
+        g[1] = a * sr * ((1.0 - w) * (-ph) + w * (-gamma * ed));
+        double dw_dlg = -k * w * (1.0 - w) * gamma;
+        double dpr_dlg = ed * (-beta + (1.0 - beta * gamma) / tf) * gamma;
+        g[2] = a * sr * (dw_dlg * (pr - pl) + w * dpr_dlg);
+        double dsig_dt0 = -sr * (1.0 - sr) / tr;
+        double dw_dt0 = -k * w * (1.0 - w);
+        double dpiece_dt0 = dw_dt0 * (pr - pl) + (1.0 - w) * beta + w * (1.0 - beta * gamma) * ed / tf;
+        g[3] = a * (dsig_dt0 * piece + sr * dpiece_dt0);
+        g[4] = a * piece * sr * (1.0 - sr) * (-ph / tr);
+        g[5] = a * sr * w * (pr * (ph - gamma) / tf);
+        g[6] = 0.0;
+        return true;
+    }
+};
+
+template <> struct ModelGrad<2, double> : Model<2, double> {
+    __device__ __forceinline__ bool pred_grad(double t, double& f, double* g) const {
+        if (!pred(t, f)) return false;
+        double ph = t - t0;
+        double sig = 1.0 / (1.0 + ::exp(-ph / tr));
+        double ps = ref_softplus(ph), sp = ref_softplus_grad(ph);
+        double w = 1.0 + ps / tf;
+        double decay = ::pow(w, -alpha);
+        g[0] = a * sig * decay; g[1] = 1.0;
+        g[2] = a * ((-sig * (1.0 - sig) / tr) * decay + sig * (alpha * ::pow(w, -alpha - 1.0) * sp / tf));
+        g[3] = a * decay * (-sig * (1.0 - sig) * ph / tr);
+        g[4] = a * sig * alpha * ::pow(w, -alpha - 1.0) * ps / tf;
+        g[5] = a * sig * (-::log(w)) * decay;
+        g[6] = 0.0;
+        return true;
+    }
+};
+
+template <> struct ModelGrad<3, double> : Model<3, double> {
+    __device__ __forceinline__ bool pred_grad(double t, double& f, double* g) const {
+        if (!pred(t, f)) return false;
+        double ph = t - t0;
+        double ps = ref_softplus(ph), sp = ref_softplus_grad(ph);
+        double e_ni = ::exp(-ps / 8.8), e_co = ::exp(-ps / 111.3);
+        double heat = fr * e_ni + (1.0 - fr) * e_co;
+        double x = ps / tm, ex2 = ::exp(-x * x), trap = 1.0 - ex2;
+        g[0] = a * heat * trap;
+        double dh = -fr * e_ni / 8.8 - (1.0 - fr) * e_co / 111.3;
+        double dtr = 2.0 * ps * ex2 / (tm * tm);
+        g[1] = a * (-sp) * (dh * trap + heat * dtr);
+        g[2] = -2.0 * a * heat * ex2 * x * x;
+        g[3] = a * trap * (e_ni - e_co) * fr * (1.0 - fr);
+        g[4] = 0.0;
+        return true;
+    }
+};
+
+template <> struct ModelGrad<4, double> : Model<4, double> {
+    __device__ __forceinline__ bool pred_grad(double t, double& f, double* g) const {
+        if (!pred(t, f)) return false;
+        double ph = t - t0;
+        double ps = ref_softplus(ph), sp = ref_softplus_grad(ph);
+        double w = 1.0 + ps / tsd;
+        double sd = 1.0 / (w * w), w3 = w * w * w;
+        double x = ps / tdf, ex2 = ::exp(-x * x), trap = 1.0 - ex2;
+        g[0] = a * sd * trap;
+        double dsd = -2.0 / (w3 * tsd);
+        double dtr = 2.0 * ps * ex2 / (tdf * tdf);
+        g[1] = a * (-sp) * (dsd * trap + sd * dtr);
+        g[2] = a * trap * 2.0 * ps / (w3 * tsd);
+        g[3] = -2.0 * a * sd * ex2 * x * x;
+        g[4] = 0.0;
+        return true;
+    }
+};
+
+template <> struct ModelGrad<5, double> : Model<5, double> {
+    __device__ __forceinline__ bool pred_grad(double t, double& f, double* g) const {
+        if (!pred(t, f)) return false;
+        double ph = t - t0;
+        double s5 = 1.0 / (1.0 + ::exp(-ph * 5.0));
+        double ps = ref_softplus(ph), sp = ref_softplus_grad(ph);
+        double cool = ::pow(ps, -n), ratio = ps / ttr, cut = ::exp(-ratio * ratio);
+        double base = cool * cut, flux = a * s5 * base;
+        g[0] = flux;
+        g[1] = a * base * (-5.0 * s5 * (1.0 - s5) + s5 * sp * (n / ps + 2.0 * ps / (ttr * ttr)));
+        g[2] = -flux * ::log(ps);
+        g[3] = flux * 2.0 * ratio * ratio;
+        g[4] = 0.0;
+        return true;
+    }
+};
+
+template <> struct ModelGrad<6, double> : Model<6, double> {
+    __device__ __forceinline__ bool pred_grad(double t, double& f, double* g) const {
+        if (!pred(t, f)) return false;
+        double ph = t - t0;
+        double ps = ref_softplus(ph), sp = ref_softplus_grad(ph);
+        double lr = ::log(ps / tb);
+        double u1 = ::exp(2.0 * a1 * lr), u2 = ::exp(2.0 * a2 * lr), u = u1 + u2;
+        double u15 = ::pow(u, -1.5);
+        g[0] = a * ::pow(u, -0.5);
+        double dps = (2.0 * a1 * u1 + 2.0 * a2 * u2) / ps;
+        g[1] = a * (-0.5) * u15 * dps * (-sp);
+        g[2] = a * (-0.5) * u15 * (-(2.0 * a1 * u1 + 2.0 * a2 * u2));
+        g[3] = a * (-0.5) * u15 * 2.0 * lr * u1;
+        g[4] = a * (-0.5) * u15 * 2.0 * lr * u2;
+        g[5] = 0.0;
+        return true;
+    }
+};
+
 // MetzgerKN: forward differences of the per-observation ODE in FP64 with the
 // reference's step h = max(1e-5, 1e-5 |theta_j|) (cuda/svi.cu:180-192).
-template <typename R> struct ModelGrad<7, R> : Model<7, R> {
-    using B = Model<7, R>;
+template <typename R> struct ModelGrad<7, R> : ModelStable<7, R> {
+    using B = ModelStable<7, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {
         double p[4] = {B::p[0], B::p[1], B::p[2], B::p[3]};
         double f0 = metzger_kn_single(p, (double)t);

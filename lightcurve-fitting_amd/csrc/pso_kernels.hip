@@ -1,303 +1,15 @@
-// pso_kernels.hip -- batched particle-swarm model fitting for gfx950.
-//
-// One wavefront owns one band ("BatchSource" of the reference) at a time and
-// runs the swarms of every requested model on it, one after another:
-//   * particles live in lanes (particle p = lane + 64*k, k < PPL);
-//   * the band's photometry is staged once into the wave's LDS slice as packed
-//     {t, y, var, flag} records and read back as wave-uniform broadcasts;
-//   * the per-particle NLL is a lane-private Kahan sum over observations, so
-//     no cross-lane traffic is needed inside the observation loop;
-//   * the swarm state (position, velocity, personal best) stays in registers
-//     for all iterations; the global best is a wave argmin per iteration.
-// Semantics follow GpuContext::batch_pso (src/gpu_cuda.rs:368-567) with the
-// cost of batch_pso_cost (cuda/models.cu:516-584) plus the host-side prior
-// term (src/gpu_cuda.rs:461-480), and the model-selection loop of
-// batch_model_select / batch_all_models (src/gpu_cuda.rs:573-637).  Random
-// numbers come from Philox4x32-10 keyed by (seed, band index) instead of the
-// reference's sequential SmallRng, which makes every band's result
-// independent of batch composition and launch order.
+// pso_kernels.hip -- launch side of the batched PSO kernels (the device code
+// lives in pso_swarm.hpp, instantiated by pso_f32.hip / pso_f64.hip) and the
+// per-band model selection kernel.
 #include "pso_kernels.hpp"
-#include "models_fit.hpp"
 
 namespace lcf {
 
-// Box bounds (GpuModelName::pso_bounds, src/gpu_types.rs:64-103) and
-// population priors (src/parametric.rs:1588-1621; the Villar entries are the
-// log10 table values times ln 10), indexed [model][param].
-__constant__ double c_lo[8][7] = {
-    {-3.0, -1.0, -100.0, -2.0, -2.0, -5.0, 0},
-    {-1.5, -0.02, 1.0, -60.0, -1.0, 1.0, -6.0},
-    {-3.0, -1.0, -100.0, -2.0, -1.0, 0.5, -5.0},
-    {-3.0, -100.0, 0.5, -3.0, -5.0, 0, 0},
-    {-3.0, -100.0, 0.0, 0.5, -5.0, 0, 0},
-    {-3.0, -100.0, 0.1, -1.0, -5.0, 0, 0},
-    {-3.0, -100.0, -2.0, -2.0, 0.5, -5.0, 0},
-    {-3.0, -2.0, -1.0, -2.0, -5.0, 0, 0}};
-__constant__ double c_hi[8][7] = {
-    {3.0, 1.0, 100.0, 5.0, 6.0, 0.0, 0},
-    {1.5, 0.05, 5.0, 40.0, 3.0, 5.5, 0.0},
-    {3.0, 1.0, 100.0, 5.0, 6.0, 4.0, 0.0},
-    {3.0, 100.0, 4.5, 3.0, 0.0, 0, 0},
-    {3.0, 100.0, 6.0, 4.5, 0.0, 0, 0},
-    {3.0, 100.0, 3.0, 4.0, 0.0, 0, 0},
-    {3.0, 100.0, 6.0, 3.0, 5.0, 0.0, 0},
-    {-0.5, -0.5, 2.0, 1.0, 0.0, 0, 0}};
-#define LN10 2.30258509299404568402
-__constant__ double c_pc[8][7] = {
-    {0.0, 0.0, 0.0, 1.5, 3.0, -3.0, 0},
-    {-0.009 * LN10, 0.009, 1.327 * LN10, -12.0, 0.360 * LN10, 1.454 * LN10, -1.033 * LN10},
-    {0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0},
-    {0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0}};
-__constant__ double c_pw[8][7] = {
-    {2.0, 0.2, 30.0, 2.5, 2.5, 2.0, 0},
-    {0.10 * LN10, 0.010, 0.40 * LN10, 15.0, 0.35 * LN10, 0.40 * LN10, 0.50 * LN10},
-    {5, 5, 5, 5, 5, 5, 5}, {5, 5, 5, 5, 5, 5, 5}, {5, 5, 5, 5, 5, 5, 5},
-    {5, 5, 5, 5, 5, 5, 5}, {5, 5, 5, 5, 5, 5, 5}, {5, 5, 5, 5, 5, 5, 5}};
-#undef LN10
+const void* pso_kernel_f32(int model, int ppl, bool persistent);
+const void* pso_kernel_f64(int model, int ppl, bool persistent);
 
-template <typename R> struct ObsT;
-template <> struct ObsT<float> { using T = float4; };
-template <> struct ObsT<double> { using T = double4; };
-
-// Copy `cnt` packed observations into the wave's LDS slice (coalesced: lane i
-// moves records i, i+64, ...).  Single-wave workgroups: __syncthreads() is a
-// wave-local ordering point.
-template <typename R>
-__device__ __forceinline__ void stage_obs(typename ObsT<R>::T* lds,
-                                          const typename ObsT<R>::T* __restrict__ g, int cnt) {
-    __syncthreads();
-    for (int i = lane_id(); i < cnt; i += WAVE) lds[i] = g[i];
-    __syncthreads();
-}
-
-template <int MODEL, typename R, int PPL>
-struct Swarm {
-    static constexpr int NP = ModelTraits<MODEL>::NP;
-    using M = Math<R>;
-    using O = typename ObsT<R>::T;
-
-    R pos[PPL][NP], vel[PPL][NP], pb[PPL][NP], pcost[PPL];
-    R gb[NP], gcost;
-
-    // NLL/n + prior/n^2 for this lane's particles (cuda/models.cu:540-583 and
-    // src/gpu_cuda.rs:461-480).
-    __device__ __forceinline__ void costs(R (&cost)[PPL], const O* lds, const O* __restrict__ g,
-                                          int n, int tile, int P) {
-        Model<MODEL, R> mdl[PPL];
-        R se2[PPL];
-        Kahan<R> acc[PPL];
-        bool ok[PPL];
-#pragma unroll
-        for (int k = 0; k < PPL; k++) {
-            mdl[k].setup(pos[k]);
-            se2[k] = M::exp(R(2) * pos[k][NP - 1]);
-            acc[k].init();
-            ok[k] = n > 0;
-        }
-        for (int base = 0; base < n; base += tile) {
-            int cnt = n - base < tile ? n - base : tile;
-            if (n > tile) stage_obs<R>(const_cast<O*>(lds), g + base, cnt);
-            for (int i = 0; i < cnt; i++) {
-                O o = lds[i];  // wave-uniform address: LDS broadcast
-#pragma unroll
-                for (int k = 0; k < PPL; k++) {
-                    R f;
-                    bool good = mdl[k].pred(o.x, f) && M::finite(f);
-                    ok[k] = ok[k] && good;
-                    R tv = o.z + se2[k];
-                    R term;
-                    if (o.w != R(0)) {
-                        term = -log_normal_cdf<R>((o.y - f) * M::rsqrt(tv));
-                    } else {
-                        R d = f - o.y;
-                        term = d * d * M::rcp(tv) + M::log(tv);
-                    }
-                    acc[k].add(term);
-                }
-            }
-        }
-        R nn = R(n > 0 ? n : 1);
-        R inv_n = R(1) / nn;
-#pragma unroll
-        for (int k = 0; k < PPL; k++) {
-            R pri = R(0);
-#pragma unroll
-            for (int j = 0; j < NP; j++) {
-                R w = (R)c_pw[MODEL][j];
-                if (w > R(0)) {
-                    R z = (pos[k][j] - (R)c_pc[MODEL][j]) / w;
-                    pri += R(0.5) * z * z;
-                }
-            }
-            R c = ok[k] ? acc[k].s * inv_n : M::big();
-            c = c + pri * inv_n * inv_n;
-            if (!(c < M::big())) c = M::big();
-            if (!ok[k]) c = M::big();
-            cost[k] = c;
-        }
-    }
-
-    __device__ __forceinline__ void run(const O* lds, const O* __restrict__ g, int n, int tile,
-                                        uint32_t band, int P, int max_iters, int stall_iters,
-                                        uint64_t seed, uint32_t flags, int& iters_out) {
-        const int lane = lane_id();
-        const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-        constexpr int NPAIR = (NP + 1) / 2;
-        R lo[NP], hi[NP], vmax[NP];
-#pragma unroll
-        for (int d = 0; d < NP; d++) {
-            lo[d] = (R)c_lo[MODEL][d];
-            hi[d] = (R)c_hi[MODEL][d];
-            vmax[d] = R(0.5) * (hi[d] - lo[d]);
-        }
-        // initial swarm: Philox block (band, particle, 0xFFFFFFFF, pair)
-#pragma unroll
-        for (int k = 0; k < PPL; k++) {
-            uint32_t p = (uint32_t)(lane + WAVE * k);
-#pragma unroll
-            for (int q = 0; q < NPAIR; q++) {
-                U4 r = philox4x32_10(band, p, 0xFFFFFFFFu, (uint32_t)q, k0, k1);
-                int d0 = 2 * q, d1 = 2 * q + 1;
-                pos[k][d0] = lo[d0] + u24<R>(r.x) * (hi[d0] - lo[d0]);
-                vel[k][d0] = vmax[d0] * R(0.2) * (R(2) * u24<R>(r.y) - R(1));
-                if (d1 < NP) {
-                    pos[k][d1] = lo[d1] + u24<R>(r.z) * (hi[d1] - lo[d1]);
-                    vel[k][d1] = vmax[d1] * R(0.2) * (R(2) * u24<R>(r.w) - R(1));
-                }
-            }
-#pragma unroll
-            for (int d = 0; d < NP; d++) pb[k][d] = R(0);
-            pcost[k] = R(INFINITY);
-        }
-#pragma unroll
-        for (int d = 0; d < NP; d++) gb[d] = R(0);
-        gcost = R(INFINITY);
-
-        const R w_max = R(0.9), w_min = R(0.4), c1 = R(1.5), c2 = R(1.5);
-        const R inv_max = R(1) / R(max_iters);
-        R prev = R(INFINITY);
-        int stall = 0;
-        int it = 0;
-        for (; it < max_iters; it++) {
-            R w = w_max - (w_max - w_min) * R(it) * inv_max;
-            R cost[PPL];
-            costs(cost, lds, g, n, tile, P);
-            // personal bests; lowest cost this iteration (ties: lowest index)
-            R bc = R(INFINITY);
-            int bi = 0x7fffffff;
-#pragma unroll
-            for (int k = 0; k < PPL; k++) {
-                int p = lane + WAVE * k;
-                if (p < P) {
-                    if (cost[k] < pcost[k]) {
-                        pcost[k] = cost[k];
-#pragma unroll
-                        for (int d = 0; d < NP; d++) pb[k][d] = pos[k][d];
-                    }
-                    if (cost[k] < bc || (cost[k] == bc && p < bi)) { bc = cost[k]; bi = p; }
-                }
-            }
-            int win = wave_argmin<R>(bc, bi);
-            R wcost = bcast(bc, win & (WAVE - 1));
-            if (wcost < gcost) {
-                gcost = wcost;
-                int wl = win & (WAVE - 1), wk = win / WAVE;
-#pragma unroll
-                for (int k = 0; k < PPL; k++)
-                    if (k == wk) {
-#pragma unroll
-                        for (int d = 0; d < NP; d++) gb[d] = bcast(pos[k][d], wl);
-                    }
-            }
-            if ((flags & 1u) && it == 0) prev = gcost;
-            // velocity / position update with clamping and wall absorption
-#pragma unroll
-            for (int k = 0; k < PPL; k++) {
-                uint32_t p = (uint32_t)(lane + WAVE * k);
-#pragma unroll
-                for (int q = 0; q < NPAIR; q++) {
-                    U4 r = philox4x32_10(band, p, (uint32_t)it, (uint32_t)q, k0, k1);
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        int d = 2 * q + h;
-                        if (d < NP) {
-                            R r1 = u24<R>(h ? r.z : r.x), r2 = u24<R>(h ? r.w : r.y);
-                            R x = pos[k][d];
-                            R v = w * vel[k][d] + c1 * r1 * (pb[k][d] - x) + c2 * r2 * (gb[d] - x);
-                            v = v < -vmax[d] ? -vmax[d] : (v > vmax[d] ? vmax[d] : v);
-                            R nx = x + v;
-                            if (nx <= lo[d]) { pos[k][d] = lo[d]; vel[k][d] = R(0); }
-                            else if (nx >= hi[d]) { pos[k][d] = hi[d]; vel[k][d] = R(0); }
-                            else { pos[k][d] = nx; vel[k][d] = v; }
-                        }
-                    }
-                }
-            }
-            // stall rule (src/gpu_cuda.rs:537-552); prev starts at +inf as
-            // in the reference unless the caller opts into flag bit 0.
-            R ap = prev < R(0) ? -prev : prev;
-            bool improved = prev - gcost > R(0.01) * (ap > R(1e-10) ? ap : R(1e-10));
-            if (improved) { stall = 0; prev = gcost; }
-            else if (++stall >= stall_iters) { it++; break; }
-        }
-        iters_out = it;
-    }
-};
-
-// Fits model MODEL on one band and writes its slot of the per-model outputs.
-// With a gate, the band is skipped unless the gate model's cost >= gate
-// (batch_model_select, src/gpu_cuda.rs:584-619).
-template <int MODEL, typename R, int PPL>
-__device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
-    using O = typename ObsT<R>::T;
-    const int64_t slot = (int64_t)a.slot * a.n_bands + band;
-    const int lane = lane_id();
-    if (a.gate_cost && !(a.gate_cost[band] >= a.gate)) {
-        if (lane < 7) a.model_params[slot * 7 + lane] = 0.0;
-        if (lane == 0) { a.model_cost[slot] = INFINITY; a.model_iters[slot] = 0; }
-        return;
-    }
-    const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[band];
-    const int n = (int)(a.offsets[band + 1] - a.offsets[band]);
-    if (n <= a.tile) stage_obs<R>(lds, g, n);
-    Swarm<MODEL, R, PPL> s;
-    int iters = 0;
-    s.run(lds, g, n, a.tile, (uint32_t)band, a.n_particles, a.max_iters, a.stall_iters, a.seed,
-          a.flags, iters);
-    double cst = (double)s.gcost;
-    if (cst >= (double)Math<R>::big()) cst = 1e99;
-    double prm = 0.0;  // lane d holds parameter d
-#pragma unroll
-    for (int d = 0; d < Swarm<MODEL, R, PPL>::NP; d++)
-        if (lane == d) prm = (double)s.gb[d];
-    if (lane < 7) a.model_params[slot * 7 + lane] = prm;
-    if (lane == 0) {
-        a.model_cost[slot] = cst;
-        a.model_iters[slot] = iters;
-    }
-}
-
-// Persistent form: each single-wave workgroup pulls band indices from a
-// device queue ordered by decreasing band size (longest-processing-time
-// first) until the queue is drained.  One-shot form (PERSISTENT = false):
-// workgroup b fits band order[task_base + b]; used by the per-size-class
-// schedule.
-template <int MODEL, typename R, int PPL, bool PERSISTENT>
-__global__ __launch_bounds__(64) void pso_model_kernel(PsoArgs a) {
-    extern __shared__ unsigned char lds_raw[];
-    using O = typename ObsT<R>::T;
-    O* lds = reinterpret_cast<O*>(lds_raw);
-    if (PERSISTENT) {
-        for (;;) {
-            const int t = wave_next_task(a.queue);
-            if (t >= a.n_tasks) break;
-            fit_band<MODEL, R, PPL>(a, lds, a.order[t]);
-            wave_reconverge();
-        }
-    } else {
-        int t = a.task_base + (int)blockIdx.x;
-        if (t < a.n_tasks) fit_band<MODEL, R, PPL>(a, lds, a.order[t]);
-    }
+static const void* pick_kernel(int model, bool fp64, int ppl, bool persistent) {
+    return fp64 ? pso_kernel_f64(model, ppl, persistent) : pso_kernel_f32(model, ppl, persistent);
 }
 
 // Best model per band over the fitted slots: strict '<' in slot order
@@ -317,40 +29,6 @@ __global__ void select_best(const double* __restrict__ mp, const double* __restr
     if (best_cost) best_cost[b] = bc;
     if (best_params)
         for (int d = 0; d < 7; d++) best_params[(int64_t)b * 7 + d] = mp[((int64_t)bs * n_bands + b) * 7 + d];
-}
-
-template <int MODEL, typename R, int PPL>
-static const void* kernel_ptr(bool persistent) {
-    return persistent ? (const void*)pso_model_kernel<MODEL, R, PPL, true>
-                      : (const void*)pso_model_kernel<MODEL, R, PPL, false>;
-}
-
-template <typename R, int PPL>
-static const void* kernel_ptr_m(int model, bool persistent) {
-    switch (model) {
-        case 0: return kernel_ptr<0, R, PPL>(persistent);
-        case 1: return kernel_ptr<1, R, PPL>(persistent);
-        case 2: return kernel_ptr<2, R, PPL>(persistent);
-        case 3: return kernel_ptr<3, R, PPL>(persistent);
-        case 4: return kernel_ptr<4, R, PPL>(persistent);
-        case 5: return kernel_ptr<5, R, PPL>(persistent);
-        case 6: return kernel_ptr<6, R, PPL>(persistent);
-        case 7: return kernel_ptr<7, R, PPL>(persistent);
-        default: return nullptr;
-    }
-}
-
-static const void* pick_kernel(int model, bool fp64, int ppl, bool persistent) {
-    if (fp64) {
-        if (ppl == 1) return kernel_ptr_m<double, 1>(model, persistent);
-        if (ppl == 2) return kernel_ptr_m<double, 2>(model, persistent);
-        if (ppl == 4) return kernel_ptr_m<double, 4>(model, persistent);
-    } else {
-        if (ppl == 1) return kernel_ptr_m<float, 1>(model, persistent);
-        if (ppl == 2) return kernel_ptr_m<float, 2>(model, persistent);
-        if (ppl == 4) return kernel_ptr_m<float, 4>(model, persistent);
-    }
-    return nullptr;
 }
 
 size_t pso_lds_bytes(bool fp64, int tile) {

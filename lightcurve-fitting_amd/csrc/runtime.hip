@@ -146,6 +146,14 @@ int lcf_ctx_set_schedule(lcf_ctx* c, int s) {
 
 int lcf_ctx_num_cus(const lcf_ctx* c) { return c ? c->n_cu : 0; }
 
+int lcf_ctx_synchronize(lcf_ctx* c) {
+    if (!c) return LCF_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipDeviceSynchronize());
+    return LCF_OK;
+}
+
 double lcf_last_kernel_ms(const lcf_ctx* c) { return c ? c->last_ms : 0.0; }
 
 double lcf_last_model_ms(const lcf_ctx* c, int slot) {

@@ -1,0 +1,419 @@
+// svi_job.hpp -- batched stochastic variational inference for gfx950: the
+// device code, instantiated per precision by svi_f32.hip and svi_f64.hip.
+//
+// One wavefront owns one (band, model) job for its whole optimisation: the
+// variational state (mu, log sigma, Adam moments) stays in registers across
+// all steps, so the step loop costs no launches and no HBM traffic.  The 64
+// lanes are split into S' = pow2 >= n_samples sample slots x G = 64/S'
+// observation groups: lane (s, g) draws the reparameterised sample
+// theta_s = mu + sigma * eps_s and walks observations g, g+G, ... of the
+// band's LDS copy, accumulating the log-likelihood (Kahan) and its adjoint
+// dLL/dtheta.  The adjoint is formed per observation from dLL/df and the
+// model's parameter gradient, recomputed with the forward value (nothing per
+// observation is stored).  One wave reduction per step combines
+// sum LL, sum dF/dtheta and sum dF/dtheta * sigma * eps, which are exactly the
+// reparameterisation gradients w.r.t. mu and log sigma.
+//
+// Semantics follow the reference's GPU SVI (cuda/svi.cu:48-330): PSO point as
+// the initial mean, log sigma = -1, Gaussian prior with the given centres and
+// widths, entropy sum(log sigma) + np/2 ln(2 pi e), Adam (0.9, 0.999, 1e-8)
+// skipping non-finite components, log sigma clamped to [-6, 2], EMA early stop
+// after 50 steps with a 20-step stall.  The noise is Philox4x32-10 keyed by the
+// seed with counter (job, step, sample, pair) instead of the reference's
+// per-job xorshift64, so every (job, step, sample) draw is independent of how
+// lanes and jobs are scheduled.
+#pragma once
+#include "svi_kernels.hpp"
+#include "models_grad.hpp"
+
+namespace lcf {
+
+template <typename R> struct ObsR;
+template <> struct ObsR<float> { using T = float4; };
+template <> struct ObsR<double> { using T = double4; };
+
+// phi(z) / max(Phi(z), 1e-300) with the A&S 7.1.26 erfc.  FP64 evaluates the
+// reference's expression as written (cuda/svi.cu:200-204); FP32 reproduces
+// the value that expression takes in binary64, including its rounding of
+// 1 - (1 - tiny) and the 1e-300 floor far in the tail.
+template <typename R> __device__ __forceinline__ R mills_ratio(R z);
+template <> __device__ __forceinline__ double mills_ratio<double>(double z) {
+    double a = fabs(z * 0.70710678118654752440);
+    double t = 1.0 / (1.0 + 0.3275911 * a);
+    double poly = t * (0.254829592 + t * (-0.284496736 + t * (1.421413741 + t * (-1.453152027 + t * 1.061405429))));
+    double val = 1.0 - poly * ::exp(-a * a);
+    double erfv = z >= 0.0 ? val : -val;
+    double phi = ::exp(-0.5 * z * z) * 0.39894228040143267794;
+    double cdf = fmax(0.5 * (1.0 + erfv), 1e-300);
+    return phi / cdf;
+}
+template <> __device__ __forceinline__ float mills_ratio<float>(float z) {
+    using M = Math<float>;
+    float a = fabsf(z) * 0.707106781f;
+    float t = M::rcp(1.0f + 0.3275911f * a);
+    float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    if (z >= 0.0f) {
+        float e = M::exp(-0.5f * z * z);
+        return 0.398942280f * e * M::rcp(1.0f - 0.5f * poly * e);
+    }
+    // z < 0: the reference's binary64 cdf is 0.5 (1 - fl(1 - x)), x = poly e^{-a^2},
+    // floored at 1e-300.  For x >= 2^-20 that is 0.5 x to FP32 accuracy; below,
+    // 1 - x rounds to a multiple k of 2^-53, so cdf = 0.5 k 2^-53 (k = 0: the
+    // floor).  The ratio is formed in the log domain; it is 0 where the
+    // binary64 phi(z) underflows (z^2 > 1490.27) and +inf where the
+    // reference's value exceeds the FP32 range (then the step is skipped).
+    float ln_x = M::log(poly) - a * a;
+    if (ln_x > -13.8629436f) return 0.797884561f * M::rcp(poly);  // (1/sqrt(2 pi)) / (poly / 2)
+    if (z * z > 1490.26644f) return 0.0f;
+    float k = rintf(M::exp(ln_x + 36.7368005f));                   // x 2^53
+    float ln_phi = -0.5f * z * z - 0.918938533f;
+    float ln_cdf = k >= 1.0f ? M::log(0.5f * k) - 36.7368005f : -690.775528f;
+    return M::exp(ln_phi - ln_cdf);
+}
+
+// phi(z) and max(Phi(z), 1e-300) as the reference forms them for the
+// upper-limit adjoint (cuda/svi.cu:200-203, erf_approx of src/parametric.rs:867-880).
+template <typename R>
+__device__ __forceinline__ void phi_cdf(R z, R& phi, R& cdf) {
+    using M = Math<R>;
+    phi = M::exp(R(-0.5) * z * z) / M::sqrt(R(2.0 * 3.14159265358979323846));
+    R x = z * R(0.70710678118654752440);
+    R ax = x < R(0) ? -x : x;
+    R t = R(1) / (R(1) + R(0.3275911) * ax);
+    R poly = t * (R(0.254829592) + t * (R(-0.284496736) + t * (R(1.421413741) + t * (R(-1.453152027) + t * R(1.061405429)))));
+    R val = R(1) - poly * M::exp(-ax * ax);
+    R erfv = x >= R(0) ? val : -val;
+    R c = R(0.5) * (R(1) + erfv);
+    cdf = c > R(1e-300) ? c : R(1e-300);
+}
+
+// 2 pi u in revolutions: v_cos_f32 / v_sin_f32 take their argument in units of 2 pi.
+__device__ __forceinline__ void sincos_rev(float u, float& s, float& c) {
+    s = __builtin_amdgcn_sinf(u);
+    c = __builtin_amdgcn_cosf(u);
+}
+__device__ __forceinline__ void sincos_rev(double u, double& s, double& c) {
+    ::sincos(6.28318530717958647692 * u, &s, &c);
+}
+
+// Philox normals of (job, step, sample): eps[2q], eps[2q+1] from pair q.
+template <typename R, int NP>
+__device__ __forceinline__ void draw_normals(uint64_t seed, uint32_t job, uint32_t step, uint32_t sample,
+                                             R (&eps)[NP]) {
+#pragma unroll
+    for (int q = 0; q < (NP + 1) / 2; q++) {
+        U4 r = philox4x32_10(job, step, sample, (uint32_t)q, (uint32_t)seed, (uint32_t)(seed >> 32));
+        R u1 = (R((r.x >> 9)) + R(0.5)) * R(1.0 / 8388608.0);
+        R u2 = R(r.y >> 8) * R(1.0 / 16777216.0);
+        R rad = Math<R>::sqrt(R(-2) * Math<R>::log(u1));
+        R s, c;
+        sincos_rev(u2, s, c);
+        eps[2 * q] = rad * c;
+        if (2 * q + 1 < NP) eps[2 * q + 1] = rad * s;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void stage(typename ObsR<R>::T* lds, const typename ObsR<R>::T* __restrict__ g, int cnt) {
+    __syncthreads();
+    for (int i = lane_id(); i < cnt; i += WAVE) lds[i] = g[i];
+    __syncthreads();
+}
+
+template <int MODEL, typename R, bool FD>
+struct SviJob {
+    static constexpr int NP = ModelTraits<MODEL>::NP;
+    static constexpr int SE = NP - 1;
+    using M = Math<R>;
+    using O = typename ObsR<R>::T;
+
+    // Model value and gradient: analytic adjoint or forward differences with
+    // the reference's step h = max(1e-5, 1e-5 |theta_j|).
+    struct Eval {
+        ModelGrad<MODEL, R> base;
+        Model<MODEL, R> pert[FD ? NP - 1 : 1];
+        R h[NP];
+        __device__ __forceinline__ void setup(const R (&th)[NP]) {
+            base.setup(th);
+            if (FD) {
+#pragma unroll
+                for (int j = 0; j < NP - 1; j++) {
+                    R tp[NP];
+#pragma unroll
+                    for (int k = 0; k < NP; k++) tp[k] = th[k];
+                    R a = th[j] < R(0) ? -th[j] : th[j];
+                    h[j] = R(1e-5) * (a > R(1) ? a : R(1));
+                    tp[j] = th[j] + h[j];
+                    pert[j].setup(tp);
+                }
+            }
+        }
+        __device__ __forceinline__ bool operator()(R t, R& f, R (&g)[NP]) const {
+            if (!FD) return base.pred_grad(t, f, g);
+            if (!base.pred(t, f)) return false;
+#pragma unroll
+            for (int j = 0; j < NP - 1; j++) {
+                R fp;
+                bool ok = pert[j].pred(t, fp) && M::finite(fp);
+                g[j] = ok ? (fp - f) / h[j] : R(0);
+            }
+            g[NP - 1] = R(0);
+            return true;
+        }
+    };
+
+    // LL of this lane's observations for sample theta and its gradient.
+    __device__ __forceinline__ R loglik(const R (&th)[NP], const O* lds, const O* __restrict__ gobs, int n,
+                                        int tile, int grp, int G, R (&dll)[NP]) {
+        constexpr bool F64 = sizeof(R) == 8;
+        Eval ev;
+        ev.setup(th);
+        R se2;
+        if (F64) {  // the reference squares exp(ln sigma_x) (cuda/svi.cu:164-165)
+            R se = M::exp(th[SE]);
+            se2 = se * se;
+        } else {
+            se2 = M::exp(R(2) * th[SE]);
+        }
+        Kahan<R> ll;
+        ll.init();
+#pragma unroll
+        for (int j = 0; j < NP; j++) dll[j] = R(0);
+        for (int base = 0; base < n; base += tile) {
+            int cnt = n - base < tile ? n - base : tile;
+            if (n > tile) stage<R>(const_cast<O*>(lds), gobs + base, cnt);
+            for (int i = grp; i < cnt; i += G) {
+                O o = lds[i];
+                R f, g[NP];
+                if (!ev(o.x, f, g) || !M::finite(f)) continue;
+                R tv = o.z + se2;
+                if (o.w != R(0)) {
+                    R st = M::sqrt(tv);
+                    R z = (o.y - f) / st;
+                    R lz = log_normal_cdf<R>(z);
+                    if (F64) ll.s += lz; else ll.add(lz);
+                    R ratio, dpred;
+                    if (F64) {  // cuda/svi.cu:200-204 as written
+                        R phi, cdf;
+                        phi_cdf(z, phi, cdf);
+                        ratio = phi / cdf;
+                        dpred = -phi / (cdf * st);
+                    } else {
+                        ratio = mills_ratio<R>(z);
+                        dpred = -ratio / st;
+                    }
+#pragma unroll
+                    for (int j = 0; j < NP - 1; j++)
+                        if (M::finite(g[j])) dll[j] += dpred * g[j];
+                    dll[SE] += ratio * (-(o.y - f) * se2 / (st * tv));
+                } else {
+                    R inv = M::rcp(tv);
+                    R r = o.y - f, r2 = r * r;
+                    R lt = R(-0.5) * (r2 * inv + M::log(R(6.28318530717958647692) * tv));
+                    if (F64) ll.s += lt; else ll.add(lt);
+                    R ri = r * inv;
+#pragma unroll
+                    for (int j = 0; j < NP - 1; j++)
+                        if (M::finite(g[j])) dll[j] += ri * g[j];
+                    dll[SE] += (r2 * inv * inv - inv) * se2;
+                }
+            }
+        }
+        return ll.s;
+    }
+
+    // Monte-Carlo ELBO and its gradient at (mu, ls) for step `step`:
+    // returns the ELBO, fills gmu / gls (wave-uniform).
+    __device__ __forceinline__ R elbo_grad(const R (&mu)[NP], const R (&ls)[NP], const R (&pc)[NP],
+                                           const R (&pw2)[NP], const O* lds, const O* __restrict__ gobs,
+                                           int n, int tile, uint32_t job, uint32_t step, int S, int G,
+                                           uint64_t seed, R (&gmu)[NP], R (&gls)[NP]) {
+        const int lane = lane_id();
+        const int s = lane / G, grp = lane - s * G;
+        const bool active = s < S;
+        R sig[NP], eps[NP], th[NP], dll[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) sig[j] = M::exp(ls[j]);
+        draw_normals<R, NP>(seed, job, step, (uint32_t)s, eps);
+#pragma unroll
+        for (int j = 0; j < NP; j++) th[j] = mu[j] + sig[j] * eps[j];
+        R L = loglik(th, lds, gobs, n, tile, grp, G, dll);
+        if (grp == 0) {
+            R lp = R(0);
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                R d = th[j] - pc[j];
+                lp += R(-0.5) * d * d / pw2[j];
+                dll[j] -= d / pw2[j];
+            }
+            L += lp;
+        }
+        if (!active) {
+            L = R(0);
+#pragma unroll
+            for (int j = 0; j < NP; j++) dll[j] = R(0);
+        }
+        L = wave_sum(L);
+        const R rS = R(S);
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            R b = dll[j] * sig[j] * eps[j];
+            gmu[j] = wave_sum(dll[j]) / rS;
+            gls[j] = wave_sum(b) / rS;
+        }
+        R ent = R(0);
+#pragma unroll
+        for (int j = 0; j < NP; j++) ent += ls[j];
+        // 0.5 np ln(2 pi e), as the reference forms it (cuda/svi.cu:264)
+        ent += R(0.5 * NP) * (R)::log(2.0 * 3.14159265358979323846 * 2.71828182845904523536);
+        return L / rS + ent;
+    }
+
+    __device__ void run(const SviArgs& a, O* lds, int job) {
+        const int lane = lane_id();
+        const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[job];
+        const int n = (int)(a.offsets[job + 1] - a.offsets[job]);
+        const int S = a.n_samples;
+        int sp = 1;
+        while (sp < S) sp <<= 1;
+        const int G = WAVE / sp;
+        R mu[NP], ls[NP], pc[NP], pw2[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            mu[j] = (R)a.mu_in[(int64_t)job * 7 + j];
+            ls[j] = a.ls_in ? (R)a.ls_in[(int64_t)job * 7 + j] : R(-1);
+            pc[j] = (R)a.prior_c[(int64_t)job * 7 + j];
+            R w = (R)a.prior_w[(int64_t)job * 7 + j];
+            pw2[j] = w * w;
+        }
+        if (n <= 0) {
+            if (lane < 7) {
+                a.out_mu[(int64_t)job * 7 + lane] = 0.0;
+                a.out_ls[(int64_t)job * 7 + lane] = 0.0;
+            }
+            if (lane == 0) {
+                a.out_elbo[job] = -1e99;
+                if (a.out_steps) a.out_steps[job] = 0;
+            }
+            return;
+        }
+        if (n <= a.tile) stage<R>(lds, g, n);
+        R gmu[NP], gls[NP];
+        if (a.grad_only) {
+            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, (uint32_t)job, (uint32_t)a.step0, S, G,
+                            a.seed, gmu, gls);
+            double om = 0.0, ol = 0.0;
+#pragma unroll
+            for (int j = 0; j < NP; j++)  // + 1: derivative of the entropy term
+                if (lane == j) { om = (double)gmu[j]; ol = (double)gls[j] + 1.0; }
+            if (lane < 7) {
+                a.out_mu[(int64_t)job * 7 + lane] = om;
+                a.out_ls[(int64_t)job * 7 + lane] = ol;
+            }
+            if (lane == 0) a.out_elbo[job] = (double)e;
+            return;
+        }
+        R am[NP], av[NP], bm[NP], bv[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) am[j] = av[j] = bm[j] = bv[j] = R(0);
+        const int min_it = (a.flags & LCF_SVI_CPU_STOPPING_FLAG) ? 200 : 50;
+        const int stall_it = (a.flags & LCF_SVI_CPU_STOPPING_FLAG) ? 50 : 20;
+        const R lr = (R)a.lr;
+        R b1t = R(1), b2t = R(1);
+        R final_elbo = R(-1e30);
+        double ema = -1e300, best = -1e300;
+        int stall = 0, step = 0;
+        for (; step < a.n_steps;) {
+            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, (uint32_t)job, (uint32_t)step, S, G, a.seed,
+                            gmu, gls);
+            final_elbo = e;
+            step++;
+            // Adam (cuda/svi.cu:266-291), non-finite components skipped
+            b1t *= R(0.9);
+            b2t *= R(0.999);
+            R bc1 = R(1) - b1t, bc2 = R(1) - b2t;
+            if (sizeof(R) == 8) {  // cuda/svi.cu:269-270 as written
+                bc1 = (R)(1.0 - ::pow(0.9, (double)step));
+                bc2 = (R)(1.0 - ::pow(0.999, (double)step));
+            }
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                R gm = -gmu[j];
+                if (M::finite(gm)) {
+                    am[j] = R(0.9) * am[j] + R(0.1) * gm;
+                    av[j] = R(0.999) * av[j] + R(0.001) * gm * gm;
+                    mu[j] -= lr * (am[j] / bc1) / (M::sqrt(av[j] / bc2) + R(1e-8));
+                }
+                R gl = -(gls[j] + R(1));
+                if (M::finite(gl)) {
+                    bm[j] = R(0.9) * bm[j] + R(0.1) * gl;
+                    bv[j] = R(0.999) * bv[j] + R(0.001) * gl * gl;
+                    ls[j] -= lr * (bm[j] / bc1) / (M::sqrt(bv[j] / bc2) + R(1e-8));
+                }
+                ls[j] = ls[j] < R(-6) ? R(-6) : (ls[j] > R(2) ? R(2) : ls[j]);
+            }
+            // EMA early stop (cuda/svi.cu:297-312)
+            if (step - 1 >= min_it) {
+                double fe = (double)final_elbo;
+                ema = ema < -1e290 ? fe : 0.1 * fe + 0.9 * ema;
+                if (ema - best > 0.01 * fmax(fabs(best), 1e-10)) { best = ema; stall = 0; }
+                else if (++stall >= stall_it) break;
+            }
+        }
+        const double infl = (a.flags & LCF_SVI_INFLATE_FLAG) ? 1.3862943611198906 : 0.0;
+        double om = 0.0, ol = 0.0;
+#pragma unroll
+        for (int j = 0; j < NP; j++)
+            if (lane == j) { om = (double)mu[j]; ol = (double)ls[j] + infl; }
+        if (lane < 7) {
+            a.out_mu[(int64_t)job * 7 + lane] = om;
+            a.out_ls[(int64_t)job * 7 + lane] = ol;
+        }
+        if (lane == 0) {
+            a.out_elbo[job] = step > 0 ? (double)final_elbo : -1e300;
+            if (a.out_steps) a.out_steps[job] = step;
+        }
+    }
+};
+
+template <int MODEL, typename R, bool FD, bool PERSISTENT>
+__global__ __launch_bounds__(64) void svi_kernel(SviArgs a) {
+    extern __shared__ unsigned char lds_raw[];
+    using O = typename ObsR<R>::T;
+    O* lds = reinterpret_cast<O*>(lds_raw);
+    SviJob<MODEL, R, FD> job;
+    if (PERSISTENT) {
+        for (;;) {
+            const int t = wave_next_task(a.queue);
+            if (t >= a.n_tasks) break;
+            job.run(a, lds, a.order[t]);
+            wave_reconverge();
+        }
+    } else {
+        int t = a.task_base + (int)blockIdx.x;
+        if (t < a.n_tasks) job.run(a, lds, a.order[t]);
+    }
+}
+
+template <int MODEL, typename R>
+static const void* svi_ptr(bool fd, bool persistent) {
+    if (fd) return persistent ? (const void*)svi_kernel<MODEL, R, true, true> : (const void*)svi_kernel<MODEL, R, true, false>;
+    return persistent ? (const void*)svi_kernel<MODEL, R, false, true> : (const void*)svi_kernel<MODEL, R, false, false>;
+}
+
+template <typename R>
+static const void* svi_ptr_m(int model, bool fd, bool persistent) {
+    switch (model) {
+        case 0: return svi_ptr<0, R>(fd, persistent);
+        case 1: return svi_ptr<1, R>(fd, persistent);
+        case 2: return svi_ptr<2, R>(fd, persistent);
+        case 3: return svi_ptr<3, R>(fd, persistent);
+        case 4: return svi_ptr<4, R>(fd, persistent);
+        case 5: return svi_ptr<5, R>(fd, persistent);
+        case 6: return svi_ptr<6, R>(fd, persistent);
+        case 7: return svi_ptr<7, R>(fd, persistent);
+        default: return nullptr;
+    }
+}
+
+}  // namespace lcf

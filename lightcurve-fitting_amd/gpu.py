@@ -139,6 +139,10 @@ class GpuContext:
     def num_cus(self) -> int:
         return int(self._lib.lcf_ctx_num_cus(self._h))
 
+    def synchronize(self) -> None:
+        """Wait for all work on the context's device (``lcf_ctx_synchronize``)."""
+        self._check(self._lib.lcf_ctx_synchronize(self._h))
+
     @property
     def last_kernel_ms(self) -> float:
         return float(self._lib.lcf_last_kernel_ms(self._h))

@@ -118,13 +118,14 @@ double orc_svi_elbo_grad(int model_id, const double* mu, const double* log_sigma
                          double* grad_log_sigma);
 /* GpuContext::batch_svi_fit (src/gpu_cuda.rs:1315-1422).  Per job: model id,
  * pso[7], prior centers/widths [7] (padded rows).  Outputs mu/log_sigma
- * [n_jobs*7], elbo[n_jobs], steps executed. */
+ * [n_jobs*7], elbo[n_jobs], steps executed, and (diagnostic, may be NULL) the
+ * largest |gradient| handed to Adam over the fit, +inf if one was non-finite. */
 int orc_batch_svi(int64_t n_jobs, const int64_t* offsets, const double* t, const double* f,
                   const double* v, const int32_t* up, const double* uf, const int32_t* model_ids,
                   const double* pso, const double* prior_c, const double* prior_w, int n_steps,
                   int n_samples, double lr, int rng_mode, uint64_t seed, int grad_mode,
                   uint32_t flags, double* out_mu, double* out_log_sigma, double* out_elbo,
-                  int32_t* out_steps, int n_threads);
+                  int32_t* out_steps, double* out_gmax, int n_threads);
 /* svi_prior_for_model (src/parametric.rs:3259-3273). */
 int orc_svi_prior_for_model(int model_id, const double* pso, double* centers, double* widths);
 

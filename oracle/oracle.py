@@ -45,7 +45,7 @@ SIGNATURES = {
     "orc_svi_elbo_grad": (_f64, [_i32, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _pd, _pd, _pi32, _pd,
                                  _i64, _i32, _pd, _pd]),
     "orc_batch_svi": (_i32, [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _pi32, _pd, _pd, _pd, _i32, _i32,
-                             _f64, _i32, _u64, _i32, _u32, _pd, _pd, _pd, _pi32, _i32]),
+                             _f64, _i32, _u64, _i32, _u32, _pd, _pd, _pd, _pi32, _pd, _i32]),
     "orc_svi_prior_for_model": (_i32, [_i32, _pd, _pd, _pd]),
 }
 
@@ -250,7 +250,9 @@ def svi_elbo_grad(m: int, mu, ls, pc, pw, eps, t, f, v, up, uf, grad_mode: int =
 
 def batch_svi(csr: dict, model_ids, pso, pc, pw, n_steps: int, n_samples: int, lr: float,
               rng_mode: int = SVI_RNG_PHILOX, seed: int = 42, grad_mode: int = 0, flags: int = 0,
-              n_threads: int | None = None):
+              n_threads: int | None = None, with_gmax: bool = False):
+    """Returns (mu, log_sigma, elbo, steps), plus the per-job largest |gradient|
+    handed to Adam (inf if one was non-finite) when ``with_gmax``."""
     off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
     nj = off.size - 1
     t, f, v, up, uf = _obs(csr)
@@ -265,13 +267,15 @@ def batch_svi(csr: dict, model_ids, pso, pc, pw, n_steps: int, n_samples: int, l
     pso, pc, pw = rows(pso), rows(pc), rows(pw)
     mu, ls, elbo = np.zeros((nj, 7)), np.zeros((nj, 7)), np.zeros(nj)
     steps = np.zeros(nj, dtype=np.int32)
+    gmax = np.zeros(nj)
     rc = lib().orc_batch_svi(nj, _p(off, C.c_int64), _p(t, C.c_double), _p(f, C.c_double), _p(v, C.c_double),
                              _p(up, C.c_int32), _p(uf, C.c_double), _p(mid, C.c_int32), _p(pso, C.c_double),
                              _p(pc, C.c_double), _p(pw, C.c_double), n_steps, n_samples, float(lr), rng_mode,
                              C.c_uint64(seed), grad_mode, flags, _p(mu, C.c_double), _p(ls, C.c_double),
-                             _p(elbo, C.c_double), _p(steps, C.c_int32), n_threads or default_threads())
+                             _p(elbo, C.c_double), _p(steps, C.c_int32), _p(gmax, C.c_double),
+                             n_threads or default_threads())
     assert rc == 0
-    return mu, ls, elbo, steps
+    return (mu, ls, elbo, steps, gmax) if with_gmax else (mu, ls, elbo, steps)
 
 
 GP_SUBSAMPLE_GPU, GP_SUBSAMPLE_CPU = 0, 1

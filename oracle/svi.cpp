@@ -144,6 +144,7 @@ double elbo_grad_step(int m, int np, int se, const double* mu, const double* ls,
 struct SviOut {
     double mu[7], ls[7], elbo;
     int steps;
+    double gmax;  // largest |gradient| handed to Adam over the fit (inf if any was non-finite)
 };
 
 static SviOut svi_job(int m, int64_t job, const double* pso, const double* pc, const double* pw,
@@ -154,6 +155,7 @@ static SviOut svi_job(int m, int64_t job, const double* pso, const double* pc, c
     const int np = n_params(m), se = np - 1;
     for (int j = 0; j < 7; j++) { o.mu[j] = 0.0; o.ls[j] = 0.0; }
     o.steps = 0;
+    o.gmax = 0.0;
     if (n <= 0) { o.elbo = -1e99; return o; }
     double mu[7], ls[7], am[14] = {0}, av[14] = {0}, gmu[7], gls[7];
     for (int j = 0; j < np; j++) { mu[j] = pso[j]; ls[j] = -1.0; }
@@ -177,6 +179,9 @@ static SviOut svi_job(int m, int64_t job, const double* pso, const double* pc, c
         t_adam++;
         double bc1 = 1.0 - std::pow(0.9, (double)t_adam), bc2 = 1.0 - std::pow(0.999, (double)t_adam);
         for (int j = 0; j < np; j++) {
+            double ga = std::fabs(gmu[j]), gb = std::fabs(gls[j] + 1.0);
+            o.gmax = std::max(o.gmax, std::isfinite(ga) ? ga : INFINITY);
+            o.gmax = std::max(o.gmax, std::isfinite(gb) ? gb : INFINITY);
             double g = -gmu[j];
             if (std::isfinite(g)) {
                 am[j] = 0.9 * am[j] + 0.1 * g;
@@ -229,7 +234,7 @@ int orc_batch_svi(int64_t n_jobs, const int64_t* off, const double* t, const dou
                   const int32_t* up, const double* uf, const int32_t* model_ids, const double* pso,
                   const double* pc, const double* pw, int n_steps, int n_samples, double lr, int rng_mode,
                   uint64_t seed, int grad_mode, uint32_t flags, double* out_mu, double* out_ls,
-                  double* out_elbo, int32_t* out_steps, int n_threads) {
+                  double* out_elbo, int32_t* out_steps, double* out_gmax, int n_threads) {
     if (n_jobs < 0 || n_steps < 0 || n_samples <= 0 || n_samples > 64) return -1;
     for (int64_t j = 0; j < n_jobs; j++)
         if (orc::n_params(model_ids[j]) < 0) return -1;
@@ -241,6 +246,7 @@ int orc_batch_svi(int64_t n_jobs, const int64_t* off, const double* t, const dou
         for (int d = 0; d < 7; d++) { out_mu[j * 7 + d] = o.mu[d]; out_ls[j * 7 + d] = o.ls[d]; }
         out_elbo[j] = o.elbo;
         if (out_steps) out_steps[j] = o.steps;
+        if (out_gmax) out_gmax[j] = o.gmax;
     });
     return 0;
 }

@@ -58,8 +58,8 @@ def test_model_metadata_mirrors_reference(pkg):
 
 
 def test_kernel_bounds_tables_match_oracle(orc):
-    """The __constant__ tables in pso_kernels.hip carry the same numbers as the oracle."""
-    src = (ROOT / "lightcurve-fitting_amd" / "csrc" / "pso_kernels.hip").read_text()
+    """The __constant__ tables in pso_swarm.hpp carry the same numbers as the oracle."""
+    src = (ROOT / "lightcurve-fitting_amd" / "csrc" / "pso_swarm.hpp").read_text()
 
     def table(name):
         m = re.search(name + r"\[8\]\[7\] = \{(.*?)\};", src, re.S)
