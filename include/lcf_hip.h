@@ -119,6 +119,22 @@ int lcf_batch_fit_models(lcf_ctx* ctx, const lcf_batch* data, const int32_t* mod
                          double* model_params, double* model_cost, int32_t* model_iters,
                          int32_t* best_model, double* best_params, double* best_cost);
 
+/* ---- greedy MultiBazin ---------------------------------------------------- */
+/* GpuContext::batch_pso_multi_bazin (src/gpu_cuda.rs:650-963): for K = 1..4
+ * Bazin components a swarm over [ln A, t0, ln tau_rise, ln tau_fall] x K, B,
+ * ln sigma_x with t0 bounded by the batch time range -/+ 30 d; particle 0 of
+ * K >= 2 starts from the K-1 solution plus a component at the largest
+ * detection residual; best K by BIC = 2 cost n + (4K+2) ln n, stopping once
+ * a K's BIC exceeds the previous one by more than 2.  Outputs per source:
+ * best_k, params[18] (4 best_k + 2 used, zero padded; all zero and bic = inf
+ * when no K gave a finite BIC), cost, bic, per_k_cost[4] / per_k_bic[4]
+ * (NaN after the stop).  The reference's `sources` argument (time ranges and
+ * residuals) is read from `data` on the device. */
+int lcf_batch_multi_bazin(lcf_ctx* ctx, const lcf_batch* data, int n_particles, int max_iters,
+                          int stall_iters, uint64_t seed, int32_t* out_best_k, double* out_params,
+                          double* out_cost, double* out_bic, double* out_per_k_cost,
+                          double* out_per_k_bic);
+
 /* ---- batched SVI ----------------------------------------------------------- */
 /* SVI flags */
 #define LCF_SVI_CPU_STOPPING 1u  /* early stop after 200 steps / stall 50 (svi_fit constants,

@@ -24,6 +24,7 @@ from .gpu import (  # noqa: F401
     GpuContext,
     GpuError,
     GpuModelName,
+    MultiBazinResult,
     SviBatchInput,
     SviBatchOutput,
     sources_to_csr,

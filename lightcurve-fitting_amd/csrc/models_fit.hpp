@@ -29,6 +29,14 @@ template <> struct ModelTraits<4> { static constexpr int NP = 5; };  // Magnetar
 template <> struct ModelTraits<5> { static constexpr int NP = 5; };  // ShockCooling
 template <> struct ModelTraits<6> { static constexpr int NP = 6; };  // Afterglow
 template <> struct ModelTraits<7> { static constexpr int NP = 5; };  // MetzgerKN
+// MultiBazin with K = 1..4 components (internal ids 8..11): 4K + 2 parameters
+// [ln A_c, t0_c, ln tau_rise_c, ln tau_fall_c]_c, B, ln sigma_x
+// (cuda/models.cu:286-317, src/parametric.rs:905-965).
+constexpr int MB_MODEL0 = 8;
+template <> struct ModelTraits<8> { static constexpr int NP = 6; };
+template <> struct ModelTraits<9> { static constexpr int NP = 10; };
+template <> struct ModelTraits<10> { static constexpr int NP = 14; };
+template <> struct ModelTraits<11> { static constexpr int NP = 18; };
 
 // logistic(x) = 1/(1+e^-x) without overflow: returns (q, e) with
 // e = e^-|x|, q = 1/(1+e); logistic = x>=0 ? q : e*q, 1-logistic = x>=0 ? e*q : q.
@@ -399,6 +407,67 @@ template <typename R> struct ModelStable<7, R> {
         return true;
     }
 };
+
+// MultiBazin, overflow-free form: each component as the Bazin model above
+// without its baseline; a component whose reference value is non-finite makes
+// the prediction non-finite, one whose logistic is exactly 0 contributes 0.
+template <int K, typename R> struct MultiBazinStable {
+    R la[K], t0[K], itr[K], itf[K], b;
+    __device__ __forceinline__ void setup(const R* p) {
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            la[c] = p[4 * c]; t0[c] = p[4 * c + 1];
+            itr[c] = Math<R>::exp(-p[4 * c + 2]);
+            itf[c] = Math<R>::exp(-p[4 * c + 3]);
+        }
+        b = p[4 * K];
+    }
+    __device__ __forceinline__ bool pred(R t, R& f) const {
+        R sum = R(0);
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            R dt = t - t0[c];
+            R x1 = -dt * itf[c], x2 = -dt * itr[c];
+            if (x1 > R(EXP_OVF64) || la[c] + x1 > R(EXP_OVF64)) return false;
+            if (x2 > R(EXP_OVF64)) continue;
+            sum += Math<R>::exp(la[c] + x1 - softplus(x2));
+        }
+        f = sum + b;
+        return true;
+    }
+};
+template <typename R> struct ModelStable<8, R> : MultiBazinStable<1, R> {};
+template <typename R> struct ModelStable<9, R> : MultiBazinStable<2, R> {};
+template <typename R> struct ModelStable<10, R> : MultiBazinStable<3, R> {};
+template <typename R> struct ModelStable<11, R> : MultiBazinStable<4, R> {};
+
+// MultiBazin in FP64, the reference's expression as written
+// (bazin_component_at / multi_bazin_at, cuda/models.cu:297-313).
+template <int K> struct MultiBazinRef {
+    double a[K], t0[K], tr[K], tf[K], b;
+    __device__ __forceinline__ void setup(const double* p) {
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            a[c] = ::exp(p[4 * c]); t0[c] = p[4 * c + 1];
+            tr[c] = ::exp(p[4 * c + 2]); tf[c] = ::exp(p[4 * c + 3]);
+        }
+        b = p[4 * K];
+    }
+    __device__ __forceinline__ bool pred(double t, double& f) const {
+        double sum = 0.0;
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            double dt = t - t0[c];
+            sum += a[c] * ::exp(-dt / tf[c]) * (1.0 / (1.0 + ::exp(-(dt / tr[c]))));
+        }
+        f = sum + b;
+        return __builtin_isfinite(f);
+    }
+};
+template <> struct Model<8, double> : MultiBazinRef<1> {};
+template <> struct Model<9, double> : MultiBazinRef<2> {};
+template <> struct Model<10, double> : MultiBazinRef<3> {};
+template <> struct Model<11, double> : MultiBazinRef<4> {};
 
 // ln Phi(x) with the Abramowitz & Stegun 7.1.26 erfc polynomial
 // (src/parametric.rs:882-901).

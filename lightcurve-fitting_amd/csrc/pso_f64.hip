@@ -5,4 +5,5 @@
 
 namespace lcf {
 const void* pso_kernel_f64(int model, int ppl, bool persistent) { return pick_kernel_r<double>(model, ppl, persistent); }
+const void* mb_kernel_f64(int ppl) { return pick_mb_r<double>(ppl); }
 }  // namespace lcf

@@ -7,6 +7,8 @@ namespace lcf {
 
 const void* pso_kernel_f32(int model, int ppl, bool persistent);
 const void* pso_kernel_f64(int model, int ppl, bool persistent);
+const void* mb_kernel_f32(int ppl);
+const void* mb_kernel_f64(int ppl);
 
 static const void* pick_kernel(int model, bool fp64, int ppl, bool persistent) {
     return fp64 ? pso_kernel_f64(model, ppl, persistent) : pso_kernel_f32(model, ppl, persistent);
@@ -55,6 +57,19 @@ hipError_t launch_select_best(const double* mp, const double* mc, const int32_t*
     hipLaunchKernelGGL(select_best, dim3((n_bands + 255) / 256), dim3(256), 0, st, mp, mc, models,
                        n_models, n_bands, best_model, best_params, best_cost);
     return hipGetLastError();
+}
+
+hipError_t launch_multi_bazin(const MbArgs& a, bool fp64, int ppl, int grid, hipStream_t st) {
+    const void* k = fp64 ? mb_kernel_f64(ppl) : mb_kernel_f32(ppl);
+    if (!k || grid <= 0) return hipErrorInvalidValue;
+    void* args[] = {const_cast<MbArgs*>(&a)};
+    return hipLaunchKernel(k, dim3(grid), dim3(64), args, pso_lds_bytes(fp64, a.tile), st);
+}
+
+hipError_t multi_bazin_occupancy(bool fp64, int ppl, int tile, int* blocks_per_cu) {
+    const void* k = fp64 ? mb_kernel_f64(ppl) : mb_kernel_f32(ppl);
+    if (!k) return hipErrorInvalidValue;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k, 64, pso_lds_bytes(fp64, tile));
 }
 
 }  // namespace lcf

@@ -26,6 +26,32 @@ struct PsoArgs {
     int32_t* model_iters;       // [n_slots][n_bands]
 };
 
+// Greedy MultiBazin fit (batch_pso_multi_bazin, src/gpu_cuda.rs:650-963).
+struct MbArgs {
+    const void* obs;            // packed records of the swarm's precision
+    const double4* obs64;       // FP64 records (residual seeding)
+    const int64_t* offsets;
+    const int32_t* order;       // band indices, decreasing size
+    int n_tasks;
+    int* queue;
+    double t_lo, t_hi;          // t0 bounds: global time range -/+ 30 days
+    int n_particles, max_iters, stall_iters;
+    uint64_t seed;
+    int tile;
+    int32_t* best_k;            // [n_bands]
+    double* params;             // [n_bands][18] best-K parameters, zero padded
+    double* cost;               // [n_bands]
+    double* bic;                // [n_bands]
+    double* per_k_cost;         // [n_bands][4] (NaN after the early stop)
+    double* per_k_bic;          // [n_bands][4]
+};
+
+constexpr int MB_MAX_K = 4;
+constexpr int MB_MAX_PARAMS = 4 * MB_MAX_K + 2;
+
+hipError_t launch_multi_bazin(const MbArgs& a, bool fp64, int ppl, int grid, hipStream_t st);
+hipError_t multi_bazin_occupancy(bool fp64, int ppl, int tile, int* blocks_per_cu);
+
 size_t pso_lds_bytes(bool fp64, int tile);
 hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool persistent, int grid,
                       hipStream_t st);

@@ -142,12 +142,14 @@ struct Swarm {
 #pragma unroll
         for (int k = 0; k < PPL; k++) {
             R pri = R(0);
+            if constexpr (MODEL < MB_MODEL0) {  // MultiBazin has no population prior
 #pragma unroll
-            for (int j = 0; j < NP; j++) {
-                R w = (R)c_pw[MODEL][j];
-                if (w > R(0)) {
-                    R z = (pos[k][j] - (R)c_pc[MODEL][j]) / w;
-                    pri += R(0.5) * z * z;
+                for (int j = 0; j < NP; j++) {
+                    R w = (R)c_pw[MODEL][j];
+                    if (w > R(0)) {
+                        R z = (pos[k][j] - (R)c_pc[MODEL][j]) / w;
+                        pri += R(0.5) * z * z;
+                    }
                 }
             }
             R c;
@@ -163,32 +165,41 @@ struct Swarm {
         }
     }
 
+    // Box bounds lo/hi; with `seedp` particle 0 starts there with velocity
+    // (hi - lo) 0.02 (2u - 1) (batch_pso_multi_bazin, src/gpu_cuda.rs:767-777).
+    // Random numbers: Philox block (band, particle, iteration or 0xFFFFFFFF,
+    // rng_tag + dimension pair).
     __device__ __forceinline__ void run(const O* lds, const O* __restrict__ g, int n, int tile,
                                         uint32_t band, int P, int max_iters, int stall_iters,
-                                        uint64_t seed, uint32_t flags, int& iters_out) {
+                                        uint64_t seed, uint32_t flags, int& iters_out,
+                                        const R (&lo)[NP], const R (&hi)[NP], const R* seedp = nullptr,
+                                        uint32_t rng_tag = 0u) {
         const int lane = lane_id();
         const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
         constexpr int NPAIR = (NP + 1) / 2;
-        R lo[NP], hi[NP], vmax[NP];
+        R vmax[NP];
 #pragma unroll
-        for (int d = 0; d < NP; d++) {
-            lo[d] = (R)c_lo[MODEL][d];
-            hi[d] = (R)c_hi[MODEL][d];
-            vmax[d] = R(0.5) * (hi[d] - lo[d]);
-        }
-        // initial swarm: Philox block (band, particle, 0xFFFFFFFF, pair)
+        for (int d = 0; d < NP; d++) vmax[d] = R(0.5) * (hi[d] - lo[d]);
 #pragma unroll
         for (int k = 0; k < PPL; k++) {
             uint32_t p = (uint32_t)(lane + WAVE * k);
+            const bool seeded = seedp != nullptr && p == 0;
 #pragma unroll
             for (int q = 0; q < NPAIR; q++) {
-                U4 r = philox4x32_10(band, p, 0xFFFFFFFFu, (uint32_t)q, k0, k1);
-                int d0 = 2 * q, d1 = 2 * q + 1;
-                pos[k][d0] = lo[d0] + u24<R>(r.x) * (hi[d0] - lo[d0]);
-                vel[k][d0] = vmax[d0] * R(0.2) * (R(2) * u24<R>(r.y) - R(1));
-                if (d1 < NP) {
-                    pos[k][d1] = lo[d1] + u24<R>(r.z) * (hi[d1] - lo[d1]);
-                    vel[k][d1] = vmax[d1] * R(0.2) * (R(2) * u24<R>(r.w) - R(1));
+                U4 r = philox4x32_10(band, p, 0xFFFFFFFFu, rng_tag + (uint32_t)q, k0, k1);
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    int d = 2 * q + h;
+                    if (d < NP) {
+                        R ux = u24<R>(h ? r.z : r.x), uv = u24<R>(h ? r.w : r.y);
+                        if (seeded) {
+                            pos[k][d] = seedp[d];
+                            vel[k][d] = (hi[d] - lo[d]) * R(0.02) * (R(2) * uv - R(1));
+                        } else {
+                            pos[k][d] = lo[d] + ux * (hi[d] - lo[d]);
+                            vel[k][d] = vmax[d] * R(0.2) * (R(2) * uv - R(1));
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -242,7 +253,7 @@ struct Swarm {
                 uint32_t p = (uint32_t)(lane + WAVE * k);
 #pragma unroll
                 for (int q = 0; q < NPAIR; q++) {
-                    U4 r = philox4x32_10(band, p, (uint32_t)it, (uint32_t)q, k0, k1);
+                    U4 r = philox4x32_10(band, p, (uint32_t)it, rng_tag + (uint32_t)q, k0, k1);
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
                         int d = 2 * q + h;
@@ -286,10 +297,17 @@ __device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
     const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[band];
     const int n = (int)(a.offsets[band + 1] - a.offsets[band]);
     if (n <= a.tile) stage_obs<R>(lds, g, n);
+    constexpr int NP = Swarm<MODEL, R, PPL>::NP;
+    R lo[NP], hi[NP];
+#pragma unroll
+    for (int d = 0; d < NP; d++) {
+        lo[d] = (R)c_lo[MODEL][d];
+        hi[d] = (R)c_hi[MODEL][d];
+    }
     Swarm<MODEL, R, PPL> s;
     int iters = 0;
     s.run(lds, g, n, a.tile, (uint32_t)band, a.n_particles, a.max_iters, a.stall_iters, a.seed,
-          a.flags, iters);
+          a.flags, iters, lo, hi);
     double cst = (double)s.gcost;
     if (cst >= (double)Math<R>::big()) cst = 1e99;
     double prm = 0.0;  // lane d holds parameter d
@@ -324,6 +342,168 @@ __global__ __launch_bounds__(64) void pso_model_kernel(PsoArgs a) {
         int t = a.task_base + (int)blockIdx.x;
         if (t < a.n_tasks) fit_band<MODEL, R, PPL>(a, lds, a.order[t]);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Greedy MultiBazin (batch_pso_multi_bazin, src/gpu_cuda.rs:650-963): for
+// K = 1..4 a swarm over 4K + 2 parameters (no population prior); from K = 2
+// particle 0 starts at the K - 1 solution plus a component seeded at the
+// largest detection residual; BIC = 2 cost n + (4K + 2) ln n selects K
+// (strict '<'), and a band stops once K's BIC exceeds K - 1's by more than 2.
+// One wavefront per band runs all K in registers.
+// ---------------------------------------------------------------------------
+struct MbState {
+    double prev[MB_MAX_PARAMS];   // gbest of the last K fitted (wave-uniform)
+    double best[MB_MAX_PARAMS];
+    double best_bic, best_cost;
+    double pk_cost[MB_MAX_K], pk_bic[MB_MAX_K];
+    int best_k;
+    bool stopped;
+};
+
+template <int K, typename R, int PPL>
+__device__ __forceinline__ void mb_fit_k(const MbArgs& a, typename ObsT<R>::T* lds, int band, int n,
+                                         MbState& st) {
+    using O = typename ObsT<R>::T;
+    constexpr int MODEL = MB_MODEL0 + K - 1;
+    constexpr int NP = 4 * K + 2;
+    if (st.stopped) {
+        st.pk_cost[K - 1] = NAN;
+        st.pk_bic[K - 1] = NAN;
+        return;
+    }
+    R lo[NP], hi[NP];
+#pragma unroll
+    for (int c = 0; c < K; c++) {
+        lo[4 * c] = R(-3); lo[4 * c + 1] = (R)a.t_lo; lo[4 * c + 2] = R(-2); lo[4 * c + 3] = R(-2);
+        hi[4 * c] = R(3); hi[4 * c + 1] = (R)a.t_hi; hi[4 * c + 2] = R(5); hi[4 * c + 3] = R(6);
+    }
+    lo[4 * K] = R(-0.3); hi[4 * K] = R(0.3);
+    lo[4 * K + 1] = R(-5); hi[4 * K + 1] = R(0);
+    R seedv[NP];
+    if (K > 1) {
+        // residual peak of the K - 1 fit over detections, in FP64 with the
+        // reference's expression (src/gpu_cuda.rs:718-745); ties: first index
+        const double4* g64 = a.obs64 + a.offsets[band];
+        const double* pv = st.prev;
+        double bv = INFINITY;
+        int bi = 0x7fffffff;
+        for (int i = lane_id(); i < n; i += WAVE) {
+            double4 o = g64[i];
+            if (o.w != 0.0) continue;
+            double pred = 0.0;
+#pragma unroll
+            for (int c = 0; c < K - 1; c++) {
+                double aa = ::exp(pv[4 * c]), t0 = pv[4 * c + 1];
+                double tr = ::exp(pv[4 * c + 2]), tf = ::exp(pv[4 * c + 3]);
+                double dt = o.x - t0;
+                double sig = 1.0 / (1.0 + ::exp(-dt / tr));
+                pred += aa * ::exp(-dt / tf) * sig;
+            }
+            pred += pv[4 * (K - 1)];
+            double resid = o.y - pred;
+            if (!(-resid < bv)) continue;   // NaN never wins; strict: first index kept
+            bv = -resid;
+            bi = i;
+        }
+        int win = wave_argmin<double>(bv, bi);
+        double wv = bcast(bv, win & (WAVE - 1));
+        int wi = bcast_int(bi, win & (WAVE - 1));
+        double peak = -wv;
+        int pidx = (wv < INFINITY) ? wi : 0;
+        if (!(wv < INFINITY)) peak = -INFINITY;
+        double tpk = n > 0 ? g64[pidx].x : 0.0;
+#pragma unroll
+        for (int d = 0; d < 4 * (K - 1); d++) seedv[d] = (R)pv[d];
+        seedv[4 * K - 4] = (R)::log(fmax(peak, 1e-10));
+        seedv[4 * K - 3] = (R)tpk;
+        seedv[4 * K - 2] = R(1);
+        seedv[4 * K - 1] = R(1);
+        seedv[4 * K] = (R)pv[4 * (K - 1)];
+        seedv[4 * K + 1] = (R)pv[4 * (K - 1) + 1];
+#pragma unroll
+        for (int d = 0; d < NP; d++) seedv[d] = seedv[d] < lo[d] ? lo[d] : (seedv[d] > hi[d] ? hi[d] : seedv[d]);
+    }
+    const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[band];
+    Swarm<MODEL, R, PPL> s;
+    int iters = 0;
+    s.run(lds, g, n, a.tile, (uint32_t)band, a.n_particles, a.max_iters, a.stall_iters, a.seed, 0u, iters,
+          lo, hi, K > 1 ? seedv : nullptr, 0x4D420000u + ((uint32_t)K << 8));
+    double cost = (double)s.gcost;
+    if (sizeof(R) == 4 && cost >= (double)Math<R>::big()) cost = 1e99;
+    const double nn = (double)n;
+    const double bic = 2.0 * cost * nn + (double)NP * ::log(nn);
+    st.pk_cost[K - 1] = cost;
+    st.pk_bic[K - 1] = bic;
+    if (bic < st.best_bic) {
+        st.best_bic = bic;
+        st.best_cost = cost;
+        st.best_k = K;
+#pragma unroll
+        for (int d = 0; d < MB_MAX_PARAMS; d++) st.best[d] = d < NP ? (double)s.gb[d] : 0.0;
+    }
+#pragma unroll
+    for (int d = 0; d < MB_MAX_PARAMS; d++) st.prev[d] = d < NP ? (double)s.gb[d] : 0.0;
+    if (K > 1 && bic > st.pk_bic[K - 2] + 2.0) st.stopped = true;
+}
+
+template <typename R, int PPL>
+__device__ void mb_band(const MbArgs& a, typename ObsT<R>::T* lds, int band) {
+    using O = typename ObsT<R>::T;
+    const int lane = lane_id();
+    const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[band];
+    const int n = (int)(a.offsets[band + 1] - a.offsets[band]);
+    if (n <= a.tile) stage_obs<R>(lds, g, n);
+    MbState st;
+    st.best_bic = INFINITY;
+    st.best_cost = INFINITY;
+    st.best_k = 1;
+    st.stopped = false;
+#pragma unroll
+    for (int d = 0; d < MB_MAX_PARAMS; d++) { st.prev[d] = 0.0; st.best[d] = 0.0; }
+    mb_fit_k<1, R, PPL>(a, lds, band, n, st);
+    mb_fit_k<2, R, PPL>(a, lds, band, n, st);
+    mb_fit_k<3, R, PPL>(a, lds, band, n, st);
+    mb_fit_k<4, R, PPL>(a, lds, band, n, st);
+    double prm = 0.0;  // lane d holds parameter d
+#pragma unroll
+    for (int d = 0; d < MB_MAX_PARAMS; d++)
+        if (lane == d) prm = st.best[d];
+    if (lane < MB_MAX_PARAMS) a.params[(int64_t)band * MB_MAX_PARAMS + lane] = prm;
+    double pc = 0.0, pb = 0.0;
+#pragma unroll
+    for (int k = 0; k < MB_MAX_K; k++)
+        if (lane == k) { pc = st.pk_cost[k]; pb = st.pk_bic[k]; }
+    if (lane < MB_MAX_K) {
+        a.per_k_cost[(int64_t)band * MB_MAX_K + lane] = pc;
+        a.per_k_bic[(int64_t)band * MB_MAX_K + lane] = pb;
+    }
+    if (lane == 0) {
+        a.best_k[band] = st.best_k;
+        a.cost[band] = st.best_cost;
+        a.bic[band] = st.best_bic;
+    }
+}
+
+template <typename R, int PPL>
+__global__ __launch_bounds__(64) void multi_bazin_kernel(MbArgs a) {
+    extern __shared__ unsigned char lds_raw[];
+    using O = typename ObsT<R>::T;
+    O* lds = reinterpret_cast<O*>(lds_raw);
+    for (;;) {
+        const int t = wave_next_task(a.queue);
+        if (t >= a.n_tasks) break;
+        mb_band<R, PPL>(a, lds, a.order[t]);
+        wave_reconverge();
+    }
+}
+
+template <typename R>
+static const void* pick_mb_r(int ppl) {
+    if (ppl == 1) return (const void*)multi_bazin_kernel<R, 1>;
+    if (ppl == 2) return (const void*)multi_bazin_kernel<R, 2>;
+    if (ppl == 4) return (const void*)multi_bazin_kernel<R, 4>;
+    return nullptr;
 }
 
 template <int MODEL, typename R, int PPL>

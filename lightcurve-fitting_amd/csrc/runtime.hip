@@ -40,6 +40,7 @@ struct lcf_batch {
     int32_t* d_order = nullptr;   // source indices by decreasing observation count
     std::vector<int32_t> h_order;
     int max_n = 0;
+    double t_min = INFINITY, t_max = -INFINITY;  // over all observations
 };
 
 namespace {
@@ -185,6 +186,10 @@ int lcf_batch_create(lcf_ctx* c, int64_t n_src, const int64_t* off, const double
     });
     for (int64_t s = 0; s < n_src; s++)
         b->max_n = (int)std::max<int64_t>(b->max_n, off[s + 1] - off[s]);
+    for (int64_t i = 0; i < n; i++) {
+        b->t_min = std::min(b->t_min, t[i]);
+        b->t_max = std::max(b->t_max, t[i]);
+    }
     auto cleanup = [&](int code) { lcf_batch_destroy(b); return code; };
     int rc = LCF_OK;
     do {
@@ -391,6 +396,74 @@ int lcf_batch_fit_models(lcf_ctx* c, const lcf_batch* b, const int32_t* models, 
     if (best_params) HIPCHK(c, hipMemcpyAsync(best_params, d_bp.p, sizeof(double) * ns * 7, hipMemcpyDeviceToHost, c->stream));
     if (best_cost) HIPCHK(c, hipMemcpyAsync(best_cost, d_bc.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
     if (best_model) HIPCHK(c, hipMemcpyAsync(best_model, d_bm.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+int lcf_batch_multi_bazin(lcf_ctx* c, const lcf_batch* b, int P, int max_iters, int stall_iters,
+                          uint64_t seed, int32_t* out_best_k, double* out_params, double* out_cost,
+                          double* out_bic, double* out_per_k_cost, double* out_per_k_bic) {
+    if (!c) return LCF_E_INVALID;
+    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (P <= 0 || P > 4 * 64) return fail(c, LCF_E_INVALID, "n_particles must be in [1, 256], got %d", P);
+    if (max_iters <= 0) return fail(c, LCF_E_INVALID, "max_iters must be positive");
+    if (stall_iters < 0) return fail(c, LCF_E_INVALID, "stall_iters must be non-negative");
+    const int64_t ns = b->n_sources;
+    if (ns == 0) return LCF_OK;
+    if (!out_best_k || !out_params || !out_cost || !out_bic || !out_per_k_cost || !out_per_k_bic)
+        return fail(c, LCF_E_INVALID, "null output pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool fp64 = c->precision == LCF_FP64;
+    const int ppl = P <= 64 ? 1 : (P <= 128 ? 2 : 4);
+    const int rec = fp64 ? (int)sizeof(double4) : (int)sizeof(float4);
+    const int tile = std::max(1, std::min(16384 / rec, std::max(b->max_n, 1)));
+    constexpr int MP = lcf::MB_MAX_PARAMS, MK = lcf::MB_MAX_K;
+    DevBuf<int32_t> d_k;
+    DevBuf<double> d_p, d_c, d_b, d_pc, d_pb;
+    HIPCHK(c, d_k.alloc(ns));
+    HIPCHK(c, d_p.alloc(ns * MP));
+    HIPCHK(c, d_c.alloc(ns));
+    HIPCHK(c, d_b.alloc(ns));
+    HIPCHK(c, d_pc.alloc(ns * MK));
+    HIPCHK(c, d_pb.alloc(ns * MK));
+    lcf::MbArgs a{};
+    a.obs = fp64 ? (const void*)b->d_obs64 : (const void*)b->d_obs32;
+    a.obs64 = b->d_obs64;
+    a.offsets = b->d_off;
+    a.order = b->d_order;
+    a.n_tasks = (int)ns;
+    a.queue = c->d_queue;
+    // t0 bounds from the batch-wide time range (src/gpu_cuda.rs:678-679)
+    a.t_lo = b->t_min - 30.0;
+    a.t_hi = b->t_max + 30.0;
+    a.n_particles = P;
+    a.max_iters = max_iters;
+    a.stall_iters = stall_iters;
+    a.seed = seed;
+    a.tile = tile;
+    a.best_k = d_k.p;
+    a.params = d_p.p;
+    a.cost = d_c.p;
+    a.bic = d_b.p;
+    a.per_k_cost = d_pc.p;
+    a.per_k_bic = d_pb.p;
+    int per_cu = 0;
+    HIPCHK(c, lcf::multi_bazin_occupancy(fp64, ppl, tile, &per_cu));
+    per_cu = std::max(per_cu, 1);
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+    HIPCHK(c, lcf::launch_multi_bazin(a, fp64, ppl, (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, ns), c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    HIPCHK(c, hipMemcpyAsync(out_best_k, d_k.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_params, d_p.p, sizeof(double) * ns * MP, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_cost, d_c.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_bic, d_b.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_per_k_cost, d_pc.p, sizeof(double) * ns * MK, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_per_k_bic, d_pb.p, sizeof(double) * ns * MK, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LCF_OK;
 }

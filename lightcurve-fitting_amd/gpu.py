@@ -88,6 +88,18 @@ class BatchPsoResult:
     cost: float = float("inf")
 
 
+@dataclass
+class MultiBazinResult:
+    """Greedy MultiBazin fit of one source (src/parametric.rs:1269-1283)."""
+
+    best_k: int
+    params: list
+    cost: float
+    bic: float
+    per_k_cost: list
+    per_k_bic: list
+
+
 def _f64(a) -> np.ndarray:
     return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
 
@@ -231,6 +243,39 @@ class GpuContext:
         for k, m in enumerate(ALL_GPU_MODELS):
             out.append((m, [BatchPsoResult(list(prm[k, s, :m.n_params()]), float(cost[k, s]))
                             for s in range(ns)]))
+        return out
+
+    # -- greedy MultiBazin -------------------------------------------------------
+    def multi_bazin_arrays(self, data: "GpuBatchData", n_particles: int, max_iters: int, stall_iters: int,
+                           seed: int) -> dict:
+        """Array form of ``lcf_batch_multi_bazin``: best_k [n], params [n, 18],
+        cost, bic [n], per_k_cost / per_k_bic [n, 4]."""
+        ns = data.n_sources
+        out = {"best_k": np.zeros(ns, dtype=np.int32), "params": np.zeros((ns, 18)), "cost": np.zeros(ns),
+               "bic": np.zeros(ns), "per_k_cost": np.zeros((ns, 4)), "per_k_bic": np.zeros((ns, 4))}
+        self._check(self._lib.lcf_batch_multi_bazin(
+            self._h, data._h, int(n_particles), int(max_iters), int(stall_iters),
+            C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), _lib.ptr(out["best_k"], C.c_int32),
+            _lib.ptr(out["params"], C.c_double), _lib.ptr(out["cost"], C.c_double), _lib.ptr(out["bic"], C.c_double),
+            _lib.ptr(out["per_k_cost"], C.c_double), _lib.ptr(out["per_k_bic"], C.c_double)))
+        return out
+
+    def batch_pso_multi_bazin(self, data: "GpuBatchData", sources: Sequence["BatchSource"] | None,
+                              n_particles: int, max_iters: int, stall_iters: int,
+                              seed: int) -> list["MultiBazinResult"]:
+        """``GpuContext::batch_pso_multi_bazin`` (src/gpu_cuda.rs:650-963).  The
+        reference also takes the sources on the host for their time ranges
+        and residuals; here those come from ``data`` on the device, and
+        ``sources`` is only checked for length."""
+        if sources is not None and len(sources) != data.n_sources:
+            raise GpuError(f"{len(sources)} sources for a batch of {data.n_sources}")
+        r = self.multi_bazin_arrays(data, n_particles, max_iters, stall_iters, seed)
+        out = []
+        for s in range(data.n_sources):
+            k = int(r["best_k"][s])
+            prm = list(r["params"][s, :4 * k + 2]) if np.isfinite(r["bic"][s]) or r["bic"][s] < 0 else []
+            out.append(MultiBazinResult(k, prm, float(r["cost"][s]), float(r["bic"][s]),
+                                        list(r["per_k_cost"][s]), list(r["per_k_bic"][s])))
         return out
 
     # -- batched SVI -------------------------------------------------------------
