@@ -100,6 +100,19 @@ int orc_batch_model_select(int64_t n_src, const int64_t* offsets,
                            int32_t* out_model, double* out_params,
                            double* out_cost, int n_threads);
 
+/* GpuContext::batch_pso_multi_bazin (src/gpu_cuda.rs:650-963), Philox
+ * stream; outputs per source best_k, params[18] (zero padded), cost, bic,
+ * per_k_cost[4], per_k_bic[4] (NaN after the early stop). */
+/* batch_pso_cost_multi_bazin (cuda/models.cu:319-367) for one parameter
+ * vector with k components. */
+double orc_multi_bazin_cost(const double* p, int k, const double* t, const double* f, const double* v,
+                            const int32_t* up, const double* uf, int64_t n);
+int orc_batch_multi_bazin(int64_t n_src, const int64_t* offsets, const double* t, const double* f,
+                          const double* v, const int32_t* up, const double* uf, int n_particles,
+                          int max_iters, int stall_iters, uint64_t seed, int32_t* best_k,
+                          double* params, double* cost, double* bic, double* per_k_cost,
+                          double* per_k_bic, int n_threads);
+
 /* ---- SVI (GPU semantics, cuda/svi.cu:48-330) ------------------------------ */
 #define ORC_SVI_RNG_PHILOX 0     /* stream of the HIP kernels */
 #define ORC_SVI_RNG_REFERENCE 1  /* reference xorshift64, seeded 42 + job*123456789 */

@@ -385,3 +385,34 @@ def batch_gp2d_fit(csr: dict, qt, qw, amps, lst, lsw, cap: int, subsample_rule: 
                               n_threads or default_threads())
     assert rc == 0
     return {"state": state, "pred": pred, "std": std, "dpred": dpred, "pred_obs": pobs}
+
+
+def batch_multi_bazin(csr: dict, n_particles: int, max_iters: int, stall_iters: int, seed: int,
+                      n_threads: int | None = None) -> dict:
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, f, v, up, uf = _obs(csr)
+    out = {"best_k": np.zeros(ns, dtype=np.int32), "params": np.zeros((ns, 18)), "cost": np.zeros(ns),
+           "bic": np.zeros(ns), "per_k_cost": np.zeros((ns, 4)), "per_k_bic": np.zeros((ns, 4))}
+    L = lib()
+    L.orc_batch_multi_bazin.restype = _i32
+    L.orc_batch_multi_bazin.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _i32, _i32, _i32, _u64, _pi32,
+                                        _pd, _pd, _pd, _pd, _pd, _i32]
+    rc = L.orc_batch_multi_bazin(ns, _p(off, C.c_int64), _p(t, C.c_double), _p(f, C.c_double), _p(v, C.c_double),
+                                 _p(up, C.c_int32), _p(uf, C.c_double), n_particles, max_iters, stall_iters,
+                                 C.c_uint64(seed), _p(out["best_k"], C.c_int32), _p(out["params"], C.c_double),
+                                 _p(out["cost"], C.c_double), _p(out["bic"], C.c_double),
+                                 _p(out["per_k_cost"], C.c_double), _p(out["per_k_bic"], C.c_double),
+                                 n_threads or default_threads())
+    assert rc == 0
+    return out
+
+
+def multi_bazin_cost(p, k: int, t, f, v, up, uf) -> float:
+    L = lib()
+    L.orc_multi_bazin_cost.restype = _f64
+    L.orc_multi_bazin_cost.argtypes = [_pd, _i32, _pd, _pd, _pd, _pi32, _pd, _i64]
+    p, t, f, v, uf = f64(p), f64(t), f64(f), f64(v), f64(uf)
+    up = i32(up)
+    return L.orc_multi_bazin_cost(_p(p, C.c_double), k, _p(t, C.c_double), _p(f, C.c_double), _p(v, C.c_double),
+                                  _p(up, C.c_int32), _p(uf, C.c_double), t.size)

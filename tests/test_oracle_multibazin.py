@@ -1,0 +1,114 @@
+"""Pins the oracle's greedy MultiBazin (CPU only).
+
+  * cost: K = 1 equals the Bazin GPU cost without its population prior
+    (parameters permuted); a zero-amplitude extra component changes nothing;
+    a non-finite prediction and an empty band give 1e99;
+  * selection: per-K BIC = 2 cost n + (4K + 2) ln n; best K is the first
+    minimum over the K tried; a K whose BIC exceeds the previous one by more
+    than 2 ends the search (later entries NaN);
+  * bounds: every returned parameter inside its box, t0 within the batch
+    time range -/+ 30;
+  * behaviour: two well separated bursts are fitted with at least two
+    components, one near each burst.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def bands(pkg, seed, n_src=12):
+    return pkg.synthetic.ragged_selection_batch(n_src, seed=seed, n_min=8, n_max=150)
+
+
+def test_cost_k1_is_bazin_without_prior(orc, pkg):
+    csr = bands(pkg, 3)
+    off = csr["offsets"]
+    rng = np.random.default_rng(0)
+    lo, hi = orc.bounds(0)
+    for b in range(off.size - 1):
+        a, e = off[b], off[b + 1]
+        obs = [csr[k][a:e] for k in ("times", "flux", "obs_var", "is_upper", "upper_flux")]
+        for _ in range(5):
+            p = lo + rng.uniform(size=6) * (hi - lo)          # Bazin [lnA, B, t0, lntr, lntf, lnsx]
+            mb = np.array([p[0], p[2], p[3], p[4], p[1], p[5]])
+            c, w = orc.population_priors(0)
+            prior = 0.5 * np.sum(((p - c) / w) ** 2)
+            n = max(e - a, 1)
+            want = orc.pso_cost(0, p, *obs) - prior / n ** 2
+            got = orc.multi_bazin_cost(mb, 1, *obs)
+            if math.isinf(want):       # the sum overflowed: +inf in both
+                assert got == math.inf
+            else:                      # includes the 1e99 of a non-finite prediction
+                assert got == pytest.approx(want, rel=1e-12, abs=1e-12)
+            mb2 = np.concatenate([mb[:4], [-800.0, 50.0, 0.0, 0.0], mb[4:]])  # exp(-800) = 0 exactly
+            assert orc.multi_bazin_cost(mb2, 2, *obs) == got
+
+
+def test_cost_special_cases(orc):
+    t = np.array([0.0, 1.0, 2.0])
+    f, v = np.ones(3), np.full(3, 0.01)
+    up, uf = np.zeros(3, np.int32), np.zeros(3)
+    assert orc.multi_bazin_cost(np.zeros(6), 1, t[:0], f[:0], v[:0], up[:0], uf[:0]) == 1e99
+    # tau_fall = e^-2, t0 far after the data: exp(-dt/tf) overflows, the logistic is 0 -> NaN
+    p = np.array([0.0, 500.0, -2.0, -2.0, 0.0, -3.0])
+    assert orc.multi_bazin_cost(p, 1, t, f, v, up, uf) == 1e99
+
+
+@pytest.fixture(scope="module")
+def fit(orc, pkg):
+    csr = bands(pkg, 11, n_src=15)
+    return csr, orc.batch_multi_bazin(csr, 24, 30, 30, 5)
+
+
+def test_bic_and_selection_rules(fit):
+    csr, r = fit
+    n = np.diff(csr["offsets"]).astype(float)
+    for s in range(n.size):
+        pk_c, pk_b = r["per_k_cost"][s], r["per_k_bic"][s]
+        kk = np.arange(1, 5)
+        live = ~np.isnan(pk_b)
+        assert live[0]
+        # NaN only as a suffix, right after the stopping K
+        last = np.nonzero(live)[0].max()
+        assert np.all(live[:last + 1]) and not np.any(live[last + 1:])
+        with np.errstate(invalid="ignore", divide="ignore"):
+            want = 2 * pk_c[live] * n[s] + (4 * kk[live] + 2) * math.log(n[s]) if n[s] > 0 else None
+        if n[s] > 0:
+            assert np.allclose(pk_b[live], want, rtol=1e-14, atol=0)
+        for k in range(1, last):
+            assert not pk_b[k] > pk_b[k - 1] + 2.0  # no stop before the last K tried
+        if last < 3:
+            assert pk_b[last] > pk_b[last - 1] + 2.0
+        best = int(np.argmin(pk_b[live]))  # first minimum
+        assert r["best_k"][s] == best + 1 and r["bic"][s] == pk_b[best] and r["cost"][s] == pk_c[best]
+
+
+def test_params_inside_bounds(fit):
+    csr, r = fit
+    t_lo, t_hi = csr["times"].min() - 30.0, csr["times"].max() + 30.0
+    for s in range(r["best_k"].size):
+        k = int(r["best_k"][s])
+        p = r["params"][s]
+        assert np.all(p[4 * k + 2:] == 0.0)
+        for c in range(k):
+            la, t0, ltr, ltf = p[4 * c:4 * c + 4]
+            assert -3 <= la <= 3 and t_lo <= t0 <= t_hi and -2 <= ltr <= 5 and -2 <= ltf <= 6
+        assert -0.3 <= p[4 * k] <= 0.3 and -5 <= p[4 * k + 1] <= 0
+
+
+def test_two_bursts_need_two_components(orc):
+    t = np.linspace(0.0, 200.0, 120)
+
+    def burst(a, t0, tr, tf):
+        dt = t - t0
+        return a * np.exp(-dt / tf) / (1.0 + np.exp(-dt / tr))
+
+    flux = burst(1.0, 40.0, 2.0, 10.0) + burst(0.8, 140.0, 2.0, 10.0)
+    csr = {"offsets": np.array([0, t.size], np.int64), "times": t, "flux": flux,
+           "obs_var": np.full(t.size, 1e-4), "is_upper": np.zeros(t.size, np.int32), "upper_flux": np.zeros(t.size)}
+    r = orc.batch_multi_bazin(csr, 64, 300, 300, 1)
+    k = int(r["best_k"][0])
+    assert k >= 2, r["per_k_bic"][0]
+    t0s = np.sort(r["params"][0][1:4 * k:4])
+    assert np.any(np.abs(t0s - 40.0) < 8.0) and np.any(np.abs(t0s - 140.0) < 8.0), t0s
