@@ -15,7 +15,11 @@ Also reported on the same JSON line:
   * cpu_baseline: the oracle (FP64 C++ restatement, all host threads up to 16)
     on a bounded sample of the same workload, in sources/second;
   * plain: the same fit launched once per power-of-two size class instead of
-    one persistent size-sorted launch per model.
+    one persistent size-sorted launch per model;
+  * svi: the SVI workload (10k Villar sources x 3 bands x 60 points, 16
+    samples, up to 2000 Adam steps, FP32) fitted by the batched adjoint kernel
+    and by the plain form (forward-difference gradients, one launch per size
+    class), with the agreement of the two posteriors.
 """
 from __future__ import annotations
 
@@ -72,6 +76,63 @@ def cpu_baseline(pkg, n_sources: int, particles: int, iters: int, threads: int) 
                       f"oracle FP64 C++, {threads} threads, {dt:.1f} s"}
 
 
+def svi_leg(pkg, n_sources: int, n_steps: int, n_samples: int, lr: float, reps: int) -> dict:
+    """Villar SVI workload: n_sources x 3 bands x 60 points, PSO start points
+    (untimed), then the SVI fit timed twice over: the batched adjoint form
+    (analytic forward + adjoint in one kernel, one persistent launch) and the
+    plain form (forward-difference gradients, one launch per power-of-two size
+    class).  Agreement of the two posteriors is reported beside the times."""
+    csr = pkg.synthetic.villar_svi_batch(n_sources, seed=77)
+    nj = csr["offsets"].size - 1
+    ctx = pkg.GpuContext(0, precision=pkg.FP32, schedule=pkg.SCHED_PERSISTENT)
+    data = pkg.GpuBatchData(ctx, csr=csr)
+    fit = ctx.fit_models(data, [1], 64, 200, 200, seed=42)
+    pso = fit["model_params"][0]
+    mids = np.full(nj, 1, dtype=np.int32)
+    pc, pw = np.zeros((nj, 7)), np.zeros((nj, 7))
+    for j in range(nj):
+        c, w = pkg.svi_prior_for_model(1, pso[j, :7])
+        pc[j], pw[j] = c, w
+    log(f"svi: {nj} Villar bands, PSO start points ready")
+
+    def run(flags, schedule):
+        ctx.set_schedule(schedule)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        out = ctx.svi_fit(data, mids, pso, pc, pw, n_steps, n_samples, lr, seed=42, flags=flags)
+        ctx.synchronize()
+        return out, time.perf_counter() - t0, ctx.last_kernel_ms
+
+    run(0, pkg.SCHED_PERSISTENT)  # warm-up
+    best = None
+    for _ in range(reps):
+        r = run(0, pkg.SCHED_PERSISTENT)
+        best = r if best is None or r[1] < best[1] else best
+    adj, dt_adj, kms_adj = best
+    log(f"svi adjoint: {dt_adj * 1e3:.1f} ms")
+    plain, dt_plain, kms_plain = run(4, pkg.SCHED_PER_CLASS)
+    ctx.set_schedule(pkg.SCHED_PERSISTENT)
+    log(f"svi plain: {dt_plain * 1e3:.1f} ms")
+    sig = np.exp(plain["log_sigma"][:, :7])
+    dmu = np.abs(adj["mu"][:, :7] - plain["mu"][:, :7]) / sig
+    dsg = np.abs(np.exp(adj["log_sigma"][:, :7]) - sig) / sig
+    evals = float(adj["steps"].astype(np.float64).sum() * n_samples * 60)
+    return {"workload": "villar_svi", "sources": n_sources, "bands": nj, "points_per_band": 60,
+            "samples": n_samples, "max_steps": n_steps, "lr": lr, "dtype": "fp32",
+            "value": round(n_sources / dt_adj, 2), "unit": "sources/s", "ms": round(dt_adj * 1e3, 3),
+            "kernel_ms": round(kms_adj, 3),
+            "plain": {"value": round(n_sources / dt_plain, 2), "ms": round(dt_plain * 1e3, 3),
+                      "kernel_ms": round(kms_plain, 3),
+                      "form": "forward-difference gradients, one launch per power-of-two size class"},
+            "speedup_vs_plain": round(dt_plain / dt_adj, 3),
+            "agreement": {"mu_q50_over_sigma": float(np.quantile(dmu, 0.5)),
+                          "mu_q95_over_sigma": float(np.quantile(dmu, 0.95)),
+                          "sigma_q95_rel": float(np.quantile(dsg, 0.95))},
+            "steps_mean": float(adj["steps"].mean()),
+            "sample_obs_step_evals": evals,
+            "evals_per_s": round(evals / (kms_adj * 1e-3), 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +144,9 @@ def main() -> int:
     ap.add_argument("--cpu-sources", type=int, default=192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-svi", action="store_true")
+    ap.add_argument("--svi-sources", type=int, default=10_000)
+    ap.add_argument("--svi-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -154,6 +218,10 @@ def main() -> int:
         log(f"cpu baseline: {args.cpu_sources} sources on {threads} threads")
         cpu = cpu_baseline(pkg, args.cpu_sources, args.particles, args.iters, threads)
 
+    svi = None
+    if not args.no_svi:
+        svi = svi_leg(pkg, args.svi_sources, args.svi_steps, 16, 0.01, reps=max(args.steps, 1))
+
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -177,6 +245,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "plain": plain,
         "per_model_ms": {MODEL_NAMES[m]: round(float(slot_ms[i]), 3) for i, m in enumerate(MODELS)},
+        "svi": svi,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -196,7 +196,7 @@ int lcf_batch_gp_fit(lcf_ctx* ctx, int64_t n_bands, const int64_t* offsets, cons
                      int n_query, const double* amp_candidates, int n_amp,
                      const double* ls_candidates, int n_ls, int max_subsample, double* out_state,
                      double* out_pred, double* out_std, double* out_pred_obs);
-/* General form: training-set cap up to 128 (LDS-resident factorisation),
+/* General form: training-set cap up to 200 (LDS-resident factorisation),
  * subsample rule (0: cuda/gp.cu:189-191, 1: src/gp.rs:15-19), criterion
  * (0: NLML, 1: train RMS, src/nonparametric.rs:1318-1335), query grid shared
  * (query_per_band = 0, query[n_query]) or per band (query[n_bands][n_query]),
@@ -222,7 +222,7 @@ int lcf_batch_gp2d_fit(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, 
                        const double* amp_candidates, int n_amp, const double* lst_candidates,
                        int n_lst, const double* lsw_candidates, int n_lsw, int max_subsample,
                        double* out_state, double* out_pred, double* out_std, double* out_train_rms);
-/* General 2D form, as lcf_batch_gp_fit_ex: cap up to 128, state rows of
+/* General 2D form, as lcf_batch_gp_fit_ex: cap up to 200, state rows of
  * 3*cap+5 doubles (alpha[cap], t[cap], w[cap], amp, 1/(2 lt^2), 1/(2 lw^2),
  * mean, m), per-source query grids, d(mean)/dt, mean at every observation. */
 int lcf_batch_gp2d_fit_ex(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,

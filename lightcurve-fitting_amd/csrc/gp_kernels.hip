@@ -4,14 +4,17 @@
 // One 256-thread workgroup owns one band (1D) or source (2D) at a time
 // (persistent, pulled from a device queue in decreasing training-set size).
 // For every hyper-parameter candidate it builds K + diag(noise) as a packed
-// lower triangle in LDS (m <= 128: 66 KB), factorises it in place with a
+// lower triangle in LDS (m <= 200: 157 KB), factorises it in place with a
 // right-looking Cholesky whose column scaling and trailing updates are spread
 // over all 256 threads, and solves for alpha on wavefront 0 with the right-hand
-// side held in registers (lane l owns rows l and l+64), so the triangular
-// solves need no barriers.  Candidates are scored by the exact negative log
-// marginal likelihood or by the train RMS; the winner is refitted and
-// predicted (mean, time derivative of the mean, standard deviation) at the
-// query grid and at every observation.
+// side held in registers (lane l owns rows l, l+64, l+128, l+192), so the
+// triangular solves need no barriers.  The training set is addressed through
+// an index list in LDS into the band's observations (L1/L2 resident), which
+// leaves the whole LDS budget to the factor.  Candidates are scored by the
+// exact negative log marginal likelihood or by the train RMS; the winner is
+// refitted and predicted at the query grid (mean and time derivative per
+// thread; standard deviation by a wave-level triangular solve per query) and
+// at every observation.
 //
 // Arithmetic follows DenseGP (src/sparse_gp.rs:16-176), batch_gp_fit
 // (cuda/gp.cu:23-367) and batch_gp2d_fit_predict (cuda/gp2d.cu:21-359): the
@@ -23,30 +26,36 @@
 namespace lcf {
 
 constexpr int GP_THREADS = 256;
+constexpr int GP_WAVES = GP_THREADS / WAVE;
+constexpr int GP_ROWS = 4;  // rows per lane in the wave-level solves (m <= 256)
 
 __device__ __forceinline__ int tri(int i, int j) { return (i * (i + 1) >> 1) + j; }
 
 struct GpShared {
     double* L;      // packed lower triangle, cap*(cap+1)/2
-    double* x;      // training times [cap]
-    double* w;      // training log wavelengths [cap] (2D)
-    double* y;      // training values [cap]
-    double* nv;     // training noise variances [cap]
     double* alpha;  // [cap]
     double* misc;   // scalars
+    int* idx;       // [cap] training-set indices into the band
 };
 
 __device__ __forceinline__ GpShared gp_carve(double* base, int cap) {
     GpShared s;
     s.L = base;
-    s.x = s.L + cap * (cap + 1) / 2;
-    s.w = s.x + cap;
-    s.y = s.w + cap;
-    s.nv = s.y + cap;
-    s.alpha = s.nv + cap;
+    s.alpha = s.L + cap * (cap + 1) / 2;
     s.misc = s.alpha + cap;
+    s.idx = reinterpret_cast<int*>(s.misc + 8);
     return s;
 }
+
+// The band's training set, read through the subsample indices.
+struct Train {
+    const double *t, *w, *y, *nv;
+    const int* idx;
+    __device__ __forceinline__ double x(int i) const { return t[idx[i]]; }
+    __device__ __forceinline__ double wl(int i) const { return w ? w[idx[i]] : 0.0; }
+    __device__ __forceinline__ double yy(int i) const { return y[idx[i]]; }
+    __device__ __forceinline__ double v(int i) const { return nv[idx[i]]; }
+};
 
 struct Hyp {
     double amp, it, iw;  // amplitude, 1/(2 l_t^2), 1/(2 l_w^2) (0 in 1D)
@@ -56,16 +65,23 @@ struct Hyp {
     }
 };
 
+// Row k's value from rows held as (lane + 64 r): wave-uniform k.
+__device__ __forceinline__ double row_of(const double (&v)[GP_ROWS], int k) {
+    const int r = k >> 6;
+    double c = r == 0 ? v[0] : (r == 1 ? v[1] : (r == 2 ? v[2] : v[3]));
+    return bcast(c, k & (WAVE - 1));
+}
+
 // Build K + diag(max(nv, 1e-10)) and factorise in place.  Returns false
 // (uniformly) when a pivot is not positive.
-__device__ bool gp_factor(const GpShared& s, int m, const Hyp& h) {
+__device__ bool gp_factor(const GpShared& s, const Train& tr, int m, const Hyp& h) {
     const int tid = threadIdx.x;
     const int mm = m * m;
     for (int e = tid; e < mm; e += GP_THREADS) {
         int i = e / m, j = e - i * m;
         if (j <= i) {
-            double v = h.k(s.x[i], s.w[i], s.x[j], s.w[j]);
-            if (i == j) v += fmax(s.nv[i], 1e-10);
+            double v = h.k(tr.x(i), tr.wl(i), tr.x(j), tr.wl(j));
+            if (i == j) v += fmax(tr.v(i), 1e-10);
             s.L[tri(i, j)] = v;
         }
     }
@@ -93,53 +109,67 @@ __device__ bool gp_factor(const GpShared& s, int m, const Hyp& h) {
 }
 
 // Wavefront 0: alpha = L^-T L^-1 (y - ybar); returns 0.5 yc.alpha + sum ln L_ii
-// (wave-uniform).  Lane l owns rows l and l + 64 in registers.
-__device__ double gp_solve_alpha(const GpShared& s, int m, double ybar) {
+// (wave-uniform).  Lane l owns rows l + 64 r in registers.
+__device__ double gp_solve_alpha(const GpShared& s, const Train& tr, int m, double ybar) {
     const int lane = lane_id();
-    double b0 = lane < m ? s.y[lane] - ybar : 0.0;
-    double b1 = lane + 64 < m ? s.y[lane + 64] - ybar : 0.0;
-    const double yc0 = b0, yc1 = b1;
+    double b[GP_ROWS], yc[GP_ROWS];
+#pragma unroll
+    for (int r = 0; r < GP_ROWS; r++) {
+        int i = lane + WAVE * r;
+        b[r] = i < m ? tr.yy(i) - ybar : 0.0;
+        yc[r] = b[r];
+    }
     for (int j = 0; j < m; j++) {  // forward: L z = yc
-        double bj = j < 64 ? bcast(b0, j) : bcast(b1, j - 64);
-        double zj = bj / s.L[tri(j, j)];
-        if (lane == (j & 63)) { if (j < 64) b0 = zj; else b1 = zj; }
-        if (lane > j && lane < m) b0 -= s.L[tri(lane, j)] * zj;
-        if (lane + 64 > j && lane + 64 < m) b1 -= s.L[tri(lane + 64, j)] * zj;
+        double zj = row_of(b, j) / s.L[tri(j, j)];
+#pragma unroll
+        for (int r = 0; r < GP_ROWS; r++) {
+            int i = lane + WAVE * r;
+            if (i == j) b[r] = zj;
+            else if (i > j && i < m) b[r] -= s.L[tri(i, j)] * zj;
+        }
     }
     for (int j = m - 1; j >= 0; j--) {  // backward: L^T a = z
-        double bj = j < 64 ? bcast(b0, j) : bcast(b1, j - 64);
-        double aj = bj / s.L[tri(j, j)];
-        if (lane == (j & 63)) { if (j < 64) b0 = aj; else b1 = aj; }
-        if (lane < j) b0 -= s.L[tri(j, lane)] * aj;
-        if (lane + 64 < j) b1 -= s.L[tri(j, lane + 64)] * aj;
+        double aj = row_of(b, j) / s.L[tri(j, j)];
+#pragma unroll
+        for (int r = 0; r < GP_ROWS; r++) {
+            int i = lane + WAVE * r;
+            if (i == j) b[r] = aj;
+            else if (i < j) b[r] -= s.L[tri(j, i)] * aj;
+        }
     }
-    if (lane < m) s.alpha[lane] = b0;
-    if (lane + 64 < m) s.alpha[lane + 64] = b1;
-    double part = (lane < m ? yc0 * b0 + ::log(s.L[tri(lane, lane)]) * 2.0 : 0.0) +
-                  (lane + 64 < m ? yc1 * b1 + ::log(s.L[tri(lane + 64, lane + 64)]) * 2.0 : 0.0);
+    double part = 0.0;
+#pragma unroll
+    for (int r = 0; r < GP_ROWS; r++) {
+        int i = lane + WAVE * r;
+        if (i < m) {
+            s.alpha[i] = b[r];
+            part += yc[r] * b[r] + ::log(s.L[tri(i, i)]) * 2.0;
+        }
+    }
     return 0.5 * wave_sum(part);
 }
 
 // Wavefront 0: RMS of the predictive mean at the training points.
-__device__ double gp_train_rms(const GpShared& s, int m, const Hyp& h, double ybar) {
+__device__ double gp_train_rms(const GpShared& s, const Train& tr, int m, const Hyp& h, double ybar) {
     const int lane = lane_id();
     double rss = 0.0;
-    for (int i = lane; i < m; i += 64) {
+    for (int i = lane; i < m; i += WAVE) {
+        const double xi = tr.x(i), wi = tr.wl(i);
         double mu = 0.0;
-        for (int k = 0; k < m; k++) mu += h.k(s.x[i], s.w[i], s.x[k], s.w[k]) * s.alpha[k];
-        double r = mu + ybar - s.y[i];
+        for (int k = 0; k < m; k++) mu += h.k(xi, wi, tr.x(k), tr.wl(k)) * s.alpha[k];
+        double r = mu + ybar - tr.yy(i);
         rss += r * r;
     }
     return ::sqrt(wave_sum(rss) / (double)m);
 }
 
-__device__ __forceinline__ Hyp candidate(const GpArgs& a, int c, bool& valid) {
+__device__ __forceinline__ Hyp candidate(const GpArgs& a, int band, int c, bool& valid) {
     const int nw = a.n_lsw > 0 ? a.n_lsw : 1;
     const int ia = c / (a.n_ls * nw), il = (c / nw) % a.n_ls, iw = c % nw;
     Hyp h;
     h.amp = a.amps[ia];
-    const double ls = a.lss[il];
-    valid = ls >= 0.1;
+    const double ls = a.lss_band ? a.lss_band[(int64_t)band * a.n_ls + il] : a.lss[il];
+    valid = ls >= 0.1 && (!a.min_ls || ls >= a.min_ls[band]);
     h.it = 0.5 / (ls * ls);
     h.iw = 0.0;
     if (a.w) {
@@ -176,17 +206,14 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
             double step = (double)(n - 1) / (double)(m - 1);
             idx = (int)(i * step + 0.5);
         } else {
-            idx = (int)(((double)i + 0.5) * n / m);
+            idx = (int)(((double)i + 0.5) * ((double)n / (double)m));
         }
-        idx = idx < n - 1 ? idx : n - 1;
-        s.x[i] = a.t[o0 + idx];
-        s.w[i] = two_d ? a.w[o0 + idx] : 0.0;
-        s.y[i] = a.y[o0 + idx];
-        s.nv[i] = a.nv[o0 + idx];
+        s.idx[i] = idx < n - 1 ? idx : n - 1;
     }
     __syncthreads();
+    Train tr{a.t + o0, two_d ? a.w + o0 : nullptr, a.y + o0, a.nv + o0, s.idx};
     double ybar = 0.0;  // all threads, same summation order
-    for (int i = 0; i < m; i++) ybar += s.y[i];
+    for (int i = 0; i < m; i++) ybar += tr.yy(i);
     ybar /= (double)m;
 
     double best = 1e99;
@@ -195,15 +222,15 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
     const int nc = a.n_amp * a.n_ls * (a.n_lsw > 0 ? a.n_lsw : 1);
     for (int c = 0; c < nc; c++) {
         bool valid;
-        const Hyp h = candidate(a, c, valid);
+        const Hyp h = candidate(a, band, c, valid);
         double score = 1e99;
         if (valid) {
-            if (gp_factor(s, m, h)) {
-                if (tid < 64) {
-                    double half_fit_logdet = gp_solve_alpha(s, m, ybar);
+            if (gp_factor(s, tr, m, h)) {
+                if (tid < WAVE) {
+                    double half_fit_logdet = gp_solve_alpha(s, tr, m, ybar);
                     double sc;
                     if (a.criterion == 0) sc = half_fit_logdet + 0.5 * (double)m * 1.83787706640934548356;
-                    else sc = gp_train_rms(s, m, h, ybar);
+                    else sc = gp_train_rms(s, tr, m, h, ybar);
                     if (tid == 0) s.misc[0] = __builtin_isfinite(sc) ? sc : 1e99;
                 }
                 __syncthreads();
@@ -215,17 +242,19 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
         if (score < best) { best = score; best_c = c; any = true; }
     }
     if (!any) return;
+    // the CPU dense path keeps no fit when no candidate has a finite score
+    if (a.criterion == 1 && !(best < 1e99)) return;
     bool valid;
-    const Hyp h = candidate(a, best_c, valid);
-    if (!gp_factor(s, m, h)) return;
-    if (tid < 64) gp_solve_alpha(s, m, ybar);
+    const Hyp h = candidate(a, band, best_c, valid);
+    if (!gp_factor(s, tr, m, h)) return;
+    if (tid < WAVE) gp_solve_alpha(s, tr, m, ybar);
     __syncthreads();
     // state 1D: alpha[cap], x[cap], amp, 1/(2 lt^2), ybar, m
     //       2D: alpha[cap], x[cap], w[cap], amp, 1/(2 lt^2), 1/(2 lw^2), ybar, m
     for (int i = tid; i < m; i += GP_THREADS) {
         st[i] = s.alpha[i];
-        st[cap + i] = s.x[i];
-        if (two_d) st[2 * cap + i] = s.w[i];
+        st[cap + i] = tr.x(i);
+        if (two_d) st[2 * cap + i] = tr.wl(i);
     }
     if (tid == 0) {
         int o = two_d ? 3 * cap : 2 * cap;
@@ -235,33 +264,49 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
         st[o++] = ybar;
         st[o] = (double)m;
     }
-    // predictive mean, time derivative and std at the query grid
+    // predictive mean and its time derivative at the query grid (one query per thread)
     for (int k = tid; k < a.n_query; k += GP_THREADS) {
         const double t = q[k], wq = qw ? qw[k] : 0.0;
         double mu = 0.0, dmu = 0.0;
         for (int i = 0; i < m; i++) {
-            double kv = h.k(t, wq, s.x[i], s.w[i]);
+            const double xi = tr.x(i);
+            double kv = h.k(t, wq, xi, tr.wl(i));
             mu += kv * s.alpha[i];
-            dmu += -2.0 * h.it * (t - s.x[i]) * kv * s.alpha[i];
-        }
-        double vtv = 0.0;  // v = L^-1 k*, var = amp - |v|^2 (floored at 1e-10)
-        double vloc[GP_LDS_CAP];
-        for (int i = 0; i < m; i++) {
-            double si = h.k(t, wq, s.x[i], s.w[i]);
-            for (int j = 0; j < i; j++) si -= s.L[tri(i, j)] * vloc[j];
-            double vi = si / s.L[tri(i, i)];
-            vloc[i] = vi;
-            vtv += vi * vi;
+            dmu += -2.0 * h.it * (t - xi) * kv * s.alpha[i];
         }
         const int64_t o = (int64_t)band * a.n_query + k;
         a.pred[o] = mu + ybar;
-        a.stdv[o] = ::sqrt(fmax(h.amp - vtv, 1e-10));
         if (a.dpred) a.dpred[o] = dmu;
+    }
+    // predictive std: v = L^-1 k*, var = amp - |v|^2 floored at 1e-10; one
+    // query per wavefront, lanes holding the rows of v
+    {
+        const int wave = tid / WAVE, lane = tid - wave * WAVE;
+        for (int k = wave; k < a.n_query; k += GP_WAVES) {
+            const double t = q[k], wq = qw ? qw[k] : 0.0;
+            double sv[GP_ROWS];
+#pragma unroll
+            for (int r = 0; r < GP_ROWS; r++) {
+                int i = lane + WAVE * r;
+                sv[r] = i < m ? h.k(t, wq, tr.x(i), tr.wl(i)) : 0.0;
+            }
+            double vtv = 0.0;
+            for (int j = 0; j < m; j++) {
+                double vj = row_of(sv, j) / s.L[tri(j, j)];
+                vtv += vj * vj;
+#pragma unroll
+                for (int r = 0; r < GP_ROWS; r++) {
+                    int i = lane + WAVE * r;
+                    if (i > j && i < m) sv[r] -= s.L[tri(i, j)] * vj;
+                }
+            }
+            if (lane == 0) a.stdv[(int64_t)band * a.n_query + k] = ::sqrt(fmax(h.amp - vtv, 1e-10));
+        }
     }
     for (int i = tid; i < n; i += GP_THREADS) {  // predictive mean at every observation
         const double t = a.t[o0 + i], wi = two_d ? a.w[o0 + i] : 0.0;
         double mu = 0.0;
-        for (int k = 0; k < m; k++) mu += h.k(t, wi, s.x[k], s.w[k]) * s.alpha[k];
+        for (int k = 0; k < m; k++) mu += h.k(t, wi, tr.x(k), tr.wl(k)) * s.alpha[k];
         a.pred_obs[o0 + i] = mu + ybar;
     }
 }
@@ -281,7 +326,7 @@ __global__ __launch_bounds__(GP_THREADS) void gp_fit_kernel(GpArgs a) {
 }
 
 size_t gp_lds_bytes(int cap) {
-    return sizeof(double) * ((size_t)cap * (cap + 1) / 2 + 5 * (size_t)cap + 8);
+    return sizeof(double) * ((size_t)cap * (cap + 1) / 2 + (size_t)cap + 8) + sizeof(int) * (size_t)cap;
 }
 
 hipError_t launch_gp_fit(const GpArgs& a, int grid, hipStream_t st) {

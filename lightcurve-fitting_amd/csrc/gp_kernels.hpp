@@ -6,7 +6,10 @@
 
 namespace lcf {
 
-constexpr int GP_LDS_CAP = 128;  // largest training set factorised in LDS
+// Largest training set factorised in LDS: the packed FP64 triangle of a
+// 200-point kernel matrix plus alpha and the training indices fill the 160 KB
+// a gfx950 workgroup can address.
+constexpr int GP_LDS_CAP = 200;
 
 struct GpArgs {
     const double* t;           // [n_obs] observation times
@@ -25,11 +28,13 @@ struct GpArgs {
     int query_per_band;
     const double* amps;
     int n_amp;
-    const double* lss;
+    const double* lss;         // [n_ls] shared length-scale candidates, or
+    const double* lss_band;    // [n_bands][n_ls] per-band candidates (null: use lss)
+    const double* min_ls;      // [n_bands] smallest admissible length scale, or null
     int n_ls;
     int cap;                   // training-set cap (subsample size)
     int subsample_rule;        // 0: GPU rule, 1: CPU rule
-    int criterion;             // 0: NLML, 1: train RMS
+    int criterion;             // 0: NLML (GPU path), 1: train RMS (CPU dense path)
     double* state;             // [n_bands][state_stride]
     int state_stride;          // >= 2*cap + 4
     double* pred;              // [n_bands][n_query]

@@ -658,7 +658,8 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
            const double* y, const double* nv, const double* query, const double* query_w, int n_query,
            int query_per_band, const double* amps, int n_amp, const double* lss, int n_ls,
            const double* lsws, int n_lsw, int cap, int subsample_rule, int criterion, double* out_state,
-           double* out_pred, double* out_std, double* out_dpred, double* out_pred_obs) {
+           double* out_pred, double* out_std, double* out_dpred, double* out_pred_obs,
+           const double* lss_band = nullptr, const double* min_ls = nullptr) {
     if (!c) return LCF_E_INVALID;
     if (w && (n_lsw <= 0 || !lsws || (n_query > 0 && !query_w)))
         return fail(c, LCF_E_INVALID, "2D fit needs wavelengths, wavelength length scales and query wavelengths");
@@ -666,7 +667,15 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
     if (n_bands == 0) return LCF_OK;
     if (cap < 3 || cap > lcf::GP_LDS_CAP)
         return fail(c, LCF_E_INVALID, "training-set cap must be in [3, %d], got %d", lcf::GP_LDS_CAP, cap);
-    if (n_amp <= 0 || n_ls <= 0 || !amps || !lss) return fail(c, LCF_E_INVALID, "empty hyperparameter grid");
+    if (n_amp <= 0 || n_ls <= 0 || !amps || (!lss && !lss_band))
+        return fail(c, LCF_E_INVALID, "empty hyperparameter grid");
+    {
+        int lds_max = 0;
+        HIPCHK(c, hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device));
+        if (lcf::gp_lds_bytes(cap) > (size_t)lds_max)
+            return fail(c, LCF_E_INVALID, "training-set cap %d needs %zu bytes of LDS, the device allows %d", cap,
+                        lcf::gp_lds_bytes(cap), lds_max);
+    }
     if (n_query < 0 || (n_query > 0 && !query)) return fail(c, LCF_E_INVALID, "invalid query grid");
     if (subsample_rule < 0 || subsample_rule > 1 || criterion < 0 || criterion > 1)
         return fail(c, LCF_E_INVALID, "unknown subsample rule or criterion");
@@ -700,6 +709,9 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
     }
     HIPCHK(c, da.alloc(n_amp));
     HIPCHK(c, dl.alloc(n_ls));
+    DevBuf<double> dlb, dml;
+    if (lss_band) HIPCHK(c, dlb.alloc(n_bands * (int64_t)n_ls));
+    if (min_ls) HIPCHK(c, dml.alloc(n_bands));
     HIPCHK(c, dst.alloc(n_bands * stride));
     HIPCHK(c, dp.alloc(n_bands * (int64_t)n_query));
     HIPCHK(c, ds.alloc(n_bands * (int64_t)n_query));
@@ -719,7 +731,9 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
         HIPCHK(c, up(dlw.p, lsws, sizeof(double) * n_lsw));
     }
     HIPCHK(c, up(da.p, amps, sizeof(double) * n_amp));
-    HIPCHK(c, up(dl.p, lss, sizeof(double) * n_ls));
+    if (lss) HIPCHK(c, up(dl.p, lss, sizeof(double) * n_ls));
+    if (lss_band) HIPCHK(c, up(dlb.p, lss_band, sizeof(double) * n_bands * n_ls));
+    if (min_ls) HIPCHK(c, up(dml.p, min_ls, sizeof(double) * n_bands));
     HIPCHK(c, up(doff.p, off, sizeof(int64_t) * (n_bands + 1)));
     HIPCHK(c, up(dord.p, order.data(), sizeof(int32_t) * n_bands));
     lcf::GpArgs a{};
@@ -739,7 +753,9 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
     a.query_per_band = query_per_band;
     a.amps = da.p;
     a.n_amp = n_amp;
-    a.lss = dl.p;
+    a.lss = lss ? dl.p : nullptr;
+    a.lss_band = lss_band ? dlb.p : nullptr;
+    a.min_ls = min_ls ? dml.p : nullptr;
     a.n_ls = n_ls;
     a.cap = cap;
     a.subsample_rule = subsample_rule;

@@ -387,7 +387,7 @@ class GpuContext:
 
     def gp_fit(self, csr: dict, query, amp_candidates, ls_candidates, *, cap: int = 128,
                subsample_rule: int = 0, criterion: int = 0, query_per_band: bool = False) -> dict:
-        """Array form (``lcf_batch_gp_fit_ex``): training-set cap up to 128,
+        """Array form (``lcf_batch_gp_fit_ex``): training-set cap up to 200,
         subsample rule (0 GPU, 1 CPU), criterion (0 NLML, 1 train RMS),
         shared or per-band query grid; returns state, pred, std, dpred, pred_obs."""
         off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))

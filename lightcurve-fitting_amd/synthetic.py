@@ -156,3 +156,39 @@ def subset(csr: dict, bands: np.ndarray) -> dict:
     for k in ("times", "flux", "obs_var", "is_upper", "upper_flux"):
         out[k] = csr[k][idx]
     return out
+
+
+def villar_svi_batch(n_sources: int, seed: int = 0, n_bands: int = 3, n_per_band: int = 60,
+                     t_max: float = 120.0) -> dict:
+    """SVI workload: ``n_sources`` Villar light curves with ``n_bands`` bands of
+    ``n_per_band`` points each, times uniform on [0, t_max] (sorted).  True
+    parameters: A ~ U(100, 1000) uJy scaled per band by U(0.7, 1.3), beta ~
+    U(0, 0.02), gamma ~ U(5, 30) d, t0 ~ U(10, 60) d, tau_rise ~ U(1, 10) d,
+    tau_fall ~ U(10, 60) d; Gaussian noise sigma = 0.05 peak + 2 uJy with peak
+    the band's largest true flux.  Returned normalised per band as above,
+    with ``band_source`` and the true internal parameters (amplitude of the
+    normalised band)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    nb = n_sources * n_bands
+    band_src = np.repeat(np.arange(n_sources), n_bands)
+    off = np.arange(nb + 1, dtype=np.int64) * n_per_band
+    t = np.sort(rng.uniform(0.0, t_max, size=(nb, n_per_band)), axis=1).ravel()
+    a = rng.uniform(100.0, 1000.0, size=n_sources)[band_src] * rng.uniform(0.7, 1.3, size=nb)
+    beta = rng.uniform(0.0, 0.02, size=n_sources)[band_src]
+    gam = rng.uniform(5.0, 30.0, size=n_sources)[band_src]
+    t0 = rng.uniform(10.0, 60.0, size=n_sources)[band_src]
+    tr = rng.uniform(1.0, 10.0, size=n_sources)[band_src]
+    tf = rng.uniform(10.0, 60.0, size=n_sources)[band_src]
+    p = np.stack([np.log(a), beta, np.log(gam), t0, np.log(tr), np.log(tf), np.full(nb, -5.0)], axis=1)
+    band_of = np.repeat(np.arange(nb), n_per_band)
+    clean = model_flux(1, p[band_of], t)
+    peak = clean.reshape(nb, n_per_band).max(axis=1)
+    sig = (0.05 * peak + 2.0)[band_of]
+    fobs = clean + rng.normal(size=t.size) * sig
+    csr = normalise_bands(off, t, fobs, sig)
+    kept = csr["kept_bands"]
+    csr["band_source"] = band_src[kept]
+    p[:, 0] -= np.log(peak)
+    csr["true_params"] = p[kept]
+    csr["n_sources"] = n_sources
+    return csr

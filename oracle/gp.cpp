@@ -170,7 +170,7 @@ static int batch_fit(int64_t n_bands, const int64_t* off, const double* t, const
                 double step = (double)(n - 1) / (double)(m - 1);
                 idx = std::min((int)(i * step + 0.5), n - 1);
             } else {
-                idx = std::min((int)(((double)i + 0.5) * n / m), n - 1);  // src/gp.rs:15-19
+                idx = std::min((int)(((double)i + 0.5) * ((double)n / (double)m)), n - 1);  // src/gp.rs:15-19
             }
             st_t[i] = t[a + idx];
             if (two_d) st_w[i] = w[a + idx];
@@ -213,6 +213,9 @@ static int batch_fit(int64_t n_bands, const int64_t* off, const double* t, const
             if (score < best) { best = score; best_c = c; any = true; }
         }
         if (!any) return;
+        // the CPU dense path keeps no fit when no candidate has a finite score
+        // (src/nonparametric.rs:1318-1339)
+        if (criterion == ORC_GP_SELECT_TRAIN_RMS && !(best < 1e99)) return;
         bool valid;
         Kern k = cand(best_c, valid);
         Dense g = dense_fit(st_t.data(), st_w.data(), st_y.data(), st_v.data(), m, k);

@@ -1,7 +1,7 @@
 """GPU parity of the batched dense GP (FP64) against the oracle (needs an
 MI355X): selected hyperparameters, alpha, predictive mean / derivative / std
 on the grid and the mean at every observation, to 1e-8 relative, for
-training sets up to the LDS cap (128), both subsample rules and both
+training sets up to the LDS cap (200), both subsample rules and both
 selection criteria; plus the reference API (GpBandInput / batch_gp_fit)."""
 import numpy as np
 import pytest
@@ -33,9 +33,9 @@ def close(a, b, rel=1e-8):
     return np.array_equal(fin, np.isfinite(a)) and np.allclose(a[fin], b[fin], rtol=rel, atol=rel * np.abs(b[fin]).max(initial=1.0))
 
 
-@pytest.mark.parametrize("cap,rule,crit", [(25, 0, 0), (64, 1, 1), (128, 0, 0), (128, 1, 1)])
+@pytest.mark.parametrize("cap,rule,crit", [(25, 0, 0), (64, 1, 1), (128, 0, 0), (128, 1, 1), (200, 0, 0), (200, 1, 1)])
 def test_gp_matches_oracle(ctx, orc, cap, rule, crit):
-    sizes = [0, 2, 3, 5, 17, 25, 26, 63, 64, 65, 100, 127, 128, 129, 200]
+    sizes = [0, 2, 3, 5, 17, 25, 26, 63, 64, 65, 100, 127, 128, 129, 199, 200, 201, 333]
     csr = bands(cap + rule, sizes)
     q = np.linspace(0, 150, 200)
     g = ctx.gp_fit(csr, q, AMPS, LSS, cap=cap, subsample_rule=rule, criterion=crit)
@@ -97,7 +97,7 @@ LST2 = np.array([0.05, 5.0, 15.0, 40.0])
 LSW2 = np.array([0.0005, 0.05, 0.2, 1.0])
 
 
-@pytest.mark.parametrize("cap,rule,crit", [(40, 0, 0), (100, 1, 1), (128, 0, 0)])
+@pytest.mark.parametrize("cap,rule,crit", [(40, 0, 0), (100, 1, 1), (128, 0, 0), (200, 1, 1)])
 def test_gp2d_matches_oracle(ctx, orc, cap, rule, crit):
     sizes = [0, 2, 3, 7, 39, 40, 41, 99, 100, 128, 129, 300]
     csr = sources_2d(cap + rule, sizes)
