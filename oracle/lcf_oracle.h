@@ -173,6 +173,39 @@ int orc_batch_gp2d_fit(int64_t n_src, const int64_t* offsets, const double* t, c
                        double* state, int state_stride, double* pred, double* stdv, double* dpred,
                        double* pred_obs, int n_threads);
 
+/* ---- nonparametric band pipeline (src/nonparametric.rs:1228-1525) ------- */
+#define ORC_NP_NFEAT 50  /* NonparametricBandResult fields (src/nonparametric.rs:74-160), band name excluded */
+/* process_band per CSR band of raw magnitudes (times, mags, mag errors),
+ * grid of n_grid points on [t_lo[b], t_hi[b]] (duration t_hi - t_lo), dense
+ * GP when detections <= dense_max, else FITC with n_inducing points and a
+ * dense GP on max_sub subsampled detections for thermal reuse.  status[b]:
+ * 0 no result, 1 dense, 2 FITC.  feat [n_bands][ORC_NP_NFEAT] (NaN = None),
+ * pred / std (rescaled) / d(mean)/dt [n_bands][n_grid], state rows (stride
+ * >= 2 cap + 4, cap >= max(dense_max, max_sub)): alpha[cap], x[cap], amp,
+ * 1/(2 l^2), y mean, m of the GP kept for thermal reuse. */
+int orc_batch_nonparametric(int64_t n_bands, const int64_t* offsets, const double* times, const double* mags,
+                            const double* mag_errs, const double* t_lo, const double* t_hi, int n_grid,
+                            const double* amps, int n_amp, const double* ls_factors, int n_ls, int dense_max,
+                            int max_sub, int n_inducing, int32_t* status, double* feat, double* pred,
+                            double* stdv, double* dpred, double* state, int state_stride, int n_threads);
+/* compute_fine_grid_metrics' eleven metrics of a curve with a given peak
+ * index, in FineGridMetrics order (src/nonparametric.rs:490-554). */
+int orc_np_curve_metrics(const double* t, const double* m, int n, int peak_idx, double* out11);
+/* von Neumann ratio, pre-peak rms, rise amplitude / noise, Stetson K, excess
+ * variance, CUSUM range (src/nonparametric.rs:586-724). */
+int orc_np_raw_features(const double* t, const double* m, const double* e, int n, double t_peak,
+                        double peak_mag, double* out6);
+/* Grid statistics of compute_predictive_features (sigma_f, peak_to_peak,
+ * dfdt_max, dfdt_min, frac_of_peak, post_var_mean, post_var_max, skewness,
+ * kurtosis, n_inflections), then power-law index and chi2, post-peak
+ * monotonicity, local maxima (src/nonparametric.rs:266-472, 730-774). */
+int orc_np_grid_features(const double* tg, const double* p, const double* sd, int n, int peak_idx,
+                         double* out14);
+/* SparseGP (FITC) fit with m inducing points + predict_with_std + approx_nlml
+ * (src/sparse_gp.rs:183-328).  Returns 1 when a factorisation fails. */
+int orc_sparse_gp(const double* t, const double* y, const double* nv, int n, double amp, double ls, int m,
+                  const double* q, int nq, double* mean, double* stdv, double* nlml);
+
 /* ---- Laplace (src/parametric.rs:2464-2693) ---------------------------------- */
 #define ORC_LAPLACE_FD4 0      /* reference: 4-point central differences of f */
 #define ORC_LAPLACE_GRAD_FD 1  /* central differences of the analytic gradient */
