@@ -192,3 +192,48 @@ def villar_svi_batch(n_sources: int, seed: int = 0, n_bands: int = 3, n_per_band
     csr["true_params"] = p[kept]
     csr["n_sources"] = n_sources
     return csr
+
+
+def bazin_mag_batch(n_sources: int, seed: int = 0, n_bands: int = 3, n_min: int = 10, n_max: int = 200,
+                    t_span: float = 150.0, sort_times: bool = True) -> dict:
+    """Nonparametric workload in magnitudes: ``n_sources`` Bazin light curves
+    with ``n_bands`` bands of U{n_min..n_max} points each, times uniform on
+    [0, t_span] d.  Flux F = A e^{-(t - t0)/tau_fall} / (1 + e^{-(t - t0)/tau_rise})
+    + B with A ~ U(100, 1000) uJy scaled per band by U(0.7, 1.3), t0 ~ U(20, 80),
+    tau_rise ~ U(1, 10), tau_fall ~ U(10, 60), B ~ N(0, 5); Gaussian noise of
+    sigma = 0.03 A + 2 uJy; mag = 23.9 - 2.5 log10(max(F, 1)), magerr = 1.0857
+    sigma / max(F, 1).  Times are made relative to the source's earliest
+    observation (as build_mag_bands does), and ``t_lo`` / ``t_hi`` hold each
+    band's source time range (the span of the prediction grid)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    nb = n_sources * n_bands
+    band_src = np.repeat(np.arange(n_sources), n_bands)
+    sizes = rng.integers(n_min, n_max + 1, size=nb)
+    off = np.zeros(nb + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    band_of = np.repeat(np.arange(nb), sizes)
+    t = rng.uniform(0.0, t_span, size=int(off[-1]))
+    if sort_times:
+        t = t + band_of * (2.0 * t_span)
+        t = np.sort(t) - band_of * (2.0 * t_span)
+    a = rng.uniform(100.0, 1000.0, size=n_sources)[band_src] * rng.uniform(0.7, 1.3, size=nb)
+    t0 = rng.uniform(20.0, 80.0, size=n_sources)[band_src]
+    tr = rng.uniform(1.0, 10.0, size=n_sources)[band_src]
+    tf = rng.uniform(10.0, 60.0, size=n_sources)[band_src]
+    b = rng.normal(0.0, 5.0, size=n_sources)[band_src]
+    dt = t - t0[band_of]
+    with np.errstate(over="ignore"):
+        flux = a[band_of] * np.exp(-dt / tf[band_of]) / (1.0 + np.exp(-dt / tr[band_of])) + b[band_of]
+    sig = 0.03 * a[band_of] + 2.0
+    flux = flux + rng.normal(size=t.size) * sig
+    fc = np.maximum(flux, 1.0)
+    mags = 23.9 - 2.5 * np.log10(fc)
+    errs = 1.0857 * sig / fc
+    src_of = band_src[band_of]
+    tmin = np.full(n_sources, np.inf)
+    np.minimum.at(tmin, src_of, t)
+    t = t - tmin[src_of]
+    tmax = np.full(n_sources, -np.inf)
+    np.maximum.at(tmax, src_of, t)
+    return {"offsets": off, "times": t, "mags": mags, "mag_errs": errs, "band_source": band_src,
+            "t_lo": np.zeros(nb), "t_hi": tmax[band_src]}

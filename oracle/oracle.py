@@ -416,3 +416,97 @@ def multi_bazin_cost(p, k: int, t, f, v, up, uf) -> float:
     up = i32(up)
     return L.orc_multi_bazin_cost(_p(p, C.c_double), k, _p(t, C.c_double), _p(f, C.c_double), _p(v, C.c_double),
                                   _p(up, C.c_int32), _p(uf, C.c_double), t.size)
+
+
+# ---- nonparametric band pipeline (src/nonparametric.rs:1228-1525) ----------
+NP_NFEAT = 50
+NP_FEATURES = (
+    "rise_time", "rise_halfmax", "rise_efold", "decay_efold", "dm15", "decay_halfmax", "t0", "peak_mag",
+    "chi2", "baseline_chi2", "n_obs", "fwhm", "rise_rate", "decay_rate", "near_peak_rise_rate",
+    "near_peak_decay_rate", "gp_dfdt_now", "gp_dfdt_next", "gp_d2fdt2_now", "gp_predicted_mag_1d",
+    "gp_predicted_mag_2d", "gp_time_to_peak", "gp_extrap_slope", "gp_sigma_f", "gp_peak_to_peak",
+    "gp_snr_max", "gp_dfdt_max", "gp_dfdt_min", "gp_frac_of_peak", "gp_post_var_mean", "gp_post_var_max",
+    "gp_skewness", "gp_kurtosis", "gp_n_inflections", "decay_power_law_index", "decay_power_law_chi2",
+    "mag_at_30d", "mag_at_60d", "mag_at_90d", "von_neumann_ratio", "pre_peak_rms",
+    "rise_amplitude_over_noise", "post_peak_monotonicity", "n_local_maxima", "n_upper_limits",
+    "gp_fit_amp", "gp_fit_lengthscale", "stetson_k", "excess_variance", "cusum_range")
+NP_AMPS = (0.3, 1.0, 3.0, 10.0)             # src/nonparametric.rs:1110
+NP_LS_FACTORS = (3.0, 6.0, 12.0, 24.0, 48.0)  # src/nonparametric.rs:1111
+
+
+def batch_nonparametric(csr: dict, t_lo, t_hi, n_grid: int = 50, amps=NP_AMPS, ls_factors=NP_LS_FACTORS,
+                        dense_max: int = 100, max_sub: int = 25, n_inducing: int = 30,
+                        n_threads: int | None = None) -> dict:
+    """process_band over CSR bands of raw magnitudes (times, mags, mag_errs)."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    nb = off.size - 1
+    t, m, e = f64(csr["times"]), f64(csr["mags"]), f64(csr["mag_errs"])
+    lo, hi, a, lf = f64(t_lo), f64(t_hi), f64(amps), f64(ls_factors)
+    assert lo.size == nb and hi.size == nb
+    cap = max(dense_max, max_sub)
+    stride = 2 * cap + 4
+    status = np.zeros(nb, np.int32)
+    feat = np.zeros((nb, NP_NFEAT))
+    pred, std, dpred = (np.zeros((nb, n_grid)) for _ in range(3))
+    state = np.zeros((nb, stride))
+    L = lib()
+    L.orc_batch_nonparametric.restype = _i32
+    L.orc_batch_nonparametric.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _i32, _pd, _i32,
+                                          _i32, _i32, _i32, _pi32, _pd, _pd, _pd, _pd, _pd, _i32, _i32]
+    rc = L.orc_batch_nonparametric(nb, _p(off, C.c_int64), _p(t, C.c_double), _p(m, C.c_double),
+                                   _p(e, C.c_double), _p(lo, C.c_double), _p(hi, C.c_double), n_grid,
+                                   _p(a, C.c_double), a.size, _p(lf, C.c_double), lf.size, dense_max, max_sub,
+                                   n_inducing, _p(status, C.c_int32), _p(feat, C.c_double), _p(pred, C.c_double),
+                                   _p(std, C.c_double), _p(dpred, C.c_double), _p(state, C.c_double), stride,
+                                   n_threads or default_threads())
+    assert rc == 0
+    return {"status": status, "features": feat, "pred": pred, "std": std, "dpred": dpred, "state": state}
+
+
+def np_curve_metrics(t, m, peak_idx: int) -> np.ndarray:
+    t, m = f64(t), f64(m)
+    out = np.zeros(11)
+    L = lib()
+    L.orc_np_curve_metrics.restype = _i32
+    L.orc_np_curve_metrics.argtypes = [_pd, _pd, _i32, _i32, _pd]
+    assert L.orc_np_curve_metrics(_p(t, C.c_double), _p(m, C.c_double), t.size, peak_idx,
+                                  _p(out, C.c_double)) == 0
+    return out
+
+
+def np_raw_features(t, m, e, t_peak: float, peak_mag: float) -> np.ndarray:
+    t, m, e = f64(t), f64(m), f64(e)
+    out = np.zeros(6)
+    L = lib()
+    L.orc_np_raw_features.restype = _i32
+    L.orc_np_raw_features.argtypes = [_pd, _pd, _pd, _i32, _f64, _f64, _pd]
+    assert L.orc_np_raw_features(_p(t, C.c_double), _p(m, C.c_double), _p(e, C.c_double), t.size, t_peak,
+                                 peak_mag, _p(out, C.c_double)) == 0
+    return out
+
+
+def np_grid_features(tg, p, sd, peak_idx: int) -> np.ndarray:
+    tg, p, sd = f64(tg), f64(p), f64(sd)
+    out = np.zeros(14)
+    L = lib()
+    L.orc_np_grid_features.restype = _i32
+    L.orc_np_grid_features.argtypes = [_pd, _pd, _pd, _i32, _i32, _pd]
+    assert L.orc_np_grid_features(_p(tg, C.c_double), _p(p, C.c_double), _p(sd, C.c_double), tg.size,
+                                  peak_idx, _p(out, C.c_double)) == 0
+    return out
+
+
+def sparse_gp(t, y, nv, amp: float, ls: float, m: int, query):
+    """SparseGP (FITC) fit; (mean, std) at query and approx_nlml, None on failure."""
+    t, y, nv, q = f64(t), f64(y), f64(nv), f64(query)
+    mean, std = np.zeros(q.size), np.zeros(q.size)
+    nl = C.c_double()
+    L = lib()
+    L.orc_sparse_gp.restype = _i32
+    L.orc_sparse_gp.argtypes = [_pd, _pd, _pd, _i32, _f64, _f64, _i32, _pd, _i32, _pd, _pd,
+                                C.POINTER(C.c_double)]
+    rc = L.orc_sparse_gp(_p(t, C.c_double), _p(y, C.c_double), _p(nv, C.c_double), t.size, amp, ls, m,
+                         _p(q, C.c_double), q.size, _p(mean, C.c_double), _p(std, C.c_double), C.byref(nl))
+    if rc != 0:
+        return None
+    return mean, std, nl.value
