@@ -19,7 +19,11 @@ Also reported on the same JSON line:
   * svi: the SVI workload (10k Villar sources x 3 bands x 60 points, 16
     samples, up to 2000 Adam steps, FP32) fitted by the batched adjoint kernel
     and by the plain form (forward-difference gradients, one launch per size
-    class), with the agreement of the two posteriors.
+    class), with the agreement of the two posteriors;
+  * nonparametric: the GP workload (10k Bazin sources x 3 bands x U{10..200}
+    magnitudes, 200-point grid, dense GP up to 200 detections) through the
+    whole band pipeline (fit + features) on the device, with the oracle's
+    process_band on a bounded sample as its CPU baseline.
 """
 from __future__ import annotations
 
@@ -133,6 +137,48 @@ def svi_leg(pkg, n_sources: int, n_steps: int, n_samples: int, lr: float, reps: 
             "evals_per_s": round(evals / (kms_adj * 1e-3), 1)}
 
 
+def np_leg(pkg, n_sources: int, n_grid: int, dense_max: int, reps: int, cpu_sources: int, threads: int,
+           do_cpu: bool) -> dict:
+    """Nonparametric workload: n_sources Bazin sources x 3 bands x U{10..200}
+    points (magnitudes), per-source grid of n_grid points, dense GP up to
+    dense_max detections; the whole process_band (fit + features) per band
+    on the device.  Time = device time of the launches (inputs resident);
+    the full call with transfers is reported beside it.  CPU baseline: the
+    oracle's process_band on a bounded sample of the same workload."""
+    csr = pkg.synthetic.bazin_mag_batch(n_sources, seed=2024)
+    nb = csr["offsets"].size - 1
+    ctx = pkg.GpuContext(0)
+    kw = dict(n_grid=n_grid, dense_max=dense_max)
+    ctx.nonparametric_arrays(csr, csr["t_lo"], csr["t_hi"], **kw)  # warm-up
+    best, best_call = None, None
+    for _ in range(max(reps, 1)):
+        t0 = time.perf_counter()
+        r = ctx.nonparametric_arrays(csr, csr["t_lo"], csr["t_hi"], **kw)
+        call = time.perf_counter() - t0
+        k = ctx.last_kernel_ms
+        best = k if best is None or k < best else best
+        best_call = call if best_call is None or call < best_call else best_call
+    log(f"nonparametric: {nb} bands, {best:.1f} ms device, {best_call * 1e3:.1f} ms with transfers")
+    out = {"workload": "gp_nonparametric", "sources": n_sources, "bands": nb,
+           "observations": int(csr["offsets"][-1]), "points_per_band": "U{10..200}", "grid": n_grid,
+           "dense_max": dense_max, "dtype": "f64", "value": round(n_sources / (best * 1e-3), 2),
+           "unit": "sources/s", "ms": round(best, 3), "ms_with_transfers": round(best_call * 1e3, 3),
+           "status_counts": np.bincount(r["status"], minlength=3).tolist()}
+    if do_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+
+        sub = pkg.synthetic.bazin_mag_batch(cpu_sources, seed=2024)
+        t0 = time.perf_counter()
+        oracle.batch_nonparametric(sub, sub["t_lo"], sub["t_hi"], n_threads=threads, **kw)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cpu_sources / dt, 2), "unit": "sources/s", "cores": threads,
+                               "kind": "port", "sample": f"{cpu_sources} sources of the same workload, oracle "
+                                                         f"FP64 C++ process_band, {threads} threads, {dt:.1f} s"}
+        log(f"nonparametric cpu baseline: {cpu_sources / dt:.1f} sources/s")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,6 +193,9 @@ def main() -> int:
     ap.add_argument("--no-svi", action="store_true")
     ap.add_argument("--svi-sources", type=int, default=10_000)
     ap.add_argument("--svi-steps", type=int, default=2000)
+    ap.add_argument("--no-np", action="store_true")
+    ap.add_argument("--np-sources", type=int, default=10_000)
+    ap.add_argument("--np-cpu-sources", type=int, default=1000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -222,6 +271,11 @@ def main() -> int:
     if not args.no_svi:
         svi = svi_leg(pkg, args.svi_sources, args.svi_steps, 16, 0.01, reps=max(args.steps, 1))
 
+    npl = None
+    if not args.no_np:
+        npl = np_leg(pkg, args.np_sources, 200, 200, max(args.steps, 1), args.np_cpu_sources,
+                     min(os.cpu_count() or 1, 16), not args.no_cpu)
+
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -246,6 +300,7 @@ def main() -> int:
         "plain": plain,
         "per_model_ms": {MODEL_NAMES[m]: round(float(slot_ms[i]), 3) for i, m in enumerate(MODELS)},
         "svi": svi,
+        "nonparametric": npl,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -35,6 +35,7 @@ extern "C" {
 #define LCF_E_INVALID (-1)   /* bad argument (size, model id, null pointer) */
 #define LCF_E_HIP (-2)       /* HIP runtime error (message has the HIP text) */
 #define LCF_E_NOMEM (-3)     /* device allocation failed */
+#define LCF_E_INTERNAL (-4)  /* inconsistent internal state (a bug: please report) */
 
 /* precision of model evaluation / swarm state in the fitting kernels */
 #define LCF_FP32 0           /* FP32 + Kahan-compensated chi-square (default) */
@@ -134,6 +135,37 @@ int lcf_batch_multi_bazin(lcf_ctx* ctx, const lcf_batch* data, int n_particles, 
                           int stall_iters, uint64_t seed, int32_t* out_best_k, double* out_params,
                           double* out_cost, double* out_bic, double* out_per_k_cost,
                           double* out_per_k_bic);
+
+/* ---- nonparametric band pipeline (FP64) --------------------------------- */
+#define LCF_NP_NFEAT 50  /* NonparametricBandResult fields (src/nonparametric.rs:74-160), band name excluded */
+/* process_band (src/nonparametric.rs:1228-1525) for every band of a batch, in
+ * one pass on the device: replaces the per-band CPU work of fit_nonparametric
+ * (src/nonparametric.rs:1085-1149) and the GPU fit + host feature loop of
+ * fit_nonparametric_batch_gpu_with_opts (src/nonparametric.rs:799-1081).
+ * Bands are CSR (offsets[n_bands+1]) over raw magnitudes: times, mags,
+ * mag_errs; the prediction grid of band b is n_grid points on
+ * [t_lo[b], t_hi[b]] (the source's time range; reference: 50 points) and its
+ * duration sets the length-scale candidates max(duration / f, 0.1) for f in
+ * ls_factors (reference: 3, 6, 12, 24, 48) crossed with amp_candidates
+ * (reference: 0.3, 1, 3, 10).  Detections (SNR >= 3) of at most dense_max
+ * (reference: 100; at most 200) are fitted by a dense GP selected by train
+ * RMS, larger bands by FITC with n_inducing points (reference: 30, at most
+ * 32) selected by approx_nlml, plus a dense GP on max_subsample detections
+ * (reference: 25) kept for thermal reuse.
+ * Outputs: out_status[b] 0 no result (fewer than 5 observations or
+ * detections, no usable fit, empty time range), 1 dense, 2 FITC;
+ * out_features [n_bands][LCF_NP_NFEAT] in NonparametricBandResult field
+ * order (n_obs and n_upper_limits as doubles, NaN for None);
+ * out_pred / out_std (rescaled) / out_dpred (analytic d mean/dt)
+ * [n_bands][n_grid]; out_state [n_bands][2 S + 4] with S = max(dense_max,
+ * max_subsample): alpha[S], x[S], amp, 1/(2 l^2), mean, m of the dense GP
+ * kept for thermal reuse (the reference's trained_gps map). */
+int lcf_batch_nonparametric(lcf_ctx* ctx, int64_t n_bands, const int64_t* offsets, const double* times,
+                            const double* mags, const double* mag_errs, const double* t_lo,
+                            const double* t_hi, int n_grid, const double* amp_candidates, int n_amp,
+                            const double* ls_factors, int n_ls, int dense_max, int max_subsample,
+                            int n_inducing, int32_t* out_status, double* out_features, double* out_pred,
+                            double* out_std, double* out_dpred, double* out_state);
 
 /* ---- batched SVI ----------------------------------------------------------- */
 /* SVI flags */

@@ -21,15 +21,9 @@
 // right-looking updates subtract l_ik l_jk in the same k order as the
 // reference's Crout loops, so factors agree to rounding.
 #include "gp_kernels.hpp"
-#include "lcf_device.hpp"
+#include "gp_device.hpp"
 
 namespace lcf {
-
-constexpr int GP_THREADS = 256;
-constexpr int GP_WAVES = GP_THREADS / WAVE;
-constexpr int GP_ROWS = 4;  // rows per lane in the wave-level solves (m <= 256)
-
-__device__ __forceinline__ int tri(int i, int j) { return (i * (i + 1) >> 1) + j; }
 
 struct GpShared {
     double* L;      // packed lower triangle, cap*(cap+1)/2
@@ -65,13 +59,6 @@ struct Hyp {
     }
 };
 
-// Row k's value from rows held as (lane + 64 r): wave-uniform k.
-__device__ __forceinline__ double row_of(const double (&v)[GP_ROWS], int k) {
-    const int r = k >> 6;
-    double c = r == 0 ? v[0] : (r == 1 ? v[1] : (r == 2 ? v[2] : v[3]));
-    return bcast(c, k & (WAVE - 1));
-}
-
 // Build K + diag(max(nv, 1e-10)) and factorise in place.  Returns false
 // (uniformly) when a pivot is not positive.
 __device__ bool gp_factor(const GpShared& s, const Train& tr, int m, const Hyp& h) {
@@ -86,26 +73,7 @@ __device__ bool gp_factor(const GpShared& s, const Train& tr, int m, const Hyp& 
         }
     }
     __syncthreads();
-    for (int j = 0; j < m; j++) {
-        double piv = s.L[tri(j, j)];
-        if (piv <= 0.0) return false;  // uniform: every thread read the same pivot
-        double ljj = ::sqrt(piv);
-        __syncthreads();  // all have read the pivot before it is overwritten
-        if (tid == 0) s.L[tri(j, j)] = ljj;
-        for (int i = j + 1 + tid; i < m; i += GP_THREADS) s.L[tri(i, j)] /= ljj;
-        __syncthreads();
-        const int r = m - j - 1;
-        const int rr = r * r;
-        for (int e = tid; e < rr; e += GP_THREADS) {
-            int a = e / r, b = e - a * r;
-            if (b <= a) {
-                int i = j + 1 + a, k = j + 1 + b;
-                s.L[tri(i, k)] -= s.L[tri(i, j)] * s.L[tri(k, j)];
-            }
-        }
-        __syncthreads();
-    }
-    return true;
+    return chol_packed(s.L, m);
 }
 
 // Wavefront 0: alpha = L^-T L^-1 (y - ybar); returns 0.5 yc.alpha + sum ln L_ii

@@ -6,6 +6,7 @@
 #include "svi_kernels.hpp"
 #include "gp_kernels.hpp"
 #include "laplace_kernels.hpp"
+#include "np_kernels.hpp"
 
 #include <algorithm>
 #include <cstdarg>
@@ -950,6 +951,153 @@ int lcf_batch_gp_fit(lcf_ctx* c, int64_t n_bands, const int64_t* off, const doub
         for (int i = 0; i < cap; i++) { o[i] = s[i]; o[25 + i] = s[cap + i]; }
         for (int i = 0; i < 4; i++) o[50 + i] = s[2 * cap + i];
     }
+    return LCF_OK;
+}
+
+int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* mag,
+                            const double* err, const double* t_lo, const double* t_hi, int n_grid,
+                            const double* amps, int n_amp, const double* ls_factors, int n_ls, int dense_max,
+                            int max_subsample, int n_inducing, int32_t* out_status, double* out_features,
+                            double* out_pred, double* out_std, double* out_dpred, double* out_state) {
+    if (!c) return LCF_E_INVALID;
+    if (n_bands < 0 || n_bands > INT32_MAX - 1 || !off) return fail(c, LCF_E_INVALID, "invalid band count");
+    if (n_bands == 0) return LCF_OK;
+    if (n_grid < 2) return fail(c, LCF_E_INVALID, "the prediction grid needs at least 2 points, got %d", n_grid);
+    if (n_amp <= 0 || n_ls <= 0 || !amps || !ls_factors) return fail(c, LCF_E_INVALID, "empty hyperparameter grid");
+    if (dense_max < 0 || max_subsample < 1 || n_inducing < 1 || n_inducing > lcf::NP_MAX_IND)
+        return fail(c, LCF_E_INVALID, "dense_max >= 0, max_subsample >= 1 and n_inducing in [1, %d] required",
+                    lcf::NP_MAX_IND);
+    const int state_cap = std::max(dense_max, max_subsample);
+    if (state_cap > lcf::NP_CAP_MAX)
+        return fail(c, LCF_E_INVALID, "dense GPs of up to %d points are factorised on chip, got %d",
+                    lcf::NP_CAP_MAX, state_cap);
+    if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
+    for (int64_t b = 0; b < n_bands; b++)
+        if (off[b + 1] < off[b]) return fail(c, LCF_E_INVALID, "offsets not non-decreasing at %lld", (long long)b);
+    const int64_t n = off[n_bands];
+    if (n > INT32_MAX) return fail(c, LCF_E_INVALID, "too many observations");
+    if ((n > 0 && (!t || !mag || !err)) || !t_lo || !t_hi)
+        return fail(c, LCF_E_INVALID, "null observation array");
+    if (!out_status || !out_features || !out_pred || !out_std || !out_dpred || !out_state)
+        return fail(c, LCF_E_INVALID, "null output");
+    HIPCHK(c, hipSetDevice(c->device));
+    // Size classes by raw band size: a band of at most dense_max observations
+    // never needs FITC and goes to the smallest dense class that holds it;
+    // larger bands go to the class with the FITC work area.
+    struct Cls { int cap, fitc; std::vector<int32_t> bands; };
+    std::vector<Cls> cls;
+    for (int cp : {64, 128})
+        if (cp < state_cap) cls.push_back({cp, 0, {}});
+    cls.push_back({state_cap, 0, {}});
+    cls.push_back({state_cap, 1, {}});
+    for (int64_t b = 0; b < n_bands; b++) {
+        const int64_t nb = off[b + 1] - off[b];
+        size_t k = cls.size() - 1;
+        if (nb <= dense_max)
+            for (size_t q = 0; q + 1 < cls.size(); q++)
+                if (nb <= cls[q].cap) { k = q; break; }
+        cls[k].bands.push_back((int32_t)b);
+    }
+    for (auto& q : cls)
+        std::stable_sort(q.bands.begin(), q.bands.end(), [&](int32_t a, int32_t b) {
+            return off[a + 1] - off[a] > off[b + 1] - off[b];
+        });
+    std::vector<int32_t> order;
+    for (auto& q : cls) order.insert(order.end(), q.bands.begin(), q.bands.end());
+    const int stride = 2 * state_cap + 4;
+    const int64_t ng = n_grid;
+    DevBuf<double> dt, dm, de, dlo, dhi, da, dl, dfeat, dp, ds, dd, dst, dbuf;
+    DevBuf<int64_t> doff;
+    DevBuf<int32_t> dord, dstat, ddet;
+    HIPCHK(c, dt.alloc(n));
+    HIPCHK(c, dm.alloc(n));
+    HIPCHK(c, de.alloc(n));
+    HIPCHK(c, dbuf.alloc(n));
+    HIPCHK(c, ddet.alloc(n));
+    HIPCHK(c, dlo.alloc(n_bands));
+    HIPCHK(c, dhi.alloc(n_bands));
+    HIPCHK(c, da.alloc(n_amp));
+    HIPCHK(c, dl.alloc(n_ls));
+    HIPCHK(c, doff.alloc(n_bands + 1));
+    HIPCHK(c, dord.alloc(n_bands));
+    HIPCHK(c, dstat.alloc(n_bands));
+    HIPCHK(c, dfeat.alloc(n_bands * lcf::NP_NFEAT));
+    HIPCHK(c, dp.alloc(n_bands * ng));
+    HIPCHK(c, ds.alloc(n_bands * ng));
+    HIPCHK(c, dd.alloc(n_bands * ng));
+    HIPCHK(c, dst.alloc(n_bands * stride));
+    auto up = [&](void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, up(dt.p, t, sizeof(double) * n));
+    HIPCHK(c, up(dm.p, mag, sizeof(double) * n));
+    HIPCHK(c, up(de.p, err, sizeof(double) * n));
+    HIPCHK(c, up(dlo.p, t_lo, sizeof(double) * n_bands));
+    HIPCHK(c, up(dhi.p, t_hi, sizeof(double) * n_bands));
+    HIPCHK(c, up(da.p, amps, sizeof(double) * n_amp));
+    HIPCHK(c, up(dl.p, ls_factors, sizeof(double) * n_ls));
+    HIPCHK(c, up(doff.p, off, sizeof(int64_t) * (n_bands + 1)));
+    HIPCHK(c, up(dord.p, order.data(), sizeof(int32_t) * n_bands));
+    lcf::NpArgs a{};
+    a.offsets = doff.p;
+    a.t = dt.p;
+    a.mag = dm.p;
+    a.err = de.p;
+    a.t_lo = dlo.p;
+    a.t_hi = dhi.p;
+    a.queue = c->d_queue;
+    a.n_grid = n_grid;
+    a.amps = da.p;
+    a.n_amp = n_amp;
+    a.lsf = dl.p;
+    a.n_ls = n_ls;
+    a.dense_max = dense_max;
+    a.max_sub = max_subsample;
+    a.n_ind = n_inducing;
+    a.state_cap = state_cap;
+    a.det = ddet.p;
+    a.buf = dbuf.p;
+    a.status = dstat.p;
+    a.feat = dfeat.p;
+    a.pred = dp.p;
+    a.stdv = ds.p;
+    a.dpred = dd.p;
+    a.state = dst.p;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    int64_t first = 0;
+    for (auto& q : cls) {
+        const int64_t nq = (int64_t)q.bands.size();
+        if (nq == 0) continue;
+        a.order = dord.p + first;
+        a.n_tasks = (int)nq;
+        a.cap = q.cap;
+        a.fitc = q.fitc;
+        int per_cu = 0;
+        HIPCHK(c, lcf::np_occupancy(q.cap, q.fitc, &per_cu));
+        per_cu = std::max(per_cu, 1);
+        const int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, nq);
+        HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+        HIPCHK(c, lcf::launch_np(a, grid, c->stream));
+        first += nq;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    auto down = [&](void* h, const void* d, size_t bytes) {
+        return bytes ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, down(out_status, dstat.p, sizeof(int32_t) * n_bands));
+    HIPCHK(c, down(out_features, dfeat.p, sizeof(double) * n_bands * lcf::NP_NFEAT));
+    HIPCHK(c, down(out_pred, dp.p, sizeof(double) * n_bands * ng));
+    HIPCHK(c, down(out_std, ds.p, sizeof(double) * n_bands * ng));
+    HIPCHK(c, down(out_dpred, dd.p, sizeof(double) * n_bands * ng));
+    HIPCHK(c, down(out_state, dst.p, sizeof(double) * n_bands * stride));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int64_t b = 0; b < n_bands; b++)
+        if (out_status[b] < 0) return fail(c, LCF_E_INTERNAL, "band %lld was routed to a launch too small for it",
+                                           (long long)b);
     return LCF_OK;
 }
 

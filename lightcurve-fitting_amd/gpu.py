@@ -18,6 +18,7 @@ import numpy as np
 from . import _lib
 
 MAX_PARAMS = 7
+NP_NFEAT = 50  # NonparametricBandResult fields without the band name
 FP32, FP64 = 0, 1
 SCHED_PERSISTENT, SCHED_PER_CLASS = 0, 1
 PSO_STALL_FROM_FIRST_EVAL = 1
@@ -408,6 +409,33 @@ class GpuContext:
             _lib.ptr(out["pred_obs"], C.c_double)))
         return out
 
+
+    def nonparametric_arrays(self, csr: dict, t_lo, t_hi, *, n_grid: int = 50,
+                             amp_candidates=(0.3, 1.0, 3.0, 10.0), ls_factors=(3.0, 6.0, 12.0, 24.0, 48.0),
+                             dense_max: int = 100, max_subsample: int = 25, n_inducing: int = 30) -> dict:
+        """Array form of the nonparametric band pipeline
+        (``lcf_batch_nonparametric``): CSR bands of raw magnitudes (offsets,
+        times, mags, mag_errs), per-band grid range; returns status, features
+        [n_bands, 50], pred / std / dpred [n_bands, n_grid] and state."""
+        off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+        nb = off.size - 1
+        t, m, e = _f64(csr["times"]), _f64(csr["mags"]), _f64(csr["mag_errs"])
+        lo, hi = _f64(t_lo), _f64(t_hi)
+        if lo.size != nb or hi.size != nb:
+            raise GpuError("t_lo / t_hi need one value per band")
+        a, lf = _f64(amp_candidates), _f64(ls_factors)
+        cap = max(dense_max, max_subsample)
+        out = {"status": np.zeros(nb, np.int32), "features": np.zeros((nb, NP_NFEAT)),
+               "pred": np.zeros((nb, n_grid)), "std": np.zeros((nb, n_grid)), "dpred": np.zeros((nb, n_grid)),
+               "state": np.zeros((nb, 2 * cap + 4))}
+        self._check(self._lib.lcf_batch_nonparametric(
+            self._h, nb, _lib.ptr(off, C.c_int64), _lib.ptr(t, C.c_double), _lib.ptr(m, C.c_double),
+            _lib.ptr(e, C.c_double), _lib.ptr(lo, C.c_double), _lib.ptr(hi, C.c_double), n_grid,
+            _lib.ptr(a, C.c_double), a.size, _lib.ptr(lf, C.c_double), lf.size, dense_max, max_subsample,
+            n_inducing, _lib.ptr(out["status"], C.c_int32), _lib.ptr(out["features"], C.c_double),
+            _lib.ptr(out["pred"], C.c_double), _lib.ptr(out["std"], C.c_double),
+            _lib.ptr(out["dpred"], C.c_double), _lib.ptr(out["state"], C.c_double)))
+        return out
 
 def _gp2d_csr(sources) -> dict:
     off = [0]
