@@ -54,6 +54,144 @@ __device__ __forceinline__ bool chol_packed(double* L, int m) {
     return true;
 }
 
+// Row start of row i in the packed lower triangle.
+__device__ __forceinline__ int rs(int i) { return (i * (i + 1)) >> 1; }
+
+// (i, j) of packed index e (j <= i), from a single-precision square root
+// corrected to the exact integer row.
+__device__ __forceinline__ void tri_ij(int e, int& i, int& j) {
+    int r = (int)((__builtin_amdgcn_sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+    r += rs(r + 1) <= e;
+    r -= rs(r) > e;
+    i = r;
+    j = e - rs(r);
+}
+
+constexpr int CHOL_NB = 16;  // block-column width of the blocked Cholesky
+
+// In-place Cholesky of the packed m x m matrix in L over the workgroup,
+// left-looking by block columns of CHOL_NB: (A) the panel below and including
+// the diagonal block loses sum_{k < J0} l_ik l_jk, register-tiled 4 x 4 per
+// thread; (B) wavefront 0 factors the diagonal block with its rows in
+// registers (lane r holds row J0 + r; column j is scaled, then the columns to
+// its right lose l_ij l_cj); (C) every row below solves against the diagonal
+// block.  Each element is reduced term by term in increasing k, as the
+// row-major Cholesky-Crout of src/sparse_gp.rs:27-49 does, so the factor
+// rounds identically when built without multiply-add contraction.
+// rows = m + 1 appends row m (its first m entries, e.g. y - ybar): it is
+// reduced like any row below the factor and ends as L^-1 times itself, the
+// forward substitution of solve_l (src/sparse_gp.rs:52-58) term for term.
+// Three barriers per block column; returns false (uniformly, after a
+// barrier) on a pivot <= 0.  The matrix must be complete and visible.
+__device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* flag) {
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    for (int J0 = 0; J0 < m; J0 += CHOL_NB) {
+        const int jb = m - J0 < CHOL_NB ? m - J0 : CHOL_NB;
+        // (A) panel update, k < J0
+        if (J0 > 0) {
+            const int nrt = (rows - J0 + 3) >> 2;
+            for (int t = tid; t < nrt * (CHOL_NB / 4); t += GP_THREADS) {
+                const int i0 = J0 + 4 * (t >> 2), j0 = J0 + 4 * (t & 3);
+                if (j0 >= J0 + jb || j0 > i0 + 3) continue;
+                double acc[4][4];
+                int ri[4], rj[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int i = i0 + r < rows ? i0 + r : rows - 1;
+                    const int j = j0 + r < m ? j0 + r : m - 1;
+                    ri[r] = rs(i);
+                    rj[r] = rs(j);
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int i = i0 + r, j = j0 + c;
+                        acc[r][c] = (i < rows && j <= i && j < m) ? L[ri[r] + j] : 0.0;
+                    }
+#pragma unroll 2
+                for (int k = 0; k < J0; k++) {
+                    double a[4], b[4];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        a[r] = L[ri[r] + k];
+                        b[r] = L[rj[r] + k];
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[r][c] -= a[r] * b[c];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int i = i0 + r, j = j0 + c;
+                        if (i < rows && j <= i && j < J0 + jb) L[ri[r] + j] = acc[r][c];
+                    }
+            }
+            __syncthreads();
+        }
+        // (B) diagonal block on wavefront 0: lane r holds row J0 + r
+        if (tid < WAVE) {
+            double v[CHOL_NB];
+            const bool own = lane < jb;
+            const int rr = own ? rs(J0 + lane) + J0 : 0;
+#pragma unroll
+            for (int c = 0; c < CHOL_NB; c++) v[c] = (own && c <= lane) ? L[rr + c] : 0.0;
+            int bad = 0;
+#pragma unroll
+            for (int j = 0; j < CHOL_NB; j++) {
+                if (j < jb) {
+                    const double piv = bcast(v[j], j);
+                    bad |= piv <= 0.0;
+                    const double ljj = ::sqrt(piv);
+                    v[j] = lane == j ? ljj : (lane > j ? v[j] / ljj : v[j]);
+                    // columns c > j of the rows below c lose l_ij l_cj
+#pragma unroll
+                    for (int c = j + 1; c < CHOL_NB; c++) {
+                        const double lcj = bcast(v[j], c);
+                        if (lane >= c) v[c] -= v[j] * lcj;
+                    }
+                }
+            }
+            if (own)
+#pragma unroll
+                for (int c = 0; c < CHOL_NB; c++)
+                    if (c <= lane) L[rr + c] = v[c];
+            if (tid == 0) *flag = bad;
+        }
+        __syncthreads();
+        if (*flag) {
+            __syncthreads();
+            return false;
+        }
+        // (C) rows below the diagonal block: forward solve against it
+        for (int i = J0 + jb + tid; i < rows; i += GP_THREADS) {
+            const int ri0 = rs(i) + J0;
+            double v[CHOL_NB];
+#pragma unroll
+            for (int c = 0; c < CHOL_NB; c++) v[c] = c < jb ? L[ri0 + c] : 0.0;
+#pragma unroll
+            for (int j = 0; j < CHOL_NB; j++) {
+                if (j < jb) {
+                    const int rj0 = rs(J0 + j) + J0;
+                    double acc = v[j];
+#pragma unroll
+                    for (int k = 0; k < j; k++) acc -= v[k] * L[rj0 + k];
+                    v[j] = acc / L[rj0 + j];
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CHOL_NB; c++)
+                if (c < jb) L[ri0 + c] = v[c];
+        }
+        __syncthreads();
+    }
+    return true;
+}
+
 // One wavefront: b <- L^-1 b (rows lane + 64 r in registers); returns the
 // squared norm of the solution (wave-uniform order j = 0..m-1).
 __device__ __forceinline__ double wave_solve_l(const double* L, int m, double (&b)[GP_ROWS]) {

@@ -37,18 +37,23 @@ constexpr double OFF_EFOLD = 1.0857362047581294;    // 2.5 log10(e)
 constexpr double OFF_HALF = 0.7525749891599529;     // 2.5 log10(2)
 constexpr int FITC_TRI = NP_MAX_IND * (NP_MAX_IND + 1) / 2;
 constexpr int FITC_DOUBLES = 2 * FITC_TRI + NP_CHUNK * NP_MAX_IND + 2 * NP_CHUNK + NP_MAX_IND;
-constexpr int MISC_DOUBLES = 64;
+constexpr int MISC_DOUBLES = 48;
 
 __host__ __device__ constexpr int tri_n(int c) { return c * (c + 1) / 2; }
 
+__host__ __device__ inline int alpha_doubles(int cap) { return cap > 2 * NP_MAX_IND ? cap : 2 * NP_MAX_IND; }
+
+// r0 holds, in turn: the packed factor (tri_n(cap)), the FITC work area, the
+// fine grid (NP_FINE_MAX); its last alpha_doubles(cap) doubles are the weight
+// slot (alpha, or FITC inducing points + weights), written once the factor is
+// no longer read.
 __host__ __device__ inline int r0_doubles(int cap, int fitc) {
+    const int ac = alpha_doubles(cap);
     int r = tri_n(cap);
-    if (r < NP_FINE_MAX) r = NP_FINE_MAX;
-    if (fitc && r < FITC_DOUBLES) r = FITC_DOUBLES;
+    if (r < NP_FINE_MAX + ac) r = NP_FINE_MAX + ac;
+    if (fitc && r < FITC_DOUBLES + ac) r = FITC_DOUBLES + ac;
     return r;
 }
-
-__host__ __device__ inline int alpha_doubles(int cap) { return cap > 2 * NP_MAX_IND ? cap : 2 * NP_MAX_IND; }
 
 // misc slots
 enum {
@@ -56,24 +61,25 @@ enum {
     M_RED2 = 4,     // [4]
     M_SCAL = 8,     // scalar broadcast slots
     M_Q = 16,       // [9] predictive queries
-    M_INT = 32,     // int view starts here (32 doubles = 64 ints)
+    M_INT = 32,     // int view starts here (16 doubles = 32 ints)
 };
 
 struct Lds {
     double* r0;
-    double* alpha;
+    double* al;     // weight slot at the end of r0
+    double* xs;     // [cap] training times of a dense band
     double* misc;
     int* imisc;
-    int* idx;
 };
 
 __device__ __forceinline__ Lds carve(double* base, int cap, int fitc) {
     Lds s;
+    const int r0n = r0_doubles(cap, fitc);
     s.r0 = base;
-    s.alpha = s.r0 + r0_doubles(cap, fitc);
-    s.misc = s.alpha + alpha_doubles(cap);
+    s.al = base + r0n - alpha_doubles(cap);
+    s.xs = base + r0n;
+    s.misc = s.xs + cap;
     s.imisc = reinterpret_cast<int*>(s.misc + M_INT);
-    s.idx = reinterpret_cast<int*>(s.misc + MISC_DOUBLES);
     return s;
 }
 
@@ -90,21 +96,41 @@ __device__ __forceinline__ bool is_detection(double m, double e) {
     return flux > 0.0 && ferr > 0.0 && (flux / ferr) >= 3.0;
 }
 
-// Detections of the band (times / mags / errors through an index list).
+// Detections of the band: times (LDS when staged, else the global compacted
+// copy), magnitudes and errors (global compacted copies).
 struct Det {
-    const double *t, *y, *e;
-    const int* idx;
-    __device__ __forceinline__ double x(int i) const { return t[idx[i]]; }
-    __device__ __forceinline__ double yy(int i) const { return y[idx[i]]; }
-    __device__ __forceinline__ double ee(int i) const { return e[idx[i]]; }
+    const double *xs, *gx, *y, *e;  // gx: the times in global memory
+    __device__ __forceinline__ double x(int i) const { return xs[i]; }
+    __device__ __forceinline__ double xg(int i) const { return gx[i]; }
+    __device__ __forceinline__ double yy(int i) const { return y[i]; }
+    __device__ __forceinline__ double ee(int i) const { return e[i]; }
+    __device__ __forceinline__ double nv(int i) const { return fmax(e[i] * e[i], 1e-6); }
+};
+
+// The subsample of src/gp.rs:5-27 of the detections: k(i) = floor((i + 1/2)
+// nd / ms) (identity when nd <= ms); times staged in LDS in subsample order.
+struct DetSub {
+    const double *xs, *gx, *y, *e;
+    int nd, sub;
+    double step;
+    __device__ __forceinline__ int k(int i) const {
+        if (!sub) return i;
+        int q = (int)::floor(((double)i + 0.5) * step);
+        return q < nd - 1 ? q : nd - 1;
+    }
+    __device__ __forceinline__ double x(int i) const { return xs[i]; }
+    __device__ __forceinline__ double xg(int i) const { return gx[k(i)]; }
+    __device__ __forceinline__ double yy(int i) const { return y[k(i)]; }
     __device__ __forceinline__ double nv(int i) const {
-        double v = e[idx[i]];
+        double v = e[k(i)];
         return fmax(v * v, 1e-6);
     }
 };
 
-// The fitted predictor: dense over x = detections (idx list) or FITC over
-// inducing points z (LDS).
+// The fitted predictor: dense over x = detections (times in LDS) or FITC over
+// inducing points z (LDS).  Every pointer refers to LDS and none is null: a
+// null merged with an LDS address makes the compiler test the LDS aperture,
+// which it cannot encode here.
 struct Pred {
     int sparse, m;
     double amp, it, ybar;
@@ -114,12 +140,12 @@ struct Pred {
     const double* L;     // dense factor / FITC K_mm factor
     const double* Lb;    // FITC B factor
     __device__ __forceinline__ double xk(int k) const { return sparse ? z[k] : d.x(k); }
-    __device__ double mean(double t) const {
+    __device__ __forceinline__ double mean(double t) const {
         double dot = 0.0;
         for (int k = 0; k < m; k++) dot += rbf(t, xk(k), amp, it) * alpha[k];
         return dot + ybar;
     }
-    __device__ double dmean(double t) const {
+    __device__ __forceinline__ double dmean(double t) const {
         double s = 0.0;
         for (int k = 0; k < m; k++) {
             double xk_ = xk(k);
@@ -127,52 +153,92 @@ struct Pred {
         }
         return s;
     }
-    // One wavefront, wave-uniform t; returns the std in every lane.
-    __device__ double wave_std(double t) const {
+    // One wavefront, four wave-uniform query times at once (independent
+    // chains hide the latency of the lane reads); the std of each in every
+    // lane.  var = amp - |L^-1 k*|^2 (+ |L_B^-1 k*|^2 for FITC); the solve
+    // multiplies by 1 / l_jj instead of dividing (within an ulp per step; the
+    // reference divides).
+    __device__ __forceinline__ void wave_std4(const double (&tq)[4], double (&sd)[4]) const {
+        double q[4];
+        wave_sqnorm4(L, tq, q);
+        if (!sparse) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) sd[u] = ::sqrt(fmax(amp - q[u], 1e-10));
+            return;
+        }
+        double qb[4];
+        wave_sqnorm4(Lb, tq, qb);
+#pragma unroll
+        for (int u = 0; u < 4; u++) sd[u] = ::sqrt(fmax(amp - q[u] + qb[u], 1e-10));
+    }
+    __device__ __forceinline__ void wave_sqnorm4(const double* F, const double (&tq)[4], double (&q)[4]) const {
         const int lane = lane_id();
-        double b[GP_ROWS];
+        double b[4][GP_ROWS];
 #pragma unroll
         for (int r = 0; r < GP_ROWS; r++) {
-            int i = lane + WAVE * r;
-            b[r] = i < m ? rbf(t, xk(i), amp, it) : 0.0;
-        }
-        double b2[GP_ROWS];
+            const int i = lane + WAVE * r;
+            const double xi = i < m ? xk(i) : 0.0;
 #pragma unroll
-        for (int r = 0; r < GP_ROWS; r++) b2[r] = b[r];
-        double q = wave_solve_l(L, m, b);
-        if (!sparse) return ::sqrt(fmax(amp - q, 1e-10));
-        double qb = wave_solve_l(Lb, m, b2);
-        return ::sqrt(fmax(amp - q + qb, 1e-10));
+            for (int u = 0; u < 4; u++) b[u][r] = i < m ? rbf(tq[u], xi, amp, it) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) q[u] = 0.0;
+        for (int j = 0; j < m; j++) {
+            // the reciprocal depends on j only, so it is off the chain through b
+            const double rljj = 1.0 / F[tri(j, j)];
+            double z[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                z[u] = row_of(b[u], j) * rljj;
+                q[u] += z[u] * z[u];
+            }
+#pragma unroll
+            for (int r = 0; r < GP_ROWS; r++) {
+                const int i = lane + WAVE * r;
+                if (i > j && i < m) {
+                    const double lij = F[tri(i, j)];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) b[u][r] -= lij * z[u];
+                }
+            }
+        }
     }
 };
 
 // ---- dense GP (src/sparse_gp.rs:83-116, 168-175) ----------------------------
 
-__device__ bool dense_factor(double* L, const Det& d, int m, double amp, double it) {
-    const int tid = threadIdx.x;
-    const int mm = m * m;
-    for (int e = tid; e < mm; e += GP_THREADS) {
-        int i = e / m, j = e - i * m;
-        if (j <= i) {
-            double v = rbf(d.x(i), d.x(j), amp, it);
-            if (i == j) v += fmax(d.nv(i), 1e-10);
-            L[tri(i, j)] = v;
-        }
+// K + diag(max(nv, 1e-10)) with y - ybar appended as row m, factorised:
+// the factor in rows 0..m-1, L^-1 (y - ybar) left in row m.  Row m may cover
+// the staged training times (a band of cap detections): restage them before
+// reading x again.
+template <typename D>
+__device__ __forceinline__ bool dense_factor(double* L, const D& d, int m, double amp, double it, double ybar,
+                                             int* flag) {
+    const int ne = rs(m);
+    for (int e = threadIdx.x; e < ne; e += GP_THREADS) {
+        int i, j;
+        tri_ij(e, i, j);
+        double v = rbf(d.x(i), d.x(j), amp, it);
+        if (i == j) v += fmax(d.nv(i), 1e-10);
+        L[e] = v;
     }
     __syncthreads();
-    return chol_packed(L, m);
+    for (int j = threadIdx.x; j < m; j += GP_THREADS) L[ne + j] = d.yy(j) - ybar;
+    __syncthreads();
+    return chol_blocked(L, m, m + 1, flag);
 }
 
-// Wavefront 0: alpha = L^-T L^-1 (y - ybar).
-__device__ void dense_alpha(const double* L, const Det& d, int m, double ybar, double* alpha) {
+// Wavefront 0: alpha = L^-T z with z = L^-1 (y - ybar) from row m, written
+// to alpha after the solve (alpha may overlap the tail of L).
+__device__ __forceinline__ void dense_alpha(const double* L, int m, double* alpha) {
     const int lane = lane_id();
+    const int rz = rs(m);
     double b[GP_ROWS];
 #pragma unroll
     for (int r = 0; r < GP_ROWS; r++) {
         int i = lane + WAVE * r;
-        b[r] = i < m ? d.yy(i) - ybar : 0.0;
+        b[r] = i < m ? L[rz + i] : 0.0;
     }
-    wave_solve_l(L, m, b);
     wave_solve_lt(L, m, b);
 #pragma unroll
     for (int r = 0; r < GP_ROWS; r++) {
@@ -181,7 +247,15 @@ __device__ void dense_alpha(const double* L, const Det& d, int m, double ybar, d
     }
 }
 
-__device__ double dense_train_rms(const Det& d, int m, double amp, double it, double ybar, const double* alpha,
+// All threads: the training times back into LDS, then a barrier.
+template <typename D>
+__device__ __forceinline__ void restage(double* xs, const D& d, int m) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < m; i += GP_THREADS) xs[i] = d.xg(i);
+    __syncthreads();
+}
+
+__device__ __forceinline__ double dense_train_rms(const Det& d, int m, double amp, double it, double ybar, const double* alpha,
                                   double* red) {
     double rss = 0.0;
     for (int i = threadIdx.x; i < m; i += GP_THREADS) {
@@ -199,7 +273,8 @@ __device__ double dense_train_rms(const Det& d, int m, double amp, double it, do
 
 struct Fitc {
     double *lmm, *lb, *kc, *li, *lv, *w;  // in r0
-    double *z, *az;                        // in the alpha region
+    double *z, *az;                        // in the weight slot
+    int* flag;
 };
 
 __device__ __forceinline__ Fitc fitc_carve(const Lds& s) {
@@ -210,12 +285,13 @@ __device__ __forceinline__ Fitc fitc_carve(const Lds& s) {
     f.li = f.kc + NP_CHUNK * NP_MAX_IND;
     f.lv = f.li + NP_CHUNK;
     f.w = f.lv + NP_CHUNK;
-    f.z = s.alpha;
-    f.az = s.alpha + NP_MAX_IND;
+    f.z = s.al;
+    f.az = s.al + NP_MAX_IND;
+    f.flag = s.imisc + 20;
     return f;
 }
 
-__device__ bool fitc_fit(const Fitc& f, const Det& d, int nd, int m, double amp, double it, double ybar,
+__device__ __forceinline__ bool fitc_fit(const Fitc& f, const Det& d, int nd, int m, double amp, double it, double ybar,
                          double tmin, double tmax) {
     const int tid = threadIdx.x;
     if (tid < m) f.z[tid] = tmin + (double)tid * (tmax - tmin) / (double)(m - 1 > 1 ? m - 1 : 1);
@@ -231,7 +307,7 @@ __device__ bool fitc_fit(const Fitc& f, const Det& d, int nd, int m, double amp,
     }
     if (tid < m) f.w[tid] = 0.0;
     __syncthreads();
-    if (!chol_packed(f.lmm, m)) return false;
+    if (!chol_blocked(f.lmm, m, m, f.flag)) return false;
     for (int j0 = 0; j0 < nd; j0 += NP_CHUNK) {
         const int cn = nd - j0 < NP_CHUNK ? nd - j0 : NP_CHUNK;
         for (int e = tid; e < cn * m; e += GP_THREADS) {
@@ -283,7 +359,7 @@ __device__ bool fitc_fit(const Fitc& f, const Det& d, int nd, int m, double amp,
         }
         __syncthreads();
     }
-    if (!chol_packed(f.lb, m)) return false;
+    if (!chol_blocked(f.lb, m, m, f.flag)) return false;
     if (tid < WAVE) {
         double b[GP_ROWS];
 #pragma unroll
@@ -299,7 +375,7 @@ __device__ bool fitc_fit(const Fitc& f, const Det& d, int nd, int m, double amp,
     return true;
 }
 
-__device__ double fitc_nlml(const Fitc& f, const Det& d, int nd, int m, double amp, double it, double ybar,
+__device__ __forceinline__ double fitc_nlml(const Fitc& f, const Det& d, int nd, int m, double amp, double it, double ybar,
                             double* red) {
     double acc = 0.0;
     for (int i = threadIdx.x; i < nd; i += GP_THREADS) {
@@ -323,7 +399,7 @@ struct Curve {
 };
 
 // First i in [a, b) with m[i] >= thr, or INT_MAX.
-__device__ int first_at_least(const Curve& c, int a, int b, double thr, int* ired) {
+__device__ __forceinline__ int first_at_least(const Curve& c, int a, int b, double thr, int* ired) {
     int best = 0x7fffffff;
     for (int i = a + threadIdx.x; i < b; i += GP_THREADS)
         if (c.m[i] >= thr && i < best) best = i;
@@ -331,7 +407,7 @@ __device__ int first_at_least(const Curve& c, int a, int b, double thr, int* ire
 }
 
 // Last i in [a, b) with m[i] >= thr, or -1.
-__device__ int last_at_least(const Curve& c, int a, int b, double thr, int* ired) {
+__device__ __forceinline__ int last_at_least(const Curve& c, int a, int b, double thr, int* ired) {
     int best = -1;
     for (int i = a + threadIdx.x; i < b; i += GP_THREADS)
         if (c.m[i] >= thr && i > best) best = i;
@@ -339,7 +415,7 @@ __device__ int last_at_least(const Curve& c, int a, int b, double thr, int* ired
 }
 
 // Least-squares slope over i in [a, b) (all threads).
-__device__ double slope_range(const Curve& c, int a, int b, double tiny, double* red) {
+__device__ __forceinline__ double slope_range(const Curve& c, int a, int b, double tiny, double* red) {
     const int n = b - a;
     if (n < 2) return NAN;
     double st = 0, sm = 0, stt = 0, stm = 0;
@@ -361,7 +437,7 @@ __device__ double slope_range(const Curve& c, int a, int b, double tiny, double*
 }
 
 // compute_fine_grid_metrics' eleven metrics (FineGridMetrics order).
-__device__ void curve_metrics(const Curve& c, int pk, double* red, int* ired, double* o) {
+__device__ __forceinline__ void curve_metrics(const Curve& c, int pk, double* red, int* ired, double* o) {
     const int n = c.n;
     const double pm = c.m[pk];
     // decay crossings (find_decay_crossing)
@@ -505,7 +581,7 @@ __device__ __forceinline__ double finite_or_nan(double v) { return __builtin_isf
 
 // Inclusive prefix sum across the workgroup of one value per thread; returns
 // this thread's inclusive sum and sets *total.
-__device__ double block_scan(double v, double* red, double* total) {
+__device__ __forceinline__ double block_scan(double v, double* red, double* total) {
     const int lane = lane_id();
 #pragma unroll
     for (int off = 1; off < WAVE; off <<= 1) {
@@ -526,7 +602,7 @@ __device__ double block_scan(double v, double* red, double* total) {
     return before + v;
 }
 
-__device__ void np_band(const NpArgs& a, const Lds& s, int band) {
+__device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band) {
     const int tid = threadIdx.x;
     const int lane = lane_id(), wave = tid / WAVE;
     double* red = s.misc + M_RED;
@@ -544,8 +620,11 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
     const double* bt = a.t + o0;
     const double* bm = a.mag + o0;
     const double* be = a.err + o0;
-    int* det = a.det + o0;
+    double* dx = a.dx + o0;  // compacted detections (times, mags, errors)
+    double* dy = a.dy + o0;
+    double* de = a.de + o0;
     double* buf = a.buf + o0;
+    int* flag = s.imisc + 20;
 
     for (int k = tid; k < NP_NFEAT; k += GP_THREADS) feat[k] = k == 10 ? (double)n : NAN;
     for (int k = tid; k < ng; k += GP_THREADS) gpred[k] = gstd[k] = gdpred[k] = NAN;
@@ -570,7 +649,11 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
             if (k < wave) base += ired[k];
             tot += ired[k];
         }
-        if (dd) det[base + below] = i;
+        if (dd) {
+            dx[base + below] = bt[i];
+            dy[base + below] = bm[i];
+            de[base + below] = be[i];
+        }
         nd += tot;
     }
     __syncthreads();
@@ -579,10 +662,10 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
 
     // minimum length scale: 2 x median positive gap of the sorted detection times
     for (int i = tid; i < nd; i += GP_THREADS) {
-        const double v = bt[det[i]];
+        const double v = dx[i];
         int r = 0;
         for (int j = 0; j < nd; j++) {
-            double u = bt[det[j]];
+            double u = dx[j];
             r += (u < v) || (u == v && j < i);
         }
         buf[r] = v;
@@ -622,20 +705,21 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
         if (tid == 0) a.status[band] = -1;
         return;
     }
-    // detection indices of a dense band into LDS
+    // training times of a dense band into LDS
     if (!sparse)
-        for (int i = tid; i < nd; i += GP_THREADS) s.idx[i] = det[i];
+        for (int i = tid; i < nd; i += GP_THREADS) s.xs[i] = dx[i];
     __syncthreads();
-    const Det dall{bt, bm, be, sparse ? det : s.idx};
+    const Det dgl{dx, dx, dy, de};    // all detections, times from global memory
+    const Det dls{s.xs, dx, dy, de};  // a dense band's detections, times staged in LDS
     double ybar = 0.0;  // sequential, as the reference
-    for (int i = 0; i < nd; i++) ybar += dall.yy(i);
+    for (int i = 0; i < nd; i++) ybar += dy[i];
     ybar = ybar / (double)nd;
     double tmin = INFINITY, tmax = -INFINITY;
     if (sparse) {
         double lo = INFINITY, hi = -INFINITY;
         for (int i = tid; i < nd; i += GP_THREADS) {
-            lo = fmin(lo, dall.x(i));
-            hi = fmax(hi, dall.x(i));
+            lo = fmin(lo, dx[i]);
+            hi = fmax(hi, dx[i]);
         }
         tmin = block_min(lo, red);
         tmax = block_max(hi, red);
@@ -654,13 +738,14 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
         const double it = 0.5 / (ls * ls);
         double score;
         if (!sparse) {
-            if (!dense_factor(s.r0, dall, nd, amp, it)) continue;
-            if (tid < WAVE) dense_alpha(s.r0, dall, nd, ybar, s.alpha);
-            __syncthreads();
-            score = dense_train_rms(dall, nd, amp, it, ybar, s.alpha, red);
+            const bool ok = dense_factor(s.r0, dls, nd, amp, it, ybar, flag);
+            if (ok && tid < WAVE) dense_alpha(s.r0, nd, s.al);
+            restage(s.xs, dls, nd);
+            if (!ok) continue;
+            score = dense_train_rms(dls, nd, amp, it, ybar, s.al, red);
         } else {
-            if (!fitc_fit(fw, dall, nd, m_ind, amp, it, ybar, tmin, tmax)) continue;
-            score = fitc_nlml(fw, dall, nd, m_ind, amp, it, ybar, red);
+            if (!fitc_fit(fw, dgl, nd, m_ind, amp, it, ybar, tmin, tmax)) continue;
+            score = fitc_nlml(fw, dgl, nd, m_ind, amp, it, ybar, red);
         }
         if (__builtin_isfinite(score) && score < best) {
             best = score;
@@ -673,46 +758,37 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
     const double fls = fmax(dur / a.lsf[best_c % a.n_ls], 0.1);
     const double fit = 0.5 / (fls * fls);
 
-    // refit the winner; a FITC band first fits its subsampled dense GP
+    // refit the winner; a FITC band first fits its subsampled dense GP.  The
+    // dense weights go to the state row first: the factor is still needed
+    // for the standard deviations, and the weight slot overlaps its tail.
     Pred P;
     P.sparse = sparse;
     P.amp = famp;
     P.it = fit;
     P.ybar = ybar;
-    P.d = dall;
+    P.d = dls;
     if (!sparse) {
-        dense_factor(s.r0, dall, nd, famp, fit);
-        if (tid < WAVE) dense_alpha(s.r0, dall, nd, ybar, s.alpha);
-        __syncthreads();
-        P.m = nd;
-        P.z = nullptr;
-        P.alpha = s.alpha;
-        P.L = s.r0;
-        P.Lb = nullptr;
-        for (int i = tid; i < nd; i += GP_THREADS) {
-            st[i] = s.alpha[i];
-            st[a.state_cap + i] = dall.x(i);
-        }
+        dense_factor(s.r0, dls, nd, famp, fit, ybar, flag);
+        if (tid < WAVE) dense_alpha(s.r0, nd, st);
+        restage(s.xs, dls, nd);
+        for (int i = tid; i < nd; i += GP_THREADS) st[a.state_cap + i] = dx[i];
         if (tid == 0) {
             st[2 * a.state_cap] = famp;
             st[2 * a.state_cap + 1] = fit;
             st[2 * a.state_cap + 2] = ybar;
             st[2 * a.state_cap + 3] = (double)nd;
         }
+        __syncthreads();
+        P.m = nd;
+        P.z = fw.z;  // unused by a dense predictor (no null LDS pointers: see Pred)
+        P.alpha = s.al;
+        P.L = s.r0;
+        P.Lb = fw.lb;
     } else {
         const int ms = nd < a.max_sub ? nd : a.max_sub;
-        for (int i = tid; i < ms; i += GP_THREADS) {
-            int k;
-            if (nd <= a.max_sub) k = i;
-            else {
-                const double step = (double)nd / (double)a.max_sub;
-                k = (int)::floor(((double)i + 0.5) * step);
-                k = k < nd - 1 ? k : nd - 1;
-            }
-            s.idx[i] = det[k];
-        }
+        const DetSub dsub{s.xs, dx, dy, de, nd, nd > a.max_sub, (double)nd / (double)a.max_sub};
+        for (int i = tid; i < ms; i += GP_THREADS) s.xs[i] = dx[dsub.k(i)];
         __syncthreads();
-        const Det dsub{bt, bm, be, s.idx};
         double ys = 0.0;
         for (int i = 0; i < ms; i++) ys += dsub.yy(i);
         ys = ys / (double)ms;
@@ -720,13 +796,11 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
             const double amp = a.amps[c / a.n_ls];
             const double ls = fmax(dur / a.lsf[c % a.n_ls], 0.1);
             const double it = 0.5 / (ls * ls);
-            if (!dense_factor(s.r0, dsub, ms, amp, it)) continue;
-            if (tid < WAVE) dense_alpha(s.r0, dsub, ms, ys, s.alpha);
-            __syncthreads();
-            for (int i = tid; i < ms; i += GP_THREADS) {
-                st[i] = s.alpha[i];
-                st[a.state_cap + i] = dsub.x(i);
-            }
+            const bool ok = dense_factor(s.r0, dsub, ms, amp, it, ys, flag);
+            if (ok && tid < WAVE) dense_alpha(s.r0, ms, st);
+            restage(s.xs, dsub, ms);
+            if (!ok) continue;
+            for (int i = tid; i < ms; i += GP_THREADS) st[a.state_cap + i] = dsub.x(i);
             if (tid == 0) {
                 st[2 * a.state_cap] = amp;
                 st[2 * a.state_cap + 1] = it;
@@ -736,7 +810,7 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
             __syncthreads();
             break;
         }
-        if (!fitc_fit(fw, dall, nd, m_ind, famp, fit, ybar, tmin, tmax)) return;
+        if (!fitc_fit(fw, dgl, nd, m_ind, famp, fit, ybar, tmin, tmax)) return;
         P.m = m_ind;
         P.z = fw.z;
         P.alpha = fw.az;
@@ -745,22 +819,58 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
     }
     if (tid == 0) a.status[band] = sparse ? 2 : 1;
 
-    // 3. grid predictions, residuals, std rescale
+    // 3. predictive std (needs the factors) on the grid and at (a 200-point
+    // subsample of) the detection times
     const double gstep_den = (double)(ng - 1);
+    for (int k0 = 4 * wave; k0 < ng; k0 += 4 * GP_WAVES) {
+        double tq[4], sd[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int k = k0 + u < ng ? k0 + u : ng - 1;
+            tq[u] = t_lo + (double)k * dur / gstep_den;
+        }
+        P.wave_std4(tq, sd);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+            if (lane == u && k0 + u < ng) gstd[k0 + u] = sd[u];
+    }
+    double ssum = 0.0, scnt = 0.0;
+    {
+        const int ns = nd > 200 ? 200 : nd;
+        for (int k0 = 4 * wave; k0 < ns; k0 += 4 * GP_WAVES) {
+            double tq[4], sd[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int k = k0 + u < ns ? k0 + u : ns - 1;
+                const int i = nd > 200 ? (int)(((double)k + 0.5) * ((double)nd / 200.0)) : k;
+                tq[u] = dx[i];
+            }
+            P.wave_std4(tq, sd);
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (k0 + u < ns && __builtin_isfinite(sd[u])) {
+                    ssum += sd[u];
+                    scnt += 1.0;
+                }
+        }
+        // every lane of a wave holds the same partial: count it once per wave
+        ssum = block_sum(lane == 0 ? ssum : 0.0, red);
+        scnt = block_sum(lane == 0 ? scnt : 0.0, red);
+    }
+    if (!sparse) {  // dense weights into the slot, over the factor's tail
+        for (int i = tid; i < nd; i += GP_THREADS) s.al[i] = st[i];
+        __syncthreads();
+    }
+    // predictive mean and derivative on the grid, residuals at the detections
     for (int k = tid; k < ng; k += GP_THREADS) {
         const double tq = t_lo + (double)k * dur / gstep_den;
         gpred[k] = P.mean(tq);
         gdpred[k] = P.dmean(tq);
     }
-    for (int k = wave; k < ng; k += GP_WAVES) {
-        const double tq = t_lo + (double)k * dur / gstep_den;
-        double sd = P.wave_std(tq);
-        if (lane == 0) gstd[k] = sd;
-    }
     double rss = 0.0, chi = 0.0, bv = 0.0, omin = INFINITY, snr = -INFINITY;
     for (int i = tid; i < nd; i += GP_THREADS) {
-        const double y = dall.yy(i), e = dall.ee(i);
-        const double r = y - P.mean(dall.x(i));
+        const double y = dy[i], e = de[i];
+        const double r = y - P.mean(dx[i]);
         const double e2 = e * e + 1e-10;
         rss += r * r;
         chi += r * r / e2;
@@ -775,26 +885,12 @@ __device__ void np_band(const NpArgs& a, const Lds& s, int band) {
     snr = block_max(snr, red);
     const double rms = ::sqrt(rss / (double)nd);
     {
-        const int ns = nd > 200 ? 200 : nd;
-        double ssum = 0.0;
-        int scnt = 0;
-        for (int k = wave; k < ns; k += GP_WAVES) {
-            const int i = nd > 200 ? (int)(((double)k + 0.5) * ((double)nd / 200.0)) : k;
-            double sd = P.wave_std(dall.x(i));
-            if (__builtin_isfinite(sd)) {
-                ssum += sd;
-                scnt++;
-            }
-        }
-        // every lane of a wave holds the same partial: count it once per wave
-        ssum = block_sum(lane == 0 ? ssum : 0.0, red);
-        const double cnt = block_sum(lane == 0 ? (double)scnt : 0.0, red);
         double scale = 1.0;
-        if (cnt > 0.0) {
-            const double mstd = ssum / cnt;
+        if (scnt > 0.0) {
+            const double mstd = ssum / scnt;
             if (mstd > 1e-12 && __builtin_isfinite(rms)) scale = fmin(fmax(rms / mstd, 0.05), 5.0) * 0.6;
         }
-        __syncthreads();  // grid std written
+        __syncthreads();  // grid mean and std written
         if (scale != 1.0)
             for (int k = tid; k < ng; k += GP_THREADS) gstd[k] *= scale;
         __syncthreads();
@@ -1189,8 +1285,7 @@ __global__ __launch_bounds__(GP_THREADS) void np_band_kernel(NpArgs a) {
 }
 
 size_t np_lds_bytes(int cap, int fitc) {
-    return sizeof(double) * ((size_t)r0_doubles(cap, fitc) + (size_t)alpha_doubles(cap) + MISC_DOUBLES) +
-           sizeof(int) * (size_t)cap;
+    return sizeof(double) * ((size_t)r0_doubles(cap, fitc) + (size_t)cap + MISC_DOUBLES);
 }
 
 hipError_t launch_np(const NpArgs& a, int grid, hipStream_t st) {

@@ -34,7 +34,9 @@ struct NpArgs {
     int cap;                 // dense capacity of this launch's LDS
     int fitc;                // this launch's LDS holds the FITC work area
     int state_cap;           // state rows: alpha[state_cap], x[state_cap], amp, 1/(2 l^2), mean, m
-    int32_t* det;            // [n_obs] scratch: detection indices, band-relative, CSR aligned
+    double* dx;              // [n_obs] scratch: detections (times, mags, errors), CSR aligned
+    double* dy;
+    double* de;
     double* buf;             // [n_obs] scratch
     int32_t* status;         // [n_bands] 0 no result, 1 dense, 2 FITC
     double* feat;            // [n_bands][NP_NFEAT]

@@ -1006,14 +1006,16 @@ int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, con
     for (auto& q : cls) order.insert(order.end(), q.bands.begin(), q.bands.end());
     const int stride = 2 * state_cap + 4;
     const int64_t ng = n_grid;
-    DevBuf<double> dt, dm, de, dlo, dhi, da, dl, dfeat, dp, ds, dd, dst, dbuf;
+    DevBuf<double> dt, dm, de, dlo, dhi, da, dl, dfeat, dp, ds, dd, dst, dbuf, ddx, ddy, dde;
     DevBuf<int64_t> doff;
-    DevBuf<int32_t> dord, dstat, ddet;
+    DevBuf<int32_t> dord, dstat;
     HIPCHK(c, dt.alloc(n));
     HIPCHK(c, dm.alloc(n));
     HIPCHK(c, de.alloc(n));
     HIPCHK(c, dbuf.alloc(n));
-    HIPCHK(c, ddet.alloc(n));
+    HIPCHK(c, ddx.alloc(n));
+    HIPCHK(c, ddy.alloc(n));
+    HIPCHK(c, dde.alloc(n));
     HIPCHK(c, dlo.alloc(n_bands));
     HIPCHK(c, dhi.alloc(n_bands));
     HIPCHK(c, da.alloc(n_amp));
@@ -1055,7 +1057,9 @@ int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, con
     a.max_sub = max_subsample;
     a.n_ind = n_inducing;
     a.state_cap = state_cap;
-    a.det = ddet.p;
+    a.dx = ddx.p;
+    a.dy = ddy.p;
+    a.de = dde.p;
     a.buf = dbuf.p;
     a.status = dstat.p;
     a.feat = dfeat.p;
