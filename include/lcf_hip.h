@@ -167,6 +167,28 @@ int lcf_batch_nonparametric(lcf_ctx* ctx, int64_t n_bands, const int64_t* offset
                             int n_inducing, int32_t* out_status, double* out_features, double* out_pred,
                             double* out_std, double* out_dpred, double* out_state);
 
+/* ---- blackbody colour-temperature fit (FP64) ------------------------------ */
+/* fit_thermal (src/thermal.rs:222-418) for n_src sources at once, after the
+ * host's band choice (reference band = bluest of u > g > r with >= 5 points;
+ * other bands with a known effective wavelength): reference-band
+ * observations (CSR ref_off: times, mags, errors) for fit_ref_band_gp when no
+ * prefit GP is given; the other bands' observations (CSR col_off) with their
+ * wavelengths; lambda_ref and the source's duration.  gp_state rows
+ * (2 state_cap + 4 doubles: alpha[state_cap], x[state_cap], amp, 1/(2 l^2),
+ * mean, m; state_cap >= 32) hold the prefit reference GP (e.g. a row of
+ * lcf_batch_nonparametric) or m = 0 to fit it on the device (the row is then
+ * filled).  out_status: 0 None, 1 result without fit (fewer than 3 colours),
+ * 2 fitted; out [n_src][8]: log_temp_peak, log_temp_latest, cooling_rate,
+ * log_temp_peak_err, cooling_rate_err, chi2, best cost, n_color_obs;
+ * out_col_used[i] = 1 for the colours used.  The two particle-swarm runs use
+ * Philox4x32-10 keyed by seed. */
+int lcf_batch_thermal(lcf_ctx* ctx, int64_t n_src, const int64_t* ref_offsets, const double* ref_times,
+                      const double* ref_mags, const double* ref_errs, const int64_t* col_offsets,
+                      const double* col_times, const double* col_mags, const double* col_errs,
+                      const double* col_lambda, const double* lambda_ref, const double* duration,
+                      double* gp_state, int state_cap, uint64_t seed, int32_t* out_status, double* out,
+                      int32_t* out_col_used);
+
 /* ---- batched SVI ----------------------------------------------------------- */
 /* SVI flags */
 #define LCF_SVI_CPU_STOPPING 1u  /* early stop after 200 steps / stall 50 (svi_fit constants,

@@ -7,6 +7,7 @@
 #include "gp_kernels.hpp"
 #include "laplace_kernels.hpp"
 #include "np_kernels.hpp"
+#include "thermal_kernels.hpp"
 
 #include <algorithm>
 #include <cstdarg>
@@ -1102,6 +1103,107 @@ int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, con
     for (int64_t b = 0; b < n_bands; b++)
         if (out_status[b] < 0) return fail(c, LCF_E_INTERNAL, "band %lld was routed to a launch too small for it",
                                            (long long)b);
+    return LCF_OK;
+}
+
+int lcf_batch_thermal(lcf_ctx* c, int64_t n_src, const int64_t* ref_off, const double* ref_t,
+                      const double* ref_m, const double* ref_e, const int64_t* col_off, const double* col_t,
+                      const double* col_m, const double* col_e, const double* col_lambda,
+                      const double* lambda_ref, const double* duration, double* gp_state, int state_cap,
+                      uint64_t seed, int32_t* out_status, double* out, int32_t* out_col_used) {
+    if (!c) return LCF_E_INVALID;
+    if (n_src < 0 || n_src > INT32_MAX - 1 || !ref_off || !col_off)
+        return fail(c, LCF_E_INVALID, "invalid source count");
+    if (n_src == 0) return LCF_OK;
+    if (state_cap < lcf::TH_MAX_GP)
+        return fail(c, LCF_E_INVALID, "state rows need a capacity of at least %d points, got %d", lcf::TH_MAX_GP,
+                    state_cap);
+    if (ref_off[0] != 0 || col_off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
+    for (int64_t s = 0; s < n_src; s++)
+        if (ref_off[s + 1] < ref_off[s] || col_off[s + 1] < col_off[s])
+            return fail(c, LCF_E_INVALID, "offsets not non-decreasing at %lld", (long long)s);
+    const int64_t nr = ref_off[n_src], nc = col_off[n_src];
+    if ((nr > 0 && (!ref_t || !ref_m || !ref_e)) || (nc > 0 && (!col_t || !col_m || !col_e || !col_lambda)) ||
+        !lambda_ref || !duration || !gp_state)
+        return fail(c, LCF_E_INVALID, "null input array");
+    if (!out_status || !out || (nc > 0 && !out_col_used)) return fail(c, LCF_E_INVALID, "null output");
+    const int stride = 2 * state_cap + 4;
+    for (int64_t s = 0; s < n_src; s++) {
+        const double m = gp_state[s * stride + 2 * state_cap + 3];
+        if (!(m >= 0.0 && m <= state_cap) || m != (double)(int64_t)m)
+            return fail(c, LCF_E_INVALID, "state row %lld: point count %g outside [0, %d]", (long long)s, m,
+                        state_cap);
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf<double> drt, drm, dre, dct, dcm, dce, dcl, dlr, ddu, dst, dsc, dout;
+    DevBuf<int64_t> dro, dco;
+    DevBuf<int32_t> dstat, dused;
+    auto up = [&](void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, drt.alloc(nr));
+    HIPCHK(c, drm.alloc(nr));
+    HIPCHK(c, dre.alloc(nr));
+    HIPCHK(c, dct.alloc(nc));
+    HIPCHK(c, dcm.alloc(nc));
+    HIPCHK(c, dce.alloc(nc));
+    HIPCHK(c, dcl.alloc(nc));
+    HIPCHK(c, dsc.alloc(4 * nc));
+    HIPCHK(c, dused.alloc(nc));
+    HIPCHK(c, dlr.alloc(n_src));
+    HIPCHK(c, ddu.alloc(n_src));
+    HIPCHK(c, dst.alloc(n_src * stride));
+    HIPCHK(c, dout.alloc(n_src * 8));
+    HIPCHK(c, dstat.alloc(n_src));
+    HIPCHK(c, dro.alloc(n_src + 1));
+    HIPCHK(c, dco.alloc(n_src + 1));
+    HIPCHK(c, up(drt.p, ref_t, sizeof(double) * nr));
+    HIPCHK(c, up(drm.p, ref_m, sizeof(double) * nr));
+    HIPCHK(c, up(dre.p, ref_e, sizeof(double) * nr));
+    HIPCHK(c, up(dct.p, col_t, sizeof(double) * nc));
+    HIPCHK(c, up(dcm.p, col_m, sizeof(double) * nc));
+    HIPCHK(c, up(dce.p, col_e, sizeof(double) * nc));
+    HIPCHK(c, up(dcl.p, col_lambda, sizeof(double) * nc));
+    HIPCHK(c, up(dlr.p, lambda_ref, sizeof(double) * n_src));
+    HIPCHK(c, up(ddu.p, duration, sizeof(double) * n_src));
+    HIPCHK(c, up(dst.p, gp_state, sizeof(double) * n_src * stride));
+    HIPCHK(c, up(dro.p, ref_off, sizeof(int64_t) * (n_src + 1)));
+    HIPCHK(c, up(dco.p, col_off, sizeof(int64_t) * (n_src + 1)));
+    lcf::ThermalArgs a{};
+    a.n_src = (int)n_src;
+    a.ref_off = dro.p;
+    a.ref_t = drt.p;
+    a.ref_m = drm.p;
+    a.ref_e = dre.p;
+    a.col_off = dco.p;
+    a.col_t = dct.p;
+    a.col_m = dcm.p;
+    a.col_e = dce.p;
+    a.col_lambda = dcl.p;
+    a.lambda_ref = dlr.p;
+    a.duration = ddu.p;
+    a.gp_state = dst.p;
+    a.cap = state_cap;
+    a.seed = seed;
+    a.scratch = dsc.p;
+    a.status = dstat.p;
+    a.out = dout.p;
+    a.col_used = dused.p;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, lcf::launch_thermal(a, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    auto down = [&](void* h, const void* d, size_t bytes) {
+        return bytes ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, down(out_status, dstat.p, sizeof(int32_t) * n_src));
+    HIPCHK(c, down(out, dout.p, sizeof(double) * n_src * 8));
+    HIPCHK(c, down(out_col_used, dused.p, sizeof(int32_t) * nc));
+    HIPCHK(c, down(gp_state, dst.p, sizeof(double) * n_src * stride));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return LCF_OK;
 }
 

@@ -437,6 +437,28 @@ class GpuContext:
             _lib.ptr(out["dpred"], C.c_double), _lib.ptr(out["state"], C.c_double)))
         return out
 
+    def thermal_arrays(self, pack: dict, seed: int = 42) -> dict:
+        """Array form of the colour-temperature fit (``lcf_batch_thermal``):
+        ``pack`` as built by ``thermal.pack_sources``; returns status, out
+        [n_src, 8], col_used and the (filled) gp_state rows."""
+        ro, co = (np.ascontiguousarray(np.asarray(pack[k], dtype=np.int64)) for k in ("ref_off", "col_off"))
+        ns = ro.size - 1
+        st = np.array(pack["gp_state"], dtype=np.float64, copy=True, order="C")
+        cap = (st.shape[1] - 4) // 2
+        arrs = [_f64(pack[k]) for k in ("ref_t", "ref_m", "ref_e", "col_t", "col_m", "col_e", "col_lambda",
+                                        "lambda_ref", "duration")]
+        out = {"status": np.zeros(ns, np.int32), "out": np.zeros((ns, 8)),
+               "col_used": np.zeros(max(int(co[-1]), 1), np.int32)}
+        p = [_lib.ptr(x, C.c_double) for x in arrs]
+        self._check(self._lib.lcf_batch_thermal(
+            self._h, ns, _lib.ptr(ro, C.c_int64), p[0], p[1], p[2], _lib.ptr(co, C.c_int64), p[3], p[4], p[5], p[6],
+            p[7], p[8], _lib.ptr(st, C.c_double), cap, C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF),
+            _lib.ptr(out["status"], C.c_int32), _lib.ptr(out["out"], C.c_double),
+            _lib.ptr(out["col_used"], C.c_int32)))
+        out["col_used"] = out["col_used"][:int(co[-1])]
+        out["gp_state"] = st
+        return out
+
 def _gp2d_csr(sources) -> dict:
     off = [0]
     for s in sources:

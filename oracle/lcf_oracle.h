@@ -206,6 +206,23 @@ int orc_np_grid_features(const double* tg, const double* p, const double* sd, in
 int orc_sparse_gp(const double* t, const double* y, const double* nv, int n, double amp, double ls, int m,
                   const double* q, int nq, double* mean, double* stdv, double* nlml);
 
+/* ---- blackbody colour-temperature fit (src/thermal.rs:222-418) ---------- */
+/* bb_color_mag (src/thermal.rs:75-92). */
+double orc_bb_color_mag(double lambda_ref, double lambda, double temp);
+/* fit_thermal per source after the host's band choice: reference-band
+ * observations (CSR ref_off) for fit_ref_band_gp, the other bands'
+ * observations with their wavelengths (CSR col_off), lambda_ref, duration,
+ * gp_state rows (alpha[cap], x[cap], amp, 1/(2 l^2), mean, m; m = 0: fit the
+ * reference GP here, the row is then filled).  status: 0 None, 1 result
+ * without a fit (fewer than 3 colours), 2 fitted; out[8]: log_temp_peak,
+ * log_temp_latest, cooling_rate, log_temp_peak_err, cooling_rate_err, chi2,
+ * best cost, n_color_obs; col_used[i] = 1 for colours kept. */
+int orc_batch_thermal(int64_t n_src, const int64_t* ref_off, const double* ref_t, const double* ref_m,
+                      const double* ref_e, const int64_t* col_off, const double* col_t, const double* col_m,
+                      const double* col_e, const double* col_lambda, const double* lambda_ref,
+                      const double* duration, double* gp_state, int state_stride, uint64_t seed, int32_t* status,
+                      double* out, int32_t* col_used, int n_threads);
+
 /* ---- Laplace (src/parametric.rs:2464-2693) ---------------------------------- */
 #define ORC_LAPLACE_FD4 0      /* reference: 4-point central differences of f */
 #define ORC_LAPLACE_GRAD_FD 1  /* central differences of the analytic gradient */

@@ -510,3 +510,37 @@ def sparse_gp(t, y, nv, amp: float, ls: float, m: int, query):
     if rc != 0:
         return None
     return mean, std, nl.value
+
+
+# ---- blackbody colour-temperature fit (src/thermal.rs:222-418) -------------
+THERMAL_OUT = ("log_temp_peak", "log_temp_latest", "cooling_rate", "log_temp_peak_err", "cooling_rate_err",
+               "chi2", "best_cost", "n_color_obs")
+
+
+def bb_color_mag(lambda_ref: float, lam: float, temp: float) -> float:
+    L = lib()
+    L.orc_bb_color_mag.restype = _f64
+    L.orc_bb_color_mag.argtypes = [_f64, _f64, _f64]
+    return L.orc_bb_color_mag(lambda_ref, lam, temp)
+
+
+def batch_thermal(pack: dict, seed: int = 42, n_threads: int | None = None) -> dict:
+    """fit_thermal over packed sources (see lcf_oracle.h); gp_state is copied."""
+    ro, co = (np.ascontiguousarray(np.asarray(pack[k], dtype=np.int64)) for k in ("ref_off", "col_off"))
+    ns = ro.size - 1
+    st = np.array(pack["gp_state"], dtype=np.float64, copy=True, order="C")
+    status = np.zeros(ns, np.int32)
+    out = np.zeros((ns, 8))
+    used = np.zeros(max(int(co[-1]), 1), np.int32)
+    arrs = [f64(pack[k]) for k in ("ref_t", "ref_m", "ref_e", "col_t", "col_m", "col_e", "col_lambda",
+                                   "lambda_ref", "duration")]
+    L = lib()
+    L.orc_batch_thermal.restype = _i32
+    L.orc_batch_thermal.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _i32,
+                                    _u64, _pi32, _pd, _pi32, _i32]
+    p = [_p(a, C.c_double) for a in arrs]
+    rc = L.orc_batch_thermal(ns, _p(ro, C.c_int64), p[0], p[1], p[2], _p(co, C.c_int64), p[3], p[4], p[5], p[6],
+                             p[7], p[8], _p(st, C.c_double), st.shape[1], seed, _p(status, C.c_int32),
+                             _p(out, C.c_double), _p(used, C.c_int32), n_threads or default_threads())
+    assert rc == 0
+    return {"status": status, "out": out, "col_used": used[:int(co[-1])], "gp_state": st}
