@@ -9,6 +9,8 @@ an upper limit when ``flux/err < 3`` and then ``upper_flux = 3 err / peak``.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 # Models of the ragged model-selection workload: Bazin, Villar, Tde, Arnett,
@@ -237,3 +239,52 @@ def bazin_mag_batch(n_sources: int, seed: int = 0, n_bands: int = 3, n_min: int 
     np.maximum.at(tmax, src_of, t)
     return {"offsets": off, "times": t, "mags": mags, "mag_errs": errs, "band_source": band_src,
             "t_lo": np.zeros(nb), "t_hi": tmax[band_src]}
+
+
+class _XorShift64:
+    """xorshift64 with 53-bit uniforms and Box-Muller normals, the generator
+    of the reference's test fixtures (tests/synthetic.rs:17-46)."""
+
+    M = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.s = max(int(seed), 1) & self.M
+
+    def next_u64(self) -> int:
+        x = self.s
+        x ^= (x << 13) & self.M
+        x ^= x >> 7
+        x ^= (x << 17) & self.M
+        self.s = x
+        return x
+
+    def uniform(self) -> float:
+        return (self.next_u64() >> 11) / float(1 << 53)
+
+    def normal(self) -> float:
+        u1 = max(self.uniform(), 1e-15)
+        u2 = self.uniform()
+        return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def reference_test_source(n_per_band: int, seed: int):
+    """generate_bazin_source (tests/synthetic.rs:53-91): g, r, i bands of
+    n_per_band points on [0, 100) d after JD 2460000, Bazin flux A = 1,
+    B = 0.01, t0 = 30, tau_rise = 3, tau_fall = 25 as magnitudes (25 where the
+    flux is not positive), colour offsets -0.5 / 0 / +0.3, errors U(0.05,
+    0.08) and Gaussian noise; returns (times, mags, mag_errs, bands)."""
+    rng = _XorShift64(seed)
+    jd = 2460000.0
+    times, mags, errs, bands = [], [], [], []
+    for name, off in (("g", -0.5), ("r", 0.0), ("i", 0.3)):
+        for j in range(n_per_band):
+            t = jd + j * 100.0 / n_per_band
+            dt = (t - jd) - 30.0
+            flux = 1.0 * math.exp(-dt / 25.0) / (1.0 + math.exp(-dt / 3.0)) + 0.01
+            mag = (-2.5 * math.log10(flux) + 23.9 if flux > 0.0 else 25.0) + off
+            sigma = 0.05 + 0.03 * rng.uniform()
+            times.append(t)
+            mags.append(mag + sigma * rng.normal())
+            errs.append(sigma)
+            bands.append(name)
+    return times, mags, errs, bands
