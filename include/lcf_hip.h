@@ -189,6 +189,16 @@ int lcf_batch_thermal(lcf_ctx* ctx, int64_t n_src, const int64_t* ref_offsets, c
                       double* gp_state, int state_cap, uint64_t seed, int32_t* out_status, double* out,
                       int32_t* out_col_used);
 
+/* fit_bb_temperature (src/gp2d.rs:275-324) for n_sets SEDs (CSR off:
+ * wavelengths in Angstrom, magnitudes, errors): the 200-point log10 T grid
+ * on [3, 5.5] with the weighted least-squares offset, chi2 / max(n - 2, 1);
+ * out_ok[s] = 0 (outputs NaN) for a set of fewer than 2 points.  Used on the
+ * 2D GP predictions by extract_thermal_from_gp2d (src/gp2d.rs:445-476) and
+ * fit_gp_2d_batch_gpu (src/gp2d.rs:516-724). */
+int lcf_batch_bb_temperature(lcf_ctx* ctx, int64_t n_sets, const int64_t* offsets, const double* wavelengths,
+                             const double* mags, const double* errs, double* out_log_t, double* out_chi2,
+                             double* out_offset, int32_t* out_ok);
+
 /* ---- batched SVI ----------------------------------------------------------- */
 /* SVI flags */
 #define LCF_SVI_CPU_STOPPING 1u  /* early stop after 200 steps / stall 50 (svi_fit constants,

@@ -8,6 +8,7 @@
 #include "laplace_kernels.hpp"
 #include "np_kernels.hpp"
 #include "thermal_kernels.hpp"
+#include "bb_kernels.hpp"
 
 #include <algorithm>
 #include <cstdarg>
@@ -1203,6 +1204,53 @@ int lcf_batch_thermal(lcf_ctx* c, int64_t n_src, const int64_t* ref_off, const d
     HIPCHK(c, down(out, dout.p, sizeof(double) * n_src * 8));
     HIPCHK(c, down(out_col_used, dused.p, sizeof(int32_t) * nc));
     HIPCHK(c, down(gp_state, dst.p, sizeof(double) * n_src * stride));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+int lcf_batch_bb_temperature(lcf_ctx* c, int64_t n_sets, const int64_t* off, const double* wavelengths,
+                             const double* mags, const double* errs, double* out_log_t, double* out_chi2,
+                             double* out_offset, int32_t* out_ok) {
+    if (!c) return LCF_E_INVALID;
+    if (n_sets < 0 || !off) return fail(c, LCF_E_INVALID, "invalid set count");
+    if (n_sets == 0) return LCF_OK;
+    if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
+    for (int64_t s = 0; s < n_sets; s++)
+        if (off[s + 1] < off[s]) return fail(c, LCF_E_INVALID, "offsets not non-decreasing at %lld", (long long)s);
+    const int64_t n = off[n_sets];
+    if (n > 0 && (!wavelengths || !mags || !errs)) return fail(c, LCF_E_INVALID, "null input array");
+    if (!out_log_t || !out_chi2 || !out_offset || !out_ok) return fail(c, LCF_E_INVALID, "null output");
+    HIPCHK(c, hipSetDevice(c->device));
+    DevBuf<double> dl, dm, de, dlt, dch, dof;
+    DevBuf<int64_t> doff;
+    DevBuf<int32_t> dok;
+    HIPCHK(c, dl.alloc(n));
+    HIPCHK(c, dm.alloc(n));
+    HIPCHK(c, de.alloc(n));
+    HIPCHK(c, dlt.alloc(n_sets));
+    HIPCHK(c, dch.alloc(n_sets));
+    HIPCHK(c, dof.alloc(n_sets));
+    HIPCHK(c, dok.alloc(n_sets));
+    HIPCHK(c, doff.alloc(n_sets + 1));
+    auto up = [&](void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, up(dl.p, wavelengths, sizeof(double) * n));
+    HIPCHK(c, up(dm.p, mags, sizeof(double) * n));
+    HIPCHK(c, up(de.p, errs, sizeof(double) * n));
+    HIPCHK(c, up(doff.p, off, sizeof(int64_t) * (n_sets + 1)));
+    lcf::BbArgs a{n_sets, doff.p, dl.p, dm.p, de.p, dlt.p, dch.p, dof.p, dok.p};
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, lcf::launch_bb(a, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    HIPCHK(c, hipMemcpyAsync(out_log_t, dlt.p, sizeof(double) * n_sets, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_chi2, dch.p, sizeof(double) * n_sets, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_offset, dof.p, sizeof(double) * n_sets, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_ok, dok.p, sizeof(int32_t) * n_sets, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LCF_OK;
 }

@@ -459,6 +459,19 @@ class GpuContext:
         out["gp_state"] = st
         return out
 
+    def bb_temperature(self, offsets, wavelengths, mags, errs) -> dict:
+        """``lcf_batch_bb_temperature``: blackbody log10 T, chi2 and offset of
+        every SED (CSR offsets); ok = 0 for sets of fewer than 2 points."""
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.int64))
+        ns = off.size - 1
+        lam, m, e = _f64(wavelengths), _f64(mags), _f64(errs)
+        out = {"log_t": np.zeros(ns), "chi2": np.zeros(ns), "offset": np.zeros(ns), "ok": np.zeros(ns, np.int32)}
+        self._check(self._lib.lcf_batch_bb_temperature(
+            self._h, ns, _lib.ptr(off, C.c_int64), _lib.ptr(lam, C.c_double), _lib.ptr(m, C.c_double),
+            _lib.ptr(e, C.c_double), _lib.ptr(out["log_t"], C.c_double), _lib.ptr(out["chi2"], C.c_double),
+            _lib.ptr(out["offset"], C.c_double), _lib.ptr(out["ok"], C.c_int32)))
+        return out
+
 def _gp2d_csr(sources) -> dict:
     off = [0]
     for s in sources:

@@ -223,6 +223,10 @@ int orc_batch_thermal(int64_t n_src, const int64_t* ref_off, const double* ref_t
                       const double* duration, double* gp_state, int state_stride, uint64_t seed, int32_t* status,
                       double* out, int32_t* col_used, int n_threads);
 
+/* fit_bb_temperature (src/gp2d.rs:275-324): out3 = log10 T, chi2, offset;
+ * returns 1 (no fit) for fewer than 2 points. */
+int orc_fit_bb_temperature(const double* lam, const double* mag, const double* err, int n, double* out3);
+
 /* ---- Laplace (src/parametric.rs:2464-2693) ---------------------------------- */
 #define ORC_LAPLACE_FD4 0      /* reference: 4-point central differences of f */
 #define ORC_LAPLACE_GRAD_FD 1  /* central differences of the analytic gradient */

@@ -544,3 +544,15 @@ def batch_thermal(pack: dict, seed: int = 42, n_threads: int | None = None) -> d
                              _p(out, C.c_double), _p(used, C.c_int32), n_threads or default_threads())
     assert rc == 0
     return {"status": status, "out": out, "col_used": used[:int(co[-1])], "gp_state": st}
+
+
+def fit_bb_temperature(lam, mag, err):
+    """(log10 T, chi2, offset) or None (src/gp2d.rs:275-324)."""
+    lam, mag, err = f64(lam), f64(mag), f64(err)
+    out = np.zeros(3)
+    L = lib()
+    L.orc_fit_bb_temperature.restype = _i32
+    L.orc_fit_bb_temperature.argtypes = [_pd, _pd, _pd, _i32, _pd]
+    rc = L.orc_fit_bb_temperature(_p(lam, C.c_double), _p(mag, C.c_double), _p(err, C.c_double), lam.size,
+                                  _p(out, C.c_double))
+    return None if rc else tuple(out)

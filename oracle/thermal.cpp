@@ -296,3 +296,48 @@ int orc_batch_thermal(int64_t n_src, const int64_t* ref_off, const double* ref_t
 }
 
 }  // extern "C"
+
+// ---- fit_bb_temperature (src/gp2d.rs:264-324) --------------------------------
+namespace orc {
+namespace bb {
+
+static double planck_relative_mag(double lam, double temp) {  // src/gp2d.rs:264-271
+    const double HC = 1.4387769e8;
+    double x = HC / (lam * temp);
+    double log_b = -5.0 * std::log(lam) - std::log(std::exp(x) - 1.0);
+    return -2.5 * log_b / 2.302585092994045684017991454684364208;
+}
+
+}  // namespace bb
+}  // namespace orc
+
+extern "C" int orc_fit_bb_temperature(const double* lam, const double* mag, const double* err, int n,
+                                      double* out3) {
+    if (n < 2) return 1;
+    double best_chi2 = INFINITY, best_lt = 4.0, best_off = 0.0;
+    const double nf = n;
+    for (int i = 0; i < 200; i++) {
+        double lt = 3.0 + (double)i * 2.5 / (200.0 - 1.0);
+        double temp = std::pow(10.0, lt);
+        std::vector<double> model(n);
+        for (int j = 0; j < n; j++) model[j] = orc::bb::planck_relative_mag(lam[j], temp);
+        double sw = 0.0, swd = 0.0;
+        for (int j = 0; j < n; j++) {
+            double w = 1.0 / std::max(err[j] * err[j], 1e-10);
+            sw += w;
+            swd += w * (mag[j] - model[j]);
+        }
+        double off = swd / sw;
+        double chi = 0.0;
+        for (int j = 0; j < n; j++) {
+            double r = mag[j] - model[j] - off;
+            chi += r * r / std::max(err[j] * err[j], 1e-10);
+        }
+        chi = chi / std::max(nf - 2.0, 1.0);
+        if (chi < best_chi2) { best_chi2 = chi; best_lt = lt; best_off = off; }
+    }
+    out3[0] = best_lt;
+    out3[1] = best_chi2;
+    out3[2] = best_off;
+    return 0;
+}
