@@ -248,6 +248,38 @@ int lcf_batch_laplace(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_
                       const double* map_params, int mode, double* out_log_sigma, double* out_elbo,
                       double* out_hessian);
 
+/* ---- batched finalize: profile refinement + magnitude chi2 (FP64) ------------ */
+#define LCF_FINALIZE_T0 1        /* profile_t0_refine */
+#define LCF_FINALIZE_ALL 2       /* profile_refine_all */
+#define LCF_FINALIZE_MAG_CHI2 4  /* magnitude-space chi2 */
+#define LCF_FINALIZE_DEFAULT 7
+
+/* The per-band post-processing that finalize_parametric_with_gpu_svi
+ * (src/parametric.rs:3298-3421), finalize_all_models_with_gpu_svi
+ * (:3429-3553) and finalize_parametric_from_gpu (:3126-3253) run on the host
+ * after the GPU fits, batched on the device: for band j of `data` (the
+ * BandFitData arrays: normalised flux, obs_var, upper flags / fluxes) with
+ * model_ids[j], PSO parameters pso_params[j][7] and posterior mu[j][7],
+ * log_sigma[j][7]:
+ *   LCF_FINALIZE_T0   profile_t0_refine (:2788-3008): 25-point t0 grid on
+ *                     [t_first - 30, t_peak + 5] with the amplitude (Bazin,
+ *                     TDE: and baseline) refitted by weighted least squares;
+ *                     best t0 -> out_mu, Delta LL = 0.5 width -> log sigma;
+ *   LCF_FINALIZE_ALL  profile_refine_all (:2705-2786): 15-point scans over
+ *                     the CPU PSO bounds, sigma = max(SVI sigma, profile
+ *                     half-width, population-prior width);
+ *   LCF_FINALIZE_MAG_CHI2  out_mag_chi2[j] (:3366-3393) from flux[],
+ *                     flux_err[] (normalised, per observation, the batch's
+ *                     order), mag_obs[] = -2.5 log10(raw flux) + 23.9 and
+ *                     peak_flux[j]; NaN when no term qualifies.
+ * Empty bands pass mu / log_sigma through with a NaN chi2.  Bands are limited
+ * to 4096 observations. */
+int lcf_batch_finalize(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_ids,
+                       const double* pso_params, const double* mu, const double* log_sigma,
+                       const double* flux, const double* flux_err, const double* mag_obs,
+                       const double* peak_flux, int flags, double* out_mu, double* out_log_sigma,
+                       double* out_mag_chi2);
+
 /* ---- batched dense GP (FP64) -------------------------------------------------- */
 /* GpuContext::batch_gp_fit (src/gpu_cuda.rs:976-1143): bands given as CSR
  * (times, mags, noise_var); training set = at most min(max_subsample, 25)

@@ -31,6 +31,7 @@ from .gpu import (  # noqa: F401
     svi_model_meta,
     svi_prior_for_model,
 )
-from . import common, gp2d, nonparametric, synthetic, thermal  # noqa: F401
+from . import common, gp2d, nonparametric, parametric, synthetic, thermal  # noqa: F401
 from .nonparametric import BandData, NonparametricBandResult  # noqa: F401
+from .parametric import GpuPsoBandResult, ParametricBandResult, UncertaintyMethod  # noqa: F401
 from ._lib import LIB_PATH  # noqa: F401

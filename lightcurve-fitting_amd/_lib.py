@@ -53,6 +53,7 @@ SIGNATURES = {
     "lcf_batch_gp_fit_ex": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _i32, _i32, _pd, _i32, _pd, _i32,
                                    _i32, _i32, _i32, _pd, _pd, _pd, _pd, _pd]),
     "lcf_batch_laplace": (_i32, [_vp, _vp, _pi32, _pd, _i32, _pd, _pd, _pd]),
+    "lcf_batch_finalize": (_i32, [_vp, _vp, _pi32, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _pd, _pd]),
     "lcf_batch_gp2d_fit": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _i32, _pd, _i32,
                                   _pd, _i32, _i32, _pd, _pd, _pd, _pd]),
     "lcf_batch_gp2d_fit_ex": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _i32, _i32, _pd, _i32,

@@ -35,7 +35,8 @@ CFLAGS = [
 # FP64 instantiations of the fitting kernels and the nonparametric band
 # pipeline are built without multiply-add contraction, so that they round like
 # the reference's unfused FP64 arithmetic.
-NO_CONTRACT = {"pso_f64.hip", "svi_f64.hip", "np_kernels.hip", "thermal_kernels.hip", "bb_kernels.hip"}
+NO_CONTRACT = {"pso_f64.hip", "svi_f64.hip", "np_kernels.hip", "thermal_kernels.hip", "bb_kernels.hip",
+               "finalize_kernels.hip"}
 
 
 def _compile(src: Path) -> Path:

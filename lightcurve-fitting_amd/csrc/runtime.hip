@@ -6,6 +6,7 @@
 #include "svi_kernels.hpp"
 #include "gp_kernels.hpp"
 #include "laplace_kernels.hpp"
+#include "finalize_kernels.hpp"
 #include "np_kernels.hpp"
 #include "thermal_kernels.hpp"
 #include "bb_kernels.hpp"
@@ -929,6 +930,98 @@ int lcf_batch_laplace(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, 
     HIPCHK(c, hipMemcpyAsync(out_log_sigma, d_ls.p, sizeof(double) * nj * 7, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(out_elbo, d_e.p, sizeof(double) * nj, hipMemcpyDeviceToHost, c->stream));
     if (out_hessian) HIPCHK(c, hipMemcpyAsync(out_hessian, d_h.p, sizeof(double) * nj * 36, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+int lcf_batch_finalize(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* pso_params,
+                       const double* mu, const double* log_sigma, const double* flux, const double* flux_err,
+                       const double* mag_obs, const double* peak_flux, int flags, double* out_mu,
+                       double* out_log_sigma, double* out_mag_chi2) {
+    if (!c) return LCF_E_INVALID;
+    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    const int64_t nj = b->n_sources, no = b->n_obs;
+    if (nj == 0) return LCF_OK;
+    if (!model_ids || !pso_params || !mu || !log_sigma || !flux || !flux_err || !mag_obs || !peak_flux ||
+        !out_mu || !out_log_sigma || !out_mag_chi2)
+        return fail(c, LCF_E_INVALID, "null pointer");
+    if (flags & ~LCF_FINALIZE_DEFAULT) return fail(c, LCF_E_INVALID, "unknown finalize flags 0x%x", flags);
+    if (b->max_n > lcf::FIN_MAX_OBS)
+        return fail(c, LCF_E_INVALID, "band of %d observations exceeds the finalize LDS limit %d", b->max_n,
+                    lcf::FIN_MAX_OBS);
+    for (int64_t j = 0; j < nj; j++)
+        if (model_ids[j] < 0 || model_ids[j] >= LCF_N_MODELS)
+            return fail(c, LCF_E_INVALID, "band %lld: unknown model id %d", (long long)j, model_ids[j]);
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int32_t> order;
+    order.reserve(nj);
+    int64_t starts[LCF_N_MODELS + 1] = {0};
+    for (int m = 0; m < LCF_N_MODELS; m++) {
+        starts[m] = (int64_t)order.size();
+        for (int32_t j : b->h_order)
+            if (model_ids[j] == m) order.push_back(j);
+    }
+    starts[LCF_N_MODELS] = (int64_t)order.size();
+    DevBuf<double> d_pso, d_mu, d_ls, d_f, d_fe, d_mag, d_pk, d_omu, d_ols, d_chi;
+    DevBuf<int32_t> d_order;
+    HIPCHK(c, d_pso.alloc(nj * 7));
+    HIPCHK(c, d_mu.alloc(nj * 7));
+    HIPCHK(c, d_ls.alloc(nj * 7));
+    HIPCHK(c, d_f.alloc(no));
+    HIPCHK(c, d_fe.alloc(no));
+    HIPCHK(c, d_mag.alloc(no));
+    HIPCHK(c, d_pk.alloc(nj));
+    HIPCHK(c, d_omu.alloc(nj * 7));
+    HIPCHK(c, d_ols.alloc(nj * 7));
+    HIPCHK(c, d_chi.alloc(nj));
+    HIPCHK(c, d_order.alloc(nj));
+    const size_t b7 = sizeof(double) * nj * 7, bo = sizeof(double) * no;
+    HIPCHK(c, hipMemcpyAsync(d_pso.p, pso_params, b7, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_mu.p, mu, b7, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_ls.p, log_sigma, b7, hipMemcpyHostToDevice, c->stream));
+    if (no > 0) {
+        HIPCHK(c, hipMemcpyAsync(d_f.p, flux, bo, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d_fe.p, flux_err, bo, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(d_mag.p, mag_obs, bo, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(d_pk.p, peak_flux, sizeof(double) * nj, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_order.p, order.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice, c->stream));
+    lcf::FinalizeArgs a{};
+    a.obs = b->d_obs64;
+    a.offsets = b->d_off;
+    a.queue = c->d_queue;
+    a.max_n = b->max_n;
+    a.flags = flags;
+    a.flux = d_f.p;
+    a.flux_err = d_fe.p;
+    a.mag_obs = d_mag.p;
+    a.peak = d_pk.p;
+    a.pso = d_pso.p;
+    a.mu_in = d_mu.p;
+    a.ls_in = d_ls.p;
+    a.mu_out = d_omu.p;
+    a.ls_out = d_ols.p;
+    a.mag_chi2 = d_chi.p;
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int m = 0; m < LCF_N_MODELS; m++) {
+        const int64_t cnt = starts[m + 1] - starts[m];
+        if (cnt == 0) continue;
+        a.order = d_order.p + starts[m];
+        a.n_tasks = (int)cnt;
+        int per_cu = 0;
+        HIPCHK(c, lcf::finalize_occupancy(m, b->max_n, &per_cu));
+        per_cu = std::max(per_cu, 1);
+        HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+        HIPCHK(c, lcf::launch_finalize(m, a, (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, cnt), c->stream));
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    HIPCHK(c, hipMemcpyAsync(out_mu, d_omu.p, b7, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_log_sigma, d_ols.p, b7, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_mag_chi2, d_chi.p, sizeof(double) * nj, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LCF_OK;
 }

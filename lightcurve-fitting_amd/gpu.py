@@ -22,6 +22,8 @@ NP_NFEAT = 50  # NonparametricBandResult fields without the band name
 FP32, FP64 = 0, 1
 SCHED_PERSISTENT, SCHED_PER_CLASS = 0, 1
 PSO_STALL_FROM_FIRST_EVAL = 1
+FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2 = 1, 2, 4
+FINALIZE_DEFAULT = FINALIZE_T0 | FINALIZE_ALL | FINALIZE_MAG_CHI2
 
 
 class GpuError(RuntimeError):
@@ -353,6 +355,35 @@ class GpuContext:
                                                 _lib.ptr(mp, C.c_double), int(mode), _lib.ptr(ls, C.c_double),
                                                 _lib.ptr(elbo, C.c_double), _lib.ptr(hess, C.c_double)))
         return {"mu": mp, "log_sigma": ls, "elbo": elbo, "hessian": hess}
+
+    # -- batched finalize -----------------------------------------------------
+    def finalize_arrays(self, data: "GpuBatchData", model_ids, pso_params, mu, log_sigma, flux, flux_err,
+                        mag_obs, peak_flux, *, flags: int = FINALIZE_DEFAULT) -> dict:
+        """Profile refinement and magnitude chi2 of every band of ``data``
+        (profile_t0_refine, profile_refine_all and the chi2 of
+        finalize_parametric_with_gpu_svi, src/parametric.rs:2705-3008,
+        3366-3393), FP64.  ``flux`` / ``flux_err`` / ``mag_obs`` are per
+        observation in the batch's order, ``peak_flux`` per band; returns the
+        refined ``mu`` and ``log_sigma`` ([n][7]) and ``mag_chi2`` ([n], NaN
+        when no term qualifies)."""
+        nj = data.n_sources
+        mid = np.ascontiguousarray(np.asarray(model_ids, dtype=np.int32))
+        if mid.size != nj:
+            raise GpuError(f"model_ids has {mid.size} entries for {nj} bands")
+        pso, m0, l0 = _rows7(pso_params, nj), _rows7(mu, nj), _rows7(log_sigma, nj)
+        per_obs = [_f64(a) for a in (flux, flux_err, mag_obs)]
+        for name, a in zip(("flux", "flux_err", "mag_obs"), per_obs):
+            if a.size != data.n_obs:
+                raise GpuError(f"{name} has {a.size} entries for {data.n_obs} observations")
+        pk = _f64(peak_flux)
+        if pk.size != nj:
+            raise GpuError(f"peak_flux has {pk.size} entries for {nj} bands")
+        om, ol, chi = np.zeros((nj, MAX_PARAMS)), np.zeros((nj, MAX_PARAMS)), np.zeros(nj)
+        d = lambda a: _lib.ptr(a, C.c_double)
+        self._check(self._lib.lcf_batch_finalize(self._h, data._h, _lib.ptr(mid, C.c_int32), d(pso), d(m0), d(l0),
+                                                 d(per_obs[0]), d(per_obs[1]), d(per_obs[2]), d(pk), int(flags),
+                                                 d(om), d(ol), d(chi)))
+        return {"mu": om, "log_sigma": ol, "mag_chi2": chi}
 
     # -- batched dense GP -----------------------------------------------------
     def batch_gp_fit(self, bands: Sequence["GpBandInput"], query_times, amp_candidates, ls_candidates,
@@ -704,6 +735,7 @@ class GpuBatchData:
         ctx._check(rc)
         self._h = h
         self.n_sources = off.size - 1
+        self.n_obs = n
         self._n_obs = np.diff(off)
 
     @classmethod

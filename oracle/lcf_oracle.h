@@ -239,6 +239,21 @@ int orc_laplace_fit(int model_id, const double* map_params, const double* t, con
                     const double* v, const int32_t* up, const double* uf, int64_t n, int mode,
                     double* log_sigma, double* elbo, double* hessian);
 
+/* ---- finalize (src/parametric.rs:2705-3008, 3298-3421) ----------------------
+ * Per band (CSR offsets, BandFitData arrays: normalised flux / flux_err, obs
+ * variance, upper-limit flags and fluxes, observed magnitudes, band peak flux)
+ * and model: profile_t0_refine, then profile_refine_all, then the magnitude
+ * chi2; mu / log_sigma / pso are [n_bands][7].  Empty bands pass through with
+ * a NaN chi2. */
+#define ORC_FINALIZE_T0 1
+#define ORC_FINALIZE_ALL 2
+#define ORC_FINALIZE_MAG_CHI2 4
+int orc_finalize(int64_t n_bands, const int64_t* off, const double* t, const double* f, const double* v,
+                 const int32_t* up, const double* uf, const double* fe, const double* mag_obs,
+                 const double* peak, const int32_t* model_ids, const double* pso, const double* mu,
+                 const double* log_sigma, int flags, double* out_mu, double* out_log_sigma,
+                 double* out_mag_chi2, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -556,3 +556,33 @@ def fit_bb_temperature(lam, mag, err):
     rc = L.orc_fit_bb_temperature(_p(lam, C.c_double), _p(mag, C.c_double), _p(err, C.c_double), lam.size,
                                   _p(out, C.c_double))
     return None if rc else tuple(out)
+
+
+FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2 = 1, 2, 4
+FINALIZE_DEFAULT = FINALIZE_T0 | FINALIZE_ALL | FINALIZE_MAG_CHI2
+
+
+def finalize(csr: dict, flux_err, mag_obs, peak, model_ids, pso, mu, log_sigma, flags: int = FINALIZE_DEFAULT,
+             n_threads: int | None = None) -> dict:
+    """profile_t0_refine + profile_refine_all + magnitude chi2 per band
+    (src/parametric.rs:2705-3008, 3366-3393); csr holds the BandFitData arrays
+    (offsets, times, flux, obs_var, is_upper, upper_flux); mu / log_sigma / pso
+    are [n_bands][7]."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    nb = off.size - 1
+    arrs = [f64(csr[k]) for k in ("times", "flux", "obs_var")]
+    up = i32(csr["is_upper"])
+    uf, fe, mo, pk = f64(csr["upper_flux"]), f64(flux_err), f64(mag_obs), f64(peak)
+    mids = i32(model_ids)
+    pso, mu, ls = (np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(nb, 7)) for a in (pso, mu, log_sigma))
+    out_mu, out_ls, chi2 = np.zeros((nb, 7)), np.zeros((nb, 7)), np.zeros(nb)
+    L = lib()
+    L.orc_finalize.restype = _i32
+    L.orc_finalize.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _pd, _pd, _pd, _pi32, _pd, _pd, _pd, _i32,
+                               _pd, _pd, _pd, _i32]
+    d = lambda a: _p(a, C.c_double)
+    rc = L.orc_finalize(nb, _p(off, C.c_int64), d(arrs[0]), d(arrs[1]), d(arrs[2]), _p(up, C.c_int32), d(uf), d(fe),
+                        d(mo), d(pk), _p(mids, C.c_int32), d(pso), d(mu), d(ls), flags, d(out_mu), d(out_ls), d(chi2),
+                        n_threads or default_threads())
+    assert rc == 0
+    return {"mu": out_mu, "log_sigma": out_ls, "mag_chi2": chi2}

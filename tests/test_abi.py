@@ -58,8 +58,9 @@ def test_model_metadata_mirrors_reference(pkg):
 
 
 def test_kernel_bounds_tables_match_oracle(orc):
-    """The __constant__ tables in pso_swarm.hpp carry the same numbers as the oracle."""
-    src = (ROOT / "lightcurve-fitting_amd" / "csrc" / "pso_swarm.hpp").read_text()
+    """The __constant__ tables in model_tables.hpp carry the same numbers as
+    the oracle (GPU bounds; the CPU bounds differ in the Bazin baseline)."""
+    src = (ROOT / "lightcurve-fitting_amd" / "csrc" / "model_tables.hpp").read_text()
 
     def table(name):
         m = re.search(name + r"\[8\]\[7\] = \{(.*?)\};", src, re.S)
@@ -74,3 +75,8 @@ def test_kernel_bounds_tables_match_oracle(orc):
         assert lo[m][:n] == pytest.approx(list(olo)) and hi[m][:n] == pytest.approx(list(ohi))
         c, w = orc.population_priors(m)
         assert pc[m][:n] == pytest.approx(list(c)) and pw[m][:n] == pytest.approx(list(w))
+        clo, chi = orc.bounds(m, cpu=True)
+        diff = [j for j in range(n) if (clo[j], chi[j]) != (olo[j], ohi[j])]
+        assert diff == ([1] if m == 0 else [])
+    bb = float(re.search(r"BAZIN_B_CPU = ([0-9.]+);", src).group(1))
+    assert (-bb, bb) == tuple(x[1] for x in orc.bounds(0, cpu=True))

@@ -1,0 +1,191 @@
+"""Pins the oracle's finalize step (profile_t0_refine, profile_refine_all,
+magnitude chi2; CPU only):
+
+  * t0 recovery: noise-free Bazin (amplitude + baseline refit) and Arnett
+    (amplitude refit) bands generated at a point of the 25-point t0 grid, with
+    the posterior mean's amplitude and baseline deliberately wrong: the
+    refit must land on that grid point exactly;
+  * t0 width extremes: a flat likelihood (huge variances) spans the whole
+    grid, a sharp one gives the 0.1 floor;
+  * per-parameter scan, closed form: the Bazin baseline enters the
+    likelihood quadratically, so the points kept are exactly those with
+    |B - B0| <= 1/sqrt(sum 1/var) on the 15-point grid over the CPU bounds
+    +-0.3, and sigma = max(SVI sigma, half-width, prior width 0.2);
+  * per-parameter scan against an independent walk that takes its
+    likelihoods from the already pinned neg_log_posterior (minus its prior);
+  * magnitude chi2, closed form: observed magnitudes offset by delta_i from
+    the model curve give mean((delta_i / (1.0857 sigma_i / f_i))^2) over the
+    positive-flux points; predictions below zero switch to the PSO curve.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+def band(orc, m, p, t, err, peak=None):
+    """Noise-free BandFitData arrays of model m at p (flux already
+    normalised) with constant normalised error `err`."""
+    f = orc.eval_batch(m, p, t).reshape(-1)
+    n = t.size
+    e = np.full(n, err)
+    return {"offsets": np.array([0, n]), "times": t, "flux": f, "obs_var": e * e + 1e-10,
+            "is_upper": np.zeros(n, np.int32), "upper_flux": 3.0 * e}, e
+
+
+def grid_t0(t, f):
+    lo = t.min() - 30.0
+    hi = t[int(np.argmax(f))] + 5.0
+    return [lo + (hi - lo) * g / 24.0 for g in range(25)]
+
+
+def fin(orc, csr, e, m, pso, mu, ls, flags, peak=1.0, mag=None):
+    n = csr["times"].size
+    if mag is None:
+        mag = np.full(n, 20.0)
+    return orc.finalize(csr, e, mag, [peak], [m], np.r_[pso, np.zeros(7 - len(pso))],
+                        np.r_[mu, np.zeros(7 - len(mu))], np.r_[ls, np.zeros(7 - len(ls))], flags)
+
+
+@pytest.mark.parametrize("m,truth,t0i", [
+    (0, [0.2, 0.05, -4.0, np.log(2.0), np.log(20.0), np.log(0.01)], 2),
+    (3, [0.1, -4.0, np.log(15.0), 0.5, np.log(0.01)], 1),
+])
+def test_t0_refit_recovers_grid_point(orc, m, truth, t0i):
+    t = np.arange(0.0, 60.0, 1.5)
+    p = np.array(truth)
+    g = grid_t0(t, orc.eval_batch(m, p, t).reshape(-1))
+    for _ in range(5):  # move t0 onto the grid its own curve defines
+        k = int(np.argmin(np.abs(np.array(g) - p[t0i])))
+        p[t0i] = g[k]
+        g2 = grid_t0(t, orc.eval_batch(m, p, t).reshape(-1))
+        if g2 == g:
+            break
+        g = g2
+    csr, e = band(orc, m, p, t, 0.01)
+    assert grid_t0(t, csr["flux"]) == g and p[t0i] == g[k]
+    mu = p.copy()
+    mu[0] += 0.7  # wrong amplitude
+    if m == 0:
+        mu[1] = 0.2  # wrong baseline
+    mu[t0i] += 3.3  # off the grid
+    r = fin(orc, csr, e, m, p, mu, np.full(p.size, -1.0), orc.FINALIZE_T0)
+    assert r["mu"][0, t0i] == g[k]
+    # amplitude / baseline are not written back, only t0 and its width
+    others = [j for j in range(p.size) if j != t0i]
+    assert np.array_equal(r["mu"][0, others], mu[others])
+    assert r["log_sigma"][0, t0i] >= math.log(0.1)
+
+
+def test_t0_width_extremes(orc):
+    t = np.arange(0.0, 60.0, 1.5)
+    p = np.array([0.0, 0.0, 5.0, np.log(2.0), np.log(20.0), np.log(1e-3)])
+    csr, e = band(orc, 0, p, t, 1e3)  # variance 1e6: every grid point within Delta LL 0.5
+    g = grid_t0(t, csr["flux"])
+    r = fin(orc, csr, e, 0, p, p, np.full(6, -1.0), orc.FINALIZE_T0)
+    assert r["log_sigma"][0, 2] == pytest.approx(math.log((g[24] - g[0]) / 2.0), rel=1e-14)
+    csr, e = band(orc, 0, p, t, 1e-4)  # sharp: only the best point
+    r = fin(orc, csr, e, 0, p, p, np.full(6, -1.0), orc.FINALIZE_T0)
+    assert r["log_sigma"][0, 2] == math.log(0.1)
+
+
+def test_scan_baseline_closed_form(orc):
+    t = np.linspace(0.0, 80.0, 64)
+    p = np.array([0.0, 0.0, 10.0, np.log(3.0), np.log(25.0), np.log(1e-4)])
+    # sum 1/var = 64 / var = 1 / 0.25^2  ->  var = 4
+    err = math.sqrt(4.0 - 1e-10)
+    csr, e = band(orc, 0, p, t, err)
+    w = np.sum(1.0 / (csr["obs_var"] + math.exp(p[5]) ** 2))
+    half = 1.0 / math.sqrt(w)
+    vals = [-0.3 + 0.6 * g / 14.0 for g in range(15)]
+    kept = [v for v in vals if abs(v) <= half] + [0.0]
+    expect = max((max(kept) - min(kept)) / 2.0, 1e-6)
+    for ls_in, floor in ((math.log(0.01), 0.2), (math.log(0.5), 0.5)):
+        r = fin(orc, csr, e, 0, p, p, np.full(6, ls_in), orc.FINALIZE_ALL)
+        assert r["log_sigma"][0, 1] == pytest.approx(math.log(max(expect, floor)), rel=1e-14)
+    assert 0.2 < expect < 0.25
+    # sigma_x untouched, mu untouched
+    assert r["log_sigma"][0, 5] == math.log(0.5) and np.array_equal(r["mu"][0, :6], p)
+
+
+def _ll_from_nlp(orc, m, p, csr):
+    pc, pw = orc.prior_params(m)
+    prior = sum(0.5 * (p[j] - pc[j]) ** 2 / pw[j] ** 2 for j in range(len(p)))
+    nlp = orc.neg_log_posterior(m, p, csr["times"], csr["flux"], csr["obs_var"], csr["is_upper"], csr["upper_flux"])
+    assert nlp < 1e98
+    return -(nlp - prior)
+
+
+@pytest.mark.parametrize("err", [0.05, 1.0])
+@pytest.mark.parametrize("m,p", [
+    (1, [0.1, 0.01, np.log(20.0), 3.0, np.log(3.0), np.log(30.0), np.log(0.05)]),
+    (2, [0.0, 0.02, 2.0, np.log(4.0), np.log(10.0), 1.5, np.log(0.05)]),
+    (6, [0.0, -3.0, np.log(10.0), 0.5, 1.5, np.log(0.05)]),
+])
+def test_scan_against_independent_walk(orc, m, p, err):
+    """With err = 1 the profile half-width exceeds the prior floor for
+    several parameters (Villar beta / gamma / tau_rise, TDE t0)."""
+    rng = np.random.default_rng(m)
+    t = np.sort(rng.uniform(0.0, 90.0, 45))
+    p = np.array(p)
+    csr, e = band(orc, m, p, t, err)
+    csr["flux"] = csr["flux"] + err * rng.normal(size=t.size)
+    csr["is_upper"][::7] = 1
+    ls = np.log(rng.uniform(0.005, 0.05, p.size))
+    r = fin(orc, csr, e, m, p, p, ls, orc.FINALIZE_ALL)
+    lo, hi = orc.bounds(m, cpu=True)
+    _, pw = orc.population_priors(m)
+    thr = _ll_from_nlp(orc, m, p, csr) - 0.5
+    for j in range(p.size - 1):
+        a = b = p[j]
+        for g in range(15):
+            q = p.copy()
+            q[j] = lo[j] + (hi[j] - lo[j]) * g / 14.0
+            try:
+                ok = _ll_from_nlp(orc, m, q, csr) >= thr
+            except AssertionError:
+                continue  # non-finite prediction: the reference skips those terms instead
+            if ok:
+                a, b = min(a, q[j]), max(b, q[j])
+        want = math.log(max(math.exp(ls[j]), max((b - a) / 2.0, 1e-6), pw[j]))
+        assert r["log_sigma"][0, j] == pytest.approx(want, rel=1e-12, abs=1e-12), j
+
+
+def test_mag_chi2_closed_form(orc):
+    t = np.linspace(0.0, 50.0, 30)
+    p = np.array([0.0, 0.0, 10.0, np.log(2.0), np.log(15.0), np.log(1e-3)])
+    csr, e = band(orc, 0, p, t, 0.02)
+    peak = 250.0
+    f = csr["flux"].copy()
+    f[3] = -0.1  # negative flux: excluded
+    f[4] = 0.0   # zero flux: excluded
+    csr["flux"] = f
+    csr["is_upper"][5] = 1  # upper limits still count when the flux is positive
+    rng = np.random.default_rng(3)
+    delta = rng.normal(0.0, 0.1, t.size)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        model = orc.eval_batch(0, p, t).reshape(-1)
+        mag = -2.5 * np.log10(model * peak) + 23.9 + delta
+        me = 1.0857 * e / f
+    keep = f > 0
+    want = np.mean((delta[keep] / me[keep]) ** 2)
+    r = fin(orc, csr, e, 0, p, p, np.full(6, -1.0), orc.FINALIZE_MAG_CHI2, peak=peak, mag=mag)
+    assert r["mag_chi2"][0] == pytest.approx(want, rel=1e-9)
+    # SVI mean with negative predictions everywhere: the PSO curve is used
+    bad = p.copy()
+    bad[1] = -5.0
+    r2 = fin(orc, csr, e, 0, p, bad, np.full(6, -1.0), orc.FINALIZE_MAG_CHI2, peak=peak, mag=mag)
+    assert r2["mag_chi2"][0] == r["mag_chi2"][0]
+    # no positive flux at all: NaN
+    csr["flux"] = -np.abs(f) - 0.1
+    r3 = fin(orc, csr, e, 0, p, p, np.full(6, -1.0), orc.FINALIZE_MAG_CHI2, peak=peak, mag=mag)
+    assert math.isnan(r3["mag_chi2"][0])
+
+
+def test_empty_band_passes_through(orc):
+    csr = {"offsets": np.array([0, 0]), "times": np.zeros(0), "flux": np.zeros(0), "obs_var": np.zeros(0),
+           "is_upper": np.zeros(0, np.int32), "upper_flux": np.zeros(0)}
+    mu = np.arange(7.0)
+    r = orc.finalize(csr, np.zeros(0), np.zeros(0), [1.0], [1], mu, mu, -mu)
+    assert np.array_equal(r["mu"][0], mu) and np.array_equal(r["log_sigma"][0], -mu)
+    assert math.isnan(r["mag_chi2"][0])
