@@ -329,6 +329,37 @@ int lcf_batch_gp2d_fit_ex(lcf_ctx* ctx, int64_t n_sources, const int64_t* offset
                           int n_lsw, int cap, int subsample_rule, int criterion, double* out_state,
                           double* out_pred, double* out_std, double* out_dpred, double* out_pred_obs);
 
+/* ---- 2D GP hyper-parameter fit with the Cholesky adjoint (FP64) -------------
+ * The optimised counterpart of the hyper-parameter grid of batch_gp_2d /
+ * fit_gp_2d (src/gpu_cuda.rs:1152-1306, src/gp2d.rs:336-439, 516-724): per
+ * source (CSR offsets, times, log10 wavelengths, magnitudes, noise variances,
+ * at most 2047 points) the separable squared-exponential GP
+ *   K = a exp(-dt^2 / (2 lt^2) - dw^2 / (2 lw^2)) + diag(noise_var + sn^2)
+ * on y - mean(y), theta = ln (a, lt, lw, sn) ([n][4]).  The negative log
+ * marginal likelihood and its exact gradient 1/2 tr((K^-1 - alpha alpha^T)
+ * dK/dtheta) come from a blocked FP64 Cholesky and its adjoint; L-BFGS
+ * (6 pairs, projected backtracking Armijo search, box a in [1e-6, 1e6],
+ * lt in [0.1, 1e4], lw in [1e-3, 10], sn in [1e-6, 10]) runs at most
+ * max_iters iterations from theta0, stopping when the projected gradient's
+ * largest component is <= grad_tol.  out_status: 0 converged, 1 iteration
+ * limit, 2 no further decrease along the search direction, 3 K not positive
+ * definite at theta0 (or no points).  Optional queries (n_query > 0; shared,
+ * or [n][n_query] when query_per_source) give the posterior mean (+ mean(y))
+ * and standard deviation at the returned theta. */
+int lcf_batch_gp2d_train(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,
+                         const double* log_waves, const double* mags, const double* noise_var,
+                         const double* theta0, int max_iters, double grad_tol, const double* query_times,
+                         const double* query_waves, int n_query, int query_per_source, double* out_theta,
+                         double* out_nlml, double* out_grad, int32_t* out_iters, int32_t* out_status,
+                         double* out_pred, double* out_std);
+
+/* The same objective and gradient at theta ([n][4], not clamped): one
+ * evaluation per source; out_status 0, or 3 when K is not positive definite
+ * (NaN outputs).  Backs the autograd binding and the finite-difference check. */
+int lcf_gp2d_nlml_grad(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,
+                       const double* log_waves, const double* mags, const double* noise_var,
+                       const double* theta, double* out_nlml, double* out_grad, int32_t* out_status);
+
 /* Timing of the last lcf_batch_fit_models-family call on this context:
  * milliseconds between HIP events around the kernel launches (inputs already
  * resident, outputs still on the device). */

@@ -7,6 +7,7 @@
 #include "gp_kernels.hpp"
 #include "laplace_kernels.hpp"
 #include "finalize_kernels.hpp"
+#include "gp2d_hyper.hpp"
 #include "np_kernels.hpp"
 #include "thermal_kernels.hpp"
 #include "bb_kernels.hpp"
@@ -1024,6 +1025,153 @@ int lcf_batch_finalize(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids,
     HIPCHK(c, hipMemcpyAsync(out_mag_chi2, d_chi.p, sizeof(double) * nj, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return LCF_OK;
+}
+
+namespace {
+
+// Shared body of lcf_batch_gp2d_train and lcf_gp2d_nlml_grad.
+int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, const double* w,
+                   const double* y, const double* nv, const double* theta0, int max_iters, double gtol,
+                   const double* qt, const double* qw, int n_query, int q_per_src, double* out_theta,
+                   double* out_nlml, double* out_grad, int32_t* out_iters, int32_t* out_status, double* out_pred,
+                   double* out_std) {
+    if (!c) return LCF_E_INVALID;
+    if (ns < 0) return fail(c, LCF_E_INVALID, "negative source count");
+    if (ns == 0) return LCF_OK;
+    if (!off || !theta0 || !out_theta || !out_nlml || !out_grad) return fail(c, LCF_E_INVALID, "null pointer");
+    if (n_query < 0 || (n_query > 0 && (!qt || !qw || !out_pred || !out_std)))
+        return fail(c, LCF_E_INVALID, "queries need times, wavelengths and outputs");
+    if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets must start at 0");
+    int max_n = 0;
+    for (int64_t s = 0; s < ns; s++) {
+        const int64_t m = off[s + 1] - off[s];
+        if (m < 0) return fail(c, LCF_E_INVALID, "offsets not monotone at source %lld", (long long)s);
+        if (m > lcf::GPH_MAX_N)
+            return fail(c, LCF_E_INVALID, "source %lld has %lld points (limit %d)", (long long)s, (long long)m,
+                        lcf::GPH_MAX_N);
+        max_n = std::max<int>(max_n, (int)m);
+    }
+    const int64_t no = off[ns];
+    if (no > 0 && (!t || !w || !y || !nv)) return fail(c, LCF_E_INVALID, "null pointer");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int32_t> order((size_t)ns);
+    for (int64_t s = 0; s < ns; s++) order[(size_t)s] = (int32_t)s;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+    const int ld = ((max_n + 1 + lcf::GPH_TILE - 1) / lcf::GPH_TILE) * lcf::GPH_TILE;
+    int per_cu = 0;
+    HIPCHK(c, lcf::gp2d_hyper_occupancy(&per_cu));
+    per_cu = std::max(per_cu, 1);
+    const size_t slot = lcf::gp2d_hyper_slot_doubles(ld);
+    const int64_t budget = (int64_t)(8ull << 30) / (int64_t)(slot * sizeof(double));  // 8 GiB of slots
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)per_cu * c->n_cu, ns, budget}));
+    const int64_t nq_tot = n_query > 0 ? (q_per_src ? ns * (int64_t)n_query : (int64_t)n_query) : 0;
+    DevBuf<int64_t> d_off;
+    DevBuf<double> d_t, d_w, d_y, d_nv, d_th0, d_work, d_th, d_f, d_g, d_qt, d_qw, d_pred, d_std;
+    DevBuf<int32_t> d_order, d_it, d_st;
+    HIPCHK(c, d_off.alloc(ns + 1));
+    HIPCHK(c, d_t.alloc(no));
+    HIPCHK(c, d_w.alloc(no));
+    HIPCHK(c, d_y.alloc(no));
+    HIPCHK(c, d_nv.alloc(no));
+    HIPCHK(c, d_th0.alloc(ns * 4));
+    HIPCHK(c, d_work.alloc(slot * (size_t)grid));
+    HIPCHK(c, d_th.alloc(ns * 4));
+    HIPCHK(c, d_f.alloc(ns));
+    HIPCHK(c, d_g.alloc(ns * 4));
+    HIPCHK(c, d_order.alloc(ns));
+    HIPCHK(c, d_it.alloc(ns));
+    HIPCHK(c, d_st.alloc(ns));
+    if (n_query > 0) {
+        HIPCHK(c, d_qt.alloc(nq_tot));
+        HIPCHK(c, d_qw.alloc(nq_tot));
+        HIPCHK(c, d_pred.alloc(ns * (int64_t)n_query));
+        HIPCHK(c, d_std.alloc(ns * (int64_t)n_query));
+    }
+    auto up = [&](void* d, const void* h, size_t bytes) {
+        return bytes ? hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, up(d_off.p, off, sizeof(int64_t) * (ns + 1)));
+    HIPCHK(c, up(d_t.p, t, sizeof(double) * no));
+    HIPCHK(c, up(d_w.p, w, sizeof(double) * no));
+    HIPCHK(c, up(d_y.p, y, sizeof(double) * no));
+    HIPCHK(c, up(d_nv.p, nv, sizeof(double) * no));
+    HIPCHK(c, up(d_th0.p, theta0, sizeof(double) * ns * 4));
+    HIPCHK(c, up(d_order.p, order.data(), sizeof(int32_t) * ns));
+    if (n_query > 0) {
+        HIPCHK(c, up(d_qt.p, qt, sizeof(double) * nq_tot));
+        HIPCHK(c, up(d_qw.p, qw, sizeof(double) * nq_tot));
+    }
+    lcf::Gp2dHyperArgs a{};
+    a.off = d_off.p;
+    a.t = d_t.p;
+    a.w = d_w.p;
+    a.y = d_y.p;
+    a.nv = d_nv.p;
+    a.order = d_order.p;
+    a.n_tasks = (int)ns;
+    a.queue = c->d_queue;
+    a.theta0 = d_th0.p;
+    a.max_iters = max_iters;
+    a.gtol = gtol;
+    a.ld = ld;
+    a.work = d_work.p;
+    a.qt = d_qt.p;
+    a.qw = d_qw.p;
+    a.n_query = n_query;
+    a.q_per_src = q_per_src ? 1 : 0;
+    a.theta = d_th.p;
+    a.nlml = d_f.p;
+    a.grad = d_g.p;
+    a.iters = d_it.p;
+    a.status = d_st.p;
+    a.pred = n_query > 0 ? d_pred.p : nullptr;
+    a.stdv = n_query > 0 ? d_std.p : nullptr;
+    HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, lcf::launch_gp2d_hyper(a, grid, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    auto down = [&](void* h, const void* d, size_t bytes) {
+        return (h && bytes) ? hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, c->stream) : hipSuccess;
+    };
+    HIPCHK(c, down(out_theta, d_th.p, sizeof(double) * ns * 4));
+    HIPCHK(c, down(out_nlml, d_f.p, sizeof(double) * ns));
+    HIPCHK(c, down(out_grad, d_g.p, sizeof(double) * ns * 4));
+    HIPCHK(c, down(out_iters, d_it.p, sizeof(int32_t) * ns));
+    HIPCHK(c, down(out_status, d_st.p, sizeof(int32_t) * ns));
+    if (n_query > 0) {
+        HIPCHK(c, down(out_pred, d_pred.p, sizeof(double) * ns * n_query));
+        HIPCHK(c, down(out_std, d_std.p, sizeof(double) * ns * n_query));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
+}  // namespace
+
+int lcf_batch_gp2d_train(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, const double* times,
+                         const double* log_waves, const double* mags, const double* noise_var, const double* theta0,
+                         int max_iters, double grad_tol, const double* query_times, const double* query_waves,
+                         int n_query, int query_per_source, double* out_theta, double* out_nlml, double* out_grad,
+                         int32_t* out_iters, int32_t* out_status, double* out_pred, double* out_std) {
+    if (c && max_iters < 0) return fail(c, LCF_E_INVALID, "max_iters must be >= 0");
+    if (c && (!out_iters || !out_status)) return fail(c, LCF_E_INVALID, "null pointer");
+    return gp2d_hyper_run(c, n_sources, offsets, times, log_waves, mags, noise_var, theta0, max_iters, grad_tol,
+                          query_times, query_waves, n_query, query_per_source, out_theta, out_nlml, out_grad,
+                          out_iters, out_status, out_pred, out_std);
+}
+
+int lcf_gp2d_nlml_grad(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, const double* times,
+                       const double* log_waves, const double* mags, const double* noise_var, const double* theta,
+                       double* out_nlml, double* out_grad, int32_t* out_status) {
+    if (c && !out_status) return fail(c, LCF_E_INVALID, "null pointer");
+    std::vector<double> th((size_t)std::max<int64_t>(n_sources, 0) * 4);
+    return gp2d_hyper_run(c, n_sources, offsets, times, log_waves, mags, noise_var, theta, -1, 0.0, nullptr, nullptr,
+                          0, 0, th.data(), out_nlml, out_grad, nullptr, out_status, nullptr, nullptr);
 }
 
 int lcf_batch_gp_fit(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* mags,

@@ -568,8 +568,53 @@ def _gp2d_fit(self, csr: dict, query_t, query_w, amp_candidates, lst_candidates,
     return out
 
 
+def _gp2d_train(self, csr: dict, theta0, *, max_iters: int = 100, grad_tol: float = 1e-6, query_t=None,
+                query_w=None, query_per_source: bool = False) -> dict:
+    """2D GP hyper-parameter fit with the Cholesky adjoint
+    (``lcf_batch_gp2d_train``): csr holds offsets / times / waves (log10
+    wavelength) / mags / noise_var; theta0 [n][4] = ln (a, l_t, l_w, sigma_n).
+    Returns theta, nlml, grad, iters, status and, with queries, pred / std."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, w, y, nv = (_f64(csr[k]) for k in ("times", "waves", "mags", "noise_var"))
+    th0 = np.ascontiguousarray(np.asarray(theta0, dtype=np.float64).reshape(ns, 4))
+    nq = 0
+    if query_t is not None:
+        qt, qw = _f64(query_t), _f64(query_w)
+        nq = qt.shape[-1]
+    else:
+        qt = qw = np.zeros(1)
+    out = {"theta": np.zeros((ns, 4)), "nlml": np.zeros(ns), "grad": np.zeros((ns, 4)),
+           "iters": np.zeros(ns, np.int32), "status": np.zeros(ns, np.int32),
+           "pred": np.zeros((ns, max(nq, 1))), "std": np.zeros((ns, max(nq, 1)))}
+    d = lambda a: _lib.ptr(a, C.c_double)
+    self._check(self._lib.lcf_batch_gp2d_train(
+        self._h, ns, _lib.ptr(off, C.c_int64), d(t), d(w), d(y), d(nv), d(th0), int(max_iters), float(grad_tol),
+        d(qt), d(qw), nq, int(query_per_source), d(out["theta"]), d(out["nlml"]), d(out["grad"]),
+        _lib.ptr(out["iters"], C.c_int32), _lib.ptr(out["status"], C.c_int32), d(out["pred"]), d(out["std"])))
+    if nq == 0:
+        del out["pred"], out["std"]
+    return out
+
+
+def _gp2d_nlml_grad(self, csr: dict, theta) -> dict:
+    """NLML and gradient of the 2D GP at theta ([n][4]), one evaluation per
+    source (``lcf_gp2d_nlml_grad``)."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, w, y, nv = (_f64(csr[k]) for k in ("times", "waves", "mags", "noise_var"))
+    th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(ns, 4))
+    f, g, st = np.zeros(ns), np.zeros((ns, 4)), np.zeros(ns, np.int32)
+    d = lambda a: _lib.ptr(a, C.c_double)
+    self._check(self._lib.lcf_gp2d_nlml_grad(self._h, ns, _lib.ptr(off, C.c_int64), d(t), d(w), d(y), d(nv), d(th),
+                                             d(f), d(g), _lib.ptr(st, C.c_int32)))
+    return {"nlml": f, "grad": g, "status": st}
+
+
 GpuContext.batch_gp_2d = _batch_gp_2d
 GpuContext.gp2d_fit = _gp2d_fit
+GpuContext.gp2d_train = _gp2d_train
+GpuContext.gp2d_nlml_grad = _gp2d_nlml_grad
 
 
 @dataclass

@@ -254,6 +254,17 @@ int orc_finalize(int64_t n_bands, const int64_t* off, const double* t, const dou
                  const double* log_sigma, int flags, double* out_mu, double* out_log_sigma,
                  double* out_mag_chi2, int n_threads);
 
+/* ---- 2D GP hyper-parameter fit (new; R & W 2.30 / 5.9, L-BFGS) -------------
+ * theta = ln (amplitude, time length scale, log10-wavelength length scale,
+ * extra noise sigma).  nlml_grad returns 1 when K is not positive definite.
+ * train: status 0 gradient below gtol, 1 max_iters reached, 2 line search
+ * failed, 3 K not positive definite at theta0 (or empty source). */
+int orc_gp2d_nlml_grad(const double* t, const double* w, const double* y, const double* nv, int n,
+                       const double* theta, double* nlml, double* grad);
+int orc_batch_gp2d_train(int64_t n_src, const int64_t* off, const double* t, const double* w, const double* y,
+                         const double* nv, const double* theta0, int max_iters, double gtol, double* theta,
+                         double* nlml, double* grad, int32_t* iters, int32_t* status, int n_threads);
+
 #ifdef __cplusplus
 }
 #endif

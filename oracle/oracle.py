@@ -586,3 +586,43 @@ def finalize(csr: dict, flux_err, mag_obs, peak, model_ids, pso, mu, log_sigma, 
                         n_threads or default_threads())
     assert rc == 0
     return {"mu": out_mu, "log_sigma": out_ls, "mag_chi2": chi2}
+
+
+GP2D_TH_LO = np.log([1e-6, 0.1, 1e-3, 1e-6])
+GP2D_TH_HI = np.log([1e6, 1e4, 10.0, 10.0])
+
+
+def gp2d_nlml_grad(t, w, y, nv, theta):
+    """(NLML, gradient[4]) of the 2D GP at theta = ln(a, lt, lw, sn), or None
+    when K is not positive definite."""
+    t, w, y, nv, th = f64(t), f64(w), f64(y), f64(nv), f64(theta)
+    g = np.zeros(4)
+    f = C.c_double()
+    L = lib()
+    L.orc_gp2d_nlml_grad.restype = _i32
+    L.orc_gp2d_nlml_grad.argtypes = [_pd, _pd, _pd, _pd, _i32, _pd, _pd, _pd]
+    rc = L.orc_gp2d_nlml_grad(_p(t, C.c_double), _p(w, C.c_double), _p(y, C.c_double), _p(nv, C.c_double),
+                              t.size, _p(th, C.c_double), C.byref(f), _p(g, C.c_double))
+    return None if rc else (f.value, g)
+
+
+def batch_gp2d_train(csr: dict, theta0, max_iters: int = 100, gtol: float = 0.0,
+                     n_threads: int | None = None) -> dict:
+    """L-BFGS fit of theta per source (CSR offsets / times / log_waves / mags / noise_var)."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    arrs = [f64(csr[k]) for k in ("times", "log_waves", "mags", "noise_var")]
+    th0 = np.ascontiguousarray(np.asarray(theta0, dtype=np.float64).reshape(ns, 4))
+    th, g = np.zeros((ns, 4)), np.zeros((ns, 4))
+    f = np.zeros(ns)
+    it, st = np.zeros(ns, np.int32), np.zeros(ns, np.int32)
+    L = lib()
+    L.orc_batch_gp2d_train.restype = _i32
+    L.orc_batch_gp2d_train.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pd, _pd, _i32, _f64, _pd, _pd, _pd, _pi32,
+                                       _pi32, _i32]
+    d = lambda a: _p(a, C.c_double)
+    rc = L.orc_batch_gp2d_train(ns, _p(off, C.c_int64), d(arrs[0]), d(arrs[1]), d(arrs[2]), d(arrs[3]), d(th0),
+                                max_iters, gtol, d(th), d(f), d(g), _p(it, C.c_int32), _p(st, C.c_int32),
+                                n_threads or default_threads())
+    assert rc == 0
+    return {"theta": th, "nlml": f, "grad": g, "iters": it, "status": st}

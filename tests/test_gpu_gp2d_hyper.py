@@ -1,0 +1,121 @@
+"""GPU parity of the 2D GP hyper-parameter fit (needs an MI355X), through
+lcf_gp2d_nlml_grad / lcf_batch_gp2d_train:
+
+  * NLML (1e-10 relative) and adjoint gradient (1e-8) against the oracle on
+    sources of 1-513 points, across the 64-point tile and block boundaries,
+    at several theta;
+  * the device gradient against central differences of the device NLML;
+  * L-BFGS: the same minimum as the oracle's driver (NLML within 1e-8
+    relative; the paths differ only by summation order), never above the
+    starting NLML;
+  * predictions at query points against numpy at the returned theta;
+  * K not positive definite and empty sources -> status 3 with NaN outputs.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def make(rng, sizes, noise=0.03):
+    off = np.zeros(len(sizes) + 1, dtype=np.int64)
+    np.cumsum(sizes, out=off[1:])
+    t, w, y = [], [], []
+    for n in sizes:
+        tt = np.sort(rng.uniform(0.0, 100.0, n))
+        ww = np.log10(rng.choice([4770.0, 6231.0, 7625.0], n))
+        t.append(tt)
+        w.append(ww)
+        y.append(18.0 + np.sin(tt / 15.0 + rng.uniform(0, 3)) + 0.5 * (ww - 3.8) + noise * rng.normal(size=n))
+    cat = lambda v: np.concatenate(v) if v else np.zeros(0)
+    return {"offsets": off, "times": cat(t), "waves": cat(w), "log_waves": cat(w), "mags": cat(y),
+            "noise_var": np.full(int(off[-1]), noise ** 2)}
+
+
+def oracle_eval(orc, csr, th):
+    off = csr["offsets"]
+    f, g = np.zeros(off.size - 1), np.zeros((off.size - 1, 4))
+    for s in range(off.size - 1):
+        a, b = off[s], off[s + 1]
+        r = orc.gp2d_nlml_grad(csr["times"][a:b], csr["waves"][a:b], csr["mags"][a:b], csr["noise_var"][a:b], th[s])
+        f[s], g[s] = r
+    return f, g
+
+
+SIZES = [1, 5, 63, 64, 65, 127, 128, 129, 200, 300, 513]
+
+
+@pytest.mark.parametrize("th", [[1.0, 20.0, 0.3, 0.01], [0.2, 3.0, 0.05, 0.1], [5.0, 60.0, 1.0, 0.003]])
+def test_nlml_grad_matches_oracle(orc, ctx, th):
+    rng = np.random.default_rng(1)
+    csr = make(rng, SIZES)
+    theta = np.tile(np.log(th), (len(SIZES), 1))
+    g = ctx.gp2d_nlml_grad(csr, theta)
+    fo, go = oracle_eval(orc, csr, theta)
+    assert (g["status"] == 0).all()
+    assert np.allclose(g["nlml"], fo, rtol=1e-10, atol=1e-9), np.abs(g["nlml"] - fo).max()
+    assert np.allclose(g["grad"], go, rtol=1e-8, atol=1e-8), np.abs(g["grad"] - go).max()
+
+
+def test_gradient_against_finite_differences(ctx):
+    rng = np.random.default_rng(2)
+    csr = make(rng, [40, 150, 260])
+    th = np.tile(np.log([1.0, 15.0, 0.2, 0.02]), (3, 1))
+    g = ctx.gp2d_nlml_grad(csr, th)["grad"]
+    h = 1e-6
+    for k in range(4):
+        e = np.zeros(4)
+        e[k] = h
+        fp = ctx.gp2d_nlml_grad(csr, th + e)["nlml"]
+        fm = ctx.gp2d_nlml_grad(csr, th - e)["nlml"]
+        assert np.allclose(g[:, k], (fp - fm) / (2 * h), rtol=1e-5, atol=1e-5), k
+
+
+def test_training_matches_oracle(orc, ctx):
+    rng = np.random.default_rng(3)
+    sizes = [30, 64, 97, 150, 222, 310]
+    csr = make(rng, sizes)
+    th0 = np.tile(np.log([1.0, 20.0, 0.3, 0.01]), (len(sizes), 1))
+    g = ctx.gp2d_train(csr, th0, max_iters=100, grad_tol=1e-6)
+    o = orc.batch_gp2d_train(csr, th0, 100, 1e-6)
+    f0, _ = oracle_eval(orc, csr, th0)
+    assert (g["status"] <= 2).all() and (g["nlml"] <= f0 + 1e-9).all()
+    assert np.allclose(g["nlml"], o["nlml"], rtol=1e-8, atol=1e-7), np.abs(g["nlml"] - o["nlml"]).max()
+    # returned gradient is the one at the returned theta
+    fr, gr = oracle_eval(orc, csr, g["theta"])
+    assert np.allclose(g["nlml"], fr, rtol=1e-10) and np.allclose(g["grad"], gr, rtol=1e-7, atol=1e-7)
+
+
+def test_predictions(ctx):
+    rng = np.random.default_rng(4)
+    sizes = [20, 90, 140]
+    csr = make(rng, sizes)
+    th0 = np.tile(np.log([1.0, 20.0, 0.3, 0.01]), (3, 1))
+    qt = np.repeat(np.linspace(0.0, 100.0, 30), 3)
+    qw = np.tile(np.log10([4770.0, 6231.0, 7625.0]), 30)
+    g = ctx.gp2d_train(csr, th0, max_iters=20, query_t=qt, query_w=qw)
+    off = csr["offsets"]
+    for s in range(3):
+        a, b = off[s], off[s + 1]
+        t, w, y, nv = csr["times"][a:b], csr["waves"][a:b], csr["mags"][a:b], csr["noise_var"][a:b]
+        A, lt, lw, sn = np.exp(g["theta"][s])
+        k = lambda t1, w1, t2, w2: A * np.exp(-(t1[:, None] - t2[None]) ** 2 / (2 * lt * lt)
+                                              - (w1[:, None] - w2[None]) ** 2 / (2 * lw * lw))
+        K = k(t, w, t, w) + np.diag(nv + sn * sn)
+        ks = k(qt, qw, t, w)
+        mu = ks @ np.linalg.solve(K, y - y.mean()) + y.mean()
+        var = A - np.einsum("ij,ji->i", ks, np.linalg.solve(K, ks.T))
+        assert np.allclose(g["pred"][s], mu, rtol=1e-9, atol=1e-9)
+        assert np.allclose(g["std"][s], np.sqrt(np.maximum(var, 1e-10)), rtol=1e-6, atol=1e-8)
+
+
+def test_failures(ctx):
+    rng = np.random.default_rng(5)
+    csr = make(rng, [10, 0, 12])
+    csr["noise_var"][:10] = -1.0
+    th0 = np.tile(np.log([1.0, 20.0, 0.3, 1e-3]), (3, 1))
+    g = ctx.gp2d_train(csr, th0, max_iters=5)
+    assert list(g["status"][:2]) == [3, 3] and g["status"][2] in (0, 1, 2)
+    assert np.isnan(g["nlml"][:2]).all() and np.isfinite(g["nlml"][2])
+    e = ctx.gp2d_nlml_grad(csr, th0)
+    assert list(e["status"]) == [3, 3, 0]

@@ -1,0 +1,65 @@
+"""Pins the oracle's 2D GP hyper-parameter objective and driver (CPU only):
+
+  * NLML against numpy (slogdet + solve of the same K) on random sources;
+  * the adjoint gradient against central finite differences of the NLML;
+  * the L-BFGS driver against scipy's L-BFGS-B on the same objective and box:
+    the same minimum (NLML within 1e-7 absolute + 1e-9 relative);
+  * K not positive definite (negative noise variances) -> status 3.
+"""
+import numpy as np
+import pytest
+
+
+def source(rng, n, noise=0.03):
+    t = np.sort(rng.uniform(0.0, 100.0, n))
+    w = np.log10(rng.choice([4770.0, 6231.0, 7625.0], n))
+    y = 18.0 + np.sin(t / 15.0) + 0.5 * (w - 3.8) + noise * rng.normal(size=n)
+    return t, w, y, np.full(n, noise ** 2)
+
+
+def numpy_nlml(t, w, y, nv, th):
+    a, lt, lw, sn = np.exp(th)
+    K = a * np.exp(-(t[:, None] - t[None]) ** 2 / (2 * lt * lt) - (w[:, None] - w[None]) ** 2 / (2 * lw * lw))
+    K += np.diag(nv + sn * sn)
+    r = y - y.mean()
+    return 0.5 * r @ np.linalg.solve(K, r) + 0.5 * np.linalg.slogdet(K)[1] + 0.5 * y.size * np.log(2 * np.pi)
+
+
+@pytest.mark.parametrize("n", [1, 7, 40, 90])
+def test_nlml_and_gradient(orc, n):
+    rng = np.random.default_rng(n)
+    t, w, y, nv = source(rng, n)
+    for th in (np.log([1.0, 20.0, 0.3, 0.01]), np.log([0.2, 3.0, 0.05, 0.1])):
+        f, g = orc.gp2d_nlml_grad(t, w, y, nv, th)
+        assert f == pytest.approx(numpy_nlml(t, w, y, nv, th), rel=1e-11, abs=1e-11)
+        h = 1e-6
+        fd = [(orc.gp2d_nlml_grad(t, w, y, nv, th + h * e)[0] - orc.gp2d_nlml_grad(t, w, y, nv, th - h * e)[0])
+              / (2 * h) for e in np.eye(4)]
+        assert np.allclose(g, fd, rtol=1e-5, atol=1e-6)
+
+
+def test_driver_matches_scipy(orc):
+    from scipy.optimize import minimize
+
+    rng = np.random.default_rng(3)
+    for n in (25, 60):
+        t, w, y, nv = source(rng, n)
+        th0 = np.log([1.0, 20.0, 0.3, 0.01])
+        csr = {"offsets": np.array([0, n]), "times": t, "log_waves": w, "mags": y, "noise_var": nv}
+        r = orc.batch_gp2d_train(csr, th0[None], 200, 1e-9)
+        fun = lambda x: orc.gp2d_nlml_grad(t, w, y, nv, x)
+        ref = minimize(lambda x: fun(x)[0], th0, jac=lambda x: fun(x)[1], method="L-BFGS-B",
+                       bounds=list(zip(orc.GP2D_TH_LO, orc.GP2D_TH_HI)), options={"ftol": 1e-15, "gtol": 1e-10})
+        assert r["status"][0] in (0, 1, 2)
+        assert r["nlml"][0] == pytest.approx(ref.fun, rel=1e-9, abs=1e-7)
+        assert r["nlml"][0] <= fun(th0)[0]
+
+
+def test_not_positive_definite(orc):
+    rng = np.random.default_rng(4)
+    t, w, y, nv = source(rng, 10)
+    assert orc.gp2d_nlml_grad(t, w, y, -np.ones(10), np.log([1.0, 20.0, 0.3, 1e-3])) is None
+    csr = {"offsets": np.array([0, 10, 10]), "times": t, "log_waves": w, "mags": y, "noise_var": -np.ones(10)}
+    th0 = np.tile(np.log([1.0, 20.0, 0.3, 1e-3]), (2, 1))
+    r = orc.batch_gp2d_train(csr, th0, 10, 0.0)
+    assert list(r["status"]) == [3, 3] and np.isnan(r["nlml"]).all()
