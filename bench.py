@@ -217,6 +217,75 @@ def np_leg(pkg, n_sources: int, n_grid: int, dense_max: int, reps: int, cpu_sour
     return out
 
 
+def gp2d_leg(pkg, n_sources: int, max_iters: int, n_times: int, reps: int, cpu_sources: int, threads: int,
+             do_cpu: bool) -> dict:
+    """2D GP hyper-parameter workload: n_sources cooling-blackbody x Bazin
+    sources of U{30..512} points over g/r/i, L-BFGS (at most max_iters
+    iterations) on the exact Cholesky-adjoint gradient of the log marginal
+    likelihood per source, predictions on n_times x 3 bands at the trained
+    hyper-parameters, then the blackbody temperature fit per (source, time).
+    Time = device time of the two launches (inputs resident).  Roofline: the
+    O(n^3) FP64 work per NLML evaluation (Cholesky, n^3/3) and gradient
+    evaluation (inverse + X^T X, 2 n^3 / 3), counted from the kernel's own
+    evaluation counters.  CPU baseline: the oracle's driver on a bounded
+    sample of the same workload."""
+    c = pkg.synthetic.cooling_bb_batch(n_sources, seed=2025)
+    ns = n_sources
+    wl = np.asarray(pkg.synthetic.GRI_WAVELENGTH)
+    tq = np.linspace(0.0, 100.0, n_times)
+    qt, qw = np.repeat(tq, 3), np.tile(np.log10(wl), n_times)
+    ctx = pkg.GpuContext(0)
+    run = lambda: ctx.gp2d_train(c, c["theta0"], max_iters=max_iters, grad_tol=1e-6, query_t=qt, query_w=qw)
+    run()  # warm-up
+    best = None
+    for _ in range(max(reps, 1)):
+        r = run()
+        k = ctx.last_kernel_ms
+        best = (k, r) if best is None or k < best[0] else best
+    ms_train, r = best
+    ok = r["status"] < 3
+    off = np.zeros(ns * n_times + 1, dtype=np.int64)
+    off[1:] = np.cumsum(np.repeat(np.where(ok, 3, 0), n_times))
+    sel = np.repeat(ok, n_times)
+    lam = np.tile(wl, ns * n_times).reshape(ns * n_times, 3)[sel].ravel()
+    mag = r["pred"].reshape(ns * n_times, 3)[sel].ravel()
+    err = r["std"].reshape(ns * n_times, 3)[sel].ravel()
+    bb = ctx.bb_temperature(off, lam, mag, err)
+    ms_bb = ctx.last_kernel_ms
+    ms = ms_train + ms_bb
+    n = np.diff(c["offsets"]).astype(np.float64)
+    ev = r["evals"].astype(np.float64)
+    flops = float(np.sum(ev[:, 0] * (n + 1) ** 3 / 3.0 + ev[:, 1] * 2.0 * n ** 3 / 3.0))
+    achieved = flops / (ms_train * 1e-3) / 1e12
+    log(f"gp2d: {ns} sources, {ms_train:.1f} ms training + {ms_bb:.2f} ms blackbody, "
+        f"{achieved:.3f} TFLOP/s, evals {ev.sum(axis=0).astype(int).tolist()}")
+    out = {"workload": "gp2d_hyperparameter_adjoint_lbfgs", "sources": ns, "observations": int(c["offsets"][-1]),
+           "points_per_source": "U{30..512}", "lbfgs_max_iters": max_iters, "dtype": "f64",
+           "value": round(ns / (ms * 1e-3), 2), "unit": "sources/s", "ms": round(ms, 3),
+           "ms_training": round(ms_train, 3), "ms_blackbody": round(ms_bb, 3),
+           "status_counts": np.bincount(r["status"], minlength=4).tolist(),
+           "mean_iterations": round(float(r["iters"].mean()), 2),
+           "evaluations": {"nlml": int(ev[:, 0].sum()), "gradient": int(ev[:, 1].sum())},
+           "temperatures_fitted": int(np.sum(bb["ok"])),
+           "roofline": {"bound": "fp64 vector ALU", "kernel": "gp2d_hyper_kernel",
+                        "achieved": round(achieved, 4), "peak": 78.6, "unit": "TFLOP/s",
+                        "frac": round(achieved / 78.6, 5), "flops": flops}}
+    if do_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+
+        sub = pkg.synthetic.cooling_bb_batch(cpu_sources, seed=2025)
+        t0 = time.perf_counter()
+        oracle.batch_gp2d_train(sub, sub["theta0"], max_iters, 1e-6, n_threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(cpu_sources / dt, 3), "unit": "sources/s", "cores": threads,
+                               "kind": "port", "sample": f"{cpu_sources} sources of the same workload, oracle FP64 "
+                                                         f"C++ NLML + adjoint + L-BFGS (training only), {threads} "
+                                                         f"threads, {dt:.1f} s"}
+        log(f"gp2d cpu baseline: {cpu_sources / dt:.2f} sources/s")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -234,6 +303,9 @@ def main() -> int:
     ap.add_argument("--no-np", action="store_true")
     ap.add_argument("--np-sources", type=int, default=10_000)
     ap.add_argument("--np-cpu-sources", type=int, default=3000)
+    ap.add_argument("--no-gp2d", action="store_true")
+    ap.add_argument("--gp2d-sources", type=int, default=5000)
+    ap.add_argument("--gp2d-cpu-sources", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
@@ -314,6 +386,11 @@ def main() -> int:
         npl = np_leg(pkg, args.np_sources, 200, 200, max(args.steps, 1), args.np_cpu_sources,
                      min(os.cpu_count() or 1, 16), not args.no_cpu)
 
+    gp2 = None
+    if not args.no_gp2d:
+        gp2 = gp2d_leg(pkg, args.gp2d_sources, 100, 20, max(args.steps, 1), args.gp2d_cpu_sources,
+                       min(os.cpu_count() or 1, 16), not args.no_cpu)
+
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -338,6 +415,7 @@ def main() -> int:
         "plain": plain,
         "per_model_ms": {MODEL_NAMES[m]: round(float(slot_ms[i]), 3) for i, m in enumerate(MODELS)},
         "svi": svi,
+        "gp2d": gp2,
         "nonparametric": npl,
     }
     print(json.dumps(line), flush=True)

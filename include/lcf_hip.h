@@ -338,20 +338,22 @@ int lcf_batch_gp2d_fit_ex(lcf_ctx* ctx, int64_t n_sources, const int64_t* offset
  * on y - mean(y), theta = ln (a, lt, lw, sn) ([n][4]).  The negative log
  * marginal likelihood and its exact gradient 1/2 tr((K^-1 - alpha alpha^T)
  * dK/dtheta) come from a blocked FP64 Cholesky and its adjoint; L-BFGS
- * (6 pairs, projected backtracking Armijo search, box a in [1e-6, 1e6],
+ * (6 pairs, projected backtracking Armijo search that gives up once a step
+ * moves theta by <= 1e-10, box a in [1e-6, 1e6],
  * lt in [0.1, 1e4], lw in [1e-3, 10], sn in [1e-6, 10]) runs at most
  * max_iters iterations from theta0, stopping when the projected gradient's
  * largest component is <= grad_tol.  out_status: 0 converged, 1 iteration
  * limit, 2 no further decrease along the search direction, 3 K not positive
  * definite at theta0 (or no points).  Optional queries (n_query > 0; shared,
  * or [n][n_query] when query_per_source) give the posterior mean (+ mean(y))
- * and standard deviation at the returned theta. */
+ * and standard deviation at the returned theta.  out_evals ([n][2], may be
+ * NULL) counts the NLML and gradient evaluations of each source. */
 int lcf_batch_gp2d_train(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,
                          const double* log_waves, const double* mags, const double* noise_var,
                          const double* theta0, int max_iters, double grad_tol, const double* query_times,
                          const double* query_waves, int n_query, int query_per_source, double* out_theta,
                          double* out_nlml, double* out_grad, int32_t* out_iters, int32_t* out_status,
-                         double* out_pred, double* out_std);
+                         double* out_pred, double* out_std, int32_t* out_evals);
 
 /* The same objective and gradient at theta ([n][4], not clamped): one
  * evaluation per source; out_status 0, or 3 when K is not positive definite

@@ -16,8 +16,9 @@
 //       blocks as X_IJ = -X_II sum_K L_IK X_KJ), alpha = X^T z, and
 //       K^-1 = X^T X formed tile by tile and contracted at once with the
 //       recomputed kernel derivatives, so K^-1 is never stored;
-//   L-BFGS (m = 6) with a projected backtracking Armijo search and box
-//       bounds, exactly as the oracle's driver; thread 0 holds the 4-vector
+//   L-BFGS (m = 6) with a projected backtracking Armijo search (given up
+//       once a step moves theta by <= 1e-10) and box bounds, exactly as the
+//       oracle's driver; thread 0 holds the 4-vector
 //       algebra, every thread takes part in the evaluations.
 // Predictions at query points use alpha and X of the returned point.
 #include "gp2d_hyper.hpp"
@@ -50,7 +51,7 @@ struct Smem {
     double b[T * T];
     double red[NT];
     double st[S_END];
-    int flag[8];  // 0 pivot failure, 1 stop, 2 accepted, 3 nm, 4 head
+    int flag[8];  // 0 pivot failure, 1 stop, 2 accepted, 3 nm, 4 head, 5 step too small
 };
 
 struct Src {
@@ -429,13 +430,15 @@ __device__ bool lbfgs_direction(Smem& sm, double gtol) {
 // decrease term g^T (tt - th).
 __device__ void trial_point(Smem& sm) {
     double* st = sm.st;
-    double dec = 0.0;
+    double dec = 0.0, mv = 0.0;
     for (int k = 0; k < 4; k++) {
         const double v = fmin(fmax(st[S_TH + k] + st[S_STEP] * st[S_D + k], c_th_lo[k]), c_th_hi[k]);
         st[S_TT + k] = v;
         dec += st[S_G + k] * (v - st[S_TH + k]);
+        mv = fmax(mv, fabs(v - st[S_TH + k]));
     }
     st[S_DEC] = dec;
+    sm.flag[5] = mv <= 1e-10;  // the step no longer moves theta
 }
 
 // Thread 0: store the pair (s, y) and move to the trial point.
@@ -530,6 +533,10 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
             a.nlml[src] = NAN;
             a.iters[src] = 0;
             a.status[src] = 3;
+            if (a.evals) {
+                a.evals[2 * src] = 0;
+                a.evals[2 * src + 1] = 0;
+            }
         }
         for (int q = tid; q < a.n_query && a.pred; q += NT) {
             a.pred[(int64_t)src * a.n_query + q] = NAN;
@@ -554,11 +561,12 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
     double th[4], g[4], f = NAN;
 #pragma unroll
     for (int k = 0; k < 4; k++) th[k] = st[S_TH + k];
-    int status = 1, it = 0;
+    int status = 1, it = 0, nf = 1, ng = 0;
     if (!eval_f(A, a.ld, s, th, ybar, zv, sm, f)) {
         status = 3;
     } else {
         gradient(A, Xm, ld, s, th, zv, av, sm, g);
+        ng++;
         if (tid == 0) {
             for (int k = 0; k < 4; k++) st[S_G + k] = g[k];
             st[S_F] = f;
@@ -578,10 +586,12 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
                 for (int ls = 0; ls < 30; ls++) {
                     if (tid == 0) trial_point(sm);
                     __syncthreads();
+                    if (sm.flag[5]) break;
                     double tt[4], ft = NAN;
 #pragma unroll
                     for (int k = 0; k < 4; k++) tt[k] = st[S_TT + k];
                     const bool pd = eval_f(A, ld, s, tt, ybar, zv, sm, ft);
+                    nf++;
                     if (tid == 0) {
                         sm.flag[2] = pd && ft <= st[S_F] + 1e-4 * st[S_DEC];
                         st[S_FT] = ft;
@@ -601,6 +611,7 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
 #pragma unroll
                 for (int k = 0; k < 4; k++) tt[k] = st[S_TT + k];
                 gradient(A, Xm, ld, s, tt, zv, av, sm, gn);
+                ng++;
                 if (tid == 0) {
                     for (int k = 0; k < 4; k++) st[S_GN + k] = gn[k];
                     accept_step(sm);
@@ -612,6 +623,8 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
             for (int k = 0; k < 4; k++) th[k] = st[S_TH + k];
             eval_f(A, ld, s, th, ybar, zv, sm, f);
             gradient(A, Xm, ld, s, th, zv, av, sm, g);
+            nf++;
+            ng++;
         }
     }
     if (tid == 0) {
@@ -622,6 +635,10 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
         a.nlml[src] = status == 3 ? NAN : f;
         a.iters[src] = it;
         a.status[src] = status;
+        if (a.evals) {
+            a.evals[2 * src] = nf;
+            a.evals[2 * src + 1] = ng;
+        }
     }
     if (a.pred && a.n_query > 0) {
         if (status == 3) {

@@ -34,6 +34,7 @@ struct Gp2dHyperArgs {
     double* grad;             // [n_src][4]
     int32_t* iters;           // [n_src]
     int32_t* status;          // [n_src]
+    int32_t* evals;           // [n_src][2] NLML and gradient evaluations (may be null)
     double* pred;             // [n_src][n_query] (may be null)
     double* stdv;             // [n_src][n_query]
 };

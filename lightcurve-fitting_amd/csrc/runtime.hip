@@ -1034,7 +1034,7 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
                    const double* y, const double* nv, const double* theta0, int max_iters, double gtol,
                    const double* qt, const double* qw, int n_query, int q_per_src, double* out_theta,
                    double* out_nlml, double* out_grad, int32_t* out_iters, int32_t* out_status, double* out_pred,
-                   double* out_std) {
+                   double* out_std, int32_t* out_evals) {
     if (!c) return LCF_E_INVALID;
     if (ns < 0) return fail(c, LCF_E_INVALID, "negative source count");
     if (ns == 0) return LCF_OK;
@@ -1068,8 +1068,9 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
     const int64_t nq_tot = n_query > 0 ? (q_per_src ? ns * (int64_t)n_query : (int64_t)n_query) : 0;
     DevBuf<int64_t> d_off;
     DevBuf<double> d_t, d_w, d_y, d_nv, d_th0, d_work, d_th, d_f, d_g, d_qt, d_qw, d_pred, d_std;
-    DevBuf<int32_t> d_order, d_it, d_st;
+    DevBuf<int32_t> d_order, d_it, d_st, d_ev;
     HIPCHK(c, d_off.alloc(ns + 1));
+    HIPCHK(c, d_ev.alloc(ns * 2));
     HIPCHK(c, d_t.alloc(no));
     HIPCHK(c, d_w.alloc(no));
     HIPCHK(c, d_y.alloc(no));
@@ -1125,6 +1126,7 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
     a.grad = d_g.p;
     a.iters = d_it.p;
     a.status = d_st.p;
+    a.evals = d_ev.p;
     a.pred = n_query > 0 ? d_pred.p : nullptr;
     a.stdv = n_query > 0 ? d_std.p : nullptr;
     HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
@@ -1143,6 +1145,7 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
     HIPCHK(c, down(out_grad, d_g.p, sizeof(double) * ns * 4));
     HIPCHK(c, down(out_iters, d_it.p, sizeof(int32_t) * ns));
     HIPCHK(c, down(out_status, d_st.p, sizeof(int32_t) * ns));
+    HIPCHK(c, down(out_evals, d_ev.p, sizeof(int32_t) * ns * 2));
     if (n_query > 0) {
         HIPCHK(c, down(out_pred, d_pred.p, sizeof(double) * ns * n_query));
         HIPCHK(c, down(out_std, d_std.p, sizeof(double) * ns * n_query));
@@ -1157,12 +1160,13 @@ int lcf_batch_gp2d_train(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, 
                          const double* log_waves, const double* mags, const double* noise_var, const double* theta0,
                          int max_iters, double grad_tol, const double* query_times, const double* query_waves,
                          int n_query, int query_per_source, double* out_theta, double* out_nlml, double* out_grad,
-                         int32_t* out_iters, int32_t* out_status, double* out_pred, double* out_std) {
+                         int32_t* out_iters, int32_t* out_status, double* out_pred, double* out_std,
+                         int32_t* out_evals) {
     if (c && max_iters < 0) return fail(c, LCF_E_INVALID, "max_iters must be >= 0");
     if (c && (!out_iters || !out_status)) return fail(c, LCF_E_INVALID, "null pointer");
     return gp2d_hyper_run(c, n_sources, offsets, times, log_waves, mags, noise_var, theta0, max_iters, grad_tol,
                           query_times, query_waves, n_query, query_per_source, out_theta, out_nlml, out_grad,
-                          out_iters, out_status, out_pred, out_std);
+                          out_iters, out_status, out_pred, out_std, out_evals);
 }
 
 int lcf_gp2d_nlml_grad(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, const double* times,
@@ -1171,7 +1175,7 @@ int lcf_gp2d_nlml_grad(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, co
     if (c && !out_status) return fail(c, LCF_E_INVALID, "null pointer");
     std::vector<double> th((size_t)std::max<int64_t>(n_sources, 0) * 4);
     return gp2d_hyper_run(c, n_sources, offsets, times, log_waves, mags, noise_var, theta, -1, 0.0, nullptr, nullptr,
-                          0, 0, th.data(), out_nlml, out_grad, nullptr, out_status, nullptr, nullptr);
+                          0, 0, th.data(), out_nlml, out_grad, nullptr, out_status, nullptr, nullptr, nullptr);
 }
 
 int lcf_batch_gp_fit(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, const double* mags,

@@ -573,7 +573,8 @@ def _gp2d_train(self, csr: dict, theta0, *, max_iters: int = 100, grad_tol: floa
     """2D GP hyper-parameter fit with the Cholesky adjoint
     (``lcf_batch_gp2d_train``): csr holds offsets / times / waves (log10
     wavelength) / mags / noise_var; theta0 [n][4] = ln (a, l_t, l_w, sigma_n).
-    Returns theta, nlml, grad, iters, status and, with queries, pred / std."""
+    Returns theta, nlml, grad, iters, status, evals ([n][2] NLML / gradient
+    evaluations) and, with queries, pred / std."""
     off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
     ns = off.size - 1
     t, w, y, nv = (_f64(csr[k]) for k in ("times", "waves", "mags", "noise_var"))
@@ -585,13 +586,14 @@ def _gp2d_train(self, csr: dict, theta0, *, max_iters: int = 100, grad_tol: floa
     else:
         qt = qw = np.zeros(1)
     out = {"theta": np.zeros((ns, 4)), "nlml": np.zeros(ns), "grad": np.zeros((ns, 4)),
-           "iters": np.zeros(ns, np.int32), "status": np.zeros(ns, np.int32),
+           "iters": np.zeros(ns, np.int32), "status": np.zeros(ns, np.int32), "evals": np.zeros((ns, 2), np.int32),
            "pred": np.zeros((ns, max(nq, 1))), "std": np.zeros((ns, max(nq, 1)))}
     d = lambda a: _lib.ptr(a, C.c_double)
     self._check(self._lib.lcf_batch_gp2d_train(
         self._h, ns, _lib.ptr(off, C.c_int64), d(t), d(w), d(y), d(nv), d(th0), int(max_iters), float(grad_tol),
         d(qt), d(qw), nq, int(query_per_source), d(out["theta"]), d(out["nlml"]), d(out["grad"]),
-        _lib.ptr(out["iters"], C.c_int32), _lib.ptr(out["status"], C.c_int32), d(out["pred"]), d(out["std"])))
+        _lib.ptr(out["iters"], C.c_int32), _lib.ptr(out["status"], C.c_int32), d(out["pred"]), d(out["std"]),
+        _lib.ptr(out["evals"], C.c_int32)))
     if nq == 0:
         del out["pred"], out["std"]
     return out

@@ -241,6 +241,50 @@ def bazin_mag_batch(n_sources: int, seed: int = 0, n_bands: int = 3, n_min: int 
             "t_lo": np.zeros(nb), "t_hi": tmax[band_src]}
 
 
+GRI_WAVELENGTH = (4770.0, 6231.0, 7625.0)  # effective wavelengths of g, r, i (src/gp2d.rs:84-90), Angstrom
+
+
+def cooling_bb_batch(n_sources: int, seed: int = 0, n_min: int = 30, n_max: int = 512,
+                     t_span: float = 100.0, noise: float = 0.03) -> dict:
+    """2D GP workload: ``n_sources`` sources of U{n_min..n_max} points spread
+    uniformly over g / r / i, times uniform on [0, t_span] d (sorted per
+    source).  Flux = Bazin(t) x B_lambda(T(t)) / B_6231(T(t)) with the
+    temperature cooling exponentially from 10^4 K at t = 0 to 6000 K at
+    t = t_span, Bazin A ~ U(100, 1000) uJy, t0 ~ U(20, 60), tau_rise ~ U(1, 10),
+    tau_fall ~ U(10, 60); multiplicative Gaussian noise of 3 %;
+    mag = 23.9 - 2.5 log10(max(F, 1)), noise variance (1.0857 x 0.03)^2.
+    Returns the 2D GP CSR (offsets, times, waves = log10 wavelength, mags,
+    noise_var) and ``theta0`` = ln (var(mag), span / 4, 0.2 dex, 0.01) per
+    source."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    sizes = rng.integers(n_min, n_max + 1, size=n_sources)
+    off = np.zeros(n_sources + 1, dtype=np.int64)
+    off[1:] = np.cumsum(sizes)
+    src = np.repeat(np.arange(n_sources), sizes)
+    t = rng.uniform(0.0, t_span, size=int(off[-1]))
+    t = np.sort(t + src * (2.0 * t_span)) - src * (2.0 * t_span)
+    lam = np.asarray(GRI_WAVELENGTH)[rng.integers(0, 3, size=t.size)]
+    a = rng.uniform(100.0, 1000.0, n_sources)[src]
+    t0 = rng.uniform(20.0, 60.0, n_sources)[src]
+    tr = rng.uniform(1.0, 10.0, n_sources)[src]
+    tf = rng.uniform(10.0, 60.0, n_sources)[src]
+    temp = 1e4 * (0.6 ** (t / t_span))
+    hck = 1.4387769e8  # h c / k in Angstrom K
+    with np.errstate(over="ignore"):
+        bb = (6231.0 / lam) ** 5 * np.expm1(hck / (6231.0 * temp)) / np.expm1(hck / (lam * temp))
+        f = a * np.exp(-(t - t0) / tf) / (1.0 + np.exp(-(t - t0) / tr)) * bb
+    f = f * (1.0 + noise * rng.normal(size=t.size))
+    mags = 23.9 - 2.5 * np.log10(np.maximum(f, 1.0))
+    nv = np.full(t.size, (1.0857 * noise) ** 2)
+    w = np.log10(lam)
+    th0 = np.zeros((n_sources, 4))
+    for s in range(n_sources):
+        y = mags[off[s]:off[s + 1]]
+        th0[s] = [np.log(max(float(np.var(y)), 1e-4)), np.log(max(float(np.ptp(t[off[s]:off[s + 1]])), 1.0) / 4.0),
+                  np.log(0.2), np.log(0.01)]
+    return {"offsets": off, "times": t, "waves": w, "log_waves": w, "mags": mags, "noise_var": nv, "theta0": th0}
+
+
 class _XorShift64:
     """xorshift64 with 53-bit uniforms and Box-Muller normals, the generator
     of the reference's test fixtures (tests/synthetic.rs:17-46)."""

@@ -17,7 +17,8 @@
 // The driver is limited-memory BFGS (Nocedal & Wright, algorithm 7.4; m = 6
 // pairs, H0 = s^T y / y^T y), a projected backtracking Armijo line search
 // (c1 = 1e-4, halving, at most 30 trials, first trial capped to a step of 2
-// in every log-parameter), box bounds on theta, and a final re-evaluation at
+// in every log-parameter, given up once the step moves theta by <= 1e-10),
+// box bounds on theta, and a final re-evaluation at
 // the returned point; convergence is tested on the projected gradient
 // (components pressing against their bound count as zero).  The device kernel
 // runs the same algorithm.
@@ -132,18 +133,21 @@ const double GP2D_TH_LO[4] = {-13.815510557964274, -2.302585092994046, -6.907755
 const double GP2D_TH_HI[4] = {13.815510557964274, 9.210340371976184, 2.302585092994046, 2.302585092994046};
 
 int gp2d_train_one(const Gp2dData& d, const double* th0, int max_iters, double gtol, double* th_out,
-                   double* f_out, double* g_out, int* iters_out) {
+                   double* f_out, double* g_out, int* iters_out, int* evals) {
     const int M = 6;
     double th[4], g[4], f;
     for (int k = 0; k < 4; k++) th[k] = std::min(std::max(th0[k], GP2D_TH_LO[k]), GP2D_TH_HI[k]);
     std::vector<double> L, z;
     *iters_out = 0;
+    evals[0] = 1;
+    evals[1] = 0;
     if (!nlml_eval(d, th, L, z, f)) {
         for (int k = 0; k < 4; k++) { th_out[k] = th[k]; g_out[k] = NAN; }
         *f_out = NAN;
         return 3;
     }
     grad_eval(d, th, L, z, g);
+    evals[1]++;
     double S[6][4], Y[6][4], rho[6];
     int nm = 0, head = 0, status = 1, it = 0;
     for (; it < max_iters; it++) {
@@ -198,17 +202,21 @@ int gp2d_train_one(const Gp2dData& d, const double* th0, int max_iters, double g
         double tt[4], ft = NAN;
         bool ok = false;
         for (int ls = 0; ls < 30; ls++) {
-            double dec = 0.0;
+            double dec = 0.0, mv = 0.0;
             for (int k = 0; k < 4; k++) {
                 tt[k] = std::min(std::max(th[k] + step * dd[k], GP2D_TH_LO[k]), GP2D_TH_HI[k]);
                 dec += g[k] * (tt[k] - th[k]);
+                mv = std::max(mv, std::fabs(tt[k] - th[k]));
             }
+            if (mv <= 1e-10) break;  // the step no longer moves theta: no further decrease
+            evals[0]++;
             if (nlml_eval(d, tt, L, z, ft) && ft <= f + 1e-4 * dec) { ok = true; break; }
             step *= 0.5;
         }
         if (!ok) { status = 2; break; }
         double gn[4];
         grad_eval(d, tt, L, z, gn);
+        evals[1]++;
         double sy = 0.0, sk[4], yk[4];
         for (int k = 0; k < 4; k++) {
             sk[k] = tt[k] - th[k];
@@ -228,6 +236,8 @@ int gp2d_train_one(const Gp2dData& d, const double* th0, int max_iters, double g
     // final state at the returned point
     nlml_eval(d, th, L, z, f);
     grad_eval(d, th, L, z, g);
+    evals[0]++;
+    evals[1]++;
     for (int k = 0; k < 4; k++) { th_out[k] = th[k]; g_out[k] = g[k]; }
     *f_out = f;
     return status;
@@ -249,7 +259,7 @@ extern "C" int orc_gp2d_nlml_grad(const double* t, const double* w, const double
 extern "C" int orc_batch_gp2d_train(int64_t n_src, const int64_t* off, const double* t, const double* w,
                                     const double* y, const double* nv, const double* theta0, int max_iters,
                                     double gtol, double* theta, double* nlml, double* grad, int32_t* iters,
-                                    int32_t* status, int n_threads) {
+                                    int32_t* status, int32_t* evals, int n_threads) {
     orc::parallel_for(n_src, n_threads, [&](int64_t s) {
         const int64_t a = off[s];
         orc::Gp2dData d{t + a, w + a, y + a, nv + a, (int)(off[s + 1] - a)};
@@ -259,11 +269,15 @@ extern "C" int orc_batch_gp2d_train(int64_t n_src, const int64_t* off, const dou
             nlml[s] = NAN;
             iters[s] = 0;
             status[s] = 3;
+            evals[2 * s] = evals[2 * s + 1] = 0;
             return;
         }
+        int ev[2];
         status[s] = orc::gp2d_train_one(d, theta0 + s * 4, max_iters, gtol, theta + s * 4, nlml + s, grad + s * 4,
-                                        &it);
+                                        &it, ev);
         iters[s] = it;
+        evals[2 * s] = ev[0];
+        evals[2 * s + 1] = ev[1];
     });
     return 0;
 }

@@ -258,12 +258,14 @@ int orc_finalize(int64_t n_bands, const int64_t* off, const double* t, const dou
  * theta = ln (amplitude, time length scale, log10-wavelength length scale,
  * extra noise sigma).  nlml_grad returns 1 when K is not positive definite.
  * train: status 0 gradient below gtol, 1 max_iters reached, 2 line search
- * failed, 3 K not positive definite at theta0 (or empty source). */
+ * failed, 3 K not positive definite at theta0 (or empty source); evals[n][2]
+ * counts NLML and gradient evaluations (initial, trials, final). */
 int orc_gp2d_nlml_grad(const double* t, const double* w, const double* y, const double* nv, int n,
                        const double* theta, double* nlml, double* grad);
 int orc_batch_gp2d_train(int64_t n_src, const int64_t* off, const double* t, const double* w, const double* y,
                          const double* nv, const double* theta0, int max_iters, double gtol, double* theta,
-                         double* nlml, double* grad, int32_t* iters, int32_t* status, int n_threads);
+                         double* nlml, double* grad, int32_t* iters, int32_t* status, int32_t* evals,
+                         int n_threads);
 
 #ifdef __cplusplus
 }

@@ -615,14 +615,14 @@ def batch_gp2d_train(csr: dict, theta0, max_iters: int = 100, gtol: float = 0.0,
     th0 = np.ascontiguousarray(np.asarray(theta0, dtype=np.float64).reshape(ns, 4))
     th, g = np.zeros((ns, 4)), np.zeros((ns, 4))
     f = np.zeros(ns)
-    it, st = np.zeros(ns, np.int32), np.zeros(ns, np.int32)
+    it, st, ev = np.zeros(ns, np.int32), np.zeros(ns, np.int32), np.zeros((ns, 2), np.int32)
     L = lib()
     L.orc_batch_gp2d_train.restype = _i32
     L.orc_batch_gp2d_train.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pd, _pd, _i32, _f64, _pd, _pd, _pd, _pi32,
-                                       _pi32, _i32]
+                                       _pi32, _pi32, _i32]
     d = lambda a: _p(a, C.c_double)
     rc = L.orc_batch_gp2d_train(ns, _p(off, C.c_int64), d(arrs[0]), d(arrs[1]), d(arrs[2]), d(arrs[3]), d(th0),
                                 max_iters, gtol, d(th), d(f), d(g), _p(it, C.c_int32), _p(st, C.c_int32),
-                                n_threads or default_threads())
+                                _p(ev, C.c_int32), n_threads or default_threads())
     assert rc == 0
-    return {"theta": th, "nlml": f, "grad": g, "iters": it, "status": st}
+    return {"theta": th, "nlml": f, "grad": g, "iters": it, "status": st, "evals": ev}
