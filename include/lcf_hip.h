@@ -329,6 +329,21 @@ int lcf_batch_gp2d_fit_ex(lcf_ctx* ctx, int64_t n_sources, const int64_t* offset
                           int n_lsw, int cap, int subsample_rule, int criterion, double* out_state,
                           double* out_pred, double* out_std, double* out_dpred, double* out_pred_obs);
 
+/* fit_gp_2d (src/gp2d.rs:336-439), the CPU path's 2D GP, batched: per
+ * source (concatenated bands: times, log10 wavelengths, mags, noise
+ * variances) a training subsample of at most 200 points at floor(i n / 200),
+ * the per-source grid amp {0.3, 1, 3, 10} x lt {dur/4, dur/8, dur/16} x
+ * lw {2 wr, wr, wr/2} (dur, wr: time and log-wavelength ranges of the
+ * subsample, floored at 1 d and 0.01 dex), selection by the first strict
+ * minimum of the train RMS, then predict_with_std (:181-209) at the source's
+ * queries (query_t / query_w [n][n_query]).  out_hyp[n][3] = amp, lt, lw,
+ * out_train_rms[n], out_n_train[n] (0 and NaN when no candidate factorises or
+ * the source has fewer than 3 points). */
+int lcf_batch_fit_gp_2d(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,
+                        const double* log_waves, const double* mags, const double* noise_var,
+                        const double* query_t, const double* query_w, int n_query, double* out_hyp,
+                        double* out_train_rms, int32_t* out_n_train, double* out_pred, double* out_std);
+
 /* ---- 2D GP hyper-parameter fit with the Cholesky adjoint (FP64) -------------
  * The optimised counterpart of the hyper-parameter grid of batch_gp_2d /
  * fit_gp_2d (src/gpu_cuda.rs:1152-1306, src/gp2d.rs:336-439, 516-724): per

@@ -137,12 +137,12 @@ __device__ __forceinline__ Hyp candidate(const GpArgs& a, int band, int c, bool&
     Hyp h;
     h.amp = a.amps[ia];
     const double ls = a.lss_band ? a.lss_band[(int64_t)band * a.n_ls + il] : a.lss[il];
-    valid = ls >= 0.1 && (!a.min_ls || ls >= a.min_ls[band]);
+    valid = ls >= a.ls_floor && (!a.min_ls || ls >= a.min_ls[band]);
     h.it = 0.5 / (ls * ls);
     h.iw = 0.0;
     if (a.w) {
-        const double lw = a.lsws[iw];
-        valid = valid && lw >= 0.001;
+        const double lw = a.lsws_band ? a.lsws_band[(int64_t)band * a.n_lsw + iw] : a.lsws[iw];
+        valid = valid && lw >= a.lw_floor;
         h.iw = 0.5 / (lw * lw);
     }
     return h;
@@ -173,8 +173,10 @@ __device__ void gp_band(const GpArgs& a, double* lds, int band) {
         else if (a.subsample_rule == 0) {
             double step = (double)(n - 1) / (double)(m - 1);
             idx = (int)(i * step + 0.5);
-        } else {
+        } else if (a.subsample_rule == 1) {
             idx = (int)(((double)i + 0.5) * ((double)n / (double)m));
+        } else {  // fit_gp_2d (src/gp2d.rs:373-385)
+            idx = (int)((double)i * ((double)n / (double)m));
         }
         s.idx[i] = idx < n - 1 ? idx : n - 1;
     }

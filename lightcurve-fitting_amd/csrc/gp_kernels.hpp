@@ -16,6 +16,7 @@ struct GpArgs {
     const double* w;           // [n_obs] log10 wavelengths (2D fit) or null (1D fit)
     const double* query_w;     // query wavelengths, same shape as query (2D) or null
     const double* lsws;        // wavelength length-scale candidates (2D)
+    const double* lsws_band;   // [n_bands][n_lsw] per-source candidates (null: use lsws)
     int n_lsw;                 // 0 in 1D
     const double* y;           // [n_obs] values (magnitudes)
     const double* nv;          // [n_obs] noise variances
@@ -33,7 +34,9 @@ struct GpArgs {
     const double* min_ls;      // [n_bands] smallest admissible length scale, or null
     int n_ls;
     int cap;                   // training-set cap (subsample size)
-    int subsample_rule;        // 0: GPU rule, 1: CPU rule
+    double ls_floor;           // candidates below these length scales are skipped
+    double lw_floor;
+    int subsample_rule;        // 0: GPU rule, 1: CPU 1D rule, 2: CPU 2D rule
     int criterion;             // 0: NLML (GPU path), 1: train RMS (CPU dense path)
     double* state;             // [n_bands][state_stride]
     int state_stride;          // >= 2*cap + 4

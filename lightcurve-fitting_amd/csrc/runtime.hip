@@ -664,9 +664,10 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
            int query_per_band, const double* amps, int n_amp, const double* lss, int n_ls,
            const double* lsws, int n_lsw, int cap, int subsample_rule, int criterion, double* out_state,
            double* out_pred, double* out_std, double* out_dpred, double* out_pred_obs,
-           const double* lss_band = nullptr, const double* min_ls = nullptr) {
+           const double* lss_band = nullptr, const double* min_ls = nullptr, const double* lsws_band = nullptr,
+           double ls_floor = 0.1, double lw_floor = 0.001) {
     if (!c) return LCF_E_INVALID;
-    if (w && (n_lsw <= 0 || !lsws || (n_query > 0 && !query_w)))
+    if (w && (n_lsw <= 0 || (!lsws && !lsws_band) || (n_query > 0 && !query_w)))
         return fail(c, LCF_E_INVALID, "2D fit needs wavelengths, wavelength length scales and query wavelengths");
     if (n_bands < 0 || n_bands > INT32_MAX - 1 || !off) return fail(c, LCF_E_INVALID, "invalid band count");
     if (n_bands == 0) return LCF_OK;
@@ -682,7 +683,7 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
                         lcf::gp_lds_bytes(cap), lds_max);
     }
     if (n_query < 0 || (n_query > 0 && !query)) return fail(c, LCF_E_INVALID, "invalid query grid");
-    if (subsample_rule < 0 || subsample_rule > 1 || criterion < 0 || criterion > 1)
+    if (subsample_rule < 0 || subsample_rule > 2 || criterion < 0 || criterion > 1)
         return fail(c, LCF_E_INVALID, "unknown subsample rule or criterion");
     if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
     for (int64_t b = 0; b < n_bands; b++)
@@ -710,7 +711,7 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
     if (w) {
         HIPCHK(c, dw.alloc(n));
         HIPCHK(c, dqw.alloc(nq_tot));
-        HIPCHK(c, dlw.alloc(n_lsw));
+        HIPCHK(c, dlw.alloc(lsws_band ? n_bands * (int64_t)n_lsw : n_lsw));
     }
     HIPCHK(c, da.alloc(n_amp));
     HIPCHK(c, dl.alloc(n_ls));
@@ -733,7 +734,8 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
     if (w) {
         HIPCHK(c, up(dw.p, w, sizeof(double) * n));
         HIPCHK(c, up(dqw.p, query_w, sizeof(double) * nq_tot));
-        HIPCHK(c, up(dlw.p, lsws, sizeof(double) * n_lsw));
+        if (lsws_band) HIPCHK(c, up(dlw.p, lsws_band, sizeof(double) * n_bands * n_lsw));
+        else HIPCHK(c, up(dlw.p, lsws, sizeof(double) * n_lsw));
     }
     HIPCHK(c, up(da.p, amps, sizeof(double) * n_amp));
     if (lss) HIPCHK(c, up(dl.p, lss, sizeof(double) * n_ls));
@@ -745,8 +747,11 @@ int gp_run(lcf_ctx* c, int64_t n_bands, const int64_t* off, const double* t, con
     a.t = dt.p;
     a.w = w ? dw.p : nullptr;
     a.query_w = w ? dqw.p : nullptr;
-    a.lsws = w ? dlw.p : nullptr;
+    a.lsws = (w && !lsws_band) ? dlw.p : nullptr;
+    a.lsws_band = (w && lsws_band) ? dlw.p : nullptr;
     a.n_lsw = w ? n_lsw : 0;
+    a.ls_floor = ls_floor;
+    a.lw_floor = lw_floor;
     a.y = dy.p;
     a.nv = dnv.p;
     a.offsets = doff.p;
@@ -821,6 +826,83 @@ int lcf_batch_gp2d_fit_ex(lcf_ctx* c, int64_t n_src, const int64_t* off, const d
     if (!w) return fail(c, LCF_E_INVALID, "null wavelength array");
     return gp_run(c, n_src, off, t, w, y, nv, qt, qw, n_query, query_per_source, amps, n_amp, lst, n_lst, lsw,
                   n_lsw, cap, subsample_rule, criterion, out_state, out_pred, out_std, out_dpred, out_pred_obs);
+}
+
+int lcf_batch_fit_gp_2d(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t, const double* w,
+                        const double* y, const double* nv, const double* query_t, const double* query_w,
+                        int n_query, double* out_hyp, double* out_train_rms, int32_t* out_n_train, double* out_pred,
+                        double* out_std) {
+    if (!c) return LCF_E_INVALID;
+    if (n_src < 0 || !off) return fail(c, LCF_E_INVALID, "invalid source count");
+    if (n_src == 0) return LCF_OK;
+    if (!out_hyp || !out_train_rms || !out_n_train) return fail(c, LCF_E_INVALID, "null output");
+    if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
+    for (int64_t s = 0; s < n_src; s++)
+        if (off[s + 1] < off[s]) return fail(c, LCF_E_INVALID, "offsets not non-decreasing at %lld", (long long)s);
+    if (off[n_src] > 0 && (!t || !w || !y || !nv)) return fail(c, LCF_E_INVALID, "null observation array");
+    const int cap = 200;  // fit_gp_2d's max_n (src/gp2d.rs:371)
+    const double amps[4] = {0.3, 1.0, 3.0, 10.0};
+    std::vector<double> lst((size_t)n_src * 3), lsw((size_t)n_src * 3);
+    std::vector<int> msub((size_t)n_src);
+    for (int64_t s = 0; s < n_src; s++) {
+        const int64_t a = off[s];
+        const int n = (int)(off[s + 1] - a);
+        const int m = std::min(n, cap);
+        msub[(size_t)s] = m;
+        double t0 = INFINITY, t1 = -INFINITY, w0 = INFINITY, w1 = -INFINITY;
+        for (int i = 0; i < m; i++) {
+            const int idx = n <= cap ? i : (int)((double)i * ((double)n / (double)cap));
+            t0 = std::fmin(t0, t[a + idx]);
+            t1 = std::fmax(t1, t[a + idx]);
+            w0 = std::fmin(w0, w[a + idx]);
+            w1 = std::fmax(w1, w[a + idx]);
+        }
+        const double dur = std::fmax(t1 - t0, 1.0), wr = std::fmax(w1 - w0, 0.01);
+        lst[(size_t)s * 3 + 0] = dur / 4.0;
+        lst[(size_t)s * 3 + 1] = dur / 8.0;
+        lst[(size_t)s * 3 + 2] = dur / 16.0;
+        lsw[(size_t)s * 3 + 0] = wr * 2.0;
+        lsw[(size_t)s * 3 + 1] = wr;
+        lsw[(size_t)s * 3 + 2] = wr / 2.0;
+    }
+    const int stride = 3 * cap + 5;
+    std::vector<double> st((size_t)n_src * stride), po((size_t)std::max<int64_t>(off[n_src], 1));
+    const double dummy = 0.0;
+    const int rc = gp_run(c, n_src, off, t, w, y, nv, n_query > 0 ? query_t : &dummy,
+                          n_query > 0 ? query_w : &dummy, n_query, 1, amps, 4, nullptr, 3, nullptr, 3, cap, 2, 1,
+                          st.data(), out_pred, out_std, nullptr, po.data(), lst.data(), nullptr, lsw.data(), 0.0,
+                          0.0);
+    if (rc != LCF_OK) return rc;
+    for (int64_t s = 0; s < n_src; s++) {
+        const double* row = st.data() + (size_t)s * stride;
+        const int m = (int)row[3 * cap + 4];
+        double* hyp = out_hyp + s * 3;
+        if (m <= 0) {
+            hyp[0] = hyp[1] = hyp[2] = NAN;
+            out_train_rms[s] = NAN;
+            out_n_train[s] = 0;
+            continue;
+        }
+        hyp[0] = row[3 * cap];
+        hyp[1] = hyp[2] = NAN;
+        for (int k = 0; k < 3; k++) {
+            const double lt = lst[(size_t)s * 3 + k], lw = lsw[(size_t)s * 3 + k];
+            if (0.5 / (lt * lt) == row[3 * cap + 1] && std::isnan(hyp[1])) hyp[1] = lt;
+            if (0.5 / (lw * lw) == row[3 * cap + 2] && std::isnan(hyp[2])) hyp[2] = lw;
+        }
+        // DenseGP2D::train_rms over the subsample, in its order (src/gp2d.rs:212-219)
+        const int64_t a = off[s];
+        const int n = (int)(off[s + 1] - a);
+        double rss = 0.0;
+        for (int i = 0; i < m; i++) {
+            const int idx = n <= cap ? i : (int)((double)i * ((double)n / (double)cap));
+            const double r = po[(size_t)(a + idx)] - y[a + idx];
+            rss += r * r;
+        }
+        out_train_rms[s] = std::sqrt(rss / (double)m);
+        out_n_train[s] = m;
+    }
+    return LCF_OK;
 }
 
 int lcf_batch_gp2d_fit(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t, const double* w,

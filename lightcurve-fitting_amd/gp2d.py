@@ -121,3 +121,102 @@ def fit_gp_2d_batch_gpu(gpu: GpuContext, sources: Sequence[Mapping], n_time_step
         res = Gp2dResult(g.amp, g.ls_time, g.ls_wave, g.train_rms, g.n_train, len(p[2]), list(p[2]), list(p[3]))
         results[p[0]] = (res, Gp2dThermalResult(*per[gi]))
     return results
+
+
+GP2D_MAX_TRAIN = 200  # fit_gp_2d's training cap (src/gp2d.rs:371)
+
+
+def _concat_bands(bands: Mapping):
+    """Training arrays of fit_gp_2d (src/gp2d.rs:340-368): bands with a known
+    wavelength and >= 3 points, concatenated in the mapping's order; None
+    below 5 points or 2 bands."""
+    t, w, m, nv, names, lws = [], [], [], [], [], []
+    for name, bd in bands.items():
+        wav = BAND_WAVELENGTH.get(name)
+        if wav is None or len(bd.times) < 3:
+            continue
+        lw = float(np.log10(wav))
+        names.append(name)
+        lws.append(lw)
+        t.extend(float(x) for x in bd.times)
+        w.extend([lw] * len(bd.times))
+        m.extend(float(x) for x in bd.values)
+        nv.extend(float(e) * float(e) for e in bd.errors)
+    if len(t) < 5 or len(names) < 2:
+        return None
+    return np.asarray(t), np.asarray(w), np.asarray(m), np.asarray(nv), names, lws
+
+
+def _subsample_range(t: np.ndarray) -> tuple[float, float]:
+    n = t.size
+    if n <= GP2D_MAX_TRAIN:
+        sub = t
+    else:
+        step = n / GP2D_MAX_TRAIN
+        sub = t[[int(i * step) for i in range(GP2D_MAX_TRAIN)]]
+    return float(np.min(sub)), float(np.max(sub))
+
+
+def fit_gp_2d_with_thermal_batch(gpu: GpuContext, sources: Sequence[Mapping], n_time_steps: int,
+                                 thermal: bool = True) -> list:
+    """fit_gp_2d (src/gp2d.rs:336-439) and fit_gp_2d_with_thermal (:479-500)
+    for a list of sources (magnitude bands per source), on the device: per
+    source Some((Gp2dResult, Gp2dThermalResult or None)) or None.  The
+    thermal step evaluates the GP at n_time_steps times spanning the training
+    subsample and every band wavelength, then fits a blackbody per time
+    (extract_thermal_from_gp2d, :445-476)."""
+    preps = [(i, p) for i, p in ((i, _concat_bands(b)) for i, b in enumerate(sources)) if p is not None]
+    results: list = [None] * len(sources)
+    if not preps:
+        return results
+    off = np.zeros(len(preps) + 1, dtype=np.int64)
+    np.cumsum([p[0].size for _, p in preps], out=off[1:])
+    csr = {"offsets": off, "times": np.concatenate([p[0] for _, p in preps]),
+           "waves": np.concatenate([p[1] for _, p in preps]), "mags": np.concatenate([p[2] for _, p in preps]),
+           "noise_var": np.concatenate([p[3] for _, p in preps])}
+    nb_max = max(len(p[4]) for _, p in preps)
+    nq = max(n_time_steps, 1) * nb_max if thermal else 1
+    qt, qw = np.zeros((len(preps), nq)), np.zeros((len(preps), nq))
+    eval_times = []
+    for k, (_, p) in enumerate(preps):
+        t0, t1 = _subsample_range(p[0])
+        et = [t0 + (t1 - t0) * i / max(n_time_steps - 1, 1) for i in range(n_time_steps)]
+        eval_times.append(et)
+        if thermal and n_time_steps > 0:
+            nb = len(p[4])
+            qt[k, :n_time_steps * nb] = np.repeat(et, nb)
+            qw[k, :n_time_steps * nb] = np.tile(p[5], n_time_steps)
+    g = gpu.fit_gp_2d_arrays(csr, qt, qw)
+    bb_off, lam, mag, err, where = [0], [], [], [], []
+    if thermal:
+        for k, (_, p) in enumerate(preps):
+            if g["n_train"][k] == 0:
+                continue
+            nb = len(p[4])
+            for ti in range(n_time_steps):
+                sl = slice(ti * nb, (ti + 1) * nb)
+                lam.extend(BAND_WAVELENGTH[nm] for nm in p[4])
+                mag.extend(g["pred"][k, sl])
+                err.extend(g["std"][k, sl])
+                bb_off.append(len(lam))
+                where.append((k, ti))
+    bb = gpu.bb_temperature(bb_off, lam, mag, err) if len(bb_off) > 1 else None
+    per = {k: ([], [], []) for k in range(len(preps))}
+    for j, (k, ti) in enumerate(where):
+        if bb["ok"][j]:
+            per[k][0].append(eval_times[k][ti])
+            per[k][1].append(float(bb["log_t"][j]))
+            per[k][2].append(float(bb["chi2"][j]))
+    for k, (i, p) in enumerate(preps):
+        if g["n_train"][k] == 0:
+            continue
+        h = g["hyp"][k]
+        res = Gp2dResult(float(h[0]), float(h[1]), float(h[2]), float(g["train_rms"][k]), int(g["n_train"][k]),
+                         len(p[4]), list(p[4]), list(p[5]))
+        results[i] = (res, Gp2dThermalResult(*per[k]) if thermal else None)
+    return results
+
+
+def fit_gp_2d_batch(gpu: GpuContext, sources: Sequence[Mapping]) -> list:
+    """fit_gp_2d (src/gp2d.rs:336-439) per source: Gp2dResult or None."""
+    return [r[0] if r is not None else None for r in fit_gp_2d_with_thermal_batch(gpu, sources, 0, thermal=False)]

@@ -568,6 +568,26 @@ def _gp2d_fit(self, csr: dict, query_t, query_w, amp_candidates, lst_candidates,
     return out
 
 
+def _fit_gp_2d_arrays(self, csr: dict, query_t, query_w) -> dict:
+    """Batched ``fit_gp_2d`` (``lcf_batch_fit_gp_2d``): csr offsets / times /
+    waves / mags / noise_var; per-source queries [n][nq].  Returns hyp
+    ([n][3] amp, lt, lw), train_rms, n_train, pred, std."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    t, w, y, nv = (_f64(csr[k]) for k in ("times", "waves", "mags", "noise_var"))
+    qt = np.ascontiguousarray(np.asarray(query_t, dtype=np.float64).reshape(ns, -1))
+    qw = np.ascontiguousarray(np.asarray(query_w, dtype=np.float64).reshape(ns, -1))
+    nq = qt.shape[1]
+    out = {"hyp": np.zeros((ns, 3)), "train_rms": np.zeros(ns), "n_train": np.zeros(ns, np.int32),
+           "pred": np.zeros((ns, max(nq, 1))), "std": np.zeros((ns, max(nq, 1)))}
+    d = lambda a: _lib.ptr(a, C.c_double)
+    self._check(self._lib.lcf_batch_fit_gp_2d(
+        self._h, ns, _lib.ptr(off, C.c_int64), d(t), d(w), d(y), d(nv), d(qt), d(qw), nq, d(out["hyp"]),
+        d(out["train_rms"]), _lib.ptr(out["n_train"], C.c_int32), d(out["pred"]), d(out["std"])))
+    out["pred"], out["std"] = out["pred"][:, :nq], out["std"][:, :nq]
+    return out
+
+
 def _gp2d_train(self, csr: dict, theta0, *, max_iters: int = 100, grad_tol: float = 1e-6, query_t=None,
                 query_w=None, query_per_source: bool = False) -> dict:
     """2D GP hyper-parameter fit with the Cholesky adjoint
@@ -616,6 +636,7 @@ def _gp2d_nlml_grad(self, csr: dict, theta) -> dict:
 GpuContext.batch_gp_2d = _batch_gp_2d
 GpuContext.gp2d_fit = _gp2d_fit
 GpuContext.gp2d_train = _gp2d_train
+GpuContext.fit_gp_2d_arrays = _fit_gp_2d_arrays
 GpuContext.gp2d_nlml_grad = _gp2d_nlml_grad
 
 

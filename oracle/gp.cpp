@@ -289,4 +289,88 @@ int orc_batch_gp2d_fit(int64_t n_src, const int64_t* off, const double* t, const
                           subsample_rule, criterion, state, state_stride, pred, stdv, dpred, pred_obs, n_threads);
 }
 
+
+// fit_gp_2d (src/gp2d.rs:336-439) on prepared training arrays: subsample of
+// max_n = 200 points at floor(i n / 200); per-source grid amp {0.3, 1, 3, 10}
+// x lt {dur/4, dur/8, dur/16} x lw {2 wr, wr, wr/2} (dur = max(t range, 1),
+// wr = max(log10-wavelength range, 0.01) of the subsample), first strict
+// minimum of the train RMS; then predict_with_std at the source's queries
+// (src/gp2d.rs:181-209).  hyp[3] = amp, lt, lw (NaN and n_train 0 when no
+// candidate factorises).
+int orc_batch_fit_gp_2d(int64_t n_src, const int64_t* off, const double* t, const double* w, const double* y,
+                        const double* nv, const double* qt, const double* qw, int nq, double* hyp,
+                        double* train_rms, int32_t* n_train, double* pred, double* stdv, int n_threads) {
+    orc::parallel_for(n_src, n_threads, [&](int64_t s) {
+        const int64_t a = off[s];
+        const int n = (int)(off[s + 1] - a);
+        hyp[s * 3] = hyp[s * 3 + 1] = hyp[s * 3 + 2] = NAN;
+        train_rms[s] = NAN;
+        n_train[s] = 0;
+        for (int k = 0; k < nq; k++) pred[s * nq + k] = stdv[s * nq + k] = NAN;
+        if (n <= 0) return;
+        const int max_n = 200;
+        std::vector<double> st, sw, sy, sv;
+        if (n <= max_n) {
+            st.assign(t + a, t + a + n);
+            sw.assign(w + a, w + a + n);
+            sy.assign(y + a, y + a + n);
+            sv.assign(nv + a, nv + a + n);
+        } else {
+            const double step = (double)n / (double)max_n;
+            for (int i = 0; i < max_n; i++) {
+                const int idx = (int)((double)i * step);
+                st.push_back(t[a + idx]);
+                sw.push_back(w[a + idx]);
+                sy.push_back(y[a + idx]);
+                sv.push_back(nv[a + idx]);
+            }
+        }
+        const int m = (int)st.size();
+        double t0 = INFINITY, t1 = -INFINITY, w0 = INFINITY, w1 = -INFINITY;
+        for (int i = 0; i < m; i++) {
+            t0 = std::fmin(t0, st[i]);
+            t1 = std::fmax(t1, st[i]);
+            w0 = std::fmin(w0, sw[i]);
+            w1 = std::fmax(w1, sw[i]);
+        }
+        const double dur = std::fmax(t1 - t0, 1.0), wr = std::fmax(w1 - w0, 0.01);
+        const double amps[4] = {0.3, 1.0, 3.0, 10.0};
+        const double lts[3] = {dur / 4.0, dur / 8.0, dur / 16.0};
+        const double lws[3] = {wr * 2.0, wr, wr / 2.0};
+        double best = INFINITY, bh[3] = {0, 0, 0};
+        bool found = false;
+        orc::Dense bg;
+        for (double amp : amps)
+            for (double lt : lts)
+                for (double lw : lws) {
+                    orc::Kern k{amp, 0.5 / (lt * lt), 0.5 / (lw * lw)};
+                    orc::Dense g = orc::dense_fit(st.data(), sw.data(), sy.data(), sv.data(), m, k);
+                    if (!g.ok) continue;
+                    double rss = 0.0;
+                    for (int i = 0; i < m; i++) {
+                        const double r = orc::dense_mean(g, st[i], sw[i]) - sy[i];
+                        rss += r * r;
+                    }
+                    const double rms = std::sqrt(rss / (double)m);
+                    if (rms < best) {
+                        best = rms;
+                        bh[0] = amp;
+                        bh[1] = lt;
+                        bh[2] = lw;
+                        bg = g;
+                        found = true;
+                    }
+                }
+        if (!found) return;
+        for (int k = 0; k < 3; k++) hyp[s * 3 + k] = bh[k];
+        train_rms[s] = best;
+        n_train[s] = m;
+        for (int k = 0; k < nq; k++) {
+            pred[s * nq + k] = orc::dense_mean(bg, qt[s * nq + k], qw[s * nq + k]);
+            stdv[s * nq + k] = orc::dense_std(bg, qt[s * nq + k], qw[s * nq + k]);
+        }
+    });
+    return 0;
+}
+
 }  // extern "C"

@@ -254,6 +254,14 @@ int orc_finalize(int64_t n_bands, const int64_t* off, const double* t, const dou
                  const double* log_sigma, int flags, double* out_mu, double* out_log_sigma,
                  double* out_mag_chi2, int n_threads);
 
+/* fit_gp_2d (src/gp2d.rs:336-439) per source on concatenated training arrays
+ * (times, log10 wavelengths, mags, noise variances): hyp[n][3] = amp, lt, lw,
+ * train_rms[n], n_train[n] (0: no fit), predict_with_std at per-source
+ * queries qt/qw[n][nq]. */
+int orc_batch_fit_gp_2d(int64_t n_src, const int64_t* off, const double* t, const double* w, const double* y,
+                        const double* nv, const double* qt, const double* qw, int nq, double* hyp,
+                        double* train_rms, int32_t* n_train, double* pred, double* stdv, int n_threads);
+
 /* ---- 2D GP hyper-parameter fit (new; R & W 2.30 / 5.9, L-BFGS) -------------
  * theta = ln (amplitude, time length scale, log10-wavelength length scale,
  * extra noise sigma).  nlml_grad returns 1 when K is not positive definite.

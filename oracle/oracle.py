@@ -626,3 +626,24 @@ def batch_gp2d_train(csr: dict, theta0, max_iters: int = 100, gtol: float = 0.0,
                                 _p(ev, C.c_int32), n_threads or default_threads())
     assert rc == 0
     return {"theta": th, "nlml": f, "grad": g, "iters": it, "status": st, "evals": ev}
+
+
+def batch_fit_gp_2d(csr: dict, qt, qw, n_threads: int | None = None) -> dict:
+    """fit_gp_2d per source (offsets / times / waves / mags / noise_var) with
+    per-source queries qt / qw [n][nq]."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    ns = off.size - 1
+    arrs = [f64(csr[k]) for k in ("times", "waves", "mags", "noise_var")]
+    qt, qw = f64(qt).reshape(ns, -1), f64(qw).reshape(ns, -1)
+    nq = qt.shape[1]
+    hyp, rms, ntr = np.zeros((ns, 3)), np.zeros(ns), np.zeros(ns, np.int32)
+    pred, sd = np.zeros((ns, nq)), np.zeros((ns, nq))
+    L = lib()
+    L.orc_batch_fit_gp_2d.restype = _i32
+    L.orc_batch_fit_gp_2d.argtypes = [_i64, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _pd, _pi32, _pd, _pd,
+                                      _i32]
+    d = lambda a: _p(a, C.c_double)
+    rc = L.orc_batch_fit_gp_2d(ns, _p(off, C.c_int64), d(arrs[0]), d(arrs[1]), d(arrs[2]), d(arrs[3]), d(qt), d(qw),
+                               nq, d(hyp), d(rms), _p(ntr, C.c_int32), d(pred), d(sd), n_threads or default_threads())
+    assert rc == 0
+    return {"hyp": hyp, "train_rms": rms, "n_train": ntr, "pred": pred, "std": sd}
