@@ -248,11 +248,49 @@ int lcf_batch_laplace(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_
                       const double* map_params, int mode, double* out_log_sigma, double* out_elbo,
                       double* out_hessian);
 
+/* ---- CPU-path swarm fits (FP64) ------------------------------------------- */
+/* The fits of the reference's CPU path, which fit_parametric runs per band
+ * (src/parametric.rs:1160-1849, 3813-4131): one job is one
+ *   LCF_CPSO_SWARM_BAND   pso_minimize / pso_minimize_seeded on
+ *                         PsoCost::cost_from_slice of band `band` and model
+ *                         `model` (population priors when use_prior; t0 held
+ *                         at t0_value and left out of the search vector when
+ *                         fixed_t0, as profile_t0_search does);
+ *   LCF_CPSO_SWARM_JOINT  pso_minimize[_seeded] on fit_parametric_multiband's
+ *                         joint Villar cost over bands band .. band+n_bands-1
+ *                         of `data` (reference band first; dim = 7 n_bands,
+ *                         n_bands <= 9);
+ *   LCF_CPSO_POLISH       lbfgs_refine from x0 (start_cost = its cost; 7 pairs,
+ *                         max_iters iterations, gradient norm 1e-6,
+ *                         backtracking Armijo line search), kept only if it
+ *                         lowers the cost.
+ * The swarm is the reference's: inertia 0.9 -> 0.4, c1 = c2 = 1.5, velocity
+ * clamp and wall absorption, global best updated after every particle,
+ * relative-gain stall rule; particle 0 at x0 when seeded.  Random numbers:
+ * Philox4x32-10 keyed by `seed`, counter word `stream` (the reference's
+ * SmallRng(seed) stream is not reproduced).  lo / hi / x0 hold `dim`
+ * entries (dim <= 64, n_particles <= 256, at most 4096 observations per
+ * job).  Outputs: out_x[n_jobs][64] (best position), out_cost, out_iters
+ * (iterations executed). */
+#define LCF_CPSO_SWARM_BAND 0
+#define LCF_CPSO_SWARM_JOINT 1
+#define LCF_CPSO_POLISH 2
+typedef struct lcf_pso_job {
+    int32_t kind, model, band, n_bands, dim, n_particles, max_iters, stall_iters, use_prior, fixed_t0, seeded;
+    uint32_t stream;
+    uint64_t seed;
+    double t0_value, start_cost;
+    double lo[64], hi[64], x0[64];
+} lcf_pso_job;
+int lcf_cpu_pso(lcf_ctx* ctx, const lcf_batch* data, const lcf_pso_job* jobs, int64_t n_jobs, double* out_x,
+                double* out_cost, int32_t* out_iters);
+
 /* ---- batched finalize: profile refinement + magnitude chi2 (FP64) ------------ */
 #define LCF_FINALIZE_T0 1        /* profile_t0_refine */
 #define LCF_FINALIZE_ALL 2       /* profile_refine_all */
 #define LCF_FINALIZE_MAG_CHI2 4  /* magnitude-space chi2 */
 #define LCF_FINALIZE_DEFAULT 7
+#define LCF_FINALIZE_SVI_CHI2 8  /* magnitude chi2 from mu only, no PSO fallback (fit_parametric_multiband, :4069-4085) */
 
 /* The per-band post-processing that finalize_parametric_with_gpu_svi
  * (src/parametric.rs:3298-3421), finalize_all_models_with_gpu_svi

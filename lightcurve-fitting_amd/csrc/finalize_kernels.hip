@@ -317,7 +317,7 @@ struct Fin {
         int cnt = 0;
         for (int i = lane; i < n; i += WAVE) cnt += (o[i].z * peak > 0.0) ? 1 : 0;
         cnt = wave_sum(cnt);
-        const bool use_pso = cnt < n / 2;
+        const bool use_pso = !(a.flags & FIN_SVI_CHI2) && cnt < n / 2;
         for (int i = lane; i < n; i += WAVE) {
             const double pf = (use_pso ? o[i].w : o[i].z) * peak;
             const double f = a.flux[base + i];

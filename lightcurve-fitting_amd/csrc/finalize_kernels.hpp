@@ -13,6 +13,7 @@ constexpr int FIN_MAX_OBS = 4096;
 constexpr int FIN_T0 = 1;        // profile_t0_refine
 constexpr int FIN_ALL = 2;       // profile_refine_all
 constexpr int FIN_MAG_CHI2 = 4;  // magnitude chi2
+constexpr int FIN_SVI_CHI2 = 8;  // chi2 from the posterior mean only (no PSO fallback)
 
 struct FinalizeArgs {
     const void* obs;          // packed double4 {t, flux or upper flux, obs var, upper flag}

@@ -8,6 +8,7 @@
 #include "laplace_kernels.hpp"
 #include "finalize_kernels.hpp"
 #include "gp2d_hyper.hpp"
+#include "cpu_pso.hpp"
 #include "np_kernels.hpp"
 #include "thermal_kernels.hpp"
 #include "bb_kernels.hpp"
@@ -1017,6 +1018,124 @@ int lcf_batch_laplace(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, 
     return LCF_OK;
 }
 
+int lcf_cpu_pso(lcf_ctx* c, const lcf_batch* b, const lcf_pso_job* jobs, int64_t n_jobs, double* out_x,
+                double* out_cost, int32_t* out_iters) {
+    if (!c) return LCF_E_INVALID;
+    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (n_jobs < 0 || n_jobs > INT32_MAX) return fail(c, LCF_E_INVALID, "invalid job count");
+    if (n_jobs == 0) return LCF_OK;
+    if (!jobs || !out_x || !out_cost || !out_iters) return fail(c, LCF_E_INVALID, "null pointer");
+    const int64_t nb = b->n_sources;
+    std::vector<lcf::CpuPsoJob> hj((size_t)n_jobs);
+    std::vector<double> lo((size_t)n_jobs * lcf::CPSO_MAX_DIM), hi(lo.size()), x0(lo.size());
+    int max_obs = 0;
+    int64_t slot = 1;
+    for (int64_t k = 0; k < n_jobs; k++) {
+        const lcf_pso_job& J = jobs[k];
+        const int nbands = J.kind == LCF_CPSO_SWARM_JOINT ? J.n_bands : 1;
+        if (J.kind < 0 || J.kind > 2) return fail(c, LCF_E_INVALID, "job %lld: unknown kind %d", (long long)k, J.kind);
+        if (J.dim <= 0 || J.dim > lcf::CPSO_MAX_DIM)
+            return fail(c, LCF_E_INVALID, "job %lld: dim %d outside [1, 64]", (long long)k, J.dim);
+        if (J.band < 0 || nbands < 1 || (int64_t)J.band + nbands > nb)
+            return fail(c, LCF_E_INVALID, "job %lld: bands outside the batch", (long long)k);
+        if (J.kind == LCF_CPSO_SWARM_JOINT) {
+            if (nbands > lcf::CPSO_MAX_BANDS || J.dim != 7 * nbands)
+                return fail(c, LCF_E_INVALID, "job %lld: a joint fit needs dim = 7 bands, bands <= 9", (long long)k);
+        } else {
+            if (J.model < 0 || J.model >= LCF_N_MODELS)
+                return fail(c, LCF_E_INVALID, "job %lld: unknown model id %d", (long long)k, J.model);
+            const int np = N_PARAMS[J.model] - (J.fixed_t0 ? 1 : 0);
+            if (J.dim != np) return fail(c, LCF_E_INVALID, "job %lld: dim %d, model needs %d", (long long)k, J.dim, np);
+            if (J.kind == LCF_CPSO_POLISH && (J.model == 7 || J.fixed_t0))
+                return fail(c, LCF_E_INVALID, "job %lld: no polish for MetzgerKN or a fixed t0", (long long)k);
+        }
+        if (J.kind != LCF_CPSO_POLISH &&
+            (J.n_particles <= 0 || J.n_particles > lcf::CPSO_MAX_PARTICLES || J.max_iters <= 0 || J.stall_iters < 0))
+            return fail(c, LCF_E_INVALID, "job %lld: swarm size or iterations out of range", (long long)k);
+        if (J.kind == LCF_CPSO_POLISH && J.max_iters < 0)
+            return fail(c, LCF_E_INVALID, "job %lld: negative iteration count", (long long)k);
+        for (int d = 0; d < J.dim; d++)
+            if (!(J.lo[d] <= J.hi[d])) return fail(c, LCF_E_INVALID, "job %lld: lo > hi in dim %d", (long long)k, d);
+        const int64_t n = b->h_off[(size_t)J.band + nbands] - b->h_off[(size_t)J.band];
+        if (n > lcf::CPSO_MAX_OBS)
+            return fail(c, LCF_E_INVALID, "job %lld: %lld observations (limit %d)", (long long)k, (long long)n,
+                        lcf::CPSO_MAX_OBS);
+        max_obs = std::max<int>(max_obs, (int)n);
+        slot = std::max<int64_t>(slot, J.kind == LCF_CPSO_POLISH ? 2 * 7 * lcf::CPSO_MAX_DIM
+                                                                 : 3 * (int64_t)J.n_particles * J.dim + J.n_particles);
+        lcf::CpuPsoJob& h = hj[(size_t)k];
+        h.kind = J.kind;
+        h.model = J.model;
+        h.band = J.band;
+        h.n_bands = nbands;
+        h.dim = J.dim;
+        h.n_particles = J.n_particles;
+        h.max_iters = J.max_iters;
+        h.stall_iters = J.stall_iters;
+        h.use_prior = J.use_prior;
+        h.fixed_t0 = J.fixed_t0;
+        h.seeded = J.seeded;
+        h.stream = J.stream;
+        h.seed = J.seed;
+        h.t0_value = J.t0_value;
+        h.start_cost = J.start_cost;
+        for (int d = 0; d < lcf::CPSO_MAX_DIM; d++) {
+            const size_t o = (size_t)k * lcf::CPSO_MAX_DIM + d;
+            lo[o] = d < J.dim ? J.lo[d] : 0.0;
+            hi[o] = d < J.dim ? J.hi[d] : 0.0;
+            x0[o] = d < J.dim ? J.x0[d] : 0.0;
+        }
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int per_cu = 0;
+    HIPCHK(c, lcf::cpu_pso_occupancy(max_obs, &per_cu));
+    per_cu = std::max(per_cu, 1);
+    const int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, n_jobs);
+    DevBuf<lcf::CpuPsoJob> d_jobs;
+    DevBuf<double> d_lo, d_hi, d_x0, d_scr, d_x, d_cost;
+    DevBuf<int32_t> d_it;
+    HIPCHK(c, d_jobs.alloc(n_jobs));
+    HIPCHK(c, d_lo.alloc(lo.size()));
+    HIPCHK(c, d_hi.alloc(lo.size()));
+    HIPCHK(c, d_x0.alloc(lo.size()));
+    HIPCHK(c, d_scr.alloc((size_t)slot * grid));
+    HIPCHK(c, d_x.alloc(lo.size()));
+    HIPCHK(c, d_cost.alloc(n_jobs));
+    HIPCHK(c, d_it.alloc(n_jobs));
+    HIPCHK(c, hipMemcpyAsync(d_jobs.p, hj.data(), sizeof(lcf::CpuPsoJob) * n_jobs, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_lo.p, lo.data(), sizeof(double) * lo.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_hi.p, hi.data(), sizeof(double) * lo.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(d_x0.p, x0.data(), sizeof(double) * lo.size(), hipMemcpyHostToDevice, c->stream));
+    lcf::CpuPsoArgs a{};
+    a.obs = b->d_obs64;
+    a.offsets = b->d_off;
+    a.jobs = d_jobs.p;
+    a.n_jobs = (int)n_jobs;
+    a.queue = c->d_queue;
+    a.lo = d_lo.p;
+    a.hi = d_hi.p;
+    a.x0 = d_x0.p;
+    a.scratch = d_scr.p;
+    a.slot_doubles = slot;
+    a.max_obs = max_obs;
+    a.out_x = d_x.p;
+    a.out_cost = d_cost.p;
+    a.out_iters = d_it.p;
+    HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    HIPCHK(c, lcf::launch_cpu_pso(a, grid, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    HIPCHK(c, hipMemcpyAsync(out_x, d_x.p, sizeof(double) * lo.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_cost, d_cost.p, sizeof(double) * n_jobs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(out_iters, d_it.p, sizeof(int32_t) * n_jobs, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
+}
+
 int lcf_batch_finalize(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* pso_params,
                        const double* mu, const double* log_sigma, const double* flux, const double* flux_err,
                        const double* mag_obs, const double* peak_flux, int flags, double* out_mu,
@@ -1028,7 +1147,8 @@ int lcf_batch_finalize(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids,
     if (!model_ids || !pso_params || !mu || !log_sigma || !flux || !flux_err || !mag_obs || !peak_flux ||
         !out_mu || !out_log_sigma || !out_mag_chi2)
         return fail(c, LCF_E_INVALID, "null pointer");
-    if (flags & ~LCF_FINALIZE_DEFAULT) return fail(c, LCF_E_INVALID, "unknown finalize flags 0x%x", flags);
+    if (flags & ~(LCF_FINALIZE_DEFAULT | LCF_FINALIZE_SVI_CHI2))
+        return fail(c, LCF_E_INVALID, "unknown finalize flags 0x%x", flags);
     if (b->max_n > lcf::FIN_MAX_OBS)
         return fail(c, LCF_E_INVALID, "band of %d observations exceeds the finalize LDS limit %d", b->max_n,
                     lcf::FIN_MAX_OBS);

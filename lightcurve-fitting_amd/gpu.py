@@ -24,6 +24,14 @@ SCHED_PERSISTENT, SCHED_PER_CLASS = 0, 1
 PSO_STALL_FROM_FIRST_EVAL = 1
 FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2 = 1, 2, 4
 FINALIZE_DEFAULT = FINALIZE_T0 | FINALIZE_ALL | FINALIZE_MAG_CHI2
+FINALIZE_SVI_CHI2 = 8
+CPSO_SWARM_BAND, CPSO_SWARM_JOINT, CPSO_POLISH = 0, 1, 2
+# struct lcf_pso_job (include/lcf_hip.h)
+PSO_JOB_DTYPE = np.dtype([("kind", "<i4"), ("model", "<i4"), ("band", "<i4"), ("n_bands", "<i4"), ("dim", "<i4"),
+                          ("n_particles", "<i4"), ("max_iters", "<i4"), ("stall_iters", "<i4"),
+                          ("use_prior", "<i4"), ("fixed_t0", "<i4"), ("seeded", "<i4"), ("stream", "<u4"),
+                          ("seed", "<u8"), ("t0_value", "<f8"), ("start_cost", "<f8"), ("lo", "<f8", 64),
+                          ("hi", "<f8", 64), ("x0", "<f8", 64)], align=True)
 
 
 class GpuError(RuntimeError):
@@ -355,6 +363,20 @@ class GpuContext:
                                                 _lib.ptr(mp, C.c_double), int(mode), _lib.ptr(ls, C.c_double),
                                                 _lib.ptr(elbo, C.c_double), _lib.ptr(hess, C.c_double)))
         return {"mu": mp, "log_sigma": ls, "elbo": elbo, "hessian": hess}
+
+    # -- CPU-path swarm fits -------------------------------------------------
+    def cpu_pso(self, data: "GpuBatchData", jobs) -> dict:
+        """``lcf_cpu_pso``: swarm / polish jobs (PSO_JOB_DTYPE records) on the
+        bands of ``data``; returns x [n][64], cost, iters."""
+        jobs = np.ascontiguousarray(jobs, dtype=PSO_JOB_DTYPE)
+        nj = jobs.size
+        x, cost, it = np.zeros((nj, 64)), np.zeros(nj), np.zeros(nj, np.int32)
+        if nj == 0:
+            return {"x": x, "cost": cost, "iters": it}
+        self._check(self._lib.lcf_cpu_pso(self._h, data._h, jobs.ctypes.data_as(C.c_void_p), nj,
+                                          _lib.ptr(x, C.c_double), _lib.ptr(cost, C.c_double),
+                                          _lib.ptr(it, C.c_int32)))
+        return {"x": x, "cost": cost, "iters": it}
 
     # -- batched finalize -----------------------------------------------------
     def finalize_arrays(self, data: "GpuBatchData", model_ids, pso_params, mu, log_sigma, flux, flux_err,

@@ -408,3 +408,387 @@ __all__ = ["ALL_GPU_MODELS", "BandFitData", "GpuPsoBandResult", "ParametricBandR
            "UncertaintyMethod", "band_fit_data", "finalize_all_models_with_gpu_svi", "finalize_jobs",
            "finalize_parametric_from_gpu", "finalize_parametric_with_gpu_svi", "fit_parametric_batch_gpu",
            "prepare_bands"]
+
+
+# ---------------------------------------------------------------------------
+# CPU path of the reference (fit_parametric, src/parametric.rs:3563-4131) on
+# the device: the swarm fits, L-BFGS polish, model selection, t0 profile
+# search, posteriors, finalize and MultiBazin as batched device calls; the
+# host builds the jobs and assembles the records.
+# ---------------------------------------------------------------------------
+from .gpu import CPSO_POLISH, CPSO_SWARM_BAND, CPSO_SWARM_JOINT, FINALIZE_SVI_CHI2, PSO_JOB_DTYPE, _PSO_BOUNDS
+
+_T0_IDX = (2, 3, 2, 1, 1, 1, 1, 3)
+_LN10 = 2.30258509299404568402
+BAZIN_GOOD_ENOUGH = 2.0  # src/parametric.rs:1862
+# pso_model_select's order after Bazin (src/parametric.rs:2002-2011)
+_SELECT_REST = (GpuModelName.Arnett, GpuModelName.Tde, GpuModelName.Afterglow, GpuModelName.Villar,
+                GpuModelName.Magnetar, GpuModelName.ShockCooling, GpuModelName.MetzgerKN)
+_SEEDS = (42, 137, 271)
+# relative_band_priors (src/parametric.rs:3771-3785): mean, sigma per Villar parameter
+_REL_MEAN = (-0.015 * _LN10, -0.001, -0.095 * _LN10, -1.181, -0.195 * _LN10, -0.323 * _LN10, -0.025 * _LN10)
+_REL_SIG = (0.08 * _LN10, 0.008, 0.25 * _LN10, 1.5, 0.35 * _LN10, 0.25 * _LN10, 0.30 * _LN10)
+
+
+def population_priors(model: int) -> tuple[list, list]:
+    """population_priors (src/parametric.rs:1588-1621): centres, widths."""
+    if model == 1:
+        return ([-0.009 * _LN10, 0.009, 1.327 * _LN10, -12.0, 0.360 * _LN10, 1.454 * _LN10, -1.033 * _LN10],
+                [0.10 * _LN10, 0.010, 0.40 * _LN10, 15.0, 0.35 * _LN10, 0.40 * _LN10, 0.50 * _LN10])
+    if model == 0:
+        return [0.0, 0.0, 0.0, 1.5, 3.0, -3.0], [2.0, 0.2, 30.0, 2.5, 2.5, 2.0]
+    n = GpuModelName(model).n_params()
+    return [0.0] * n, [5.0] * n
+
+
+def pso_bounds_cpu(model: int) -> tuple[list, list]:
+    """pso_bounds (src/parametric.rs:1541-1579): the GPU box with the Bazin
+    baseline at +-0.3."""
+    lo, hi = (list(x) for x in _PSO_BOUNDS[int(model)])
+    if model == 0:
+        lo[1], hi[1] = -0.3, 0.3
+    return lo, hi
+
+
+def _peak_and_first(data: BandFitData) -> tuple[float, float]:
+    """Time of the largest detected flux (first occurrence; times[0] when
+    none) and the earliest time."""
+    t = data.times
+    pt, pf = float(t[0]), -math.inf
+    for i in range(t.size):
+        if not data.is_upper[i] and data.flux[i] > pf:
+            pf, pt = float(data.flux[i]), float(t[i])
+    return pt, float(np.min(t))
+
+
+def _clamp(v, lo, hi):
+    return lo if v < lo else (hi if v > hi else v)
+
+
+def _job(kind, model, band, dim, lo, hi, *, n_bands=1, particles=30, iters=60, stall=12, prior=1, fixed_t0=0,
+         t0=0.0, seeded=0, x0=None, seed=42, stream=0, start_cost=0.0):
+    j = np.zeros((), dtype=PSO_JOB_DTYPE)
+    j["kind"], j["model"], j["band"], j["n_bands"], j["dim"] = kind, model, band, n_bands, dim
+    j["n_particles"], j["max_iters"], j["stall_iters"] = particles, iters, stall
+    j["use_prior"], j["fixed_t0"], j["seeded"], j["stream"], j["seed"] = prior, fixed_t0, seeded, stream, seed
+    j["t0_value"], j["start_cost"] = t0, start_cost
+    j["lo"][:dim], j["hi"][:dim] = lo[:dim], hi[:dim]
+    if x0 is not None:
+        j["x0"][:dim] = x0[:dim]
+    return j
+
+
+def _single_model_setup(data: BandFitData, model: int):
+    """pso_fit_single_model's box with the data-informed t0 bounds and its
+    two seed particles (src/parametric.rs:1864-1960)."""
+    lo, hi = pso_bounds_cpu(model)
+    t0i = _T0_IDX[model]
+    peak_t, t_first = _peak_and_first(data)
+    t0_lo, t0_hi = max(t_first - 30.0, lo[t0i]), min(peak_t + 10.0, hi[t0i])
+    if t0_lo < t0_hi:
+        lo[t0i], hi[t0i] = t0_lo, t0_hi
+    pc, _ = population_priors(model)
+    base = [_clamp(pc[d], lo[d], hi[d]) for d in range(len(lo))]
+    at_peak, early = list(base), list(base)
+    at_peak[t0i] = _clamp(peak_t - 5.0, lo[t0i], hi[t0i])
+    early[t0i] = _clamp(t_first - 15.0, lo[t0i], hi[t0i])
+    return lo, hi, (None, at_peak, early)
+
+
+def fit_single_models(ctx: GpuContext, batch: GpuBatchData, datas: Sequence[BandFitData], pairs) -> dict:
+    """pso_fit_single_model (src/parametric.rs:1864-1999) for (band, model)
+    pairs: three restarts (seeds 42 / 137 / 271, the second seeded at the
+    peak, the third before the first observation), best in restart order,
+    then lbfgs_refine (not for MetzgerKN).  Returns {(band, model): (params,
+    cost)}; params empty when no restart found a finite cost."""
+    jobs, meta = [], []
+    for b, m in pairs:
+        lo, hi, seeds = _single_model_setup(datas[b], m)
+        np_ = len(lo)
+        for r, sd in enumerate(_SEEDS):
+            x0 = seeds[r]
+            jobs.append(_job(CPSO_SWARM_BAND, m, b, np_, lo, hi, seeded=int(x0 is not None), x0=x0, seed=sd,
+                             stream=b))
+        meta.append((b, m, lo, hi))
+    if not jobs:
+        return {}
+    r = ctx.cpu_pso(batch, np.array(jobs))
+    best = {}
+    pol, pol_meta = [], []
+    for k, (b, m, lo, hi) in enumerate(meta):
+        np_ = len(lo)
+        bc, bp = math.inf, []
+        for rr in range(3):
+            c = r["cost"][3 * k + rr]
+            if c < bc:
+                bc, bp = float(c), list(r["x"][3 * k + rr, :np_])
+        best[(b, m)] = (bp, bc)
+        if bp and m != int(GpuModelName.MetzgerKN):
+            pol.append(_job(CPSO_POLISH, m, b, np_, lo, hi, iters=50, x0=bp, start_cost=bc, stream=b))
+            pol_meta.append((b, m, np_))
+    if pol:
+        rp = ctx.cpu_pso(batch, np.array(pol))
+        for k, (b, m, np_) in enumerate(pol_meta):
+            best[(b, m)] = (list(rp["x"][k, :np_]), float(rp["cost"][k]))
+    return best
+
+
+def pso_model_select_batch(ctx: GpuContext, batch: GpuBatchData, datas: Sequence[BandFitData], bands,
+                           fit_all_models: bool = False) -> dict:
+    """pso_model_select (src/parametric.rs:2001-2066) per band: Bazin first,
+    the other seven models unless Bazin's cost is below 2 (or fit_all_models),
+    selection by BIC = 2 cost n + k ln n.  Returns {band: (model, params,
+    cost, per_model_chi2, per_model_params)}."""
+    bz = fit_single_models(ctx, batch, datas, [(b, 0) for b in bands])
+    need = [b for b in bands if fit_all_models or not (bz[(b, 0)][1] < BAZIN_GOOD_ENOUGH)]
+    rest = fit_single_models(ctx, batch, datas, [(b, int(m)) for b in need for m in _SELECT_REST])
+    out = {}
+    for b in bands:
+        n = max(datas[b].times.size, 1)
+        ln_n = math.log(n)
+        bic = lambda cost, m: 2.0 * cost * n + GpuModelName(m).n_params() * ln_n
+        p0, c0 = bz[(b, 0)]
+        chi2 = {GpuModelName.Bazin: finite_or_none(c0)}
+        params = {GpuModelName.Bazin: p0} if fit_all_models else {}
+        best = (GpuModelName.Bazin, p0, c0, bic(c0, 0))
+        if b in need:
+            for m in _SELECT_REST:
+                pm, cm = rest[(b, int(m))]
+                chi2[m] = finite_or_none(cm)
+                if fit_all_models:
+                    params[m] = pm
+                bm = bic(cm, int(m))
+                if bm < best[3]:
+                    best = (m, pm, cm, bm)
+        else:
+            for m in _SELECT_REST:
+                chi2.setdefault(m, None)
+        out[b] = (best[0], best[1], best[2], chi2, params)
+    return out
+
+
+def _linspace(a: float, b: float, n: int) -> list:
+    """linspace (src/parametric.rs:1759-1765)."""
+    if n <= 1:
+        return [a]
+    step = (b - a) / (n - 1)
+    return [a + i * step for i in range(n)]
+
+
+def profile_t0_search_batch(ctx: GpuContext, batch: GpuBatchData, datas: Sequence[BandFitData], fits: dict) -> dict:
+    """profile_t0_search (src/parametric.rs:1767-1849) for {band: (model,
+    params, cost)}: fixed-t0 swarms (30 x 60, stall 12, seed 42, no prior)
+    at 5 t0 within +-5 d of the pilot (not before t_first - 10), then 3
+    within +-0.5 d of the best; returns {band: (params, cost)}."""
+    def run(points):
+        jobs, meta = [], []
+        for b, t0 in points:
+            m, p, _ = fits[b]
+            lo, hi = pso_bounds_cpu(m)
+            t0i = _T0_IDX[m]
+            del lo[t0i], hi[t0i]
+            jobs.append(_job(CPSO_SWARM_BAND, m, b, len(lo), lo, hi, prior=0, fixed_t0=1, t0=t0, seed=42, stream=b))
+            meta.append((b, t0, t0i, len(lo)))
+        r = ctx.cpu_pso(batch, np.array(jobs)) if jobs else None
+        res = []
+        for k, (b, t0, t0i, d) in enumerate(meta):
+            red = list(r["x"][k, :d])
+            res.append((b, t0, red[:t0i] + [t0] + red[t0i:], float(r["cost"][k])))
+        return res
+    state, rng = {}, {}
+    coarse = []
+    for b, (m, p, c) in fits.items():
+        t0i = _T0_IDX[m]
+        pilot = p[t0i]
+        t_first = float(np.min(datas[b].times))
+        lo, hi = max(pilot - 5.0, t_first - 10.0), pilot + 5.0
+        rng[b] = (lo, hi)
+        state[b] = [list(p), c, pilot]
+        coarse += [(b, t) for t in _linspace(lo, hi, 5)]
+    for b, t0, full, cost in run(coarse):
+        if cost < state[b][1] and full:
+            state[b] = [full, cost, t0]
+    fine = []
+    for b, (p, c, bt) in state.items():
+        lo, hi = rng[b]
+        fine += [(b, t) for t in _linspace(max(bt - 0.5, lo), min(bt + 0.5, hi), 3)]
+    for b, t0, full, cost in run(fine):
+        if cost < state[b][1] and full:
+            state[b][0], state[b][1] = full, cost
+    return {b: (s[0], s[1]) for b, s in state.items()}
+
+
+def _posteriors(ctx, datas, models, params, method):
+    if method == UncertaintyMethod.Svi:
+        return _svi_on_device(ctx, datas, models, params, SVI_N_STEPS, SVI_N_SAMPLES, SVI_LR, True)
+    data = GpuBatchData(ctx, csr=_fit_csr(datas))
+    try:
+        r = ctx.laplace_fit(data, models, np.stack([_pad7(p) for p in params]))
+    finally:
+        data.close()
+    return [(list(params[k]), list(r["log_sigma"][k, :len(params[k])]), float(r["elbo"][k])) for k in range(len(datas))]
+
+
+def _multi_bazin_first(ctx, datas_first):
+    """MultiBazin on each source's first band (the device's greedy fit; the
+    reference's CPU fit_multi_bazin returns nothing below 5 points)."""
+    from .gpu import BatchSource
+
+    idx = [k for k, d in enumerate(datas_first) if d is not None and d.times.size >= 5]
+    out = [None] * len(datas_first)
+    if not idx:
+        return out
+    srcs = [BatchSource(datas_first[k].times, datas_first[k].flux, datas_first[k].obs_var,
+                        datas_first[k].is_upper.astype(bool), datas_first[k].upper_flux) for k in idx]
+    b = GpuBatchData(ctx, srcs)
+    try:
+        mb = ctx.batch_pso_multi_bazin(b, None, 30, 60, 12, 42)
+    finally:
+        b.close()
+    for k, r in zip(idx, mb):
+        out[k] = r if r.params else None
+    return out
+
+
+def fit_parametric_model_batch(ctx: GpuContext, sources: Sequence[Mapping[str, BandData]],
+                               fit_all_models: bool = False, method: UncertaintyMethod = UncertaintyMethod.Svi,
+                               forced_model: GpuModelName | None = None) -> list[list[ParametricBandResult]]:
+    """fit_parametric_model (src/parametric.rs:3563-3739) for a list of
+    sources: per band model selection (or the forced model), the t0 profile
+    search on each source's first band for SVI, the posterior, profile
+    refinement, magnitude chi2 and MultiBazin on the first band."""
+    prepared = [prepare_bands(fb) for fb in sources]
+    flat = [(s, i) for s, pb in enumerate(prepared) for i in range(len(pb))]
+    out: list[list[ParametricBandResult]] = [[] for _ in sources]
+    if not flat:
+        return out
+    datas = [prepared[s][i][1] for s, i in flat]
+    batch = GpuBatchData(ctx, csr=_fit_csr(datas))
+    try:
+        bands = list(range(len(flat)))
+        if forced_model is not None:
+            fm = int(forced_model)
+            fits = fit_single_models(ctx, batch, datas, [(b, fm) for b in bands])
+            sel = {b: (GpuModelName(fm), fits[(b, fm)][0], fits[(b, fm)][1],
+                       {GpuModelName(fm): finite_or_none(fits[(b, fm)][1])},
+                       {GpuModelName(fm): fits[(b, fm)][0]} if fit_all_models else {}) for b in bands}
+        else:
+            sel = pso_model_select_batch(ctx, batch, datas, bands, fit_all_models)
+        keep = [b for b in bands if len(sel[b][1]) > 0]
+        pso = {b: (list(sel[b][1]), sel[b][2]) for b in keep}
+        if method == UncertaintyMethod.Svi:
+            first = {b: (int(sel[b][0]), pso[b][0], pso[b][1]) for b in keep if flat[b][1] == 0}
+            pso.update(profile_t0_search_batch(ctx, batch, datas, first))
+    finally:
+        batch.close()
+    if not keep:
+        return out
+    models = [int(sel[b][0]) for b in keep]
+    post = _posteriors(ctx, [datas[b] for b in keep], models, [pso[b][0] for b in keep], method)
+    mbz = _multi_bazin_first(ctx, [datas[b] if flat[b][1] == 0 else None for b in keep])
+    jobs = []
+    for k, b in enumerate(keep):
+        s, i = flat[b]
+        name, data, mags = prepared[s][i]
+        g = GpuPsoBandResult(model=sel[b][0], pso_params=pso[b][0], pso_cost=pso[b][1], per_model_chi2=sel[b][3],
+                             per_model_params=sel[b][4])
+        jobs.append((name, data, mags, g, post[k], mbz[k]))
+    for b, r in zip(keep, _results(ctx, jobs, method)):
+        out[flat[b][0]].append(r)
+    return out
+
+
+def fit_parametric_multiband_batch(ctx: GpuContext, sources: Sequence[Mapping[str, BandData]],
+                                   method: UncertaintyMethod = UncertaintyMethod.Svi) -> list:
+    """fit_parametric_multiband (src/parametric.rs:3813-4131) for a list of
+    sources: the joint Villar swarm (reference band + offsets, 200 particles x
+    1500 iterations, stall 60, three restarts, the second seeded at the prior
+    means) over all bands, then per band the posterior, profile refinement
+    and magnitude chi2 (from the posterior mean), MultiBazin on the first
+    band.  A source yields [] when it has no usable band or more than 9."""
+    prepared = [prepare_bands(fb) for fb in sources]
+    out: list = [[] for _ in sources]
+    srcs = [s for s, pb in enumerate(prepared) if 0 < len(pb) <= 9]
+    if not srcs:
+        return out
+    datas, first = [], {}
+    for s in srcs:
+        first[s] = len(datas)
+        datas += [e[1] for e in prepared[s]]
+    batch = GpuBatchData(ctx, csr=_fit_csr(datas))
+    jobs, meta = [], []
+    pc, _ = population_priors(1)
+    for s in srcs:
+        B = len(prepared[s])
+        lo, hi = pso_bounds_cpu(1)
+        for _ in range(1, B):
+            lo += [-3.0 * sg for sg in _REL_SIG]
+            hi += [3.0 * sg for sg in _REL_SIG]
+        peak_t, t_first = _peak_and_first(prepared[s][0][1])
+        t0_lo, t0_hi = max(t_first - 30.0, lo[3]), min(peak_t + 10.0, hi[3])
+        if t0_lo < t0_hi:  # the reference narrows unconditionally and would panic on an empty box
+            lo[3], hi[3] = t0_lo, t0_hi
+        x0 = [_clamp(pc[j], lo[j], hi[j]) for j in range(7)]
+        x0[3] = _clamp(peak_t - 5.0, lo[3], hi[3])
+        for b in range(1, B):
+            x0 += [_clamp(_REL_MEAN[j], lo[7 * b + j], hi[7 * b + j]) for j in range(7)]
+        for r, sd in enumerate(_SEEDS):
+            jobs.append(_job(CPSO_SWARM_JOINT, 1, first[s], 7 * B, lo, hi, n_bands=B, particles=200, iters=1500,
+                             stall=60, seeded=int(r == 1), x0=x0, seed=sd, stream=first[s]))
+        meta.append((s, B))
+    try:
+        r = ctx.cpu_pso(batch, np.array(jobs))
+    finally:
+        batch.close()
+    band_jobs = []
+    for k, (s, B) in enumerate(meta):
+        bc, bp = math.inf, None
+        for rr in range(3):
+            c = r["cost"][3 * k + rr]
+            if c < bc:
+                bc, bp = float(c), r["x"][3 * k + rr, :7 * B]
+        if bp is None:
+            continue
+        ref = list(bp[:7])
+        for b in range(B):
+            params = ref if b == 0 else [ref[j] + bp[7 * b + j] for j in range(7)]
+            band_jobs.append((s, b, params, bc))
+    if not band_jobs:
+        return out
+    bdatas = [prepared[s][b][1] for s, b, _, _ in band_jobs]
+    post = _posteriors(ctx, bdatas, [1] * len(band_jobs), [p for _, _, p, _ in band_jobs], method)
+    mbz = _multi_bazin_first(ctx, [prepared[s][b][1] if b == 0 else None for s, b, _, _ in band_jobs])
+    fin = finalize_jobs(ctx, bdatas, [prepared[s][b][2] for s, b, _, _ in band_jobs], [1] * len(band_jobs),
+                        np.stack([_pad7(p) for _, _, p, _ in band_jobs]),
+                        np.stack([_pad7(q[0]) for q in post]), np.stack([_pad7(q[1]) for q in post]),
+                        flags=FINALIZE_DEFAULT | FINALIZE_SVI_CHI2)
+    for k, (s, b, params, bc) in enumerate(band_jobs):
+        name, data, _ = prepared[s][b]
+        out[s].append(ParametricBandResult(
+            band=name, model=GpuModelName.Villar, pso_params=list(params), pso_chi2=finite_or_none(bc),
+            svi_mu=[float(x) for x in fin["mu"][k, :7]], svi_log_sigma=[float(x) for x in fin["log_sigma"][k, :7]],
+            svi_elbo=finite_or_none(post[k][2]), n_obs=int(data.times.size),
+            mag_chi2=finite_or_none(fin["mag_chi2"][k]), per_model_chi2={GpuModelName.Villar: finite_or_none(bc)},
+            per_model_params={}, uncertainty_method=method, multi_bazin=mbz[k]))
+    return out
+
+
+def fit_parametric_batch(ctx: GpuContext, sources: Sequence[Mapping[str, BandData]], fit_all_models: bool = False,
+                         method: UncertaintyMethod = UncertaintyMethod.Svi) -> list[list[ParametricBandResult]]:
+    """fit_parametric (src/parametric.rs:3742-3761) for a list of sources:
+    the joint multi-band Villar fit for sources with at least two non-empty
+    bands (unless fit_all_models), the per-band path otherwise or when the
+    joint fit yields nothing."""
+    multi = [s for s, fb in enumerate(sources)
+             if not fit_all_models and sum(1 for b in fb.values() if len(b.values) > 0) >= 2]
+    out: list = [[] for _ in sources]
+    if multi:
+        mr = fit_parametric_multiband_batch(ctx, [sources[s] for s in multi], method)
+        for s, r in zip(multi, mr):
+            out[s] = r
+    rest = [s for s in range(len(sources)) if not out[s]]
+    if rest:
+        pr = fit_parametric_model_batch(ctx, [sources[s] for s in rest], fit_all_models, method)
+        for s, r in zip(rest, pr):
+            out[s] = r
+    return out

@@ -19,7 +19,8 @@
 //     (:3366-3393): SVI predictions unless fewer than n/2 are positive (then
 //     the PSO parameters), residual mag_obs - (-2.5 log10(pred * peak) + 23.9)
 //     over sigma_m = 1.0857 flux_err / flux, mean over the terms with positive
-//     predicted and observed flux.
+//     predicted and observed flux (ORC_FINALIZE_SVI_CHI2: from mu only, as
+//     fit_parametric_multiband does, :4069-4085).
 // Every sum runs over the observations in order, as the reference does.
 #include "lcf_oracle.h"
 #include "oracle_internal.hpp"
@@ -193,7 +194,7 @@ void profile_refine_all(int m, const double* mu, double* ls, const BandView& b) 
     }
 }
 
-double mag_chi2(int m, const double* mu, const double* pso, const BandView& b) {
+double mag_chi2(int m, const double* mu, const double* pso, const BandView& b, bool svi_only) {
     const int64_t n = b.n;
     std::vector<double> sv(std::max<int64_t>(n, 1)), ps(std::max<int64_t>(n, 1));
     eval_times(m, mu, b.t, n, sv.data());
@@ -201,7 +202,7 @@ double mag_chi2(int m, const double* mu, const double* pso, const BandView& b) {
     for (int64_t i = 0; i < n; i++)
         if (sv[i] * b.peak > 0.0) npos++;
     const double* mp = sv.data();
-    if (npos < n / 2) {
+    if (!svi_only && npos < n / 2) {
         eval_times(m, pso, b.t, n, ps.data());
         mp = ps.data();
     }
@@ -247,7 +248,8 @@ extern "C" int orc_finalize(int64_t n_bands, const int64_t* off, const double* t
         if (b.n == 0) return;
         if (flags & ORC_FINALIZE_T0) orc::profile_t0_refine(m, om, ol, b);
         if (flags & ORC_FINALIZE_ALL) orc::profile_refine_all(m, om, ol, b);
-        if (flags & ORC_FINALIZE_MAG_CHI2) out_mag_chi2[k] = orc::mag_chi2(m, om, pso + k * 7, b);
+        if (flags & ORC_FINALIZE_MAG_CHI2)
+            out_mag_chi2[k] = orc::mag_chi2(m, om, pso + k * 7, b, (flags & ORC_FINALIZE_SVI_CHI2) != 0);
     });
     return 0;
 }

@@ -248,6 +248,7 @@ int orc_laplace_fit(int model_id, const double* map_params, const double* t, con
 #define ORC_FINALIZE_T0 1
 #define ORC_FINALIZE_ALL 2
 #define ORC_FINALIZE_MAG_CHI2 4
+#define ORC_FINALIZE_SVI_CHI2 8  /* chi2 from mu only (no PSO fallback) */
 int orc_finalize(int64_t n_bands, const int64_t* off, const double* t, const double* f, const double* v,
                  const int32_t* up, const double* uf, const double* fe, const double* mag_obs,
                  const double* peak, const int32_t* model_ids, const double* pso, const double* mu,
@@ -274,6 +275,26 @@ int orc_batch_gp2d_train(int64_t n_src, const int64_t* off, const double* t, con
                          const double* nv, const double* theta0, int max_iters, double gtol, double* theta,
                          double* nlml, double* grad, int32_t* iters, int32_t* status, int32_t* evals,
                          int n_threads);
+
+/* ---- CPU-path swarm fits (src/parametric.rs:1160-1849, 3813-4131) ---------
+ * One job = one pso_minimize[_seeded] run (kind 0: PsoCost of band `band`,
+ * model `model`, priors when use_prior, t0 held at t0_value when fixed_t0;
+ * kind 1: the joint Villar cost over bands band .. band + n_bands - 1) or one
+ * lbfgs_refine from x0 (kind 2, start_cost = cost at x0).  lo / hi / x0 hold
+ * `dim` entries.  Outputs: x[n][64], cost[n], iterations[n]. */
+#define ORC_CPSO_SWARM_BAND 0
+#define ORC_CPSO_SWARM_JOINT 1
+#define ORC_CPSO_POLISH 2
+typedef struct orc_pso_job {
+    int32_t kind, model, band, n_bands, dim, n_particles, max_iters, stall_iters, use_prior, fixed_t0, seeded;
+    uint32_t stream;
+    uint64_t seed;
+    double t0_value, start_cost;
+    double lo[64], hi[64], x0[64];
+} orc_pso_job;
+int orc_cpu_pso(const int64_t* off, const double* t, const double* flux, const double* obs_var,
+                const int32_t* is_upper, const double* upper_flux, int64_t n_obs, const orc_pso_job* jobs,
+                int64_t n_jobs, double* out_x, double* out_cost, int32_t* out_iters, int n_threads);
 
 #ifdef __cplusplus
 }

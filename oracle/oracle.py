@@ -558,7 +558,7 @@ def fit_bb_temperature(lam, mag, err):
     return None if rc else tuple(out)
 
 
-FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2 = 1, 2, 4
+FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2, FINALIZE_SVI_CHI2 = 1, 2, 4, 8
 FINALIZE_DEFAULT = FINALIZE_T0 | FINALIZE_ALL | FINALIZE_MAG_CHI2
 
 
@@ -647,3 +647,31 @@ def batch_fit_gp_2d(csr: dict, qt, qw, n_threads: int | None = None) -> dict:
                                nq, d(hyp), d(rms), _p(ntr, C.c_int32), d(pred), d(sd), n_threads or default_threads())
     assert rc == 0
     return {"hyp": hyp, "train_rms": rms, "n_train": ntr, "pred": pred, "std": sd}
+
+
+CPSO_SWARM_BAND, CPSO_SWARM_JOINT, CPSO_POLISH = 0, 1, 2
+PSO_JOB_DTYPE = np.dtype([("kind", "<i4"), ("model", "<i4"), ("band", "<i4"), ("n_bands", "<i4"), ("dim", "<i4"),
+                          ("n_particles", "<i4"), ("max_iters", "<i4"), ("stall_iters", "<i4"),
+                          ("use_prior", "<i4"), ("fixed_t0", "<i4"), ("seeded", "<i4"), ("stream", "<u4"),
+                          ("seed", "<u8"), ("t0_value", "<f8"), ("start_cost", "<f8"), ("lo", "<f8", 64),
+                          ("hi", "<f8", 64), ("x0", "<f8", 64)], align=True)
+
+
+def cpu_pso(csr: dict, jobs, n_threads: int | None = None) -> dict:
+    """Swarm / polish jobs (PSO_JOB_DTYPE records) over the BandFitData CSR
+    (offsets, times, flux, obs_var, is_upper, upper_flux)."""
+    off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
+    t, f, v, uf = (f64(csr[k]) for k in ("times", "flux", "obs_var", "upper_flux"))
+    up = i32(csr["is_upper"])
+    jobs = np.ascontiguousarray(jobs, dtype=PSO_JOB_DTYPE)
+    nj = jobs.size
+    x, cost, it = np.zeros((nj, 64)), np.zeros(nj), np.zeros(nj, np.int32)
+    L = lib()
+    L.orc_cpu_pso.restype = _i32
+    L.orc_cpu_pso.argtypes = [_pi64, _pd, _pd, _pd, _pi32, _pd, _i64, C.c_void_p, _i64, _pd, _pd, _pi32, _i32]
+    d = lambda a: _p(a, C.c_double)
+    rc = L.orc_cpu_pso(_p(off, C.c_int64), d(t), d(f), d(v), _p(up, C.c_int32), d(uf), t.size,
+                       jobs.ctypes.data_as(C.c_void_p), nj, d(x), d(cost), _p(it, C.c_int32),
+                       n_threads or default_threads())
+    assert rc == 0
+    return {"x": x, "cost": cost, "iters": it}
