@@ -4,7 +4,8 @@ lcf_batch_finalize:
 
   * all eight models (MetzgerKN in its 200-point batch form), ragged bands of
     1-150 observations with upper limits, negative fluxes, an empty band;
-  * each step alone (flag bits) and all three together;
+  * each step alone (flag bits), all three together, and with the magnitude
+    chi2 taken from the posterior mean only (the multi-band route);
   * refined t0 identical (grid values), log sigma and chi2 within 1e-9
     relative: every likelihood is one lane's sum over the observations in
     the reference's order, so only the device exponential / logarithm
@@ -81,7 +82,7 @@ def jobs(orc):
     return random_jobs(orc, 6, seed=11)
 
 
-@pytest.mark.parametrize("flags", [1, 2, 4, 7])
+@pytest.mark.parametrize("flags", [1, 2, 4, 7, 15])
 def test_matches_oracle(pkg, orc, ctx, jobs, flags):
     csr, fe, mags, peaks, models, pso, mu, ls = jobs
     data = pkg.GpuBatchData(ctx, csr=csr)
@@ -115,7 +116,7 @@ def test_invalid_arguments(pkg, ctx, jobs):
         with pytest.raises(pkg.GpuError):
             ctx.finalize_arrays(data, bad, pso, mu, ls, csr["flux"], fe, mags, peaks)
         with pytest.raises(pkg.GpuError):
-            ctx.finalize_arrays(data, models, pso, mu, ls, csr["flux"], fe, mags, peaks, flags=8)
+            ctx.finalize_arrays(data, models, pso, mu, ls, csr["flux"], fe, mags, peaks, flags=16)
         with pytest.raises(pkg.GpuError):
             ctx.finalize_arrays(data, models, pso, mu, ls, csr["flux"][:-1], fe, mags, peaks)
     finally:

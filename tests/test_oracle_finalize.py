@@ -176,6 +176,11 @@ def test_mag_chi2_closed_form(orc):
     bad[1] = -5.0
     r2 = fin(orc, csr, e, 0, p, bad, np.full(6, -1.0), orc.FINALIZE_MAG_CHI2, peak=peak, mag=mag)
     assert r2["mag_chi2"][0] == r["mag_chi2"][0]
+    # the multi-band route never falls back to the PSO curve: with negative
+    # predictions everywhere no term qualifies
+    r4 = fin(orc, csr, e, 0, p, bad, np.full(6, -1.0), orc.FINALIZE_MAG_CHI2 | orc.FINALIZE_SVI_CHI2, peak=peak,
+             mag=mag)
+    assert math.isnan(r4["mag_chi2"][0])
     # no positive flux at all: NaN
     csr["flux"] = -np.abs(f) - 0.1
     r3 = fin(orc, csr, e, 0, p, p, np.full(6, -1.0), orc.FINALIZE_MAG_CHI2, peak=peak, mag=mag)
