@@ -9,6 +9,7 @@ const void* pso_kernel_f32(int model, int ppl, bool persistent);
 const void* pso_kernel_f64(int model, int ppl, bool persistent);
 const void* mb_kernel_f32(int ppl);
 const void* mb_kernel_f64(int ppl);
+hipError_t pso_f32_prepare(hipStream_t st);
 
 static const void* pick_kernel(int model, bool fp64, int ppl, bool persistent) {
     return fp64 ? pso_kernel_f64(model, ppl, persistent) : pso_kernel_f32(model, ppl, persistent);
@@ -41,6 +42,10 @@ hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool pers
                       hipStream_t st) {
     const void* k = pick_kernel(model, fp64, ppl, persistent);
     if (!k || grid <= 0) return hipErrorInvalidValue;
+    if (!fp64) {
+        hipError_t e = pso_f32_prepare(st);
+        if (e != hipSuccess) return e;
+    }
     void* args[] = {const_cast<PsoArgs*>(&a)};
     return hipLaunchKernel(k, dim3(grid), dim3(64), args, pso_lds_bytes(fp64, a.tile), st);
 }

@@ -21,6 +21,7 @@
 #include "pso_kernels.hpp"
 #include "model_tables.hpp"
 #include "models_fit.hpp"
+#include "pso_fast32.hpp"
 
 namespace lcf {
 
@@ -42,11 +43,81 @@ __device__ __forceinline__ void stage_obs(typename ObsT<R>::T* lds,
 template <int MODEL, typename R, int PPL>
 struct Swarm {
     static constexpr int NP = ModelTraits<MODEL>::NP;
+    // FP32 swarms of models 0-6 use the detections-first LDS tile and the
+    // instruction-lean model forms of pso_fast32.hpp.
+    static constexpr bool FAST = sizeof(R) == 4 && MODEL <= 6;
     using M = Math<R>;
     using O = typename ObsT<R>::T;
 
     R pos[PPL][NP], vel[PPL][NP], pb[PPL][NP], pcost[PPL];
     R gb[NP], gcost;
+    R ipw[NP];  // FAST: 1 / population-prior width (0 where the width is 0)
+
+    // FAST cost: the tile holds {t log2 e, y, var, flag} with the `nd`
+    // detections first (staged once by the caller when the band fits one
+    // tile, else re-staged here per tile).  Four detections per Kahan update,
+    // then the upper limits, four per update.  A non-finite prediction reaches the
+    // sum as +inf or NaN and maps to the invalid sentinel below, as the
+    // per-observation flag of the general path does.
+    __device__ __forceinline__ void costs_fast(R (&cost)[PPL], const O* lds, const O* __restrict__ g,
+                                               int n, int tile, int nd_staged) {
+        f32::Fast<MODEL> mdl[PPL];
+        float se2[PPL];
+        Kahan<float> acc[PPL];
+#pragma unroll
+        for (int k = 0; k < PPL; k++) {
+            mdl[k].setup(pos[k]);
+            se2[k] = f32::ex2((2.0f * f32::L2E) * pos[k][NP - 1]);
+            acc[k].init();
+        }
+        for (int base = 0; base < n; base += tile) {
+            const int cnt = n - base < tile ? n - base : tile;
+            const int nd = n > tile ? f32::stage_split(const_cast<O*>(lds), g + base, cnt) : nd_staged;
+            int i = 0;
+            for (; i + 4 <= nd; i += 4) {
+                const O r0 = lds[i], r1 = lds[i + 1], r2 = lds[i + 2], r3 = lds[i + 3];
+#pragma unroll
+                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_quad(mdl[k], se2[k], r0, r1, r2, r3));
+            }
+            if (i + 2 <= nd) {
+                const O r0 = lds[i], r1 = lds[i + 1];
+#pragma unroll
+                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_pair(mdl[k], se2[k], r0, r1));
+                i += 2;
+            }
+            if (i < nd) {
+                const O r0 = lds[i];
+#pragma unroll
+                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_one(mdl[k], se2[k], r0));
+                i++;
+            }
+            for (; i + 4 <= cnt; i += 4) {
+                const O r0 = lds[i], r1 = lds[i + 1], r2 = lds[i + 2], r3 = lds[i + 3];
+#pragma unroll
+                for (int k = 0; k < PPL; k++)
+                    acc[k].add(f32::cost_upper_quad(mdl[k], se2[k], r0, r1, r2, r3));
+            }
+            for (; i < cnt; i++) {
+                const O r0 = lds[i];
+#pragma unroll
+                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_upper(mdl[k], se2[k], r0));
+            }
+        }
+        const float inv_n = 1.0f / (float)(n > 0 ? n : 1);
+#pragma unroll
+        for (int k = 0; k < PPL; k++) {
+            float pri = 0.0f;
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                float z = (pos[k][j] - (float)c_pc[MODEL][j]) * ipw[j];
+                pri = __builtin_fmaf(0.5f * z, z, pri);
+            }
+            float c = n > 0 ? acc[k].s * inv_n : M::big();
+            c = __builtin_fmaf(pri * inv_n, inv_n, c);
+            if (!(c < M::big())) c = M::big();
+            cost[k] = c;
+        }
+    }
 
     // NLL/n + prior/n^2 for this lane's particles (cuda/models.cu:540-583 and
     // src/gpu_cuda.rs:461-480).  FP64 follows the reference's arithmetic
@@ -55,7 +126,11 @@ struct Swarm {
     // cost never becomes a personal best, 1e99 does).  FP32 uses a Kahan sum
     // and reciprocals, and folds every overflow into its 1e30 sentinel.
     __device__ __forceinline__ void costs(R (&cost)[PPL], const O* lds, const O* __restrict__ g,
-                                          int n, int tile, int P) {
+                                          int n, int tile, int P, int nd_staged) {
+        if constexpr (FAST) {
+            costs_fast(cost, lds, g, n, tile, nd_staged);
+            return;
+        }
         constexpr bool F64 = sizeof(R) == 8;
         Model<MODEL, R> mdl[PPL];
         R se2[PPL];
@@ -140,8 +215,15 @@ struct Swarm {
                                         uint32_t band, int P, int max_iters, int stall_iters,
                                         uint64_t seed, uint32_t flags, int& iters_out,
                                         const R (&lo)[NP], const R (&hi)[NP], const R* seedp = nullptr,
-                                        uint32_t rng_tag = 0u) {
+                                        uint32_t rng_tag = 0u, int nd_staged = 0) {
         const int lane = lane_id();
+        if constexpr (FAST) {
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                const double w = c_pw[MODEL][j];
+                ipw[j] = w > 0.0 ? (R)(1.0 / w) : R(0);
+            }
+        }
         const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
         constexpr int NPAIR = (NP + 1) / 2;
         R vmax[NP];
@@ -185,7 +267,7 @@ struct Swarm {
         for (; it < max_iters; it++) {
             R w = w_max - (w_max - w_min) * R(it) * inv_max;
             R cost[PPL];
-            costs(cost, lds, g, n, tile, P);
+            costs(cost, lds, g, n, tile, P, nd_staged);
             // personal bests; lowest cost this iteration (ties: lowest index)
             R bc = R(INFINITY);
             int bi = 0x7fffffff;
@@ -263,7 +345,12 @@ __device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
     }
     const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[band];
     const int n = (int)(a.offsets[band + 1] - a.offsets[band]);
-    if (n <= a.tile) stage_obs<R>(lds, g, n);
+    int nd = 0;
+    if constexpr (Swarm<MODEL, R, PPL>::FAST) {
+        if (n <= a.tile) nd = f32::stage_split(reinterpret_cast<float4*>(lds), reinterpret_cast<const float4*>(g), n);
+    } else {
+        if (n <= a.tile) stage_obs<R>(lds, g, n);
+    }
     constexpr int NP = Swarm<MODEL, R, PPL>::NP;
     R lo[NP], hi[NP];
 #pragma unroll
@@ -274,7 +361,7 @@ __device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
     Swarm<MODEL, R, PPL> s;
     int iters = 0;
     s.run(lds, g, n, a.tile, (uint32_t)band, a.n_particles, a.max_iters, a.stall_iters, a.seed,
-          a.flags, iters, lo, hi);
+          a.flags, iters, lo, hi, nullptr, 0u, nd);
     double cst = (double)s.gcost;
     if (cst >= (double)Math<R>::big()) cst = 1e99;
     double prm = 0.0;  // lane d holds parameter d
