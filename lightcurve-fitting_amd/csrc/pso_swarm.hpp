@@ -14,9 +14,11 @@
 // cost of batch_pso_cost (cuda/models.cu:516-584) plus the host-side prior
 // term (src/gpu_cuda.rs:461-480), and the model-selection loop of
 // batch_model_select / batch_all_models (src/gpu_cuda.rs:573-637).  Random
-// numbers come from Philox4x32-10 keyed by (seed, band index) instead of the
-// reference's sequential SmallRng, which makes every band's result
-// independent of batch composition and launch order.
+// numbers come from Philox4x32-10 keyed by the seed with the band index in
+// the counter (initial swarm, and the seed of each particle's xoshiro128**
+// velocity-update stream) instead of the reference's sequential SmallRng,
+// which makes every band's result independent of batch composition and
+// launch order.
 #pragma once
 #include "pso_kernels.hpp"
 #include "model_tables.hpp"
@@ -209,8 +211,10 @@ struct Swarm {
 
     // Box bounds lo/hi; with `seedp` particle 0 starts there with velocity
     // (hi - lo) 0.02 (2u - 1) (batch_pso_multi_bazin, src/gpu_cuda.rs:767-777).
-    // Random numbers: Philox block (band, particle, iteration or 0xFFFFFFFF,
-    // rng_tag + dimension pair).
+    // Random numbers: the initial position and velocity from the Philox block
+    // (band, particle, 0xFFFFFFFF, rng_tag + dimension pair); the velocity
+    // updates from the particle's xoshiro128** stream, seeded by the block
+    // (band, particle, 0xFFFFFFFE, rng_tag), r1 and r2 per dimension in order.
     __device__ __forceinline__ void run(const O* lds, const O* __restrict__ g, int n, int tile,
                                         uint32_t band, int P, int max_iters, int stall_iters,
                                         uint64_t seed, uint32_t flags, int& iters_out,
@@ -226,6 +230,7 @@ struct Swarm {
         }
         const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
         constexpr int NPAIR = (NP + 1) / 2;
+        Xoshiro128 rng[PPL];
         R vmax[NP];
 #pragma unroll
         for (int d = 0; d < NP; d++) vmax[d] = R(0.5) * (hi[d] - lo[d]);
@@ -251,6 +256,7 @@ struct Swarm {
                     }
                 }
             }
+            rng[k].seed(philox4x32_10(band, p, 0xFFFFFFFEu, rng_tag, k0, k1));
 #pragma unroll
             for (int d = 0; d < NP; d++) pb[k][d] = R(0);
             pcost[k] = R(INFINITY);
@@ -299,24 +305,17 @@ struct Swarm {
             // velocity / position update with clamping and wall absorption
 #pragma unroll
             for (int k = 0; k < PPL; k++) {
-                uint32_t p = (uint32_t)(lane + WAVE * k);
 #pragma unroll
-                for (int q = 0; q < NPAIR; q++) {
-                    U4 r = philox4x32_10(band, p, (uint32_t)it, rng_tag + (uint32_t)q, k0, k1);
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        int d = 2 * q + h;
-                        if (d < NP) {
-                            R r1 = u24<R>(h ? r.z : r.x), r2 = u24<R>(h ? r.w : r.y);
-                            R x = pos[k][d];
-                            R v = w * vel[k][d] + c1 * r1 * (pb[k][d] - x) + c2 * r2 * (gb[d] - x);
-                            v = v < -vmax[d] ? -vmax[d] : (v > vmax[d] ? vmax[d] : v);
-                            R nx = x + v;
-                            if (nx <= lo[d]) { pos[k][d] = lo[d]; vel[k][d] = R(0); }
-                            else if (nx >= hi[d]) { pos[k][d] = hi[d]; vel[k][d] = R(0); }
-                            else { pos[k][d] = nx; vel[k][d] = v; }
-                        }
-                    }
+                for (int d = 0; d < NP; d++) {
+                    R r1 = u24<R>(rng[k].next());
+                    R r2 = u24<R>(rng[k].next());
+                    R x = pos[k][d];
+                    R v = w * vel[k][d] + c1 * r1 * (pb[k][d] - x) + c2 * r2 * (gb[d] - x);
+                    v = v < -vmax[d] ? -vmax[d] : (v > vmax[d] ? vmax[d] : v);
+                    R nx = x + v;
+                    if (nx <= lo[d]) { pos[k][d] = lo[d]; vel[k][d] = R(0); }
+                    else if (nx >= hi[d]) { pos[k][d] = hi[d]; vel[k][d] = R(0); }
+                    else { pos[k][d] = nx; vel[k][d] = v; }
                 }
             }
             // stall rule (src/gpu_cuda.rs:537-552); prev starts at +inf as

@@ -17,9 +17,12 @@
 //     clamped), with velocity (hi - lo) 0.02 (2u - 1);
 //   * BIC = 2 cost n + (4K + 2) ln n, best K by strict '<', stop once a K's BIC
 //     exceeds the previous K's by more than 2.
-// Random numbers: Philox4x32-10 keyed by the seed, counter (band, particle,
-// iteration or 0xFFFFFFFF, 0x4D420000 + (K << 8) + dimension pair), in place of
-// the reference's SmallRng streams (not reproducible here; parity unpinned).
+// Random numbers: initial positions and velocities from Philox4x32-10 keyed by
+// the seed, counter (band, particle, 0xFFFFFFFF, 0x4D420000 + (K << 8) +
+// dimension pair); the velocity updates from a per-particle xoshiro128**
+// stream seeded by the block (band, particle, 0xFFFFFFFE, 0x4D420000 +
+// (K << 8)), r1 and r2 drawn per dimension in order.  They stand in for the
+// reference's SmallRng streams (not reproducible here; parity unpinned).
 #include "lcf_oracle.h"
 #include "oracle_internal.hpp"
 
@@ -146,6 +149,12 @@ static MbOut mb_source(int64_t s, const double* t, const double* f, const double
                     }
                 }
             }
+        std::vector<orc::Xoshiro128ss> rng(P);
+        for (int p = 0; p < P; p++) {
+            uint32_t ctr[4] = {(uint32_t)s, (uint32_t)p, 0xFFFFFFFEu, tag}, r[4];
+            orc_philox4x32_10(ctr, key, r);
+            rng[p].seed(r);
+        }
         const double w_max = 0.9, w_min = 0.4, c1 = 1.5, c2 = 1.5;
         const double inv_max = 1.0 / (double)max_iters;
         double prev_g = INFINITY;
@@ -163,13 +172,10 @@ static MbOut mb_source(int64_t s, const double* t, const double* f, const double
                     }
                 }
             for (int p = 0; p < P; p++)
-                for (int q = 0; q < npair; q++) {
-                    uint32_t ctr[4] = {(uint32_t)s, (uint32_t)p, (uint32_t)it, tag + (uint32_t)q}, r[4];
-                    orc_philox4x32_10(ctr, key, r);
-                    for (int h = 0; h < 2; h++) {
-                        int d = 2 * q + h;
-                        if (d >= np) break;
-                        double r1 = u24mb(r[2 * h]), r2 = u24mb(r[2 * h + 1]);
+                for (int d = 0; d < np; d++) {
+                    {
+                        double r1 = u24mb(rng[p].next());
+                        double r2 = u24mb(rng[p].next());
                         double x = pos[p * np + d];
                         double vv = w * vel[p * np + d] + c1 * r1 * (pb[p * np + d] - x) + c2 * r2 * (gb[d] - x);
                         vv = std::min(std::max(vv, -vmax[d]), vmax[d]);

@@ -240,6 +240,15 @@ static SwarmOut pso_one_source(int m, int64_t s, const double* t, const double* 
             }
         }
 
+    // velocity-update stream of each particle: xoshiro128** seeded from the
+    // Philox block at iteration slot 0xFFFFFFFE
+    std::vector<Xoshiro128ss> rng(P);
+    for (int p = 0; p < P; p++) {
+        uint32_t ctr[4] = {(uint32_t)s, (uint32_t)p, 0xFFFFFFFEu, 0u}, r[4];
+        philox(ctr, key, r);
+        rng[p].seed(r);
+    }
+
     const double w_max = 0.9, w_min = 0.4, c1 = 1.5, c2 = 1.5;
     const double inv_max = 1.0 / (double)max_iters;
     double prev = INFINITY;
@@ -265,13 +274,10 @@ static SwarmOut pso_one_source(int m, int64_t s, const double* t, const double* 
         first_eval = false;
         // velocity / position update (the swarm is not yet done here)
         for (int p = 0; p < P; p++)
-            for (int k = 0; k < npair; k++) {
-                uint32_t ctr[4] = {(uint32_t)s, (uint32_t)p, (uint32_t)it, (uint32_t)k}, r[4];
-                philox(ctr, key, r);
-                for (int h = 0; h < 2; h++) {
-                    int d = 2 * k + h;
-                    if (d >= np) break;
-                    double r1 = u24(r[2 * h]), r2 = u24(r[2 * h + 1]);
+            for (int d = 0; d < np; d++) {
+                {
+                    double r1 = u24(rng[p].next());
+                    double r2 = u24(rng[p].next());
                     double x = pos[p * np + d];
                     double vv = w * vel[p * np + d] + c1 * r1 * (pbest[p * np + d] - x) + c2 * r2 * (gbest[d] - x);
                     vv = std::min(std::max(vv, -vmax[d]), vmax[d]);
