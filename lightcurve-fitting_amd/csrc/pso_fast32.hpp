@@ -74,13 +74,17 @@ template <> struct Fast<0> {
         p_inf = -(OVF2 - __builtin_fmaxf(la, 0.0f)) * ex(p[4]);
         p_b = -OVF2 * ex(p[3]);
     }
+    __device__ __forceinline__ bool safe(float tmin, float) const {
+        return tmin - t0 >= __builtin_fmaxf(p_b, p_inf);
+    }
+    template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
         float e1 = __builtin_fmaf(-ph, itf, la);
         float arg = e1 - softplus2(-ph * itr);
-        arg = ph < p_b ? -INFINITY : arg;
+        if (CHECK) arg = ph < p_b ? -INFINITY : arg;
         float f = ex2(arg) + b;
-        return ph < p_inf ? INFINITY : f;
+        return CHECK && ph < p_inf ? INFINITY : f;
     }
 };
 
@@ -100,6 +104,8 @@ template <> struct Fast<1> {
         omb = 1.0f - beta * gamma;
         p_inf = gl - OVF2 * ex(p[5]);
     }
+    __device__ __forceinline__ bool safe(float tmin, float) const { return tmin - t0 >= p_inf; }
+    template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
         float sr = rcp(1.0f + ex2(-ph * itr));
@@ -111,7 +117,7 @@ template <> struct Fast<1> {
         float wexp = qz * ex2(y + __builtin_fminf(z, 0.0f));
         float inner = __builtin_fmaf(omw, __builtin_fmaf(-bn, ph, 1.0f), omb * wexp);
         float f = a * sr * inner;
-        return ph < p_inf ? INFINITY : f;
+        return CHECK && ph < p_inf ? INFINITY : f;
     }
 };
 
@@ -126,12 +132,14 @@ template <> struct Fast<2> {
         wa = itf * LN2; wb = 1.0f + 1e-6f * itf;
         al = p[5];
     }
+    __device__ __forceinline__ bool safe(float, float tmax) const { return tmax - t0 <= OVF2; }
+    template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
         float sr = rcp(1.0f + ex2(-ph * itr));
         float w = __builtin_fmaf(softplus2(ph), wa, wb);
         float f = __builtin_fmaf(ex2(__builtin_fmaf(-al, lg2(w), la)), sr, b);
-        return ph > OVF2 ? b : f;
+        return CHECK && ph > OVF2 ? b : f;
     }
 };
 
@@ -146,6 +154,8 @@ template <> struct Fast<3> {
         omfr = 1.0f - fr;
         xa = LN2 * itm * SQL2E; xb = 1e-6f * itm * SQL2E;
     }
+    __device__ __forceinline__ bool safe(float, float tmax) const { return tmax - t0 <= OVF2; }
+    template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
         float sp = softplus2(ph);  // ps = sp ln2 + 1e-6
@@ -154,7 +164,7 @@ template <> struct Fast<3> {
         float heat = __builtin_fmaf(fr, h1, omfr * h2);
         float x = __builtin_fmaf(sp, xa, xb);  // ps / tm sqrt(log2 e)
         float f = a * heat * (1.0f - ex2(-x * x));
-        return ph > OVF2 ? 0.0f : f;
+        return CHECK && ph > OVF2 ? 0.0f : f;
     }
 };
 
@@ -167,13 +177,15 @@ template <> struct Fast<4> {
         wa = LN2 * itsd; wb = 1.0f + 1e-6f * itsd;
         xa = LN2 * itd * SQL2E; xb = 1e-6f * itd * SQL2E;
     }
+    __device__ __forceinline__ bool safe(float, float tmax) const { return tmax - t0 <= OVF2; }
+    template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
         float sp = softplus2(ph);
         float w = __builtin_fmaf(sp, wa, wb);
         float x = __builtin_fmaf(sp, xa, xb);
         float f = a * rcp(w * w) * (1.0f - ex2(-x * x));
-        return ph > OVF2 ? 0.0f : f;
+        return CHECK && ph > OVF2 ? 0.0f : f;
     }
 };
 
@@ -184,13 +196,15 @@ template <> struct Fast<5> {
         la = p[0] * L2E; t0 = p[1] * L2E; n = p[2];
         rl = ex(-p[3]) * SQL2E;
     }
+    __device__ __forceinline__ bool safe(float, float tmax) const { return tmax - t0 <= OVF2; }
+    template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
         float s5 = rcp(1.0f + ex2(-5.0f * ph));
         float ps = __builtin_fmaf(softplus2_acc(ph), LN2, 1e-6f);
         float r = ps * rl;
         float f = s5 * ex2(__builtin_fmaf(-r, r, __builtin_fmaf(-n, lg2(ps), la)));
-        return ph > OVF2 ? 0.0f : f;
+        return CHECK && ph > OVF2 ? 0.0f : f;
     }
 };
 
@@ -205,6 +219,8 @@ template <> struct Fast<6> {
         a1 = 2.0f * p[3]; a2 = 2.0f * p[4];
         f_ovf = (p[3] == 0.0f || p[4] == 0.0f || (p[3] < 0.0f && p[4] < 0.0f)) ? INFINITY : 0.0f;
     }
+    __device__ __forceinline__ bool safe(float, float) const { return false; }
+    template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
         float ps = __builtin_fmaf(softplus2_acc(ph), LN2, 1e-6f);
@@ -219,13 +235,15 @@ template <> struct Fast<6> {
 };
 
 // Gaussian terms of 4, 2 and 1 detections for one particle: records are
-// {t log2 e, y, var, 0}; se2 = sigma_x^2.
-template <int MODEL>
+// {t log2 e, y, var, 0}; se2 = sigma_x^2.  CHECK = false drops the phase
+// tests of pred() (the caller has shown that no observation of the tile
+// reaches them).
+template <bool CHECK, int MODEL>
 __device__ __forceinline__ float cost_quad(const Fast<MODEL>& m, float se2, const float4& r0,
                                            const float4& r1, const float4& r2, const float4& r3) {
     float v0 = r0.z + se2, v1 = r1.z + se2, v2 = r2.z + se2, v3 = r3.z + se2;
-    float d0 = m.pred(r0.x) - r0.y, d1 = m.pred(r1.x) - r1.y;
-    float d2 = m.pred(r2.x) - r2.y, d3 = m.pred(r3.x) - r3.y;
+    float d0 = m.template pred<CHECK>(r0.x) - r0.y, d1 = m.template pred<CHECK>(r1.x) - r1.y;
+    float d2 = m.template pred<CHECK>(r2.x) - r2.y, d3 = m.template pred<CHECK>(r3.x) - r3.y;
     float q = d0 * (d0 * rcp(v0));
     q = __builtin_fmaf(d1 * rcp(v1), d1, q);
     q = __builtin_fmaf(d2 * rcp(v2), d2, q);
@@ -233,20 +251,21 @@ __device__ __forceinline__ float cost_quad(const Fast<MODEL>& m, float se2, cons
     float l = lg2(v0 * v1) + lg2(v2 * v3);
     return __builtin_fmaf(l, LN2, q);
 }
-template <int MODEL>
+template <bool CHECK, int MODEL>
 __device__ __forceinline__ float cost_pair(const Fast<MODEL>& m, float se2, const float4& r0,
                                            const float4& r1) {
     float v0 = r0.z + se2, v1 = r1.z + se2;
-    float d0 = m.pred(r0.x) - r0.y, d1 = m.pred(r1.x) - r1.y;
+    float d0 = m.template pred<CHECK>(r0.x) - r0.y, d1 = m.template pred<CHECK>(r1.x) - r1.y;
     float q = __builtin_fmaf(d1 * rcp(v1), d1, d0 * (d0 * rcp(v0)));
     return __builtin_fmaf(lg2(v0 * v1), LN2, q);
 }
-template <int MODEL>
+template <bool CHECK, int MODEL>
 __device__ __forceinline__ float cost_one(const Fast<MODEL>& m, float se2, const float4& r) {
     float v = r.z + se2;
-    float d = m.pred(r.x) - r.y;
+    float d = m.template pred<CHECK>(r.x) - r.y;
     return __builtin_fmaf(lg2(v), LN2, d * (d * rcp(v)));
 }
+
 // -ln Phi(x) of the reference's log_normal_cdf (A&S 7.1.26 erfc rational,
 // src/parametric.rs:882-901) tabulated on [-30, 8] in LNPHI_N intervals of
 // width 1/8: entry i holds {c0, c1, c2, c3} of the cubic in u = 8 (x - x_i)
@@ -258,50 +277,80 @@ constexpr float LNPHI_SCALE = 8.0f;
 static __device__ float4 g_lnphi[LNPHI_N];
 
 // Censored term -ln Phi(x), x = (y_up - f) / sqrt(v), of an upper limit: the
-// table on [-30, 8], the reference's exact 0 above x = 8 and its asymptote
-// x^2/2 + ln sqrt(2 pi) + ln(-x) below -30.  The table index is clamped
-// before the load, so any x (NaN, +-inf) reads inside the table; a NaN x
-// stays NaN and a +inf prediction (x = -inf) gives +inf.
-template <int MODEL>
+// table on [-30, 8] and the reference's asymptote x^2/2 + ln sqrt(2 pi) +
+// ln(-x) below -30.  The table index is clamped before the load, so any x
+// (NaN, +-inf) reads inside the table; a +inf prediction (x = -inf) gives
+// +inf.
+template <bool CHECK, int MODEL>
 __device__ __forceinline__ float upper_x(const Fast<MODEL>& m, float se2, const float4& r) {
-    return (r.y - m.pred(r.x)) * __builtin_amdgcn_rsqf(r.z + se2);
+    return (r.y - m.template pred<CHECK>(r.x)) * __builtin_amdgcn_rsqf(r.z + se2);
+}
+// Table part: interval index clamped to the table (v_med3), position in it
+// by v_fract; beyond x = 8 the last interval's cubic, < 1e-15 on all of
+// [0, 1], stands for the reference's exact 0.
+__device__ __forceinline__ float neg_ln_phi_tab(float x) {
+    float ug = __builtin_fmaf(x, LNPHI_SCALE, -LNPHI_X0 * LNPHI_SCALE);
+    unsigned i = (unsigned)__builtin_amdgcn_fmed3f(ug, 0.0f, (float)(LNPHI_N - 1));
+    float u = __builtin_amdgcn_fractf(ug);
+    float4 c = g_lnphi[i];
+    return __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, c.w, c.z), c.y), c.x);
+}
+__device__ __forceinline__ float neg_ln_phi_asym(float x) {
+    return __builtin_fmaf(LN2, lg2(-x), __builtin_fmaf(0.5f * x, x, 0.918938533204672742f));
 }
 __device__ __forceinline__ float neg_ln_phi(float x) {
-    float ug = __builtin_fmaf(x, LNPHI_SCALE, -LNPHI_X0 * LNPHI_SCALE);
-    ug = __builtin_fminf(__builtin_fmaxf(ug, 0.0f), (float)(LNPHI_N - 1));
-    unsigned i = (unsigned)ug;
-    float u = __builtin_fmaf(x, LNPHI_SCALE, -LNPHI_X0 * LNPHI_SCALE - (float)i);
-    float4 c = g_lnphi[i];
-    float tab = __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, c.w, c.z), c.y), c.x);
-    float asym = __builtin_fmaf(LN2, lg2(-x), __builtin_fmaf(0.5f * x, x, 0.918938533204672742f));
-    tab = x > 8.0f ? 0.0f : tab;
-    return x < -30.0f ? asym : tab;
+    return x < -30.0f ? neg_ln_phi_asym(x) : neg_ln_phi_tab(x);
 }
-template <int MODEL>
+// Four upper limits: the asymptote is evaluated only when some lane has an
+// observation below x = -30 (a wave-uniform branch).
+template <bool CHECK, int MODEL>
 __device__ __forceinline__ float cost_upper_quad(const Fast<MODEL>& m, float se2, const float4& r0,
                                                  const float4& r1, const float4& r2, const float4& r3) {
-    float x0 = upper_x(m, se2, r0), x1 = upper_x(m, se2, r1);
-    float x2 = upper_x(m, se2, r2), x3 = upper_x(m, se2, r3);
-    return (neg_ln_phi(x0) + neg_ln_phi(x1)) + (neg_ln_phi(x2) + neg_ln_phi(x3));
+    float x0 = upper_x<CHECK>(m, se2, r0), x1 = upper_x<CHECK>(m, se2, r1);
+    float x2 = upper_x<CHECK>(m, se2, r2), x3 = upper_x<CHECK>(m, se2, r3);
+    float c0 = neg_ln_phi_tab(x0), c1 = neg_ln_phi_tab(x1);
+    float c2 = neg_ln_phi_tab(x2), c3 = neg_ln_phi_tab(x3);
+    float xm = __builtin_fminf(__builtin_fminf(x0, x1), __builtin_fminf(x2, x3));
+    if (__builtin_expect(__any(xm < -30.0f), 0)) {
+        c0 = x0 < -30.0f ? neg_ln_phi_asym(x0) : c0;
+        c1 = x1 < -30.0f ? neg_ln_phi_asym(x1) : c1;
+        c2 = x2 < -30.0f ? neg_ln_phi_asym(x2) : c2;
+        c3 = x3 < -30.0f ? neg_ln_phi_asym(x3) : c3;
+    }
+    return (c0 + c1) + (c2 + c3);
 }
-template <int MODEL>
+template <bool CHECK, int MODEL>
 __device__ __forceinline__ float cost_upper(const Fast<MODEL>& m, float se2, const float4& r) {
-    return neg_ln_phi(upper_x(m, se2, r));
+    return neg_ln_phi(upper_x<CHECK>(m, se2, r));
 }
 
 // Stage `cnt` packed records {t, y, var, flag} from global memory into LDS as
 // {t log2 e, y, var, flag}, detections first and upper limits after them,
-// each group in its original order.  Returns the number of detections
-// (wave-uniform).  Single-wave workgroups.
-__device__ __forceinline__ int stage_split(float4* lds, const float4* __restrict__ g, int cnt) {
+// each group in its original order.  Returns the number of detections and
+// the range [tmin, tmax] of t log2 e (wave-uniform).  Single-wave workgroups.
+__device__ __forceinline__ int stage_split(float4* lds, const float4* __restrict__ g, int cnt,
+                                           float& tmin, float& tmax) {
     const int lane = lane_id();
     __syncthreads();
     int nd = 0;
+    float lo = INFINITY, hi = -INFINITY;
     for (int i0 = 0; i0 < cnt; i0 += WAVE) {
         int i = i0 + lane;
-        bool det = i < cnt && g[i].w == 0.0f;
-        nd += __popcll(__ballot(det));
+        bool valid = i < cnt;
+        float4 r = valid ? g[i] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        if (valid) {
+            lo = __builtin_fminf(lo, r.x);
+            hi = __builtin_fmaxf(hi, r.x);
+        }
+        nd += __popcll(__ballot(valid && r.w == 0.0f));
     }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        lo = __builtin_fminf(lo, __shfl_xor(lo, off, WAVE));
+        hi = __builtin_fmaxf(hi, __shfl_xor(hi, off, WAVE));
+    }
+    tmin = lo * L2E;
+    tmax = hi * L2E;
     int kd = 0, ku = nd;
     for (int i0 = 0; i0 < cnt; i0 += WAVE) {
         int i = i0 + lane;

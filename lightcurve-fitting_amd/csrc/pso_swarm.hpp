@@ -58,11 +58,48 @@ struct Swarm {
     // FAST cost: the tile holds {t log2 e, y, var, flag} with the `nd`
     // detections first (staged once by the caller when the band fits one
     // tile, else re-staged here per tile).  Four detections per Kahan update,
-    // then the upper limits, four per update.  A non-finite prediction reaches the
-    // sum as +inf or NaN and maps to the invalid sentinel below, as the
-    // per-observation flag of the general path does.
+    // then the upper limits, four per update.  When no particle of the wave
+    // can reach a phase test of its model on the tile's time range, the
+    // tests are compiled out (CHECK = false).  A non-finite prediction
+    // reaches the sum as +inf or NaN and maps to the invalid sentinel below,
+    // as the per-observation flag of the general path does.
+    template <bool CHECK>
+    __device__ __forceinline__ void tile_fast(Kahan<float> (&acc)[PPL], const f32::Fast<MODEL> (&mdl)[PPL],
+                                              const float (&se2)[PPL], const O* lds, int nd, int cnt) {
+        int i = 0;
+        for (; i + 4 <= nd; i += 4) {
+            const O r0 = lds[i], r1 = lds[i + 1], r2 = lds[i + 2], r3 = lds[i + 3];
+#pragma unroll
+            for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_quad<CHECK>(mdl[k], se2[k], r0, r1, r2, r3));
+        }
+        if (i + 2 <= nd) {
+            const O r0 = lds[i], r1 = lds[i + 1];
+#pragma unroll
+            for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_pair<CHECK>(mdl[k], se2[k], r0, r1));
+            i += 2;
+        }
+        if (i < nd) {
+            const O r0 = lds[i];
+#pragma unroll
+            for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_one<CHECK>(mdl[k], se2[k], r0));
+            i++;
+        }
+        for (; i + 4 <= cnt; i += 4) {
+            const O r0 = lds[i], r1 = lds[i + 1], r2 = lds[i + 2], r3 = lds[i + 3];
+#pragma unroll
+            for (int k = 0; k < PPL; k++)
+                acc[k].add(f32::cost_upper_quad<CHECK>(mdl[k], se2[k], r0, r1, r2, r3));
+        }
+        for (; i < cnt; i++) {
+            const O r0 = lds[i];
+#pragma unroll
+            for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_upper<CHECK>(mdl[k], se2[k], r0));
+        }
+    }
+
     __device__ __forceinline__ void costs_fast(R (&cost)[PPL], const O* lds, const O* __restrict__ g,
-                                               int n, int tile, int nd_staged) {
+                                               int n, int tile, int nd_staged, float tmin_staged,
+                                               float tmax_staged) {
         f32::Fast<MODEL> mdl[PPL];
         float se2[PPL];
         Kahan<float> acc[PPL];
@@ -74,36 +111,14 @@ struct Swarm {
         }
         for (int base = 0; base < n; base += tile) {
             const int cnt = n - base < tile ? n - base : tile;
-            const int nd = n > tile ? f32::stage_split(const_cast<O*>(lds), g + base, cnt) : nd_staged;
-            int i = 0;
-            for (; i + 4 <= nd; i += 4) {
-                const O r0 = lds[i], r1 = lds[i + 1], r2 = lds[i + 2], r3 = lds[i + 3];
+            int nd = nd_staged;
+            float tmin = tmin_staged, tmax = tmax_staged;
+            if (n > tile) nd = f32::stage_split(const_cast<O*>(lds), g + base, cnt, tmin, tmax);
+            bool safe = true;
 #pragma unroll
-                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_quad(mdl[k], se2[k], r0, r1, r2, r3));
-            }
-            if (i + 2 <= nd) {
-                const O r0 = lds[i], r1 = lds[i + 1];
-#pragma unroll
-                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_pair(mdl[k], se2[k], r0, r1));
-                i += 2;
-            }
-            if (i < nd) {
-                const O r0 = lds[i];
-#pragma unroll
-                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_one(mdl[k], se2[k], r0));
-                i++;
-            }
-            for (; i + 4 <= cnt; i += 4) {
-                const O r0 = lds[i], r1 = lds[i + 1], r2 = lds[i + 2], r3 = lds[i + 3];
-#pragma unroll
-                for (int k = 0; k < PPL; k++)
-                    acc[k].add(f32::cost_upper_quad(mdl[k], se2[k], r0, r1, r2, r3));
-            }
-            for (; i < cnt; i++) {
-                const O r0 = lds[i];
-#pragma unroll
-                for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_upper(mdl[k], se2[k], r0));
-            }
+            for (int k = 0; k < PPL; k++) safe = safe && mdl[k].safe(tmin, tmax);
+            if (__all(safe)) tile_fast<false>(acc, mdl, se2, lds, nd, cnt);
+            else tile_fast<true>(acc, mdl, se2, lds, nd, cnt);
         }
         const float inv_n = 1.0f / (float)(n > 0 ? n : 1);
 #pragma unroll
@@ -128,9 +143,10 @@ struct Swarm {
     // cost never becomes a personal best, 1e99 does).  FP32 uses a Kahan sum
     // and reciprocals, and folds every overflow into its 1e30 sentinel.
     __device__ __forceinline__ void costs(R (&cost)[PPL], const O* lds, const O* __restrict__ g,
-                                          int n, int tile, int P, int nd_staged) {
+                                          int n, int tile, int P, int nd_staged, float tmin_staged,
+                                          float tmax_staged) {
         if constexpr (FAST) {
-            costs_fast(cost, lds, g, n, tile, nd_staged);
+            costs_fast(cost, lds, g, n, tile, nd_staged, tmin_staged, tmax_staged);
             return;
         }
         constexpr bool F64 = sizeof(R) == 8;
@@ -219,7 +235,8 @@ struct Swarm {
                                         uint32_t band, int P, int max_iters, int stall_iters,
                                         uint64_t seed, uint32_t flags, int& iters_out,
                                         const R (&lo)[NP], const R (&hi)[NP], const R* seedp = nullptr,
-                                        uint32_t rng_tag = 0u, int nd_staged = 0) {
+                                        uint32_t rng_tag = 0u, int nd_staged = 0, float tmin_staged = 0.0f,
+                                        float tmax_staged = 0.0f) {
         const int lane = lane_id();
         if constexpr (FAST) {
 #pragma unroll
@@ -273,7 +290,7 @@ struct Swarm {
         for (; it < max_iters; it++) {
             R w = w_max - (w_max - w_min) * R(it) * inv_max;
             R cost[PPL];
-            costs(cost, lds, g, n, tile, P, nd_staged);
+            costs(cost, lds, g, n, tile, P, nd_staged, tmin_staged, tmax_staged);
             // personal bests; lowest cost this iteration (ties: lowest index)
             R bc = R(INFINITY);
             int bi = 0x7fffffff;
@@ -345,8 +362,10 @@ __device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
     const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[band];
     const int n = (int)(a.offsets[band + 1] - a.offsets[band]);
     int nd = 0;
+    float tmin = 0.0f, tmax = 0.0f;
     if constexpr (Swarm<MODEL, R, PPL>::FAST) {
-        if (n <= a.tile) nd = f32::stage_split(reinterpret_cast<float4*>(lds), reinterpret_cast<const float4*>(g), n);
+        if (n <= a.tile)
+            nd = f32::stage_split(reinterpret_cast<float4*>(lds), reinterpret_cast<const float4*>(g), n, tmin, tmax);
     } else {
         if (n <= a.tile) stage_obs<R>(lds, g, n);
     }
@@ -360,7 +379,7 @@ __device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
     Swarm<MODEL, R, PPL> s;
     int iters = 0;
     s.run(lds, g, n, a.tile, (uint32_t)band, a.n_particles, a.max_iters, a.stall_iters, a.seed,
-          a.flags, iters, lo, hi, nullptr, 0u, nd);
+          a.flags, iters, lo, hi, nullptr, 0u, nd, tmin, tmax);
     double cst = (double)s.gcost;
     if (cst >= (double)Math<R>::big()) cst = 1e99;
     double prm = 0.0;  // lane d holds parameter d
