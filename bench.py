@@ -43,11 +43,15 @@ PKG_DIR = ROOT / "lightcurve-fitting_amd"
 METRIC = "sources/sec, batched+adjoint vs plain GPU fit at equal accuracy, 100k src"
 MODELS = (0, 1, 2, 3, 4)
 MODEL_NAMES = {0: "Bazin", 1: "Villar", 2: "Tde", 3: "Arnett", 4: "Magnetar"}
-# FP32 flop-equivalents per model-observation evaluation (model + likelihood
-# term + Kahan update), counted from models_fit.hpp / pso_kernels.hip: plain
-# VALU ops count 1 (FMA 2), each transcendental (exp2/log2/rcp/rsq/sqrt)
-# counts 8 (quarter-rate issue x FMA).
-FLOPEQ_PER_MOE = {0: 64.0, 1: 91.0, 2: 94.0, 3: 86.0, 4: 74.0}
+# FP32 flop-equivalents per model-observation evaluation (MOE: prediction,
+# likelihood term and its share of the Kahan update), counted from the
+# compiled check-free cost loops of pso_kernel<MODEL, fp32> (four
+# observations per loop trip, /4): an FMA counts 2, a transcendental
+# (exp2/log2/rcp/rsq) 4 (it issues at half rate on gfx950: 2 slots x 2), any
+# other VALU instruction 1.  Detections and upper limits (censored term)
+# differ; the per-iteration swarm update is not counted.
+FLOPEQ_DET = {0: 34.5, 1: 54.5, 2: 49.5, 3: 50.5, 4: 42.5}
+FLOPEQ_UP = {0: 41.0, 1: 61.0, 2: 56.0, 3: 57.0, 4: 49.0}
 FP32_PEAK_TFLOPS = 157.3
 
 
@@ -348,16 +352,23 @@ def main() -> int:
 
     # roofline of the dominant model kernel
     npts = np.diff(csr["offsets"]).astype(np.float64)
-    moe = (iters_exec.astype(np.float64) * npts[None, :]).sum(axis=1) * args.particles
+    nup = np.add.reduceat(csr["is_upper"].astype(np.float64), csr["offsets"][:-1]) if npts.size else npts
+    nup = np.where(npts > 0, nup, 0.0)
+    it64 = iters_exec.astype(np.float64)
+    moe_up = (it64 * nup[None, :]).sum(axis=1) * args.particles
+    moe = (it64 * npts[None, :]).sum(axis=1) * args.particles
+    moe_det = moe - moe_up
     dom = int(np.argmax(slot_ms))
-    flops = moe[dom] * FLOPEQ_PER_MOE[MODELS[dom]]
+    m_dom = MODELS[dom]
+    flops = moe_det[dom] * FLOPEQ_DET[m_dom] + moe_up[dom] * FLOPEQ_UP[m_dom]
     achieved = flops / (slot_ms[dom] * 1e-3) / 1e12
     roof = {"bound": "mfma", "pipe": "fp32 vector ALU + transcendental unit (no matrix ops)",
             "kernel": f"pso_model_kernel<{MODEL_NAMES[MODELS[dom]]}, fp32>",
             "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
             "launch_ms": round(float(slot_ms[dom]), 3),
-            "moe_per_launch": float(moe[dom]), "flopeq_per_moe": FLOPEQ_PER_MOE[MODELS[dom]]}
+            "moe_per_launch": float(moe[dom]), "upper_limit_moe_fraction": round(float(moe_up[dom] / moe[dom]), 4),
+            "flopeq_per_moe": {"detection": FLOPEQ_DET[m_dom], "upper_limit": FLOPEQ_UP[m_dom]}}
 
     plain = None
     if not args.no_plain:
