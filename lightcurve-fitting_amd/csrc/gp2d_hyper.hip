@@ -1,16 +1,19 @@
 // gp2d_hyper.hip -- 2D (time x log10 wavelength) GP hyper-parameter fit, FP64.
 //
 // One 256-thread workgroup per source (persistent, sources taken from a
-// size-sorted queue), the source's matrices in a global slot of ld x ld
-// doubles (ld a multiple of 64, L2 / MALL resident), all O(n^3) work as
-// 64 x 64 tile products staged through LDS with 4 x 4 register tiles per
-// thread:
+// size-sorted queue; two workgroups per CU: the large phases are separate
+// functions so that each fits 256 registers), the source's matrices in a
+// global slot of ld x ld doubles (ld a multiple of 64, L2 / MALL resident),
+// all O(n^3) work as 64 x 64 tile products staged through LDS and run on the
+// FP64 matrix cores (v_mfma_f64_16x16x4f64, a 32 x 32 quadrant per wave):
 //   K   = a exp(-dt^2 / (2 lt^2) - dw^2 / (2 lw^2)) + diag(nv + sn^2),
 //         theta = ln (a, lt, lw, sn), r = y - mean(y);
 //   f   NLML = 1/2 z^T z + sum ln L_ii + n/2 ln 2 pi, z = L^-1 r, from a
 //       right-looking blocked Cholesky of [[K, 0], [r^T, 1]] (padded with
 //       the identity): the augmented row ends as z^T, so the forward solve
 //       comes out of the factorisation;
+//       The 64 x 64 diagonal blocks are factored left-looking in LDS by
+//       one wave, the panels solved thread per row 16 columns at a time;
 //   g   dNLML/dtheta_k = 1/2 tr((K^-1 - alpha alpha^T) dK/dtheta_k): X = L^-1
 //       by blocked inversion (diagonal blocks by one wave, off-diagonal
 //       blocks as X_IJ = -X_II sum_K L_IK X_KJ), alpha = X^T z, and
@@ -86,32 +89,51 @@ __device__ __forceinline__ void stage_kmaj(double* s, const double* M, int ld, i
     }
 }
 
-// acc[r][c] += sum_kk sa[kk][4 ty + r] sb[kk][4 tx + c]
-__device__ __forceinline__ void mma64(double (&acc)[4][4], const double* sa, const double* sb) {
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-    const double* pa = sa + 4 * ty;
-    const double* pb = sb + 4 * tx;
-#pragma unroll 8
-    for (int kk = 0; kk < T; kk++) {
-        const double4 a = *reinterpret_cast<const double4*>(pa + kk * T);
-        const double4 b = *reinterpret_cast<const double4*>(pb + kk * T);
-        const double av[4] = {a.x, a.y, a.z, a.w}, bv[4] = {b.x, b.y, b.z, b.w};
+// 64 x 64 tile products on the FP64 matrix cores: C += A B with
+// A[r][kk] = sa[kk * 64 + r] and B[kk][c] = sb[kk * 64 + c].  Wave w owns the
+// 32 x 32 quadrant (32 (w >> 1), 32 (w & 1)) as 2 x 2 blocks of 16 x 16, each
+// an accumulator of v_mfma_f64_16x16x4f64 (four doubles per lane), fed with
+// one double of A and one of B per lane per four-deep k step.  Operand and
+// result lane layouts of the instruction (checked on gfx950): A lane l ->
+// A[l % 16][l / 16], B lane l -> B[l / 16][l % 16], D lane l element i ->
+// D[4 i + l / 16][l % 16].
+typedef double v4d __attribute__((ext_vector_type(4)));
+struct Tile {
+    v4d d[2][2];
+};
+
+__device__ __forceinline__ void zero(Tile& t) {
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+    for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) acc[r][c] = fma(av[r], bv[c], acc[r][c]);
+        for (int cb = 0; cb < 2; cb++) t.d[rb][cb] = v4d{0.0, 0.0, 0.0, 0.0};
+}
+
+// Row / column inside the 64 x 64 tile of element i of block (rb, cb).
+__device__ __forceinline__ int tile_row(int rb, int i) {
+    return 32 * (threadIdx.x >> 7) + 16 * rb + 4 * i + ((threadIdx.x >> 4) & 3);
+}
+__device__ __forceinline__ int tile_col(int cb) {
+    return 32 * ((threadIdx.x >> 6) & 1) + 16 * cb + (threadIdx.x & 15);
+}
+
+__device__ __forceinline__ void mma64(Tile& t, const double* sa, const double* sb) {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double* pa = sa + (l >> 4) * T + 32 * (w >> 1) + (l & 15);
+    const double* pb = sb + (l >> 4) * T + 32 * (w & 1) + (l & 15);
+#pragma unroll 4
+    for (int k0 = 0; k0 < T; k0 += 4) {
+        const double a0 = pa[k0 * T], a1 = pa[k0 * T + 16];
+        const double b0 = pb[k0 * T], b1 = pb[k0 * T + 16];
+        t.d[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, t.d[0][0], 0, 0, 0);
+        t.d[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, t.d[0][1], 0, 0, 0);
+        t.d[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, t.d[1][0], 0, 0, 0);
+        t.d[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, t.d[1][1], 0, 0, 0);
     }
 }
 
-__device__ __forceinline__ void zero(double (&acc)[4][4]) {
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-#pragma unroll
-        for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
-}
-
 // K (lower triangle incl. padding and the augmented row) into A.
-__device__ void build_k(double* A, int ld, const Src& s, const double* th, double ybar) {
+__device__ __noinline__ void build_k(double* A, int ld, const Src& s, const double* th, double ybar) {
     const double a = ::exp(th[0]), lt = ::exp(th[1]), lw = ::exp(th[2]), sn = ::exp(th[3]);
     const double it = 1.0 / (2.0 * lt * lt), iw = 1.0 / (2.0 * lw * lw), sn2 = sn * sn;
     const int n = s.n, np = s.np;
@@ -136,39 +158,33 @@ __device__ void build_k(double* A, int ld, const Src& s, const double* th, doubl
 
 // Right-looking blocked Cholesky of A's lower triangle in place; columns
 // >= n take the identity pivot.  Returns false (uniformly) on a pivot <= 0.
-__device__ bool cholesky(double* A, int ld, const Src& s, Smem& sm) {
+__device__ __noinline__ bool cholesky(double* A, int ld, const Src& s, Smem& sm) {
     const int n = s.n, np = s.np, tid = threadIdx.x, lane = lane_id();
     const int rows = n + 1;  // rows below n are identity and never change
     for (int J0 = 0; J0 < rows; J0 += T) {
-        // (1) diagonal block on wave 0: lane r holds row J0 + r
+        // (1) diagonal block on wave 0, left-looking in LDS: the block's lower
+        // triangle column-major in sm.a (sm.a[c * T + r] = L[r][c]: a lane
+        // reads its own row coalesced, the pivot row as a broadcast); lane r
+        // computes column j of row r as A[r][j] - sum_k<j L[r][k] L[j][k] in
+        // increasing k, the order of a right-looking sweep.
         if (tid < WAVE) {
-            double v[T];
             const double* rowp = A + (size_t)(J0 + lane) * ld + J0;
-#pragma unroll
-            for (int c = 0; c < T; c++) v[c] = c <= lane ? rowp[c] : 0.0;
+            for (int c = 0; c < T; c++) sm.a[c * T + lane] = c <= lane ? rowp[c] : 0.0;
             int bad = 0;
-#pragma unroll
             for (int j = 0; j < T; j++) {
                 if (J0 + j >= n) {
-                    if (lane == j) v[j] = 1.0;
+                    if (lane == j) sm.a[j * T + j] = 1.0;
                     continue;
                 }
-                const double piv = bcast(v[j], j);
+                double acc = sm.a[j * T + lane];
+                for (int k = 0; k < j; k++) acc = fma(-sm.a[k * T + lane], sm.a[k * T + j], acc);
+                const double piv = bcast(acc, j);
                 bad |= !(piv > 0.0);
                 const double ljj = ::sqrt(piv);
-                v[j] = lane == j ? ljj : (lane > j ? v[j] / ljj : v[j]);
-#pragma unroll
-                for (int c = j + 1; c < T; c++) {
-                    const double lcj = bcast(v[j], c);
-                    if (lane >= c) v[c] = fma(-v[j], lcj, v[c]);
-                }
+                if (lane >= j) sm.a[j * T + lane] = lane == j ? ljj : acc / ljj;
             }
             double* wr = A + (size_t)(J0 + lane) * ld + J0;
-#pragma unroll
-            for (int c = 0; c < T; c++) {
-                if (c <= lane) wr[c] = v[c];
-                sm.a[lane * T + c] = c <= lane ? v[c] : 0.0;
-            }
+            for (int c = 0; c <= lane; c++) wr[c] = sm.a[c * T + lane];
             if (tid == 0) sm.flag[0] = bad;
         }
         __syncthreads();
@@ -176,40 +192,50 @@ __device__ bool cholesky(double* A, int ld, const Src& s, Smem& sm) {
             __syncthreads();
             return false;
         }
-        // (2) panel rows below the block (real rows and the augmented row)
+        // (2) panel rows below the block (real rows and the augmented row),
+        // thread per row, 16 columns at a time: the row's solved columns of
+        // the earlier chunks are read back in increasing k, so every element
+        // loses its terms in the same order as a full-row substitution.
         for (int i = J0 + T + tid; i < rows; i += NT) {
-            double v[T];
             double* rp = A + (size_t)i * ld + J0;
+            for (int cb = 0; cb < T; cb += 16) {
+                double v[16];
 #pragma unroll
-            for (int c = 0; c < T; c++) v[c] = rp[c];
+                for (int c = 0; c < 16; c++) v[c] = rp[cb + c];
+                for (int k = 0; k < cb; k++) {
+                    const double lik = rp[k];
 #pragma unroll
-            for (int j = 0; j < T; j++) {
-                double acc = v[j];
+                    for (int j = 0; j < 16; j++) v[j] = fma(-lik, sm.a[k * T + cb + j], v[j]);
+                }
 #pragma unroll
-                for (int k = 0; k < j; k++) acc = fma(-v[k], sm.a[j * T + k], acc);
-                v[j] = acc / sm.a[j * T + j];
+                for (int j = 0; j < 16; j++) {
+                    double acc = v[j];
+#pragma unroll
+                    for (int k = 0; k < j; k++) acc = fma(-v[k], sm.a[(cb + k) * T + cb + j], acc);
+                    v[j] = acc / sm.a[(cb + j) * T + cb + j];
+                }
+#pragma unroll
+                for (int c = 0; c < 16; c++) rp[cb + c] = v[c];
             }
-#pragma unroll
-            for (int c = 0; c < T; c++) rp[c] = v[c];
         }
         __syncthreads();
         // (3) trailing update of the lower tiles with rows < n + 1
         for (int I0 = J0 + T; I0 < rows; I0 += T) {
             for (int C0 = J0 + T; C0 <= I0; C0 += T) {
-                double acc[4][4];
+                Tile acc;
                 zero(acc);
                 __syncthreads();
                 stage_rows(sm.a, A, ld, I0, J0, J0 + T);
                 stage_rows(sm.b, A, ld, C0, J0, J0 + T);
                 __syncthreads();
                 mma64(acc, sm.a, sm.b);
-                const int tx = tid & 15, ty = tid >> 4;
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    double* cp = A + (size_t)(I0 + 4 * ty + r) * ld + C0 + 4 * tx;
+                for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-                    for (int c = 0; c < 4; c++) cp[c] -= acc[r][c];
-                }
+                    for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            A[(size_t)(I0 + tile_row(rb, i)) * ld + C0 + tile_col(cb)] -= acc.d[rb][cb][i];
             }
         }
         __syncthreads();
@@ -219,7 +245,7 @@ __device__ bool cholesky(double* A, int ld, const Src& s, Smem& sm) {
 }
 
 // NLML from the factor; stores z (augmented row) in zv.
-__device__ double nlml_from_factor(const double* A, int ld, const Src& s, double* zv, Smem& sm) {
+__device__ __noinline__ double nlml_from_factor(const double* A, int ld, const Src& s, double* zv, Smem& sm) {
     const int n = s.n;
     double q = 0.0, l = 0.0;
     for (int i = threadIdx.x; i < n; i += NT) {
@@ -234,10 +260,9 @@ __device__ double nlml_from_factor(const double* A, int ld, const Src& s, double
 }
 
 // Gradient at the factor in A (z in zv): X = L^-1 into Xm, alpha into av.
-__device__ void gradient(double* A, double* Xm, int ld, const Src& s, const double* th, const double* zv,
+__device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& s, const double* th, const double* zv,
                          double* av, Smem& sm, double* g) {
     const int n = s.n, tid = threadIdx.x, lane = lane_id();
-    const int tx = tid & 15, ty = tid >> 4;
     const int nb = (n + T - 1) / T;
     // the augmented row leaves the factor: [[L, 0], [0, 1]]
     for (int j = tid; j < n; j += NT) A[(size_t)n * ld + j] = 0.0;
@@ -252,16 +277,28 @@ __device__ void gradient(double* A, double* Xm, int ld, const Src& s, const doub
         }
         __syncthreads();
         if (tid < WAVE) {
-            double x[T];
+            // 16 rows at a time; x of the earlier chunks read back from Xm
+            // in increasing q (the order of a full-column substitution)
+            double* xc = Xm + (size_t)I0 * ld + I0 + lane;
+            for (int rb = 0; rb < T; rb += 16) {
+                double x[16];
 #pragma unroll
-            for (int r = 0; r < T; r++) {
-                double acc = r == lane ? 1.0 : 0.0;
+                for (int j = 0; j < 16; j++) x[j] = rb + j == lane ? 1.0 : 0.0;
+                for (int q = 0; q < rb; q++) {
+                    const double xq = xc[(size_t)q * ld];
 #pragma unroll
-                for (int q = 0; q < r; q++) acc = fma(-sm.a[r * T + q], x[q], acc);
-                x[r] = acc / sm.a[r * T + r];
+                    for (int j = 0; j < 16; j++) x[j] = fma(-sm.a[(rb + j) * T + q], xq, x[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) {
+                    double acc = x[j];
+#pragma unroll
+                    for (int q = 0; q < j; q++) acc = fma(-sm.a[(rb + j) * T + rb + q], x[q], acc);
+                    x[j] = acc / sm.a[(rb + j) * T + rb + j];
+                }
+#pragma unroll
+                for (int j = 0; j < 16; j++) xc[(size_t)(rb + j) * ld] = x[j];
             }
-#pragma unroll
-            for (int r = 0; r < T; r++) Xm[(size_t)(I0 + r) * ld + I0 + lane] = x[r];
         }
     }
     __syncthreads();
@@ -270,7 +307,7 @@ __device__ void gradient(double* A, double* Xm, int ld, const Src& s, const doub
         const int J0 = J * T;
         for (int I = J + 1; I < nb; I++) {
             const int I0 = I * T;
-            double acc[4][4];
+            Tile acc;
             zero(acc);
             for (int k0 = J0; k0 < I0; k0 += T) {
                 __syncthreads();
@@ -281,20 +318,23 @@ __device__ void gradient(double* A, double* Xm, int ld, const Src& s, const doub
             }
             __syncthreads();
 #pragma unroll
-            for (int r = 0; r < 4; r++)
+            for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) sm.b[(4 * ty + r) * T + 4 * tx + c] = acc[r][c];
+                for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++) sm.b[tile_row(rb, i) * T + tile_col(cb)] = acc.d[rb][cb][i];
             stage_rows(sm.a, Xm, ld, I0, I0, I0 + T);
             __syncthreads();
-            double x2[4][4];
+            Tile x2;
             zero(x2);
             mma64(x2, sm.a, sm.b);
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                double* xp = Xm + (size_t)(I0 + 4 * ty + r) * ld + J0 + 4 * tx;
+            for (int rb = 0; rb < 2; rb++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) xp[c] = -x2[r][c];
-            }
+                for (int cb = 0; cb < 2; cb++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        Xm[(size_t)(I0 + tile_row(rb, i)) * ld + J0 + tile_col(cb)] = -x2.d[rb][cb][i];
         }
     }
     __syncthreads();
@@ -313,7 +353,7 @@ __device__ void gradient(double* A, double* Xm, int ld, const Src& s, const doub
         const int I0 = I * T;
         for (int J = 0; J <= I; J++) {
             const int J0 = J * T;
-            double acc[4][4];
+            Tile acc;
             zero(acc);
             for (int k0 = I0; k0 < n; k0 += T) {
                 __syncthreads();
@@ -323,15 +363,17 @@ __device__ void gradient(double* A, double* Xm, int ld, const Src& s, const doub
                 mma64(acc, sm.a, sm.b);
             }
 #pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int i = I0 + 4 * ty + r;
+                const int i = I0 + tile_row(rb, r);
                 if (i >= n) continue;
                 const double ti = s.t[i], wi = s.w[i], ai = av[i];
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const int j = J0 + 4 * tx + c;
+                for (int cb = 0; cb < 2; cb++) {
+                    const int j = J0 + tile_col(cb);
                     if (j > i) continue;
-                    const double W = acc[r][c] - ai * av[j];
+                    const double W = acc.d[rb][cb][r] - ai * av[j];
                     const double dt = ti - s.t[j], dw = wi - s.w[j];
                     const double dt2 = dt * dt, dw2 = dw * dw;
                     const double ak = a * ::exp(-(dt2 * it + dw2 * iw));
@@ -467,7 +509,7 @@ __device__ void accept_step(Smem& sm) {
     st[S_F] = st[S_FT];
 }
 
-__device__ void predict(const Gp2dHyperArgs& a, int src, double* Kq, const double* Xm, int ld, const Src& s,
+__device__ __noinline__ void predict(const Gp2dHyperArgs& a, int src, double* Kq, const double* Xm, int ld, const Src& s,
                         const double* th, double ybar, const double* av, Smem& sm) {
     const int n = s.n, nq = a.n_query, tid = threadIdx.x;
     const double amp = ::exp(th[0]), lt = ::exp(th[1]), lw = ::exp(th[2]);
@@ -655,7 +697,7 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
 
 }  // namespace
 
-__global__ __launch_bounds__(NT) void gp2d_hyper_kernel(Gp2dHyperArgs a) {
+__global__ __launch_bounds__(NT, 2) void gp2d_hyper_kernel(Gp2dHyperArgs a) {
     __shared__ Smem sm;
     __shared__ int ticket;
     double* slot = a.work + (size_t)blockIdx.x * gp2d_hyper_slot_doubles(a.ld);
