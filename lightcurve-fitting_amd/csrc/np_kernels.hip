@@ -1270,7 +1270,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
 
 }  // namespace
 
-__global__ __launch_bounds__(GP_THREADS) void np_band_kernel(NpArgs a) {
+__global__ __launch_bounds__(GP_THREADS, 2) void np_band_kernel(NpArgs a) {
     extern __shared__ double np_lds[];
     const Lds s = carve(np_lds, a.cap, a.fitc);
     for (;;) {
