@@ -398,8 +398,17 @@ __device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
 // first) until the queue is drained.  One-shot form (PERSISTENT = false):
 // workgroup b fits band order[task_base + b]; used by the per-size-class
 // schedule.
+// Waves per SIMD the kernel is compiled for: the FP32 cost loops of models
+// 0-6 with one particle per lane fit 96 registers, so five single-wave
+// workgroups share a SIMD (their 8000-byte LDS tiles of 500-point bands fill
+// the CU's 160 KB exactly); the other forms keep the compiler's choice.
+template <int MODEL, typename R, int PPL>
+constexpr int pso_waves_per_simd() {
+    return (sizeof(R) == 4 && MODEL <= 6 && PPL == 1) ? 5 : 1;
+}
+
 template <int MODEL, typename R, int PPL, bool PERSISTENT>
-__global__ __launch_bounds__(64) void pso_model_kernel(PsoArgs a) {
+__global__ __launch_bounds__(64, (pso_waves_per_simd<MODEL, R, PPL>())) void pso_model_kernel(PsoArgs a) {
     extern __shared__ unsigned char lds_raw[];
     using O = typename ObsT<R>::T;
     O* lds = reinterpret_cast<O*>(lds_raw);
