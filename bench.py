@@ -50,8 +50,8 @@ MODEL_NAMES = {0: "Bazin", 1: "Villar", 2: "Tde", 3: "Arnett", 4: "Magnetar"}
 # (exp2/log2/rcp/rsq) 4 (it issues at half rate on gfx950: 2 slots x 2), any
 # other VALU instruction 1.  Detections and upper limits (censored term)
 # differ; the per-iteration swarm update is not counted.
-FLOPEQ_DET = {0: 34.5, 1: 54.5, 2: 49.5, 3: 50.5, 4: 42.5}
-FLOPEQ_UP = {0: 41.0, 1: 61.0, 2: 56.0, 3: 57.0, 4: 49.0}
+FLOPEQ_DET = {0: 34.5, 1: 53.5, 2: 49.5, 3: 49.5, 4: 41.5}
+FLOPEQ_UP = {0: 41.0, 1: 60.0, 2: 56.0, 3: 56.0, 4: 48.0}
 FP32_PEAK_TFLOPS = 157.3
 
 

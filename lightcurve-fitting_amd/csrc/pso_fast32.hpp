@@ -92,16 +92,16 @@ template <> struct Fast<0> {
 // w (1-beta gamma) e^{(gamma-ph)/tf}], w = s(10(ph-gamma)), with w e^{y}
 // merged into one exponential; non-finite when e^{(gamma-ph)/tf} overflows.
 template <> struct Fast<1> {
-    float a, bn, zc, t0, itr, itf, yc, omb, p_inf;
+    float a, abn, zc, t0, itr, itf, yc, aomb, p_inf;  // A folded into the bracket
     __device__ __forceinline__ void setup(const float* p) {
         float beta = p[1], gamma = ex(p[2]);
         a = ex(p[0]); t0 = p[3] * L2E;
         itr = ex(-p[4]); itf = ex(-p[5]);
         float gl = gamma * L2E;
-        bn = beta * LN2;
+        abn = a * beta * LN2;
         zc = -10.0f * gl;
         yc = gl * itf;
-        omb = 1.0f - beta * gamma;
+        aomb = a * (1.0f - beta * gamma);
         p_inf = gl - OVF2 * ex(p[5]);
     }
     __device__ __forceinline__ bool safe(float tmin, float) const { return tmin - t0 >= p_inf; }
@@ -115,8 +115,8 @@ template <> struct Fast<1> {
         float omw = z >= 0.0f ? ez * qz : qz;
         float y = __builtin_fmaf(-ph, itf, yc);
         float wexp = qz * ex2(y + __builtin_fminf(z, 0.0f));
-        float inner = __builtin_fmaf(omw, __builtin_fmaf(-bn, ph, 1.0f), omb * wexp);
-        float f = a * sr * inner;
+        float inner = __builtin_fmaf(omw, __builtin_fmaf(-abn, ph, a), aomb * wexp);
+        float f = sr * inner;
         return CHECK && ph < p_inf ? INFINITY : f;
     }
 };
@@ -146,12 +146,14 @@ template <> struct Fast<2> {
 // Arnett (src/parametric.rs:339-356): A [f e^{-ps/8.8} + (1-f) e^{-ps/111.3}]
 // (1 - e^{-(ps/tm)^2}), ps = softplus(ph) + 1e-6; 0 beyond the FP64 range.
 template <> struct Fast<3> {
-    float a, t0, fr, omfr, xa, xb;
+    float t0, afr, aomfr, xa, xb;  // A folded into the heating weights
     __device__ __forceinline__ void setup(const float* p) {
-        a = ex(p[0]); t0 = p[1] * L2E;
+        float a = ex(p[0]);
+        t0 = p[1] * L2E;
         float itm = ex(-p[2]);
-        fr = logistic(p[3]);
-        omfr = 1.0f - fr;
+        float fr = logistic(p[3]);
+        afr = a * fr;
+        aomfr = a * (1.0f - fr);
         xa = LN2 * itm * SQL2E; xb = 1e-6f * itm * SQL2E;
     }
     __device__ __forceinline__ bool safe(float, float tmax) const { return tmax - t0 <= OVF2; }
@@ -161,20 +163,21 @@ template <> struct Fast<3> {
         float sp = softplus2(ph);  // ps = sp ln2 + 1e-6
         float h1 = ex2(__builtin_fmaf(sp, -1.0f / 8.8f, -1e-6f * L2E / 8.8f));
         float h2 = ex2(__builtin_fmaf(sp, -1.0f / 111.3f, -1e-6f * L2E / 111.3f));
-        float heat = __builtin_fmaf(fr, h1, omfr * h2);
+        float heat = __builtin_fmaf(afr, h1, aomfr * h2);
         float x = __builtin_fmaf(sp, xa, xb);  // ps / tm sqrt(log2 e)
-        float f = a * heat * (1.0f - ex2(-x * x));
+        float f = heat * (1.0f - ex2(-x * x));
         return CHECK && ph > OVF2 ? 0.0f : f;
     }
 };
 
 // Magnetar (src/parametric.rs:387-400): A (1 + ps/tsd)^-2 (1 - e^{-(ps/td)^2}).
 template <> struct Fast<4> {
-    float a, t0, wa, wb, xa, xb;
+    float t0, wa, wb, xa, xb;  // w scaled by A^-1/2, so A / w^2 = 1 / (w')^2
     __device__ __forceinline__ void setup(const float* p) {
-        a = ex(p[0]); t0 = p[1] * L2E;
+        float isa = ex(-0.5f * p[0]);
+        t0 = p[1] * L2E;
         float itsd = ex(-p[2]), itd = ex(-p[3]);
-        wa = LN2 * itsd; wb = 1.0f + 1e-6f * itsd;
+        wa = LN2 * itsd * isa; wb = (1.0f + 1e-6f * itsd) * isa;
         xa = LN2 * itd * SQL2E; xb = 1e-6f * itd * SQL2E;
     }
     __device__ __forceinline__ bool safe(float, float tmax) const { return tmax - t0 <= OVF2; }
@@ -184,7 +187,7 @@ template <> struct Fast<4> {
         float sp = softplus2(ph);
         float w = __builtin_fmaf(sp, wa, wb);
         float x = __builtin_fmaf(sp, xa, xb);
-        float f = a * rcp(w * w) * (1.0f - ex2(-x * x));
+        float f = rcp(w * w) * (1.0f - ex2(-x * x));
         return CHECK && ph > OVF2 ? 0.0f : f;
     }
 };
