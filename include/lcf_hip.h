@@ -113,7 +113,12 @@ int lcf_batch_all_models(lcf_ctx* ctx, const lcf_batch* data, int n_particles,
  * whose models[0] cost is >= gate.  Any output pointer may be NULL.
  *   model_params[n_models][n_sources][7], model_cost[n_models][n_sources],
  *   model_iters[n_models][n_sources] (iterations executed),
- *   best_model/best_cost[n_sources], best_params[n_sources][7]. */
+ *   best_model/best_cost[n_sources], best_params[n_sources][7].
+ * Random numbers: the initial swarm from Philox4x32-10 keyed by seed with the
+ * source index in the counter, the velocity updates from one xoshiro128**
+ * stream per (source, particle) seeded by a Philox block, so a source's fit
+ * does not depend on the batch it is in (the reference's sequential
+ * SmallRng stream is not reproduced). */
 int lcf_batch_fit_models(lcf_ctx* ctx, const lcf_batch* data, const int32_t* models,
                          int n_models, int use_gate, double gate, int n_particles,
                          int max_iters, int stall_iters, uint64_t seed, uint32_t flags,
