@@ -53,6 +53,15 @@ MODEL_NAMES = {0: "Bazin", 1: "Villar", 2: "Tde", 3: "Arnett", 4: "Magnetar"}
 FLOPEQ_DET = {0: 34.5, 1: 53.5, 2: 49.5, 3: 49.5, 4: 41.5}
 FLOPEQ_UP = {0: 41.0, 1: 60.0, 2: 56.0, 3: 56.0, 4: 48.0}
 FP32_PEAK_TFLOPS = 157.3
+# HBM traffic per launch of each model kernel on the default 100k-source
+# workload, from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB
+# per dispatch) over `bench.py --steps 1 --warmup 0 --no-cpu --no-plain
+# --no-svi --no-np --no-gp2d`, each scaled by what the same counter reads for
+# `bench.py --copy-probe 1024` (a 1 GiB device copy): FETCH_SIZE 524308 KiB
+# for 1048576 KiB read (x 2.0), WRITE_SIZE 1048576 KiB written (x 1.0).
+PSO_FETCH_KIB = {0: 236585.0, 1: 240138.0, 2: 236294.0, 3: 235933.0, 4: 230035.0}
+PSO_WRITE_KIB = {0: 37904.0, 1: 42918.0, 2: 37911.0, 3: 37906.0, 4: 35398.0}
+FETCH_SCALE, WRITE_SCALE = 1048576.0 / 524308.0, 1.0
 
 
 def load_package():
@@ -310,11 +319,23 @@ def main() -> int:
     ap.add_argument("--no-gp2d", action="store_true")
     ap.add_argument("--gp2d-sources", type=int, default=5000)
     ap.add_argument("--gp2d-cpu-sources", type=int, default=48)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (a plain copy kernel of known "
+                         "size, the scale of the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported on this build "
               "(sources are independent, see the sharding notes)", file=sys.stderr)
         return 2
+    if args.copy_probe > 0:
+        import torch
+        x = torch.ones(args.copy_probe << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_probe} MiB")
+        return 0
 
     pkg = load_package()
     log(f"generating {args.sources} sources")
@@ -361,12 +382,19 @@ def main() -> int:
     dom = int(np.argmax(slot_ms))
     m_dom = MODELS[dom]
     flops = moe_det[dom] * FLOPEQ_DET[m_dom] + moe_up[dom] * FLOPEQ_UP[m_dom]
+    # measured HBM bytes per launch (default workload only); the algorithmic
+    # bytes are one read of every packed record plus the outputs
+    traffic = None
+    if args.sources == 100_000 and args.particles == 64 and args.iters == 200:
+        traffic = round((PSO_FETCH_KIB[m_dom] * FETCH_SCALE + PSO_WRITE_KIB[m_dom] * WRITE_SCALE) * 1024.0)
+    algo_bytes = n_obs * 16 + n_bands * (7 * 8 + 8 + 4 + 4 + 8)
     achieved = flops / (slot_ms[dom] * 1e-3) / 1e12
     roof = {"bound": "mfma", "pipe": "fp32 vector ALU + transcendental unit (no matrix ops)",
             "kernel": f"pso_model_kernel<{MODEL_NAMES[MODELS[dom]]}, fp32>",
             "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "launch_ms": round(float(slot_ms[dom]), 3),
+            "algorithmic_bytes": int(algo_bytes),
             "moe_per_launch": float(moe[dom]), "upper_limit_moe_fraction": round(float(moe_up[dom] / moe[dom]), 4),
             "flopeq_per_moe": {"detection": FLOPEQ_DET[m_dom], "upper_limit": FLOPEQ_UP[m_dom]}}
 
