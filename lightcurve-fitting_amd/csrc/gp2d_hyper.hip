@@ -33,6 +33,8 @@ namespace {
 
 constexpr int T = GPH_TILE;
 constexpr int NT = GPH_THREADS;
+// block_sum (gp_device.hpp) combines GP_WAVES per-wave partial sums
+static_assert(NT == GP_THREADS, "gp2d workgroups must match the block reductions' wave count");
 constexpr int LM = 6;  // L-BFGS memory
 constexpr double LN2PI = 1.83787706640934548356;
 
