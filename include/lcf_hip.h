@@ -118,7 +118,9 @@ int lcf_batch_all_models(lcf_ctx* ctx, const lcf_batch* data, int n_particles,
  * source index in the counter, the velocity updates from one xoshiro128**
  * stream per (source, particle) seeded by a Philox block, so a source's fit
  * does not depend on the batch it is in (the reference's sequential
- * SmallRng stream is not reproduced). */
+ * SmallRng stream is not reproduced).  The first FP32 call on a device builds
+ * the FP32 cost loop's censored-likelihood table and synchronises the
+ * context's stream once (not capturable into a graph); later calls do not. */
 int lcf_batch_fit_models(lcf_ctx* ctx, const lcf_batch* data, const int32_t* models,
                          int n_models, int use_gate, double gate, int n_particles,
                          int max_iters, int stall_iters, uint64_t seed, uint32_t flags,
