@@ -1,4 +1,4 @@
-// pso_fast32.hpp -- FP32 forms of models 0-6 for the PSO cost loop, written
+// pso_fast32.hpp -- FP32 forms of models 0-6 and MultiBazin for the PSO cost loop, written
 // for the gfx950 instruction stream rather than for the textbook formula.
 //
 // The swarm cost is Sum_i [(f(t_i) - y_i)^2 / v_i + ln v_i] (plus the censored
@@ -236,6 +236,53 @@ template <> struct Fast<6> {
         return ph > OVF2 ? f_ovf : f;
     }
 };
+
+// MultiBazin with K components (internal ids 8..11, src/parametric.rs:905-965):
+// B + sum_c of the Bazin form above without its baseline, components added
+// in order; a component whose FP64 amplitude term overflows makes the
+// prediction non-finite, one whose logistic underflows to 0 adds nothing.
+template <int K> struct FastMB {
+    float la[K], t0[K], itr[K], itf[K], p_inf[K], p_b[K], b;
+    __device__ __forceinline__ void setup(const float* p) {
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            la[c] = p[4 * c] * L2E;
+            t0[c] = p[4 * c + 1] * L2E;
+            itr[c] = ex(-p[4 * c + 2]);
+            itf[c] = ex(-p[4 * c + 3]);
+            p_inf[c] = -(OVF2 - __builtin_fmaxf(la[c], 0.0f)) * ex(p[4 * c + 3]);
+            p_b[c] = -OVF2 * ex(p[4 * c + 2]);
+        }
+        b = p[4 * K];
+    }
+    __device__ __forceinline__ bool safe(float tmin, float) const {
+        bool ok = true;
+#pragma unroll
+        for (int c = 0; c < K; c++) ok = ok && tmin - t0[c] >= __builtin_fmaxf(p_b[c], p_inf[c]);
+        return ok;
+    }
+    template <bool CHECK = true>
+    __device__ __forceinline__ float pred(float tl) const {
+        float sum = 0.0f;
+        bool bad = false;
+#pragma unroll
+        for (int c = 0; c < K; c++) {
+            float ph = tl - t0[c];
+            float arg = __builtin_fmaf(-ph, itf[c], la[c]) - softplus2(-ph * itr[c]);
+            if (CHECK) {
+                arg = ph < p_b[c] ? -INFINITY : arg;
+                bad = bad || ph < p_inf[c];
+            }
+            sum += ex2(arg);
+        }
+        float f = sum + b;
+        return CHECK && bad ? INFINITY : f;
+    }
+};
+template <> struct Fast<8> : FastMB<1> {};
+template <> struct Fast<9> : FastMB<2> {};
+template <> struct Fast<10> : FastMB<3> {};
+template <> struct Fast<11> : FastMB<4> {};
 
 // Gaussian terms of 4, 2 and 1 detections for one particle: records are
 // {t log2 e, y, var, 0}; se2 = sigma_x^2.  CHECK = false drops the phase

@@ -67,6 +67,10 @@ hipError_t launch_select_best(const double* mp, const double* mc, const int32_t*
 hipError_t launch_multi_bazin(const MbArgs& a, bool fp64, int ppl, int grid, hipStream_t st) {
     const void* k = fp64 ? mb_kernel_f64(ppl) : mb_kernel_f32(ppl);
     if (!k || grid <= 0) return hipErrorInvalidValue;
+    if (!fp64) {
+        hipError_t e = pso_f32_prepare(st);
+        if (e != hipSuccess) return e;
+    }
     void* args[] = {const_cast<MbArgs*>(&a)};
     return hipLaunchKernel(k, dim3(grid), dim3(64), args, pso_lds_bytes(fp64, a.tile), st);
 }

@@ -45,9 +45,9 @@ __device__ __forceinline__ void stage_obs(typename ObsT<R>::T* lds,
 template <int MODEL, typename R, int PPL>
 struct Swarm {
     static constexpr int NP = ModelTraits<MODEL>::NP;
-    // FP32 swarms of models 0-6 use the detections-first LDS tile and the
-    // instruction-lean model forms of pso_fast32.hpp.
-    static constexpr bool FAST = sizeof(R) == 4 && MODEL <= 6;
+    // FP32 swarms of models 0-6 and MultiBazin use the detections-first LDS
+    // tile and the instruction-lean model forms of pso_fast32.hpp.
+    static constexpr bool FAST = sizeof(R) == 4 && (MODEL <= 6 || MODEL >= MB_MODEL0);
     using M = Math<R>;
     using O = typename ObsT<R>::T;
 
@@ -124,10 +124,12 @@ struct Swarm {
 #pragma unroll
         for (int k = 0; k < PPL; k++) {
             float pri = 0.0f;
+            if constexpr (MODEL < MB_MODEL0) {  // MultiBazin has no population prior
 #pragma unroll
-            for (int j = 0; j < NP; j++) {
-                float z = (pos[k][j] - (float)c_pc[MODEL][j]) * ipw[j];
-                pri = __builtin_fmaf(0.5f * z, z, pri);
+                for (int j = 0; j < NP; j++) {
+                    float z = (pos[k][j] - (float)c_pc[MODEL][j]) * ipw[j];
+                    pri = __builtin_fmaf(0.5f * z, z, pri);
+                }
             }
             float c = n > 0 ? acc[k].s * inv_n : M::big();
             c = __builtin_fmaf(pri * inv_n, inv_n, c);
@@ -238,7 +240,7 @@ struct Swarm {
                                         uint32_t rng_tag = 0u, int nd_staged = 0, float tmin_staged = 0.0f,
                                         float tmax_staged = 0.0f) {
         const int lane = lane_id();
-        if constexpr (FAST) {
+        if constexpr (FAST && MODEL < MB_MODEL0) {
 #pragma unroll
             for (int j = 0; j < NP; j++) {
                 const double w = c_pw[MODEL][j];
@@ -444,7 +446,7 @@ struct MbState {
 
 template <int K, typename R, int PPL>
 __device__ __forceinline__ void mb_fit_k(const MbArgs& a, typename ObsT<R>::T* lds, int band, int n,
-                                         MbState& st) {
+                                         MbState& st, int nd, float tmin, float tmax) {
     using O = typename ObsT<R>::T;
     constexpr int MODEL = MB_MODEL0 + K - 1;
     constexpr int NP = 4 * K + 2;
@@ -509,7 +511,7 @@ __device__ __forceinline__ void mb_fit_k(const MbArgs& a, typename ObsT<R>::T* l
     Swarm<MODEL, R, PPL> s;
     int iters = 0;
     s.run(lds, g, n, a.tile, (uint32_t)band, a.n_particles, a.max_iters, a.stall_iters, a.seed, 0u, iters,
-          lo, hi, K > 1 ? seedv : nullptr, 0x4D420000u + ((uint32_t)K << 8));
+          lo, hi, K > 1 ? seedv : nullptr, 0x4D420000u + ((uint32_t)K << 8), nd, tmin, tmax);
     double cost = (double)s.gcost;
     if (sizeof(R) == 4 && cost >= (double)Math<R>::big()) cost = 1e99;
     const double nn = (double)n;
@@ -534,7 +536,14 @@ __device__ void mb_band(const MbArgs& a, typename ObsT<R>::T* lds, int band) {
     const int lane = lane_id();
     const O* g = reinterpret_cast<const O*>(a.obs) + a.offsets[band];
     const int n = (int)(a.offsets[band + 1] - a.offsets[band]);
-    if (n <= a.tile) stage_obs<R>(lds, g, n);
+    int nd = 0;
+    float tmin = 0.0f, tmax = 0.0f;
+    if constexpr (sizeof(R) == 4) {  // the FAST cost loop's detections-first tile
+        if (n <= a.tile)
+            nd = f32::stage_split(reinterpret_cast<float4*>(lds), reinterpret_cast<const float4*>(g), n, tmin, tmax);
+    } else {
+        if (n <= a.tile) stage_obs<R>(lds, g, n);
+    }
     MbState st;
     st.best_bic = INFINITY;
     st.best_cost = INFINITY;
@@ -542,10 +551,10 @@ __device__ void mb_band(const MbArgs& a, typename ObsT<R>::T* lds, int band) {
     st.stopped = false;
 #pragma unroll
     for (int d = 0; d < MB_MAX_PARAMS; d++) { st.prev[d] = 0.0; st.best[d] = 0.0; }
-    mb_fit_k<1, R, PPL>(a, lds, band, n, st);
-    mb_fit_k<2, R, PPL>(a, lds, band, n, st);
-    mb_fit_k<3, R, PPL>(a, lds, band, n, st);
-    mb_fit_k<4, R, PPL>(a, lds, band, n, st);
+    mb_fit_k<1, R, PPL>(a, lds, band, n, st, nd, tmin, tmax);
+    mb_fit_k<2, R, PPL>(a, lds, band, n, st, nd, tmin, tmax);
+    mb_fit_k<3, R, PPL>(a, lds, band, n, st, nd, tmin, tmax);
+    mb_fit_k<4, R, PPL>(a, lds, band, n, st, nd, tmin, tmax);
     double prm = 0.0;  // lane d holds parameter d
 #pragma unroll
     for (int d = 0; d < MB_MAX_PARAMS; d++)
