@@ -3,6 +3,7 @@
 // censored-likelihood table they read.
 #include <mutex>
 
+#define LCF_PSO_F32_UNIT 1
 #include "pso_swarm.hpp"
 
 namespace lcf {

@@ -324,6 +324,10 @@ __device__ __forceinline__ float cost_one(const Fast<MODEL>& m, float se2, const
 constexpr int LNPHI_N = 304;
 constexpr float LNPHI_X0 = -30.0f;
 constexpr float LNPHI_SCALE = 8.0f;
+// One copy per translation unit (no device linking across units): the
+// table is filled only in pso_f32.hip, the one unit that instantiates the
+// FP32 swarm kernels (it defines LCF_PSO_F32_UNIT before including this
+// header); a unit without it may not reach the FAST cost loop.
 static __device__ float4 g_lnphi[LNPHI_N];
 
 // Censored term -ln Phi(x), x = (y_up - f) / sqrt(v), of an upper limit: the

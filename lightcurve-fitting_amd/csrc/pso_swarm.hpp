@@ -48,6 +48,9 @@ struct Swarm {
     // FP32 swarms of models 0-6 and MultiBazin use the detections-first LDS
     // tile and the instruction-lean model forms of pso_fast32.hpp.
     static constexpr bool FAST = sizeof(R) == 4 && (MODEL <= 6 || MODEL >= MB_MODEL0);
+#ifndef LCF_PSO_F32_UNIT
+    static_assert(!FAST, "FP32 swarms are instantiated in pso_f32.hip only (it builds their table)");
+#endif
     using M = Math<R>;
     using O = typename ObsT<R>::T;
 
