@@ -116,6 +116,40 @@ def test_fp32_cost_accuracy(ctx, orc, pkg, ragged):
             assert abs(got - want) <= 1e-4 * scale, (m, b, got, want)
 
 
+def test_fp32_phase_range_cases(ctx, orc, pkg):
+    """Bands over +-1000 d around the peak: observations far before t0 make
+    the reference's FP64 amplitude terms overflow (a non-finite prediction,
+    cost 1e99) or its logistics underflow to 0 for part of the swarm, and
+    phases above 709.78 d hit the exact late-time limits, so the FP32 cost
+    loop runs its range tests.  A wrongly finite cost there could win the
+    swarm; the FP32 cost reported for the returned parameters must equal the
+    FP64 oracle's at those parameters."""
+    rng = np.random.default_rng(17)
+    srcs = []
+    for b in range(12):
+        n = 80
+        t = np.sort(rng.uniform(-1000.0, 1000.0, n))
+        f = np.exp(-(t - 20.0) / 40.0) / (1.0 + np.exp(-(t - 20.0) / 4.0))
+        f = np.nan_to_num(f) + rng.normal(0.0, 0.02, n)
+        up = ((f / 0.02) < 3.0).astype(np.int32)
+        srcs.append({"times": t, "flux": f, "obs_var": np.full(n, 4e-4), "is_upper": up,
+                     "upper_flux": np.full(n, 0.06)})
+    csr = stack_sources(srcs)
+    data = pkg.GpuBatchData(ctx, csr=csr)
+    off = csr["offsets"]
+    for m in range(7):
+        out = ctx.fit_models(data, [m], 64, 40, 40, seed=11)
+        for b in range(off.size - 1):
+            a, e = off[b], off[b + 1]
+            want = orc.pso_cost(m, out["model_params"][0, b, :orc.n_params(m)],
+                                *(csr[k][a:e] for k in ("times", "flux", "obs_var", "is_upper", "upper_flux")))
+            got = out["model_cost"][0, b]
+            if want >= 1e99:
+                assert got >= 1e99, (m, b, got)
+                continue
+            assert abs(got - want) <= 1e-4 * max(abs(want), 1.0), (m, b, got, want)
+
+
 def test_fp32_selection_statistics(ctx, orc, pkg):
     csr = pkg.synthetic.ragged_selection_batch(60, seed=21, n_min=10, n_max=200)
     models = [0, 1, 2, 3, 4]
