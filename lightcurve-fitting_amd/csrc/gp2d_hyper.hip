@@ -269,16 +269,20 @@ __device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& 
     // the augmented row leaves the factor: [[L, 0], [0, 1]]
     for (int j = tid; j < n; j += NT) A[(size_t)n * ld + j] = 0.0;
     __syncthreads();
-    // diagonal blocks of X = L^-1: lane c solves L_II x = e_c
-    for (int I = 0; I < nb; I++) {
-        const int I0 = I * T;
+    // diagonal blocks of X = L^-1, two at a time (sm.a, sm.b): wave w of the
+    // pair's block, lane c solves L_II x = e_c
+    for (int I = 0; I < nb; I += 2) {
         __syncthreads();
         for (int e = tid; e < T * T; e += NT) {
             const int r = e >> 6, c = e & 63;
-            sm.a[e] = c <= r ? A[(size_t)(I0 + r) * ld + I0 + c] : 0.0;
+            sm.a[e] = c <= r ? A[(size_t)(I * T + r) * ld + I * T + c] : 0.0;
+            if (I + 1 < nb) sm.b[e] = c <= r ? A[(size_t)((I + 1) * T + r) * ld + (I + 1) * T + c] : 0.0;
         }
         __syncthreads();
-        if (tid < WAVE) {
+        const int w = tid >> 6;
+        if (w < 2 && I + w < nb) {
+            const double* L = w ? sm.b : sm.a;
+            const int I0 = (I + w) * T;
             // 16 rows at a time; x of the earlier chunks read back from Xm
             // in increasing q (the order of a full-column substitution)
             double* xc = Xm + (size_t)I0 * ld + I0 + lane;
@@ -289,14 +293,14 @@ __device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& 
                 for (int q = 0; q < rb; q++) {
                     const double xq = xc[(size_t)q * ld];
 #pragma unroll
-                    for (int j = 0; j < 16; j++) x[j] = fma(-sm.a[(rb + j) * T + q], xq, x[j]);
+                    for (int j = 0; j < 16; j++) x[j] = fma(-L[(rb + j) * T + q], xq, x[j]);
                 }
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
                     double acc = x[j];
 #pragma unroll
-                    for (int q = 0; q < j; q++) acc = fma(-sm.a[(rb + j) * T + rb + q], x[q], acc);
-                    x[j] = acc / sm.a[(rb + j) * T + rb + j];
+                    for (int q = 0; q < j; q++) acc = fma(-L[(rb + j) * T + rb + q], x[q], acc);
+                    x[j] = acc / L[(rb + j) * T + rb + j];
                 }
 #pragma unroll
                 for (int j = 0; j < 16; j++) xc[(size_t)(rb + j) * ld] = x[j];
