@@ -118,14 +118,33 @@ int lcf_batch_all_models(lcf_ctx* ctx, const lcf_batch* data, int n_particles,
  * source index in the counter, the velocity updates from one xoshiro128**
  * stream per (source, particle) seeded by a Philox block, so a source's fit
  * does not depend on the batch it is in (the reference's sequential
- * SmallRng stream is not reproduced).  The first FP32 call on a device builds
- * the FP32 cost loop's censored-likelihood table and synchronises the
- * context's stream once (not capturable into a graph); later calls do not. */
+ * SmallRng stream is not reproduced).  The FP32 cost loop's
+ * censored-likelihood table is built once per device by lcf_ctx_create. */
 int lcf_batch_fit_models(lcf_ctx* ctx, const lcf_batch* data, const int32_t* models,
                          int n_models, int use_gate, double gate, int n_particles,
                          int max_iters, int stall_iters, uint64_t seed, uint32_t flags,
                          double* model_params, double* model_cost, int32_t* model_iters,
                          int32_t* best_model, double* best_params, double* best_cost);
+
+/* ---- batched fit without device allocation ------------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory inside the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): lcf_batch_fit_models with the same
+ * arguments, outputs and results, but its device intermediates (per-model
+ * results, best-model outputs, model ids, the work-queue counter) live in one
+ * fixed scratch array per device (LCF_NOALLOC_SCRATCH_BYTES, held until the
+ * process ends) instead of being allocated per call; the counter is cleared
+ * by a kernel, not a memset.  Batches of more than LCF_NOALLOC_MAX_BANDS
+ * sources are rejected with LCF_E_INVALID.  Every call on one device shares
+ * that device's array: calls that overlap on a device (several contexts or
+ * threads) race and must be serialised by the caller. */
+#define LCF_NOALLOC_MAX_BANDS 16384
+#define LCF_NOALLOC_SCRATCH_BYTES 10027520
+int lcf_batch_fit_models_noalloc(lcf_ctx* ctx, const lcf_batch* data, const int32_t* models,
+                                 int n_models, int use_gate, double gate, int n_particles,
+                                 int max_iters, int stall_iters, uint64_t seed, uint32_t flags,
+                                 double* model_params, double* model_cost, int32_t* model_iters,
+                                 int32_t* best_model, double* best_params, double* best_cost);
 
 /* ---- greedy MultiBazin ---------------------------------------------------- */
 /* GpuContext::batch_pso_multi_bazin (src/gpu_cuda.rs:650-963): for K = 1..4

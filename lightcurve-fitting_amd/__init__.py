@@ -10,6 +10,7 @@ from .gpu import (  # noqa: F401
     FP32,
     FP64,
     MAX_PARAMS,
+    NOALLOC_MAX_BANDS,
     PSO_STALL_FROM_FIRST_EVAL,
     SCHED_PER_CLASS,
     SCHED_PERSISTENT,

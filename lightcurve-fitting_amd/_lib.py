@@ -37,6 +37,8 @@ SIGNATURES = {
     "lcf_batch_all_models": (_i32, [_vp, _vp, _i32, _i32, _i32, _pd, _pd]),
     "lcf_batch_fit_models": (_i32, [_vp, _vp, _pi32, _i32, _i32, _f64, _i32, _i32, _i32, _u64,
                                     _u32, _pd, _pd, _pi32, _pi32, _pd, _pd]),
+    "lcf_batch_fit_models_noalloc": (_i32, [_vp, _vp, _pi32, _i32, _i32, _f64, _i32, _i32, _i32, _u64,
+                                            _u32, _pd, _pd, _pi32, _pi32, _pd, _pd]),
     "lcf_batch_multi_bazin": (_i32, [_vp, _vp, _i32, _i32, _i32, _u64, _pi32, _pd, _pd, _pd, _pd, _pd]),
     "lcf_batch_bb_temperature": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _pi32]),
     "lcf_batch_thermal": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _i32,

@@ -9,7 +9,6 @@ const void* pso_kernel_f32(int model, int ppl, bool persistent);
 const void* pso_kernel_f64(int model, int ppl, bool persistent);
 const void* mb_kernel_f32(int ppl);
 const void* mb_kernel_f64(int ppl);
-hipError_t pso_f32_prepare(hipStream_t st);
 
 static const void* pick_kernel(int model, bool fp64, int ppl, bool persistent) {
     return fp64 ? pso_kernel_f64(model, ppl, persistent) : pso_kernel_f32(model, ppl, persistent);

@@ -56,6 +56,10 @@ size_t pso_lds_bytes(bool fp64, int tile);
 hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool persistent, int grid,
                       hipStream_t st);
 hipError_t pso_occupancy(int model, bool fp64, int ppl, int tile, int* blocks_per_cu);
+// Builds the FP32 cost loop's censored-likelihood table on the current device
+// the first time it is called there (one launch, then a wait on `st`); later
+// calls only check a flag.
+hipError_t pso_f32_prepare(hipStream_t st);
 hipError_t launch_select_best(const double* mp, const double* mc, const int32_t* models, int n_models,
                               int n_bands, int32_t* best_model, double* best_params,
                               double* best_cost, hipStream_t st);

@@ -31,6 +31,7 @@ struct lcf_ctx {
     int* d_queue = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t ev_slot[LCF_N_MODELS + 1] = {};  // boundaries of the per-model launches
+    char* scratch = nullptr;  // this device's copy of the fixed scratch array (lcf_batch_fit_models_noalloc)
     double last_ms = 0.0;
     double slot_ms[LCF_N_MODELS] = {};
 };
@@ -77,6 +78,39 @@ struct DevBuf {
     hipError_t alloc(size_t n) { return hipMalloc((void**)&p, std::max<size_t>(n, 1) * sizeof(T)); }
 };
 
+// Fixed scratch of lcf_batch_fit_models_noalloc: every device that runs the
+// library holds one copy of this array until the process ends.  Host code
+// never uses the symbol itself as an address: lcf_ctx_create resolves each
+// device's copy with hipGetSymbolAddress (with that device current) and
+// keeps it in the context.  Layout (256-byte aligned regions):
+//   [SCR_QUEUE]  persistent-launch queue counter
+//   [SCR_MODELS] model ids of the call
+//   [SCR_MP]     per-model params [n_models][n_bands][7] f64, then costs
+//                [n_models][n_bands] f64, then iterations [n_models][n_bands] i32,
+//                at fixed offsets for LCF_N_MODELS x LCF_NOALLOC_MAX_BANDS
+//   [SCR_BEST]   best params [n_bands][7] f64, best cost f64, best model i32
+// All calls on one device share it; calls that overlap on a device race.
+constexpr size_t SCR_QUEUE = 0;
+constexpr size_t SCR_MODELS = 256;
+constexpr size_t SCR_MP = 512;
+constexpr size_t SCR_MP_BYTES = (size_t)LCF_N_MODELS * LCF_NOALLOC_MAX_BANDS * (7 * 8 + 8 + 4);
+constexpr size_t SCR_BEST = SCR_MP + ((SCR_MP_BYTES + 255) / 256) * 256;
+constexpr size_t SCR_BYTES = SCR_BEST + (size_t)LCF_NOALLOC_MAX_BANDS * (7 * 8 + 8 + 4);
+static_assert(SCR_BYTES == LCF_NOALLOC_SCRATCH_BYTES, "scratch size in the header is out of date");
+static __device__ __align__(256) char scratch[SCR_BYTES];
+
+__global__ void zero_ints(int* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Clears n counters with a kernel: a capturable kernel node instead of a
+// memset node.
+hipError_t launch_zero_ints(int* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(zero_ints, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
 const int N_PARAMS[LCF_N_MODELS] = {6, 7, 7, 5, 5, 5, 6, 5};
 // ALL_GPU_MODELS (src/gpu_types.rs:107-116)
 const int32_t ALL_MODELS[LCF_N_MODELS] = {0, 3, 2, 6, 1, 4, 5, 7};
@@ -90,6 +124,139 @@ __global__ void pack_obs(const double* __restrict__ t, const double* __restrict_
     double y = u ? uf[i] : f[i];
     o64[i] = make_double4(t[i], y, v[i], u ? 1.0 : 0.0);
     o32[i] = make_float4((float)t[i], (float)y, (float)v[i], u ? 1.0f : 0.0f);
+}
+
+// Device buffers of one batched fit: per-model results, model ids, best
+// model outputs, and the persistent-launch queue counter.
+struct FitBufs {
+    double* mp;
+    double* mc;
+    int32_t* mi;
+    int32_t* models;
+    double* bp;
+    double* bc;
+    int32_t* bm;
+    int* queue;
+    bool fill_queue;  // zero the counter with a kernel (no memset node) instead of hipMemsetAsync
+};
+
+int check_fit_args(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models, int P, int max_iters,
+                   int stall_iters) {
+    if (!c) return LCF_E_INVALID;
+    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!models || n_models <= 0 || n_models > LCF_N_MODELS)
+        return fail(c, LCF_E_INVALID, "n_models must be in [1, %d]", LCF_N_MODELS);
+    for (int i = 0; i < n_models; i++)
+        if (models[i] < 0 || models[i] >= LCF_N_MODELS)
+            return fail(c, LCF_E_INVALID, "unknown model id %d", models[i]);
+    if (P <= 0 || P > 4 * 64) return fail(c, LCF_E_INVALID, "n_particles must be in [1, 256], got %d", P);
+    if (max_iters <= 0) return fail(c, LCF_E_INVALID, "max_iters must be positive");
+    if (stall_iters < 0) return fail(c, LCF_E_INVALID, "stall_iters must be non-negative");
+    return LCF_OK;
+}
+
+// Launches, times and downloads one lcf_batch_fit_models call into the
+// device buffers `d` (arguments already checked, batch non-empty).
+int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models, int use_gate,
+                    double gate, int P, int max_iters, int stall_iters, uint64_t seed, uint32_t flags,
+                    const FitBufs& d, double* model_params, double* model_cost, int32_t* model_iters,
+                    int32_t* best_model, double* best_params, double* best_cost) {
+    const bool fp64 = c->precision == LCF_FP64;
+    const int ppl = P <= 64 ? 1 : (P <= 128 ? 2 : 4);
+    // LDS tile: the whole band when it fits 16 KiB per wave, else 16 KiB tiles
+    // re-staged per iteration.
+    const int rec = fp64 ? (int)sizeof(double4) : (int)sizeof(float4);
+    const int tile_cap = 16384 / rec;
+    const int tile = std::max(1, std::min(tile_cap, std::max(b->max_n, 1)));
+
+    const int64_t ns = b->n_sources, nm = n_models;
+    HIPCHK(c, hipMemcpyAsync(d.models, models, sizeof(int32_t) * nm, hipMemcpyHostToDevice, c->stream));
+
+    lcf::PsoArgs a{};
+    a.obs = fp64 ? (const void*)b->d_obs64 : (const void*)b->d_obs32;
+    a.offsets = b->d_off;
+    a.order = b->d_order;
+    a.n_bands = (int)ns;
+    a.queue = d.queue;
+    a.gate = gate;
+    a.n_particles = P;
+    a.max_iters = max_iters;
+    a.stall_iters = stall_iters;
+    a.seed = seed;
+    a.flags = flags;
+    a.tile = tile;
+    a.model_params = d.mp;
+    a.model_cost = d.mc;
+    a.model_iters = d.mi;
+
+    // the FP32 cost loop's table is built at context creation; this only
+    // confirms it (no launch, no synchronisation inside the timed region)
+    if (!fp64) HIPCHK(c, lcf::pso_f32_prepare(c->stream));
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (int mi = 0; mi < n_models; mi++) {
+        const int model = models[mi];
+        a.slot = mi;
+        HIPCHK(c, hipEventRecord(c->ev_slot[mi], c->stream));
+        a.gate_cost = (use_gate && mi > 0) ? d.mc : nullptr;
+        if (c->schedule == LCF_SCHED_PERSISTENT) {
+            int per_cu = 0;
+            HIPCHK(c, lcf::pso_occupancy(model, fp64, ppl, tile, &per_cu));
+            per_cu = std::max(per_cu, 1);
+            a.task_base = 0;
+            a.n_tasks = (int)ns;
+            int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, ns);
+            if (d.fill_queue)
+                HIPCHK(c, launch_zero_ints(d.queue, 1, c->stream));
+            else
+                HIPCHK(c, hipMemsetAsync(d.queue, 0, sizeof(int), c->stream));
+            HIPCHK(c, lcf::launch_pso(model, a, fp64, ppl, true, grid, c->stream));
+        } else {
+            // One synchronised launch per power-of-two size class, in
+            // decreasing size (the band order is already size-sorted).
+            int64_t i = 0;
+            while (i < ns) {
+                int64_t s0 = b->h_order[i];
+                int64_t n0 = b->h_off[s0 + 1] - b->h_off[s0];
+                int64_t cls = 1;
+                while (cls < n0) cls <<= 1;
+                int64_t j = i;
+                while (j < ns) {
+                    int64_t sj = b->h_order[j];
+                    int64_t nj = b->h_off[sj + 1] - b->h_off[sj];
+                    if (cls > 1 && nj * 2 <= cls) break;
+                    j++;
+                }
+                a.task_base = (int)i;
+                a.n_tasks = (int)j;
+                HIPCHK(c, lcf::launch_pso(model, a, fp64, ppl, false, (int)(j - i), c->stream));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+                i = j;
+            }
+        }
+    }
+    HIPCHK(c, hipEventRecord(c->ev_slot[n_models], c->stream));
+    if (best_model || best_params || best_cost)
+        HIPCHK(c, lcf::launch_select_best(d.mp, d.mc, d.models, n_models, (int)ns, d.bm, d.bp,
+                                          d.bc, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    for (int mi = 0; mi < LCF_N_MODELS; mi++) c->slot_ms[mi] = 0.0;
+    for (int mi = 0; mi < n_models; mi++) {
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_slot[mi], c->ev_slot[mi + 1]));
+        c->slot_ms[mi] = ms;
+    }
+
+    if (model_params) HIPCHK(c, hipMemcpyAsync(model_params, d.mp, sizeof(double) * nm * ns * 7, hipMemcpyDeviceToHost, c->stream));
+    if (model_cost) HIPCHK(c, hipMemcpyAsync(model_cost, d.mc, sizeof(double) * nm * ns, hipMemcpyDeviceToHost, c->stream));
+    if (model_iters) HIPCHK(c, hipMemcpyAsync(model_iters, d.mi, sizeof(int32_t) * nm * ns, hipMemcpyDeviceToHost, c->stream));
+    if (best_params) HIPCHK(c, hipMemcpyAsync(best_params, d.bp, sizeof(double) * ns * 7, hipMemcpyDeviceToHost, c->stream));
+    if (best_cost) HIPCHK(c, hipMemcpyAsync(best_cost, d.bc, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
+    if (best_model) HIPCHK(c, hipMemcpyAsync(best_model, d.bm, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LCF_OK;
 }
 
 }  // namespace
@@ -118,6 +285,20 @@ int lcf_ctx_create(int device, lcf_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
     if (c->n_cu <= 0) c->n_cu = 256;
+    // this device's copy of the fixed scratch array, resolved with the device
+    // current; then the FP32 cost loop's one-time table, so that no fit call
+    // builds it inside its timed region
+    void* sp = nullptr;
+    e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !sp) {
+        lcf_ctx_destroy(c);
+        return LCF_E_HIP;
+    }
+    c->scratch = (char*)sp;
+    if (lcf::pso_f32_prepare(c->stream) != hipSuccess) {
+        lcf_ctx_destroy(c);
+        return LCF_E_HIP;
+    }
     *out = c;
     return LCF_OK;
 }
@@ -291,29 +472,10 @@ int lcf_batch_fit_models(lcf_ctx* c, const lcf_batch* b, const int32_t* models, 
                          uint64_t seed, uint32_t flags, double* model_params, double* model_cost,
                          int32_t* model_iters, int32_t* best_model, double* best_params,
                          double* best_cost) {
-    if (!c) return LCF_E_INVALID;
-    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
-    if (!models || n_models <= 0 || n_models > LCF_N_MODELS)
-        return fail(c, LCF_E_INVALID, "n_models must be in [1, %d]", LCF_N_MODELS);
-    for (int i = 0; i < n_models; i++)
-        if (models[i] < 0 || models[i] >= LCF_N_MODELS)
-            return fail(c, LCF_E_INVALID, "unknown model id %d", models[i]);
-    if (P <= 0 || P > 4 * 64) return fail(c, LCF_E_INVALID, "n_particles must be in [1, 256], got %d", P);
-    if (max_iters <= 0) return fail(c, LCF_E_INVALID, "max_iters must be positive");
-    if (stall_iters < 0) return fail(c, LCF_E_INVALID, "stall_iters must be non-negative");
-    const int64_t ns = b->n_sources;
-    if (ns == 0) return LCF_OK;
+    int rc = check_fit_args(c, b, models, n_models, P, max_iters, stall_iters);
+    if (rc != LCF_OK || b->n_sources == 0) return rc;
+    const int64_t ns = b->n_sources, nm = n_models;
     HIPCHK(c, hipSetDevice(c->device));
-
-    const bool fp64 = c->precision == LCF_FP64;
-    const int ppl = P <= 64 ? 1 : (P <= 128 ? 2 : 4);
-    // LDS tile: the whole band when it fits 16 KiB per wave, else 16 KiB tiles
-    // re-staged per iteration.
-    const int rec = fp64 ? (int)sizeof(double4) : (int)sizeof(float4);
-    const int tile_cap = 16384 / rec;
-    const int tile = std::max(1, std::min(tile_cap, std::max(b->max_n, 1)));
-
-    const int64_t nm = n_models;
     DevBuf<double> d_mp, d_mc, d_bp, d_bc;
     DevBuf<int32_t> d_mi, d_bm, d_models;
     HIPCHK(c, d_mp.alloc(nm * ns * 7));
@@ -323,87 +485,37 @@ int lcf_batch_fit_models(lcf_ctx* c, const lcf_batch* b, const int32_t* models, 
     if (best_params) HIPCHK(c, d_bp.alloc(ns * 7));
     if (best_cost) HIPCHK(c, d_bc.alloc(ns));
     if (best_model) HIPCHK(c, d_bm.alloc(ns));
-    HIPCHK(c, hipMemcpyAsync(d_models.p, models, sizeof(int32_t) * nm, hipMemcpyHostToDevice, c->stream));
+    FitBufs d{d_mp.p, d_mc.p, d_mi.p, d_models.p, d_bp.p, d_bc.p, d_bm.p, c->d_queue, false};
+    return fit_models_core(c, b, models, n_models, use_gate, gate, P, max_iters, stall_iters, seed, flags, d,
+                           model_params, model_cost, model_iters, best_model, best_params, best_cost);
+}
 
-    lcf::PsoArgs a{};
-    a.obs = fp64 ? (const void*)b->d_obs64 : (const void*)b->d_obs32;
-    a.offsets = b->d_off;
-    a.order = b->d_order;
-    a.n_bands = (int)ns;
-    a.queue = c->d_queue;
-    a.gate = gate;
-    a.n_particles = P;
-    a.max_iters = max_iters;
-    a.stall_iters = stall_iters;
-    a.seed = seed;
-    a.flags = flags;
-    a.tile = tile;
-    a.model_params = d_mp.p;
-    a.model_cost = d_mc.p;
-    a.model_iters = d_mi.p;
-
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int mi = 0; mi < n_models; mi++) {
-        const int model = models[mi];
-        a.slot = mi;
-        HIPCHK(c, hipEventRecord(c->ev_slot[mi], c->stream));
-        a.gate_cost = (use_gate && mi > 0) ? d_mc.p : nullptr;
-        if (c->schedule == LCF_SCHED_PERSISTENT) {
-            int per_cu = 0;
-            HIPCHK(c, lcf::pso_occupancy(model, fp64, ppl, tile, &per_cu));
-            per_cu = std::max(per_cu, 1);
-            a.task_base = 0;
-            a.n_tasks = (int)ns;
-            int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, ns);
-            HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
-            HIPCHK(c, lcf::launch_pso(model, a, fp64, ppl, true, grid, c->stream));
-        } else {
-            // One synchronised launch per power-of-two size class, in
-            // decreasing size (the band order is already size-sorted).
-            int64_t i = 0;
-            while (i < ns) {
-                int64_t s0 = b->h_order[i];
-                int64_t n0 = b->h_off[s0 + 1] - b->h_off[s0];
-                int64_t cls = 1;
-                while (cls < n0) cls <<= 1;
-                int64_t j = i;
-                while (j < ns) {
-                    int64_t sj = b->h_order[j];
-                    int64_t nj = b->h_off[sj + 1] - b->h_off[sj];
-                    if (cls > 1 && nj * 2 <= cls) break;
-                    j++;
-                }
-                a.task_base = (int)i;
-                a.n_tasks = (int)j;
-                HIPCHK(c, lcf::launch_pso(model, a, fp64, ppl, false, (int)(j - i), c->stream));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
-                i = j;
-            }
-        }
-    }
-    HIPCHK(c, hipEventRecord(c->ev_slot[n_models], c->stream));
-    if (best_model || best_params || best_cost)
-        HIPCHK(c, lcf::launch_select_best(d_mp.p, d_mc.p, d_models.p, n_models, (int)ns, d_bm.p, d_bp.p,
-                                          d_bc.p, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-    HIPCHK(c, hipEventSynchronize(c->ev1));
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_ms = ms;
-    for (int mi = 0; mi < LCF_N_MODELS; mi++) c->slot_ms[mi] = 0.0;
-    for (int mi = 0; mi < n_models; mi++) {
-        HIPCHK(c, hipEventElapsedTime(&ms, c->ev_slot[mi], c->ev_slot[mi + 1]));
-        c->slot_ms[mi] = ms;
-    }
-
-    if (model_params) HIPCHK(c, hipMemcpyAsync(model_params, d_mp.p, sizeof(double) * nm * ns * 7, hipMemcpyDeviceToHost, c->stream));
-    if (model_cost) HIPCHK(c, hipMemcpyAsync(model_cost, d_mc.p, sizeof(double) * nm * ns, hipMemcpyDeviceToHost, c->stream));
-    if (model_iters) HIPCHK(c, hipMemcpyAsync(model_iters, d_mi.p, sizeof(int32_t) * nm * ns, hipMemcpyDeviceToHost, c->stream));
-    if (best_params) HIPCHK(c, hipMemcpyAsync(best_params, d_bp.p, sizeof(double) * ns * 7, hipMemcpyDeviceToHost, c->stream));
-    if (best_cost) HIPCHK(c, hipMemcpyAsync(best_cost, d_bc.p, sizeof(double) * ns, hipMemcpyDeviceToHost, c->stream));
-    if (best_model) HIPCHK(c, hipMemcpyAsync(best_model, d_bm.p, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    return LCF_OK;
+int lcf_batch_fit_models_noalloc(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models,
+                                 int use_gate, double gate, int P, int max_iters, int stall_iters,
+                                 uint64_t seed, uint32_t flags, double* model_params, double* model_cost,
+                                 int32_t* model_iters, int32_t* best_model, double* best_params,
+                                 double* best_cost) {
+    int rc = check_fit_args(c, b, models, n_models, P, max_iters, stall_iters);
+    if (rc != LCF_OK || b->n_sources == 0) return rc;
+    if (b->n_sources > LCF_NOALLOC_MAX_BANDS)
+        return fail(c, LCF_E_INVALID, "lcf_batch_fit_models_noalloc: %lld sources exceed the fixed scratch "
+                    "capacity of %d (use lcf_batch_fit_models)", (long long)b->n_sources, LCF_NOALLOC_MAX_BANDS);
+    if (!c->scratch) return fail(c, LCF_E_INTERNAL, "scratch address of device %d not resolved", c->device);
+    HIPCHK(c, hipSetDevice(c->device));
+    char* s = c->scratch;
+    const int64_t cap = LCF_NOALLOC_MAX_BANDS;
+    FitBufs d{};
+    d.queue = (int*)(s + SCR_QUEUE);
+    d.models = (int32_t*)(s + SCR_MODELS);
+    d.mp = (double*)(s + SCR_MP);
+    d.mc = (double*)(s + SCR_MP + sizeof(double) * LCF_N_MODELS * cap * 7);
+    d.mi = (int32_t*)(s + SCR_MP + sizeof(double) * LCF_N_MODELS * cap * 8);
+    d.bp = (double*)(s + SCR_BEST);
+    d.bc = (double*)(s + SCR_BEST + sizeof(double) * cap * 7);
+    d.bm = (int32_t*)(s + SCR_BEST + sizeof(double) * cap * 8);
+    d.fill_queue = true;
+    return fit_models_core(c, b, models, n_models, use_gate, gate, P, max_iters, stall_iters, seed, flags, d,
+                           model_params, model_cost, model_iters, best_model, best_params, best_cost);
 }
 
 int lcf_batch_multi_bazin(lcf_ctx* c, const lcf_batch* b, int P, int max_iters, int stall_iters,

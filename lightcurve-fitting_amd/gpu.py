@@ -21,6 +21,7 @@ MAX_PARAMS = 7
 NP_NFEAT = 50  # NonparametricBandResult fields without the band name
 FP32, FP64 = 0, 1
 SCHED_PERSISTENT, SCHED_PER_CLASS = 0, 1
+NOALLOC_MAX_BANDS = 16384  # LCF_NOALLOC_MAX_BANDS
 PSO_STALL_FROM_FIRST_EVAL = 1
 FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2 = 1, 2, 4
 FINALIZE_DEFAULT = FINALIZE_T0 | FINALIZE_ALL | FINALIZE_MAG_CHI2
@@ -189,9 +190,11 @@ class GpuContext:
     # -- batched PSO ----------------------------------------------------------
     def fit_models(self, data: "GpuBatchData", models: Sequence[int], n_particles: int,
                    max_iters: int, stall_iters: int, *, seed: int = 42, gate: float | None = None,
-                   flags: int = 0, per_model: bool = True) -> dict:
+                   flags: int = 0, per_model: bool = True, noalloc: bool = False) -> dict:
         """Outputs of ``lcf_batch_fit_models`` as numpy arrays (per-model
-        arrays are None when ``per_model`` is false)."""
+        arrays are None when ``per_model`` is false); ``noalloc`` calls
+        ``lcf_batch_fit_models_noalloc`` (fixed per-device scratch, at most
+        ``NOALLOC_MAX_BANDS`` sources)."""
         ms = np.ascontiguousarray(np.asarray([int(m) for m in models], dtype=np.int32))
         ns, nm = data.n_sources, ms.size
         out = {
@@ -202,7 +205,8 @@ class GpuContext:
             "best_params": np.zeros((ns, MAX_PARAMS)),
             "best_cost": np.zeros(ns),
         }
-        rc = self._lib.lcf_batch_fit_models(
+        fn = self._lib.lcf_batch_fit_models_noalloc if noalloc else self._lib.lcf_batch_fit_models
+        rc = fn(
             self._h, data._h, _lib.ptr(ms, C.c_int32), nm, 0 if gate is None else 1,
             0.0 if gate is None else float(gate), n_particles, max_iters, stall_iters,
             C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF), flags,
