@@ -43,15 +43,24 @@ PKG_DIR = ROOT / "lightcurve-fitting_amd"
 METRIC = "sources/sec, batched+adjoint vs plain GPU fit at equal accuracy, 100k src"
 MODELS = (0, 1, 2, 3, 4)
 MODEL_NAMES = {0: "Bazin", 1: "Villar", 2: "Tde", 3: "Arnett", 4: "Magnetar"}
-# FP32 flop-equivalents per model-observation evaluation (MOE: prediction,
-# likelihood term and its share of the Kahan update), counted from the
-# compiled check-free cost loops of pso_kernel<MODEL, fp32> (four
-# observations per loop trip, /4): an FMA counts 2, a transcendental
-# (exp2/log2/rcp/rsq) 4 (it issues at half rate on gfx950: 2 slots x 2), any
-# other VALU instruction 1.  Detections and upper limits (censored term)
-# differ; the per-iteration swarm update is not counted.
+# Algorithmic FP32 flop-equivalents per model-observation evaluation (MOE:
+# prediction, likelihood term and its share of the Kahan update): the fixed
+# per-model operation counts of the model formulas (Bazin 12 ops + 5
+# transcendentals, Villar 20 + 6, TDE 18 + 7, Arnett 18 + 6, Magnetar 15 + 5,
+# including the NLL's 6 ops + log + rcp and the Kahan update's 3), a
+# transcendental counted as 8 flop-equivalents (quarter-rate issue x FMA = 2).
+# They do not depend on how the kernel is compiled.
+FLOPEQ_ALGO = {0: 12 + 5 * 8, 1: 20 + 6 * 8, 2: 18 + 7 * 8, 3: 18 + 6 * 8, 4: 15 + 5 * 8}
+# Issued-instruction flop-equivalents of the shipped check-free cost loops of
+# pso_kernel<MODEL, fp32> (four observations per loop trip, /4): an FMA 2, a
+# transcendental 4 (two issue slots x 2), any other VALU instruction 1;
+# detections and upper limits (censored term) differ.  Reported beside the
+# algorithmic figure as the kernel's own issue rate.
 FLOPEQ_DET = {0: 34.5, 1: 53.5, 2: 49.5, 3: 49.5, 4: 41.5}
 FLOPEQ_UP = {0: 41.0, 1: 60.0, 2: 56.0, 3: 56.0, 4: 48.0}
+# SVI flop-equivalents per (sample, observation, step): the Villar forward
+# evaluation (20 ops + 6 transcendentals x 8) and its adjoint (about 45 ops).
+SVI_FLOPEQ_FWD, SVI_FLOPEQ_ADJ = 20 + 6 * 8, 45
 FP32_PEAK_TFLOPS = 157.3
 # HBM traffic per launch of each model kernel on the default 100k-source
 # workload, from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB
@@ -80,6 +89,60 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_threads() -> int:
+    """Host threads of the CPU baselines: every core the process may use, at
+    most 16 (the host share that comes with one GPU on the measurement
+    machines; os.cpu_count() is reported beside it)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def pso_accuracy(pkg, csr: dict, fp32_out: dict, n_src: int, particles: int, iters: int) -> dict:
+    """Accuracy of the timed FP32 fit against FP64 on the bands of the first
+    n_src sources of the workload: the FP64 kernels (which reproduce the
+    oracle's swarm exactly) on those bands, seed 42 as in the timed pass and
+    seed 43 for the method's own seed-to-seed spread.  A band's swarm depends
+    only on its data, its index and the seed, so the first bands of the batch
+    fitted alone follow the same streams as in the full batch."""
+    nb = int(np.searchsorted(csr["band_source"], n_src))
+    sub = pkg.synthetic.subset(csr, np.arange(nb))
+    c64 = pkg.GpuContext(0, precision=pkg.FP64)
+    d64 = pkg.GpuBatchData(c64, csr=sub)
+    r64 = c64.fit_models(d64, MODELS, particles, iters, iters, seed=42)
+    r64b = c64.fit_models(d64, MODELS, particles, iters, iters, seed=43)
+    d64.close()
+    c64.close()
+    valid = np.diff(sub["offsets"]) > 0
+    bm32, b32 = fp32_out["best_model"][:nb], fp32_out["best_cost"][:nb]
+    gap = ((b32 - r64["best_cost"]) / np.maximum(np.abs(r64["best_cost"]), 1.0))[valid]
+    same = (bm32 == r64["best_model"]) & valid
+    prel = []
+    for j in np.nonzero(same)[0]:
+        i = MODELS.index(int(r64["best_model"][j]))
+        k = pkg.GpuModelName(int(r64["best_model"][j])).n_params()
+        a, b = fp32_out["model_params"][i, j, :k], r64["best_params"][j, :k]
+        prel.append(float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0))))
+    prel = np.asarray(prel)
+    q = lambda x, qs: [float(np.quantile(x, v)) for v in qs]
+    out = {"reference": "FP64 kernels (the oracle's swarm), same seed, same bands", "sources": n_src,
+           "bands": int(valid.sum()),
+           "best_model_agreement": round(float((bm32 == r64["best_model"])[valid].mean()), 5),
+           "fp64_seed_to_seed_agreement": round(float((r64b["best_model"] == r64["best_model"])[valid].mean()), 5),
+           "best_cost_rel_gap_quantiles": {"q05": None, "q50": None, "q95": None},
+           "frac_fp32_worse_1e-4": round(float((gap > 1e-4).mean()), 5),
+           "frac_fp32_better_1e-4": round(float((gap < -1e-4).mean()), 5),
+           "frac_cost_within_1e-4": round(float((np.abs(gap) <= 1e-4).mean()), 5),
+           "param_rel_err_quantiles_same_model": dict(zip(("q50", "q90", "q99"), q(prel, (0.5, 0.9, 0.99)))),
+           "note": "fp64_seed_to_seed_agreement is the swarm's own spread at this budget"}
+    out["best_cost_rel_gap_quantiles"] = dict(zip(("q05", "q50", "q95"), q(gap, (0.05, 0.5, 0.95))))
+    log(f"accuracy: agreement {out['best_model_agreement']:.4f} (fp64 seed spread "
+        f"{out['fp64_seed_to_seed_agreement']:.4f}) on {out['bands']} bands")
+    return out
+
+
 def cpu_baseline(pkg, n_sources: int, particles: int, iters: int, threads: int) -> dict:
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle
@@ -95,10 +158,14 @@ def cpu_baseline(pkg, n_sources: int, particles: int, iters: int, threads: int) 
 
 def svi_leg(pkg, n_sources: int, n_steps: int, n_samples: int, lr: float, reps: int) -> dict:
     """Villar SVI workload: n_sources x 3 bands x 60 points, PSO start points
-    (untimed), then the SVI fit timed twice over: the batched adjoint form
-    (analytic forward + adjoint in one kernel, one persistent launch) and the
-    plain form (forward-difference gradients, one launch per power-of-two size
-    class).  Agreement of the two posteriors is reported beside the times."""
+    (untimed), then the SVI fit at all n_steps Adam steps (no early stop),
+    timed twice over: the batched adjoint form (analytic forward + adjoint in
+    one kernel, one persistent launch) and the plain form (forward-difference
+    gradients, one launch per power-of-two size class).  The reference's
+    early-stopped fit is timed beside them.  Agreement of the two posteriors
+    is reported with the times; the roofline counts the forward + adjoint
+    flop-equivalents of every (sample, observation, step)."""
+    FIXED, FD, EXACT = 8, 4, 16
     csr = pkg.synthetic.villar_svi_batch(n_sources, seed=77)
     nj = csr["offsets"].size - 1
     ctx = pkg.GpuContext(0, precision=pkg.FP32, schedule=pkg.SCHED_PERSISTENT)
@@ -120,22 +187,27 @@ def svi_leg(pkg, n_sources: int, n_steps: int, n_samples: int, lr: float, reps: 
         ctx.synchronize()
         return out, time.perf_counter() - t0, ctx.last_kernel_ms
 
-    run(0, pkg.SCHED_PERSISTENT)  # warm-up
+    run(FIXED | EXACT, pkg.SCHED_PERSISTENT)  # warm-up
     best = None
     for _ in range(reps):
-        r = run(0, pkg.SCHED_PERSISTENT)
+        r = run(FIXED | EXACT, pkg.SCHED_PERSISTENT)
         best = r if best is None or r[1] < best[1] else best
     adj, dt_adj, kms_adj = best
-    log(f"svi adjoint: {dt_adj * 1e3:.1f} ms")
-    plain, dt_plain, kms_plain = run(4, pkg.SCHED_PER_CLASS)
-    ctx.set_schedule(pkg.SCHED_PERSISTENT)
+    log(f"svi adjoint ({n_steps} steps): {dt_adj * 1e3:.1f} ms")
+    plain, dt_plain, kms_plain = run(FIXED | EXACT | FD, pkg.SCHED_PER_CLASS)
     log(f"svi plain: {dt_plain * 1e3:.1f} ms")
+    early, dt_early, kms_early = run(EXACT, pkg.SCHED_PERSISTENT)
+    ctx.set_schedule(pkg.SCHED_PERSISTENT)
+    log(f"svi early-stopped: {dt_early * 1e3:.1f} ms, mean {early['steps'].mean():.1f} steps")
     sig = np.exp(plain["log_sigma"][:, :7])
     dmu = np.abs(adj["mu"][:, :7] - plain["mu"][:, :7]) / sig
     dsg = np.abs(np.exp(adj["log_sigma"][:, :7]) - sig) / sig
     evals = float(adj["steps"].astype(np.float64).sum() * n_samples * 60)
+    flops = evals * (SVI_FLOPEQ_FWD + SVI_FLOPEQ_ADJ)
+    achieved = flops / (kms_adj * 1e-3) / 1e12
     return {"workload": "villar_svi", "sources": n_sources, "bands": nj, "points_per_band": 60,
-            "samples": n_samples, "max_steps": n_steps, "lr": lr, "dtype": "fp32",
+            "samples": n_samples, "steps": n_steps, "early_stop": False, "lr": lr, "dtype": "fp32",
+            "upper_limit_adjoint": "exact derivative of log_normal_cdf (LCF_SVI_EXACT_TAIL)",
             "value": round(n_sources / dt_adj, 2), "unit": "sources/s", "ms": round(dt_adj * 1e3, 3),
             "kernel_ms": round(kms_adj, 3),
             "plain": {"value": round(n_sources / dt_plain, 2), "ms": round(dt_plain * 1e3, 3),
@@ -145,9 +217,14 @@ def svi_leg(pkg, n_sources: int, n_steps: int, n_samples: int, lr: float, reps: 
             "agreement": {"mu_q50_over_sigma": float(np.quantile(dmu, 0.5)),
                           "mu_q95_over_sigma": float(np.quantile(dmu, 0.95)),
                           "sigma_q95_rel": float(np.quantile(dsg, 0.95))},
-            "steps_mean": float(adj["steps"].mean()),
+            "early_stopped": {"value": round(n_sources / dt_early, 2), "ms": round(dt_early * 1e3, 3),
+                              "steps_mean": float(early["steps"].mean()),
+                              "rule": "EMA of the ELBO after 50 steps, 20-step stall (cuda/svi.cu:297-312)"},
             "sample_obs_step_evals": evals,
-            "evals_per_s": round(evals / (kms_adj * 1e-3), 1)}
+            "roofline": {"bound": "valu", "pipe": "fp32 vector ALU + transcendental unit",
+                         "kernel": "svi_kernel<Villar, fp32>", "achieved": round(achieved, 3),
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                         "flopeq_per_sample_obs_step": SVI_FLOPEQ_FWD + SVI_FLOPEQ_ADJ, "traffic": None}}
 
 
 def np_flops(csr: dict, n_grid: int, dense_max: int, amps, lsf) -> float:
@@ -256,6 +333,25 @@ def gp2d_leg(pkg, n_sources: int, max_iters: int, n_times: int, reps: int, cpu_s
         k = ctx.last_kernel_ms
         best = (k, r) if best is None or k < best[0] else best
     ms_train, r = best
+    # plain form: the same L-BFGS on central finite differences of the NLML
+    # (eight factorisations per gradient), one launch per size class
+    ctx.set_schedule(pkg.SCHED_PER_CLASS)
+    rp = ctx.gp2d_train(c, c["theta0"], max_iters=max_iters, grad_tol=1e-6, query_t=qt, query_w=qw,
+                        fd_gradient=True)
+    ms_plain = ctx.last_kernel_ms
+    ctx.set_schedule(pkg.SCHED_PERSISTENT)
+    log(f"gp2d plain (FD gradients, per size class): {ms_plain:.1f} ms")
+    both = (r["status"] < 3) & (rp["status"] < 3)
+    drel = np.abs(r["nlml"] - rp["nlml"]) / np.maximum(np.abs(r["nlml"]), 1.0)
+    pg = r["pgrad_max"]
+    per_status = {}
+    for st_ in range(3):
+        sel = r["status"] == st_
+        if sel.any():
+            per_status[str(st_)] = {"sources": int(sel.sum()),
+                                    "pgrad_max_q50": float(np.quantile(pg[sel], 0.5)),
+                                    "pgrad_max_q95": float(np.quantile(pg[sel], 0.95)),
+                                    "pgrad_max_max": float(pg[sel].max())}
     ok = r["status"] < 3
     off = np.zeros(ns * n_times + 1, dtype=np.int64)
     off[1:] = np.cumsum(np.repeat(np.where(ok, 3, 0), n_times))
@@ -280,6 +376,16 @@ def gp2d_leg(pkg, n_sources: int, max_iters: int, n_times: int, reps: int, cpu_s
            "mean_iterations": round(float(r["iters"].mean()), 2),
            "evaluations": {"nlml": int(ev[:, 0].sum()), "gradient": int(ev[:, 1].sum())},
            "temperatures_fitted": int(np.sum(bb["ok"])),
+           "projected_gradient_by_status": per_status,
+           "plain": {"ms_training": round(ms_plain, 3), "value": round(ns / ((ms_plain + ms_bb) * 1e-3), 2),
+                     "form": "L-BFGS on central finite-difference gradients (8 NLML factorisations each), "
+                             "one launch per power-of-two size class",
+                     "status_counts": np.bincount(rp["status"], minlength=4).tolist(),
+                     "evaluations": {"factorisations": int(rp["evals"][:, 0].sum()),
+                                     "gradient": int(rp["evals"][:, 1].sum())},
+                     "nlml_rel_diff_q50": float(np.quantile(drel[both], 0.5)),
+                     "nlml_rel_diff_q99": float(np.quantile(drel[both], 0.99))},
+           "speedup_vs_plain": round((ms_plain + ms_bb) / ms, 3),
            "roofline": {"bound": "mfma", "pipe": "fp64 matrix cores (tile products) + fp64 vector ALU, both 78.6 TFLOP/s", "kernel": "gp2d_hyper_kernel",
                         "achieved": round(achieved, 4), "peak": 78.6, "unit": "TFLOP/s",
                         "frac": round(achieved / 78.6, 5), "flops": flops}}
@@ -308,6 +414,9 @@ def main() -> int:
     ap.add_argument("--particles", type=int, default=64)
     ap.add_argument("--iters", type=int, default=200)
     ap.add_argument("--cpu-sources", type=int, default=192)
+    ap.add_argument("--cpu-sources-1t", type=int, default=16)
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--accuracy-sources", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-svi", action="store_true")
@@ -381,7 +490,8 @@ def main() -> int:
     moe_det = moe - moe_up
     dom = int(np.argmax(slot_ms))
     m_dom = MODELS[dom]
-    flops = moe_det[dom] * FLOPEQ_DET[m_dom] + moe_up[dom] * FLOPEQ_UP[m_dom]
+    flops = moe[dom] * FLOPEQ_ALGO[m_dom]
+    issued = moe_det[dom] * FLOPEQ_DET[m_dom] + moe_up[dom] * FLOPEQ_UP[m_dom]
     # measured HBM bytes per launch (default workload only); the algorithmic
     # bytes are one read of every packed record plus the outputs
     traffic = None
@@ -389,14 +499,32 @@ def main() -> int:
         traffic = round((PSO_FETCH_KIB[m_dom] * FETCH_SCALE + PSO_WRITE_KIB[m_dom] * WRITE_SCALE) * 1024.0)
     algo_bytes = n_obs * 16 + n_bands * (7 * 8 + 8 + 4 + 4 + 8)
     achieved = flops / (slot_ms[dom] * 1e-3) / 1e12
-    roof = {"bound": "mfma", "pipe": "fp32 vector ALU + transcendental unit (no matrix ops)",
+    roof = {"bound": "valu", "pipe": "fp32 vector ALU + transcendental unit (no matrix ops)",
             "kernel": f"pso_model_kernel<{MODEL_NAMES[MODELS[dom]]}, fp32>",
             "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
             "launch_ms": round(float(slot_ms[dom]), 3),
             "algorithmic_bytes": int(algo_bytes),
             "moe_per_launch": float(moe[dom]), "upper_limit_moe_fraction": round(float(moe_up[dom] / moe[dom]), 4),
-            "flopeq_per_moe": {"detection": FLOPEQ_DET[m_dom], "upper_limit": FLOPEQ_UP[m_dom]}}
+            "flopeq_per_moe": FLOPEQ_ALGO[m_dom],
+            "flopeq_convention": "model formula operation count, transcendental = 8",
+            "issued_flopeq_per_moe": {"detection": FLOPEQ_DET[m_dom], "upper_limit": FLOPEQ_UP[m_dom]},
+            "issued_tflops": round(issued / (slot_ms[dom] * 1e-3) / 1e12, 3)}
+
+    # end to end: upload + pack of the observations, the fit, the downloads
+    ctx.synchronize()
+    t_up = time.perf_counter()
+    data_e2e = pkg.GpuBatchData(ctx, csr=csr)
+    ms_upload = (time.perf_counter() - t_up) * 1e3
+    t_e2e = time.perf_counter()
+    ctx.fit_models(data_e2e, MODELS, args.particles, args.iters, args.iters, seed=42, per_model=True)
+    ms_e2e = ms_upload + (time.perf_counter() - t_e2e) * 1e3
+    data_e2e.close()
+    log(f"end to end with upload: {ms_e2e:.1f} ms (upload {ms_upload:.1f} ms)")
+
+    accuracy = None
+    if not args.no_accuracy:
+        accuracy = pso_accuracy(pkg, csr, first, args.accuracy_sources, args.particles, args.iters)
 
     plain = None
     if not args.no_plain:
@@ -412,9 +540,12 @@ def main() -> int:
 
     cpu = None
     if not args.no_cpu:
-        threads = max(1, min(os.cpu_count() or 1, 16))
+        threads = cpu_threads()
         log(f"cpu baseline: {args.cpu_sources} sources on {threads} threads")
         cpu = cpu_baseline(pkg, args.cpu_sources, args.particles, args.iters, threads)
+        one = cpu_baseline(pkg, args.cpu_sources_1t, args.particles, args.iters, 1)
+        cpu["single_thread"] = {"value": one["value"], "unit": "sources/s", "sample": one["sample"]}
+        cpu["nproc"] = os.cpu_count()
 
     svi = None
     if not args.no_svi:
@@ -423,12 +554,12 @@ def main() -> int:
     npl = None
     if not args.no_np:
         npl = np_leg(pkg, args.np_sources, 200, 200, max(args.steps, 1), args.np_cpu_sources,
-                     min(os.cpu_count() or 1, 16), not args.no_cpu)
+                     cpu_threads(), not args.no_cpu)
 
     gp2 = None
     if not args.no_gp2d:
         gp2 = gp2d_leg(pkg, args.gp2d_sources, 100, 20, max(args.steps, 1), args.gp2d_cpu_sources,
-                       min(os.cpu_count() or 1, 16), not args.no_cpu)
+                       cpu_threads(), not args.no_cpu)
 
     line = {
         "metric": METRIC,
@@ -451,6 +582,9 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ms_with_upload": round(ms_e2e, 3),
+        "ms_upload": round(ms_upload, 3),
+        "accuracy": accuracy,
         "plain": plain,
         "per_model_ms": {MODEL_NAMES[m]: round(float(slot_ms[i]), 3) for i, m in enumerate(MODELS)},
         "svi": svi,

@@ -232,6 +232,11 @@ int lcf_batch_bb_temperature(lcf_ctx* ctx, int64_t n_sets, const int64_t* offset
 #define LCF_SVI_INFLATE_SIGMA 2u /* add ln 4 to the returned log sigma (src/parametric.rs:2441-2446) */
 #define LCF_SVI_FD_GRADIENT 4u   /* numerical (forward-difference) model gradients instead of the
                                     analytic adjoint: the plain comparison path */
+#define LCF_SVI_FIXED_STEPS 8u   /* run all n_steps Adam steps (no early stop) */
+#define LCF_SVI_EXACT_TAIL 16u   /* upper-limit adjoint = the exact derivative of the log_normal_cdf
+                                    the ELBO sums, instead of the reference's phi / max(Phi, 1e-300)
+                                    with Phi from erf_approx (cuda/svi.cu:200-204), which reaches
+                                    1e280 in the lower tail and freezes Adam */
 
 /* GpuContext::batch_svi_fit (src/gpu_cuda.rs:1315-1422): one job per source
  * of `data` (the SviBatchInput list is parallel to the batch).  Rows of
@@ -417,8 +422,11 @@ int lcf_batch_fit_gp_2d(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets,
  * on y - mean(y), theta = ln (a, lt, lw, sn) ([n][4]).  The negative log
  * marginal likelihood and its exact gradient 1/2 tr((K^-1 - alpha alpha^T)
  * dK/dtheta) come from a blocked FP64 Cholesky and its adjoint; L-BFGS
- * (6 pairs, projected backtracking Armijo search that gives up once a step
- * moves theta by <= 1e-10, box a in [1e-6, 1e6],
+ * (6 pairs over the free components -- a component whose gradient presses
+ * against its bound is held for the iteration, and a step the projection
+ * leaves without decrease falls back to free steepest descent -- projected
+ * backtracking Armijo search that gives up once a step moves theta by
+ * <= 1e-10, box a in [1e-6, 1e6],
  * lt in [0.1, 1e4], lw in [1e-3, 10], sn in [1e-6, 10]) runs at most
  * max_iters iterations from theta0, stopping when the projected gradient's
  * largest component is <= grad_tol.  out_status: 0 converged, 1 iteration
@@ -433,6 +441,20 @@ int lcf_batch_gp2d_train(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets
                          const double* query_waves, int n_query, int query_per_source, double* out_theta,
                          double* out_nlml, double* out_grad, int32_t* out_iters, int32_t* out_status,
                          double* out_pred, double* out_std, int32_t* out_evals);
+
+/* General form: flags LCF_GP2D_FD_GRADIENT runs the same L-BFGS on central
+ * finite differences of the NLML (h = 1e-5 max(|theta_k|, 1), eight
+ * factorisations per gradient; the plain form the adjoint replaces; out_grad
+ * is still the exact gradient at the returned point), and the context's
+ * schedule applies (LCF_SCHED_PER_CLASS: one synchronised launch per
+ * power-of-two size class).  out_evals counts every factorisation. */
+#define LCF_GP2D_FD_GRADIENT 1u
+int lcf_batch_gp2d_train_ex(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, const double* times,
+                            const double* log_waves, const double* mags, const double* noise_var,
+                            const double* theta0, int max_iters, double grad_tol, const double* query_times,
+                            const double* query_waves, int n_query, int query_per_source, uint32_t flags,
+                            double* out_theta, double* out_nlml, double* out_grad, int32_t* out_iters,
+                            int32_t* out_status, double* out_pred, double* out_std, int32_t* out_evals);
 
 /* The same objective and gradient at theta ([n][4], not clamped): one
  * evaluation per source; out_status 0, or 3 when K is not positive definite

@@ -59,6 +59,8 @@ SIGNATURES = {
                                    _pd]),
     "lcf_batch_gp2d_train": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _pd, _i32, _f64, _pd, _pd, _i32, _i32,
                                     _pd, _pd, _pd, _pi32, _pi32, _pd, _pd, _pi32]),
+    "lcf_batch_gp2d_train_ex": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _pd, _i32, _f64, _pd, _pd, _i32,
+                                       _i32, _u32, _pd, _pd, _pd, _pi32, _pi32, _pd, _pd, _pi32]),
     "lcf_gp2d_nlml_grad": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _pi32]),
     "lcf_cpu_pso": (_i32, [_vp, _vp, _vp, _i64, _pd, _pd, _pi32]),
     "lcf_batch_finalize": (_i32, [_vp, _vp, _pi32, _pd, _pd, _pd, _pd, _pd, _pd, _pd, _i32, _pd, _pd, _pd]),

@@ -9,6 +9,13 @@ forward stores the gradient, and backward scales it by the incoming gradient.
 Tensors are [n_jobs, 7] float64 (rows padded like the C ABI); they cross the
 C ABI through host memory.
 
+``Gp2dNlml.apply(theta, gpu_ctx, csr)`` returns the negative log marginal
+likelihood of each source's 2D GP (separable squared-exponential kernel,
+theta = ln(amp, l_t, l_lambda, sigma_n) as [n_sources, 4] float64) and
+back-propagates through the Cholesky adjoint kernel (``lcf_gp2d_nlml_grad``:
+1/2 tr((K^-1 - alpha alpha^T) dK/dtheta) from one blocked FP64 factorisation);
+sources whose K is not positive definite give NaN and a NaN gradient.
+
 Imported on demand (it imports torch); nothing else in the package needs it.
 """
 from __future__ import annotations
@@ -37,3 +44,16 @@ class SviElbo(torch.autograd.Function):
         gmu, gls = ctx.saved_tensors
         g = grad_out.reshape(-1, 1)
         return g * gmu, g * gls, None, None, None, None, None, None, None, None, None
+
+
+class Gp2dNlml(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, theta, gpu_ctx, csr):
+        r = gpu_ctx.gp2d_nlml_grad(csr, _np(theta))
+        ctx.save_for_backward(torch.from_numpy(r["grad"]).to(theta.device, theta.dtype))
+        return torch.from_numpy(r["nlml"]).to(theta.device, theta.dtype)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return grad_out.reshape(-1, 1) * g, None, None

@@ -56,7 +56,8 @@ struct Smem {
     double b[T * T];
     double red[NT];
     double st[S_END];
-    int flag[8];  // 0 pivot failure, 1 stop, 2 accepted, 3 nm, 4 head, 5 step too small
+    int flag[8];  // 0 pivot failure, 1 stop, 2 accepted, 3 nm, 4 head, 5 step too small,
+                  // 6 direction is the free steepest descent
 };
 
 struct Src {
@@ -411,21 +412,68 @@ __device__ bool eval_f(double* A, int ld, const Src& s, const double* th, double
     return __builtin_isfinite(f);
 }
 
-// Thread 0: L-BFGS direction (two-loop recursion) and first step length;
-// returns false when the projected gradient is within gtol.
+// Components of the gradient at th that press against their bound (the
+// projected gradient is zero there; they stay fixed for this iteration).
+__device__ __forceinline__ bool pinned_at(const double* th, const double* g, int k) {
+    return (th[k] <= c_th_lo[k] && g[k] > 0.0) || (th[k] >= c_th_hi[k] && g[k] < 0.0);
+}
+
+// Thread 0: steepest descent on the free components, scaled to at most unit
+// length; the curvature memory is dropped.
+__device__ void steepest_free(Smem& sm) {
+    double* st = sm.st;
+    const double* th = st + S_TH;
+    const double* g = st + S_G;
+    double* d = st + S_D;
+    double gn = 0.0;
+    for (int k = 0; k < 4; k++)
+        if (!pinned_at(th, g, k)) gn += g[k] * g[k];
+    const double sc = 1.0 / fmax(1.0, ::sqrt(gn));
+    for (int k = 0; k < 4; k++) d[k] = pinned_at(th, g, k) ? 0.0 : -g[k] * sc;
+    sm.flag[3] = 0;
+    sm.flag[6] = 1;
+    double dmax = 0.0;
+    for (int k = 0; k < 4; k++) dmax = fmax(dmax, fabs(d[k]));
+    st[S_STEP] = dmax > 2.0 ? 2.0 / dmax : 1.0;
+}
+
+// Central finite differences of the NLML at th (all threads), h = 1e-5
+// max(|theta_k|, 1); one-sided against f0 where one side is not positive
+// definite, 0 where neither is.  Eight factorisations per gradient: the
+// plain form the adjoint replaces.  Leaves A holding another point's factor.
+__device__ __noinline__ void fd_gradient(double* A, int ld, const Src& s, const double* th, double ybar,
+                                         double* zv, Smem& sm, double f0, double* g) {
+#pragma unroll 1
+    for (int k = 0; k < 4; k++) {
+        const double h = 1e-5 * fmax(fabs(th[k]), 1.0);
+        double tp[4], tm[4], fp = NAN, fm = NAN;
+        for (int j = 0; j < 4; j++) tp[j] = tm[j] = th[j];
+        tp[k] += h;
+        tm[k] -= h;
+        const bool okp = eval_f(A, ld, s, tp, ybar, zv, sm, fp);
+        const bool okm = eval_f(A, ld, s, tm, ybar, zv, sm, fm);
+        g[k] = okp && okm ? (fp - fm) / (2.0 * h) : (okp ? (fp - f0) / h : (okm ? (f0 - fm) / h : 0.0));
+    }
+}
+
+// Thread 0: L-BFGS direction (two-loop recursion over the free components:
+// the pinned components of g are zeroed before the recursion and of the
+// direction after it) and first step length; returns false when the
+// projected gradient is within gtol.
 __device__ bool lbfgs_direction(Smem& sm, double gtol) {
     double* st = sm.st;
     const double* th = st + S_TH;
     const double* g = st + S_G;
     double gi = 0.0;
+    bool pin[4];
     for (int k = 0; k < 4; k++) {
-        const bool pinned = (th[k] <= c_th_lo[k] && g[k] > 0.0) || (th[k] >= c_th_hi[k] && g[k] < 0.0);
-        gi = fmax(gi, pinned ? 0.0 : fabs(g[k]));
+        pin[k] = pinned_at(th, g, k);
+        gi = fmax(gi, pin[k] ? 0.0 : fabs(g[k]));
     }
     if (gi <= gtol) return false;
     const int nm = sm.flag[3], head = sm.flag[4];
     double q[4], al[LM];
-    for (int k = 0; k < 4; k++) q[k] = g[k];
+    for (int k = 0; k < 4; k++) q[k] = pin[k] ? 0.0 : g[k];
     for (int m = 0; m < nm; m++) {
         const int p = (head - 1 - m + LM) % LM;
         double s = 0.0;
@@ -444,7 +492,8 @@ __device__ bool lbfgs_direction(Smem& sm, double gtol) {
         gam = sy / yy;
     } else {
         double gn = 0.0;
-        for (int k = 0; k < 4; k++) gn += g[k] * g[k];
+        for (int k = 0; k < 4; k++)
+            if (!pin[k]) gn += g[k] * g[k];
         gam = 1.0 / fmax(1.0, ::sqrt(gn));
     }
     for (int k = 0; k < 4; k++) q[k] *= gam;
@@ -458,15 +507,13 @@ __device__ bool lbfgs_direction(Smem& sm, double gtol) {
     double* d = st + S_D;
     double gd = 0.0;
     for (int k = 0; k < 4; k++) {
-        d[k] = -q[k];
+        d[k] = pin[k] ? 0.0 : -q[k];
         gd += g[k] * d[k];
     }
+    sm.flag[6] = 0;
     if (!(gd < 0.0)) {
-        sm.flag[3] = 0;
-        double gn = 0.0;
-        for (int k = 0; k < 4; k++) gn += g[k] * g[k];
-        const double sc = 1.0 / fmax(1.0, ::sqrt(gn));
-        for (int k = 0; k < 4; k++) d[k] = -g[k] * sc;
+        steepest_free(sm);
+        return true;
     }
     double dmax = 0.0;
     for (int k = 0; k < 4; k++) dmax = fmax(dmax, fabs(d[k]));
@@ -475,18 +522,25 @@ __device__ bool lbfgs_direction(Smem& sm, double gtol) {
 }
 
 // Thread 0: trial point th + step d, projected on the box, and the Armijo
-// decrease term g^T (tt - th).
+// decrease term g^T (tt - th).  When the projection leaves an L-BFGS step
+// with no predicted decrease or no movement, the search restarts along the
+// free steepest descent direction (which cannot lose its decrease to the
+// projection).
 __device__ void trial_point(Smem& sm) {
     double* st = sm.st;
-    double dec = 0.0, mv = 0.0;
-    for (int k = 0; k < 4; k++) {
-        const double v = fmin(fmax(st[S_TH + k] + st[S_STEP] * st[S_D + k], c_th_lo[k]), c_th_hi[k]);
-        st[S_TT + k] = v;
-        dec += st[S_G + k] * (v - st[S_TH + k]);
-        mv = fmax(mv, fabs(v - st[S_TH + k]));
+    for (int pass = 0; pass < 2; pass++) {
+        double dec = 0.0, mv = 0.0;
+        for (int k = 0; k < 4; k++) {
+            const double v = fmin(fmax(st[S_TH + k] + st[S_STEP] * st[S_D + k], c_th_lo[k]), c_th_hi[k]);
+            st[S_TT + k] = v;
+            dec += st[S_G + k] * (v - st[S_TH + k]);
+            mv = fmax(mv, fabs(v - st[S_TH + k]));
+        }
+        st[S_DEC] = dec;
+        sm.flag[5] = mv <= 1e-10;  // the step no longer moves theta
+        if (sm.flag[6] || (dec < 0.0 && !sm.flag[5])) break;
+        steepest_free(sm);
     }
-    st[S_DEC] = dec;
-    sm.flag[5] = mv <= 1e-10;  // the step no longer moves theta
 }
 
 // Thread 0: store the pair (s, y) and move to the trial point.
@@ -613,7 +667,12 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
     if (!eval_f(A, a.ld, s, th, ybar, zv, sm, f)) {
         status = 3;
     } else {
-        gradient(A, Xm, ld, s, th, zv, av, sm, g);
+        if (a.fd_grad && train) {
+            fd_gradient(A, ld, s, th, ybar, zv, sm, f, g);
+            nf += 8;
+        } else {
+            gradient(A, Xm, ld, s, th, zv, av, sm, g);
+        }
         ng++;
         if (tid == 0) {
             for (int k = 0; k < 4; k++) st[S_G + k] = g[k];
@@ -658,7 +717,12 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
                 double tt[4], gn[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) tt[k] = st[S_TT + k];
-                gradient(A, Xm, ld, s, tt, zv, av, sm, gn);
+                if (a.fd_grad) {
+                    fd_gradient(A, ld, s, tt, ybar, zv, sm, st[S_FT], gn);
+                    nf += 8;
+                } else {
+                    gradient(A, Xm, ld, s, tt, zv, av, sm, gn);
+                }
                 ng++;
                 if (tid == 0) {
                     for (int k = 0; k < 4; k++) st[S_GN + k] = gn[k];

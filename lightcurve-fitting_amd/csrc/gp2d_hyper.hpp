@@ -22,6 +22,7 @@ struct Gp2dHyperArgs {
     int* queue;
     const double* theta0;     // [n_src][4] ln (a, l_t, l_w, sigma_n)
     int max_iters;            // < 0: NLML and gradient at theta0 only
+    int fd_grad;              // 1: L-BFGS on central finite differences of the NLML (the plain form)
     double gtol;
     int ld;                   // leading dimension of the slot matrices (multiple of 64)
     double* work;             // [slots][2 ld ld + 2 ld]

@@ -502,4 +502,29 @@ __device__ __forceinline__ float log_normal_cdf<float>(float x) {
     return M::log(1.0f - 0.5f * poly * M::exp(-z * z));
 }
 
+// d/dx log_normal_cdf(x), differentiated from the A&S form above rather than
+// from the exact normal CDF, so that it is the derivative of what the
+// likelihood sums: Q(t) = t P(t), t = 1 / (1 + p |z|), z = -x / sqrt 2;
+// 0 above 8, -x - 1/x on the asymptote below -30.  Free of the
+// 1 - (1 - tiny) cancellation of phi / Phi far in the lower tail.
+template <typename R>
+__device__ __forceinline__ R dlog_normal_cdf(R x) {
+    using M = Math<R>;
+    const R S2 = R(0.70710678118654752440), p = R(0.3275911);
+    if (x > R(8)) return R(0);
+    if (x < R(-30)) return -x - M::rcp(x);
+    const R z = -x * S2, az = z < R(0) ? -z : z;
+    const R t = M::rcp(R(1) + p * az);
+    const R q = t * (R(0.254829592) + t * (R(-0.284496736) + t * (R(1.421413741) +
+                t * (R(-1.453152027) + t * R(1.061405429)))));
+    const R dq = R(0.254829592) + t * (R(2.0 * -0.284496736) + t * (R(3.0 * 1.421413741) +
+                 t * (R(4.0 * -1.453152027) + t * R(5.0 * 1.061405429))));
+    const R dt = -p * t * t;
+    if (z >= R(0)) return -S2 * (dq * dt / q - R(2) * az);
+    const R e = M::exp(-az * az);
+    const R E = R(0.5) * q * e;
+    const R dE = R(0.5) * e * (dq * dt - R(2) * az * q);
+    return -S2 * dE / (R(1) - E);
+}
+
 }  // namespace lcf

@@ -1348,9 +1348,10 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
                    const double* y, const double* nv, const double* theta0, int max_iters, double gtol,
                    const double* qt, const double* qw, int n_query, int q_per_src, double* out_theta,
                    double* out_nlml, double* out_grad, int32_t* out_iters, int32_t* out_status, double* out_pred,
-                   double* out_std, int32_t* out_evals) {
+                   double* out_std, int32_t* out_evals, uint32_t flags = 0u) {
     if (!c) return LCF_E_INVALID;
     if (ns < 0) return fail(c, LCF_E_INVALID, "negative source count");
+    if (flags & ~LCF_GP2D_FD_GRADIENT) return fail(c, LCF_E_INVALID, "unknown gp2d flags 0x%x", flags);
     if (ns == 0) return LCF_OK;
     if (!off || !theta0 || !out_theta || !out_nlml || !out_grad) return fail(c, LCF_E_INVALID, "null pointer");
     if (n_query < 0 || (n_query > 0 && (!qt || !qw || !out_pred || !out_std)))
@@ -1428,6 +1429,7 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
     a.queue = c->d_queue;
     a.theta0 = d_th0.p;
     a.max_iters = max_iters;
+    a.fd_grad = (flags & LCF_GP2D_FD_GRADIENT) ? 1 : 0;
     a.gtol = gtol;
     a.ld = ld;
     a.work = d_work.p;
@@ -1443,9 +1445,32 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
     a.evals = d_ev.p;
     a.pred = n_query > 0 ? d_pred.p : nullptr;
     a.stdv = n_query > 0 ? d_std.p : nullptr;
-    HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    HIPCHK(c, lcf::launch_gp2d_hyper(a, grid, c->stream));
+    if (c->schedule == LCF_SCHED_PERSISTENT) {
+        HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+        HIPCHK(c, lcf::launch_gp2d_hyper(a, grid, c->stream));
+    } else {
+        // one synchronised launch per power-of-two size class, largest first
+        int64_t i = 0;
+        while (i < ns) {
+            const int64_t n0 = off[order[(size_t)i] + 1] - off[order[(size_t)i]];
+            int64_t cls = 1;
+            while (cls < n0) cls <<= 1;
+            int64_t j = i;
+            while (j < ns) {
+                const int64_t nj = off[order[(size_t)j] + 1] - off[order[(size_t)j]];
+                if (cls > 1 && nj * 2 <= cls) break;
+                j++;
+            }
+            lcf::Gp2dHyperArgs ac = a;
+            ac.order = d_order.p + i;
+            ac.n_tasks = (int)(j - i);
+            HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+            HIPCHK(c, lcf::launch_gp2d_hyper(ac, (int)std::min<int64_t>(grid, j - i), c->stream));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+            i = j;
+        }
+    }
     HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     HIPCHK(c, hipEventSynchronize(c->ev1));
     float ms = 0.f;
@@ -1481,6 +1506,19 @@ int lcf_batch_gp2d_train(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, 
     return gp2d_hyper_run(c, n_sources, offsets, times, log_waves, mags, noise_var, theta0, max_iters, grad_tol,
                           query_times, query_waves, n_query, query_per_source, out_theta, out_nlml, out_grad,
                           out_iters, out_status, out_pred, out_std, out_evals);
+}
+
+int lcf_batch_gp2d_train_ex(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, const double* times,
+                            const double* log_waves, const double* mags, const double* noise_var,
+                            const double* theta0, int max_iters, double grad_tol, const double* query_times,
+                            const double* query_waves, int n_query, int query_per_source, uint32_t flags,
+                            double* out_theta, double* out_nlml, double* out_grad, int32_t* out_iters,
+                            int32_t* out_status, double* out_pred, double* out_std, int32_t* out_evals) {
+    if (c && max_iters < 0) return fail(c, LCF_E_INVALID, "max_iters must be >= 0");
+    if (c && (!out_iters || !out_status)) return fail(c, LCF_E_INVALID, "null pointer");
+    return gp2d_hyper_run(c, n_sources, offsets, times, log_waves, mags, noise_var, theta0, max_iters, grad_tol,
+                          query_times, query_waves, n_query, query_per_source, out_theta, out_nlml, out_grad,
+                          out_iters, out_status, out_pred, out_std, out_evals, flags);
 }
 
 int lcf_gp2d_nlml_grad(lcf_ctx* c, int64_t n_sources, const int64_t* offsets, const double* times,

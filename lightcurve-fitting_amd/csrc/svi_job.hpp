@@ -164,7 +164,7 @@ struct SviJob {
 
     // LL of this lane's observations for sample theta and its gradient.
     __device__ __forceinline__ R loglik(const R (&th)[NP], const O* lds, const O* __restrict__ gobs, int n,
-                                        int tile, int grp, int G, R (&dll)[NP]) {
+                                        int tile, int grp, int G, bool exact_tail, R (&dll)[NP]) {
         constexpr bool F64 = sizeof(R) == 8;
         Eval ev;
         ev.setup(th);
@@ -193,7 +193,10 @@ struct SviJob {
                     R lz = log_normal_cdf<R>(z);
                     if (F64) ll.s += lz; else ll.add(lz);
                     R ratio, dpred;
-                    if (F64) {  // cuda/svi.cu:200-204 as written
+                    if (exact_tail) {  // the derivative of the lz summed above
+                        ratio = dlog_normal_cdf<R>(z);
+                        dpred = -ratio / st;
+                    } else if (F64) {  // cuda/svi.cu:200-204 as written
                         R phi, cdf;
                         phi_cdf(z, phi, cdf);
                         ratio = phi / cdf;
@@ -227,7 +230,7 @@ struct SviJob {
     __device__ __forceinline__ R elbo_grad(const R (&mu)[NP], const R (&ls)[NP], const R (&pc)[NP],
                                            const R (&pw2)[NP], const O* lds, const O* __restrict__ gobs,
                                            int n, int tile, uint32_t job, uint32_t step, int S, int G,
-                                           uint64_t seed, R (&gmu)[NP], R (&gls)[NP]) {
+                                           uint64_t seed, bool exact_tail, R (&gmu)[NP], R (&gls)[NP]) {
         const int lane = lane_id();
         const int s = lane / G, grp = lane - s * G;
         const bool active = s < S;
@@ -237,7 +240,7 @@ struct SviJob {
         draw_normals<R, NP>(seed, job, step, (uint32_t)s, eps);
 #pragma unroll
         for (int j = 0; j < NP; j++) th[j] = mu[j] + sig[j] * eps[j];
-        R L = loglik(th, lds, gobs, n, tile, grp, G, dll);
+        R L = loglik(th, lds, gobs, n, tile, grp, G, exact_tail, dll);
         if (grp == 0) {
             R lp = R(0);
 #pragma unroll
@@ -301,7 +304,7 @@ struct SviJob {
         R gmu[NP], gls[NP];
         if (a.grad_only) {
             R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, (uint32_t)job, (uint32_t)a.step0, S, G,
-                            a.seed, gmu, gls);
+                            a.seed, (a.flags & LCF_SVI_EXACT_TAIL_FLAG) != 0, gmu, gls);
             double om = 0.0, ol = 0.0;
 #pragma unroll
             for (int j = 0; j < NP; j++)  // + 1: derivative of the entropy term
@@ -316,6 +319,7 @@ struct SviJob {
         R am[NP], av[NP], bm[NP], bv[NP];
 #pragma unroll
         for (int j = 0; j < NP; j++) am[j] = av[j] = bm[j] = bv[j] = R(0);
+        const bool exact_tail = (a.flags & LCF_SVI_EXACT_TAIL_FLAG) != 0;
         const int min_it = (a.flags & LCF_SVI_CPU_STOPPING_FLAG) ? 200 : 50;
         const int stall_it = (a.flags & LCF_SVI_CPU_STOPPING_FLAG) ? 50 : 20;
         const R lr = (R)a.lr;
@@ -325,7 +329,7 @@ struct SviJob {
         int stall = 0, step = 0;
         for (; step < a.n_steps;) {
             R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, (uint32_t)job, (uint32_t)step, S, G, a.seed,
-                            gmu, gls);
+                            exact_tail, gmu, gls);
             final_elbo = e;
             step++;
             // Adam (cuda/svi.cu:266-291), non-finite components skipped
@@ -352,8 +356,8 @@ struct SviJob {
                 }
                 ls[j] = ls[j] < R(-6) ? R(-6) : (ls[j] > R(2) ? R(2) : ls[j]);
             }
-            // EMA early stop (cuda/svi.cu:297-312)
-            if (step - 1 >= min_it) {
+            // EMA early stop (cuda/svi.cu:297-312), unless all steps are asked for
+            if (!(a.flags & LCF_SVI_FIXED_STEPS_FLAG) && step - 1 >= min_it) {
                 double fe = (double)final_elbo;
                 ema = ema < -1e290 ? fe : 0.1 * fe + 0.9 * ema;
                 if (ema - best > 0.01 * fmax(fabs(best), 1e-10)) { best = ema; stall = 0; }

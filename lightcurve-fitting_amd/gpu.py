@@ -615,12 +615,14 @@ def _fit_gp_2d_arrays(self, csr: dict, query_t, query_w) -> dict:
 
 
 def _gp2d_train(self, csr: dict, theta0, *, max_iters: int = 100, grad_tol: float = 1e-6, query_t=None,
-                query_w=None, query_per_source: bool = False) -> dict:
+                query_w=None, query_per_source: bool = False, fd_gradient: bool = False) -> dict:
     """2D GP hyper-parameter fit with the Cholesky adjoint
-    (``lcf_batch_gp2d_train``): csr holds offsets / times / waves (log10
+    (``lcf_batch_gp2d_train_ex``): csr holds offsets / times / waves (log10
     wavelength) / mags / noise_var; theta0 [n][4] = ln (a, l_t, l_w, sigma_n).
-    Returns theta, nlml, grad, iters, status, evals ([n][2] NLML / gradient
-    evaluations) and, with queries, pred / std."""
+    ``fd_gradient`` runs the plain form (central finite differences of the
+    NLML).  Returns theta, nlml, grad, iters, status, evals ([n][2]
+    factorisations / gradient evaluations), pgrad_max (largest projected
+    gradient component at the returned point) and, with queries, pred / std."""
     off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
     ns = off.size - 1
     t, w, y, nv = (_f64(csr[k]) for k in ("times", "waves", "mags", "noise_var"))
@@ -635,14 +637,30 @@ def _gp2d_train(self, csr: dict, theta0, *, max_iters: int = 100, grad_tol: floa
            "iters": np.zeros(ns, np.int32), "status": np.zeros(ns, np.int32), "evals": np.zeros((ns, 2), np.int32),
            "pred": np.zeros((ns, max(nq, 1))), "std": np.zeros((ns, max(nq, 1)))}
     d = lambda a: _lib.ptr(a, C.c_double)
-    self._check(self._lib.lcf_batch_gp2d_train(
+    self._check(self._lib.lcf_batch_gp2d_train_ex(
         self._h, ns, _lib.ptr(off, C.c_int64), d(t), d(w), d(y), d(nv), d(th0), int(max_iters), float(grad_tol),
-        d(qt), d(qw), nq, int(query_per_source), d(out["theta"]), d(out["nlml"]), d(out["grad"]),
+        d(qt), d(qw), nq, int(query_per_source), GP2D_FD_GRADIENT if fd_gradient else 0,
+        d(out["theta"]), d(out["nlml"]), d(out["grad"]),
         _lib.ptr(out["iters"], C.c_int32), _lib.ptr(out["status"], C.c_int32), d(out["pred"]), d(out["std"]),
         _lib.ptr(out["evals"], C.c_int32)))
     if nq == 0:
         del out["pred"], out["std"]
+    out["pgrad_max"] = projected_grad_max(out["theta"], out["grad"])
     return out
+
+
+# box of the 2D GP hyper-parameter fit on theta = ln (a, l_t, l_w, sigma_n)
+GP2D_THETA_LO = np.log([1e-6, 0.1, 1e-3, 1e-6])
+GP2D_THETA_HI = np.log([1e6, 1e4, 10.0, 10.0])
+GP2D_FD_GRADIENT = 1  # LCF_GP2D_FD_GRADIENT
+
+
+def projected_grad_max(theta, grad) -> np.ndarray:
+    """Largest |component| of the projected gradient per source: components
+    at a bound whose gradient points out of the box count as zero."""
+    th, g = np.atleast_2d(theta), np.atleast_2d(grad)
+    pinned = ((th <= GP2D_THETA_LO) & (g > 0)) | ((th >= GP2D_THETA_HI) & (g < 0))
+    return np.max(np.where(pinned, 0.0, np.abs(g)), axis=1)
 
 
 def _gp2d_nlml_grad(self, csr: dict, theta) -> dict:

@@ -152,15 +152,22 @@ int gp2d_train_one(const Gp2dData& d, const double* th0, int max_iters, double g
     int nm = 0, head = 0, status = 1, it = 0;
     for (; it < max_iters; it++) {
         // projected gradient: a component pressing against its bound is spent
+        // and held fixed for this iteration (zeroed in the recursion's input
+        // and in the direction)
+        bool pin[4];
         double gi = 0.0;
         for (int k = 0; k < 4; k++) {
-            const bool pinned = (th[k] <= GP2D_TH_LO[k] && g[k] > 0.0) || (th[k] >= GP2D_TH_HI[k] && g[k] < 0.0);
-            gi = std::max(gi, pinned ? 0.0 : std::fabs(g[k]));
+            pin[k] = (th[k] <= GP2D_TH_LO[k] && g[k] > 0.0) || (th[k] >= GP2D_TH_HI[k] && g[k] < 0.0);
+            gi = std::max(gi, pin[k] ? 0.0 : std::fabs(g[k]));
         }
         if (gi <= gtol) { status = 0; break; }
+        double gfn = 0.0;
+        for (int k = 0; k < 4; k++)
+            if (!pin[k]) gfn += g[k] * g[k];
+        const double gsc = 1.0 / std::max(1.0, std::sqrt(gfn));
         // two-loop recursion, newest pair first
         double q[4], al[6];
-        for (int k = 0; k < 4; k++) q[k] = g[k];
+        for (int k = 0; k < 4; k++) q[k] = pin[k] ? 0.0 : g[k];
         for (int m = 0; m < nm; m++) {
             const int p = (head - 1 - m + M) % M;
             double s = 0.0;
@@ -175,9 +182,7 @@ int gp2d_train_one(const Gp2dData& d, const double* th0, int max_iters, double g
             for (int k = 0; k < 4; k++) { sy += S[p][k] * Y[p][k]; yy += Y[p][k] * Y[p][k]; }
             gam = sy / yy;
         } else {
-            double gn = 0.0;
-            for (int k = 0; k < 4; k++) gn += g[k] * g[k];
-            gam = 1.0 / std::max(1.0, std::sqrt(gn));
+            gam = gsc;
         }
         for (int k = 0; k < 4; k++) q[k] *= gam;
         for (int m = nm - 1; m >= 0; m--) {
@@ -188,25 +193,37 @@ int gp2d_train_one(const Gp2dData& d, const double* th0, int max_iters, double g
             for (int k = 0; k < 4; k++) q[k] += S[p][k] * (al[m] - b);
         }
         double dd[4], gd = 0.0;
-        for (int k = 0; k < 4; k++) { dd[k] = -q[k]; gd += g[k] * dd[k]; }
-        if (!(gd < 0.0)) {  // not a descent direction: restart from the scaled gradient
+        for (int k = 0; k < 4; k++) { dd[k] = pin[k] ? 0.0 : -q[k]; gd += g[k] * dd[k]; }
+        bool steepest = false;
+        auto to_steepest = [&]() {  // scaled free gradient; the curvature memory is dropped
             nm = 0;
-            double gn = 0.0;
-            for (int k = 0; k < 4; k++) gn += g[k] * g[k];
-            const double sc = 1.0 / std::max(1.0, std::sqrt(gn));
-            for (int k = 0; k < 4; k++) dd[k] = -g[k] * sc;
-        }
-        double dmax = 0.0;
-        for (int k = 0; k < 4; k++) dmax = std::max(dmax, std::fabs(dd[k]));
-        double step = dmax > 2.0 ? 2.0 / dmax : 1.0;
+            for (int k = 0; k < 4; k++) dd[k] = pin[k] ? 0.0 : -g[k] * gsc;
+            steepest = true;
+        };
+        auto first_step = [&]() {
+            double dmax = 0.0;
+            for (int k = 0; k < 4; k++) dmax = std::max(dmax, std::fabs(dd[k]));
+            return dmax > 2.0 ? 2.0 / dmax : 1.0;
+        };
+        if (!(gd < 0.0)) to_steepest();  // not a descent direction
+        double step = first_step();
         double tt[4], ft = NAN;
         bool ok = false;
         for (int ls = 0; ls < 30; ls++) {
             double dec = 0.0, mv = 0.0;
-            for (int k = 0; k < 4; k++) {
-                tt[k] = std::min(std::max(th[k] + step * dd[k], GP2D_TH_LO[k]), GP2D_TH_HI[k]);
-                dec += g[k] * (tt[k] - th[k]);
-                mv = std::max(mv, std::fabs(tt[k] - th[k]));
+            for (int pass = 0; pass < 2; pass++) {
+                dec = 0.0;
+                mv = 0.0;
+                for (int k = 0; k < 4; k++) {
+                    tt[k] = std::min(std::max(th[k] + step * dd[k], GP2D_TH_LO[k]), GP2D_TH_HI[k]);
+                    dec += g[k] * (tt[k] - th[k]);
+                    mv = std::max(mv, std::fabs(tt[k] - th[k]));
+                }
+                // an L-BFGS step that the projection leaves without decrease
+                // or movement: restart along the free steepest descent
+                if (steepest || (dec < 0.0 && !(mv <= 1e-10))) break;
+                to_steepest();
+                step = first_step();
             }
             if (mv <= 1e-10) break;  // the step no longer moves theta: no further decrease
             evals[0]++;

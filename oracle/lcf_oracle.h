@@ -51,6 +51,7 @@ int orc_eval_grad(int model_id, const double* p, double t, double* out);
 
 /* ---- likelihood helpers ------------------------------------------------- */
 double orc_log_normal_cdf(double x);   /* src/parametric.rs:882-901 */
+double orc_dlog_normal_cdf(double x);  /* its exact derivative (the consistent SVI tail adjoint) */
 double orc_erf_approx(double x);       /* src/parametric.rs:867-880 */
 
 /* ---- PSO cost -------------------------------------------------------------- */
@@ -118,6 +119,8 @@ int orc_batch_multi_bazin(int64_t n_src, const int64_t* offsets, const double* t
 #define ORC_SVI_RNG_REFERENCE 1  /* reference xorshift64, seeded 42 + job*123456789 */
 #define ORC_SVI_CPU_STOPPING 1u  /* flag: svi_fit constants, min 200 / stall 50 */
 #define ORC_SVI_INFLATE_SIGMA 2u /* flag: add ln 4 to log sigma (src/parametric.rs:2441-2446) */
+#define ORC_SVI_FIXED_STEPS 8u   /* flag: no early stop, all n_steps steps */
+#define ORC_SVI_EXACT_TAIL 16u   /* flag: upper-limit adjoint = exact derivative of log_normal_cdf */
 
 /* Philox normals of (job, step, sample): eps[np]. */
 int orc_svi_normals(uint64_t seed, uint32_t job, uint32_t step, uint32_t sample, int np, double* eps);

@@ -32,6 +32,7 @@ SIGNATURES = {
     "orc_metzger_kn_single": (_f64, [_pd, _f64]),
     "orc_eval_grad": (_i32, [_i32, _pd, _f64, _pd]),
     "orc_log_normal_cdf": (_f64, [_f64]),
+    "orc_dlog_normal_cdf": (_f64, [_f64]),
     "orc_erf_approx": (_f64, [_f64]),
     "orc_pso_cost_gpu": (_f64, [_i32, _pd, _pd, _pd, _pd, _pi32, _pd, _i64]),
     "orc_pso_cost_cpu": (_f64, [_i32, _pd, _pd, _pd, _pd, _pi32, _pd, _i64]),
@@ -51,6 +52,7 @@ SIGNATURES = {
 
 SVI_RNG_PHILOX, SVI_RNG_REFERENCE = 0, 1
 SVI_CPU_STOPPING, SVI_INFLATE_SIGMA = 1, 2
+SVI_FIXED_STEPS, SVI_EXACT_TAIL = 8, 16
 
 _LIB = None
 
@@ -137,6 +139,10 @@ def metzger_single(params, t: float) -> float:
 
 def log_normal_cdf(x: float) -> float:
     return lib().orc_log_normal_cdf(float(x))
+
+
+def dlog_normal_cdf(x: float) -> float:
+    return lib().orc_dlog_normal_cdf(float(x))
 
 
 def erf_approx(x: float) -> float:

@@ -13,6 +13,7 @@ double metzger_single(const double* p, double t);
 int eval_grad(int m, const double* p, double t, double* o);
 
 double log_normal_cdf(double x);
+double dlog_normal_cdf(double x);  // exact derivative of log_normal_cdf
 double erf_approx(double x);
 
 void pso_bounds_gpu(int m, double* lo, double* hi);

@@ -40,6 +40,30 @@ double log_normal_cdf(double x) {
     return std::log(std::max(phi, 1e-300));
 }
 
+// d/dx log_normal_cdf(x), differentiated from the form above (not from the
+// exact normal CDF): the A&S polynomial Q(t) = t P(t), t = 1 / (1 + p |z|),
+// z = -x / sqrt 2, with dQ/dt in Horner form; 0 above 8; -x - 1/x on the
+// asymptote below -30.
+double dlog_normal_cdf(double x) {
+    const double S2 = 0.70710678118654752440, p = 0.3275911;
+    if (x > 8.0) return 0.0;
+    if (x < -30.0) return -x - 1.0 / x;
+    const double z = -x * S2, az = std::fabs(z);
+    const double t = 1.0 / (1.0 + p * az);
+    const double q = as_poly(t);
+    const double dq = 0.254829592 + t * (2.0 * -0.284496736 + t * (3.0 * 1.421413741 +
+                      t * (4.0 * -1.453152027 + t * (5.0 * 1.061405429))));
+    const double dt = -p * t * t;  // dt / d|z|
+    if (z >= 0.0) {  // ln(0.5 Q) - z^2, z = |z|, dz/dx = -1/sqrt 2
+        return -S2 * (dq * dt / q - 2.0 * az);
+    }
+    // ln(1 - 0.5 Q e^{-z^2}), |z| = x / sqrt 2
+    const double e = std::exp(-az * az);
+    const double E = 0.5 * q * e;
+    const double dE = 0.5 * e * (dq * dt - 2.0 * az * q);
+    return -S2 * dE / (1.0 - E);
+}
+
 // ---------------------------------------------------------------------------
 // bounds and priors
 // ---------------------------------------------------------------------------
@@ -306,6 +330,7 @@ static SwarmOut pso_one_source(int m, int64_t s, const double* t, const double* 
 extern "C" {
 
 double orc_log_normal_cdf(double x) { return orc::log_normal_cdf(x); }
+double orc_dlog_normal_cdf(double x) { return orc::dlog_normal_cdf(x); }
 double orc_erf_approx(double x) { return orc::erf_approx(x); }
 
 int orc_pso_bounds(int m, double* lo, double* hi) {

@@ -58,19 +58,29 @@ static double loglik_grad(int m, const double* th, int np, int se, const double*
     for (int64_t i = 0; i < n; i++) {
         double pred = (m == 7) ? metzger_single(th, t[i]) : eval_scalar(m, th, t[i]);
         if (!std::isfinite(pred)) continue;
-        if (m == 7 || grad_mode == 1) fd_grad(m, th, np, se, t[i], pred, g);
+        if (m == 7 || (grad_mode & 1)) fd_grad(m, th, np, se, t[i], pred, g);
         else eval_grad(m, th, t[i], g);
         double tv = v[i] + se2, st = std::sqrt(tv), inv = 1.0 / tv;
         if (up[i]) {
             double z = (uf[i] - pred) / st;
             ll += log_normal_cdf(z);
-            double phi = std::exp(-0.5 * z * z) / std::sqrt(2.0 * PI);
-            double cdf = std::max(0.5 * (1.0 + erf_approx(z * 0.70710678118654752440)), 1e-300);
-            double dpred = -phi / (cdf * st);
+            // d ln Phi / dz: the reference's phi / max(Phi, 1e-300) with Phi
+            // from erf_approx, or (grad_mode bit 2) the exact derivative of
+            // the log_normal_cdf summed above
+            double ratio, dpred;
+            if (grad_mode & 2) {
+                ratio = dlog_normal_cdf(z);
+                dpred = -ratio / st;
+            } else {
+                double phi = std::exp(-0.5 * z * z) / std::sqrt(2.0 * PI);
+                double cdf = std::max(0.5 * (1.0 + erf_approx(z * 0.70710678118654752440)), 1e-300);
+                ratio = phi / cdf;
+                dpred = -phi / (cdf * st);
+            }
             for (int j = 0; j < np; j++)
                 if (j != se && std::isfinite(g[j])) dll[j] += dpred * g[j];
             double dz = -(uf[i] - pred) * se2 / (st * tv);
-            dll[se] += (phi / cdf) * dz;
+            dll[se] += ratio * dz;
         } else {
             double r = f[i] - pred, r2 = r * r;
             ll += -0.5 * (r2 * inv + std::log(2.0 * PI * tv));
@@ -197,7 +207,7 @@ static SviOut svi_job(int m, int64_t job, const double* pso, const double* pc, c
         }
         for (int j = 0; j < np; j++) ls[j] = std::max(-6.0, std::min(2.0, ls[j]));
         // early stop (cuda/svi.cu:297-312)
-        if (step >= min_it) {
+        if (!(flags & ORC_SVI_FIXED_STEPS) && step >= min_it) {
             ema = (ema < -1e290) ? final_elbo : 0.1 * final_elbo + 0.9 * ema;
             if (ema - best > 0.01 * std::max(std::fabs(best), 1e-10)) { best = ema; stall = 0; }
             else if (++stall >= stall_it) break;
@@ -242,7 +252,7 @@ int orc_batch_svi(int64_t n_jobs, const int64_t* off, const double* t, const dou
         int64_t a = off[j], b = off[j + 1];
         orc::SviOut o = orc::svi_job(model_ids[j], j, pso + j * 7, pc + j * 7, pw + j * 7, t + a, f + a,
                                      v + a, up + a, uf + a, b - a, n_steps, n_samples, lr, rng_mode, seed,
-                                     grad_mode, flags);
+                                     grad_mode | ((flags & ORC_SVI_EXACT_TAIL) ? 2 : 0), flags);
         for (int d = 0; d < 7; d++) { out_mu[j * 7 + d] = o.mu[d]; out_ls[j * 7 + d] = o.ls[d]; }
         out_elbo[j] = o.elbo;
         if (out_steps) out_steps[j] = o.steps;

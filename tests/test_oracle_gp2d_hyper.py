@@ -4,6 +4,9 @@
   * the adjoint gradient against central finite differences of the NLML;
   * the L-BFGS driver against scipy's L-BFGS-B on the same objective and box:
     the same minimum (NLML within 1e-7 absolute + 1e-9 relative);
+  * an active bound (sigma_n driven to its lower bound by over-stated
+    errors): the driver holds the pinned component and converges on the
+    free ones to scipy's minimum;
   * K not positive definite (negative noise variances) -> status 3.
 """
 import numpy as np
@@ -63,3 +66,45 @@ def test_not_positive_definite(orc):
     th0 = np.tile(np.log([1.0, 20.0, 0.3, 1e-3]), (2, 1))
     r = orc.batch_gp2d_train(csr, th0, 10, 0.0)
     assert list(r["status"]) == [3, 3] and np.isnan(r["nlml"]).all()
+
+
+def projected_max(th, g, orc):
+    lo, hi = np.asarray(orc.GP2D_TH_LO), np.asarray(orc.GP2D_TH_HI)
+    pinned = ((th <= lo) & (g > 0)) | ((th >= hi) & (g < 0))
+    return np.max(np.where(pinned, 0.0, np.abs(g)))
+
+
+def test_driver_with_an_active_bound_matches_scipy(orc):
+    """Sources whose noise variances already explain the scatter: the extra
+    noise sigma_n runs into its lower bound (1e-6) and its gradient keeps
+    pointing out of the box.  The driver must hold that component, converge
+    on the free ones (status 0, projected gradient within tolerance) and
+    reach scipy's L-BFGS-B minimum on the same objective and box."""
+    from scipy.optimize import minimize
+
+    rng = np.random.default_rng(11)
+    hits = 0
+    for n in (30, 50, 80):
+        t, w, y, nv = source(rng, n, noise=0.05)
+        nv = nv * 1.5  # over-stated errors: sigma_n -> 0
+        th0 = np.log([1.0, 20.0, 0.3, 0.01])
+        csr = {"offsets": np.array([0, n]), "times": t, "log_waves": w, "mags": y, "noise_var": nv}
+        r = orc.batch_gp2d_train(csr, th0[None], 200, 1e-5)
+        fun = lambda x: orc.gp2d_nlml_grad(t, w, y, nv, x)
+        ref = minimize(lambda x: fun(x)[0], th0, jac=lambda x: fun(x)[1], method="L-BFGS-B",
+                       bounds=list(zip(orc.GP2D_TH_LO, orc.GP2D_TH_HI)), options={"ftol": 1e-15, "gtol": 1e-10})
+        at_lo = ref.x[3] <= orc.GP2D_TH_LO[3] + 1e-9
+        hits += at_lo
+        assert r["nlml"][0] == pytest.approx(ref.fun, rel=1e-9, abs=1e-7), (n, r["status"][0])
+        if at_lo:
+            # sigma_n^2 << noise_var: the NLML is flat in ln sigma_n down there
+            assert r["theta"][0, 3] <= np.log(1e-4)
+            assert r["status"][0] == 0, r["status"][0]
+            assert projected_max(r["theta"][0], r["grad"][0], orc) <= 1e-5
+        # a tolerance below what the NLML's rounding resolves ends in "no
+        # further decrease" (status 2), but at a stationary point
+        r2 = orc.batch_gp2d_train(csr, th0[None], 200, 1e-9)
+        assert r2["status"][0] in (0, 2)
+        assert projected_max(r2["theta"][0], r2["grad"][0], orc) <= 1e-5
+        assert r2["nlml"][0] == pytest.approx(ref.fun, rel=1e-9, abs=1e-7)
+    assert hits >= 2, hits

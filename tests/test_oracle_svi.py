@@ -172,3 +172,60 @@ def test_reference_stream_vs_philox_statistics(orc, svi_jobs):
     dmu = np.abs(mu_p[:, :6] - ref[0][:, :6]) / np.maximum(sig, 1e-3)
     dsig = np.abs(np.exp(ls_p[:, :6]) - sig) / np.maximum(sig, 1e-3)
     assert np.median(dmu) < 0.35 and np.median(dsig) < 0.35, (np.median(dmu), np.median(dsig))
+
+
+def test_dlog_normal_cdf_is_the_derivative_of_log_normal_cdf(orc):
+    """The exact-tail adjoint: against central differences of the oracle's
+    log_normal_cdf on both sides of the polynomial branch point (x = 0), deep
+    in the lower tail (where the reference's phi / max(Phi, 1e-300) reaches
+    1e280) and on the asymptote below -30; and against the exact Mills
+    ratio phi / Phi of the true normal CDF to the A&S method's accuracy."""
+    import math
+
+    for x in np.concatenate([np.linspace(-29.5, 7.5, 149), [-45.0, -31.0, -0.3, 0.3, 2.0]]):
+        if abs(x) < 1e-3:
+            continue  # the A&S form's |z| kink: its one-sided slopes differ by 1e-3
+        h = 1e-6 * max(abs(x), 1.0)
+        fd = (orc.log_normal_cdf(x + h) - orc.log_normal_cdf(x - h)) / (2 * h)
+        d = orc.dlog_normal_cdf(x)
+        assert abs(d - fd) <= 1e-6 * max(abs(fd), 1.0), (x, d, fd)
+    for x in (-8.0, -3.0, -1.0, 0.0, 1.0, 3.0):
+        mills = math.exp(-0.5 * x * x) / math.sqrt(2 * math.pi) / (0.5 * math.erfc(-x / math.sqrt(2)))
+        assert abs(orc.dlog_normal_cdf(x) - mills) <= 2e-3 * max(mills, 1.0), (x, orc.dlog_normal_cdf(x), mills)
+    # the reference's ratio far in the tail, for contrast
+    z = -12.0
+    phi = math.exp(-0.5 * z * z) / math.sqrt(2 * math.pi)
+    cdf = max(0.5 * (1.0 + orc.erf_approx(z / math.sqrt(2))), 1e-300)
+    assert phi / cdf > 1e250 and abs(orc.dlog_normal_cdf(z)) < 20.0
+
+
+def test_exact_tail_gradient_is_the_elbo_derivative(orc, pkg):
+    """On the Villar SVI workload (a third of the points are upper limits),
+    the exact-tail gradient of one step's ELBO equals central differences of
+    that ELBO (fixed noise) to 1e-6 of the gradient scale, at the initial
+    guide (log sigma = -1) where the reference's ratio overflows."""
+    csr = pkg.synthetic.villar_svi_batch(4, seed=77)
+    nj = csr["offsets"].size - 1
+    prm, _, _ = orc.batch_pso(1, csr, 32, 60, 60, 42)
+    off = csr["offsets"]
+    OBS = ("times", "flux", "obs_var", "is_upper", "upper_flux")
+    for j in range(nj):
+        pc, pw = orc.svi_prior_for_model(1, prm[j])
+        eps = np.stack([orc.svi_normals(5, j, 0, s, 7) for s in range(16)])
+        obs = [csr[k][off[j]:off[j + 1]] for k in OBS]
+        mu, ls = prm[j].copy(), np.full(7, -1.0)
+        e, gm, gl = orc.svi_elbo_grad(1, mu, ls, pc, pw, eps, *obs, grad_mode=2)
+        f = lambda m_, l_: orc.svi_elbo_grad(1, m_, l_, pc, pw, eps, *obs, grad_mode=2)[0]
+        scale = max(np.abs(gm).max(), np.abs(gl).max(), 1.0)
+        assert np.isfinite(scale) and scale < 1e8, scale
+        for k in range(7):
+            h = 1e-6 * max(abs(mu[k]), 1.0)
+            p, m = mu.copy(), mu.copy()
+            p[k] += h
+            m[k] -= h
+            assert abs(gm[k] - (f(p, ls) - f(m, ls)) / (2 * h)) <= 1e-6 * scale, (j, k)
+            p, m = ls.copy(), ls.copy()
+            p[k] += 1e-6
+            m[k] -= 1e-6
+            # the entropy's d/d log sigma = 1 is included in gl
+            assert abs(gl[k] - (f(mu, p) - f(mu, m)) / 2e-6) <= 1e-6 * scale, (j, k)
