@@ -233,6 +233,9 @@ int lcf_batch_bb_temperature(lcf_ctx* ctx, int64_t n_sets, const int64_t* offset
 #define LCF_SVI_FD_GRADIENT 4u   /* numerical (forward-difference) model gradients instead of the
                                     analytic adjoint: the plain comparison path */
 #define LCF_SVI_FIXED_STEPS 8u   /* run all n_steps Adam steps (no early stop) */
+#define LCF_SVI_SHARED_STREAM 32u /* every job draws job 0's noise, so a job's fit does not depend on
+                                     its position in the batch (the CPU fit's fresh SmallRng(42) per
+                                     band, src/parametric.rs:2239) */
 #define LCF_SVI_EXACT_TAIL 16u   /* upper-limit adjoint = the exact derivative of the log_normal_cdf
                                     the ELBO sums, instead of the reference's phi / max(Phi, 1e-300)
                                     with Phi from erf_approx (cuda/svi.cu:200-204), which reaches
@@ -291,6 +294,10 @@ int lcf_batch_laplace(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_
  *                         joint Villar cost over bands band .. band+n_bands-1
  *                         of `data` (reference band first; dim = 7 n_bands,
  *                         n_bands <= 9);
+ *   LCF_CPSO_SWARM_MULTIBAZIN  pso_minimize[_seeded] on multi_bazin_cost
+ *                         (src/parametric.rs:983-1006) of band `band` with
+ *                         K = `model` components (dim = 4 K + 2), the swarms
+ *                         of fit_multi_bazin (:1008-1157);
  *   LCF_CPSO_POLISH       lbfgs_refine from x0 (start_cost = its cost; 7 pairs,
  *                         max_iters iterations, gradient norm 1e-6,
  *                         backtracking Armijo line search), kept only if it
@@ -306,6 +313,7 @@ int lcf_batch_laplace(lcf_ctx* ctx, const lcf_batch* data, const int32_t* model_
 #define LCF_CPSO_SWARM_BAND 0
 #define LCF_CPSO_SWARM_JOINT 1
 #define LCF_CPSO_POLISH 2
+#define LCF_CPSO_SWARM_MULTIBAZIN 3
 typedef struct lcf_pso_job {
     int32_t kind, model, band, n_bands, dim, n_particles, max_iters, stall_iters, use_prior, fixed_t0, seeded;
     uint32_t stream;

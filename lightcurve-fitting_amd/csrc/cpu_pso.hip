@@ -19,6 +19,10 @@
 //                     (:3905-4000): reference band + offsets, constraint per
 //                     band, band-balanced likelihood, population prior on the
 //                     reference and relative priors on the offsets;
+//   CPSO_SWARM_MULTIBAZIN  multi_bazin_cost (:983-1006) of fit_multi_bazin
+//                     (:1008-1157): K Bazin components as written
+//                     (bazin_component_eval, :919-929) plus the baseline, no
+//                     prior, 1e99 on a non-finite prediction, / n;
 //   CPSO_POLISH       lbfgs_refine (:1453-1497): L-BFGS (7 pairs, at most
 //                     max_iters iterations, gradient norm 1e-6) on the
 //                     clamped cost with the projected analytic gradient of
@@ -235,8 +239,41 @@ __device__ double joint_cost(const CpuPsoArgs& a, const CpuPsoJob& J, const doub
     return (bal + lp) / nt;
 }
 
+// multi_bazin_cost (src/parametric.rs:983-1006) of K components: lanes over
+// observations, the reference's component expression and summation order.
+__device__ double multi_bazin_job_cost(int K, const double* xs, const double4* o, int n) {
+    const int lane = lane_id();
+    const double se = ::exp(xs[4 * K + 1]);
+    const double se2 = se * se;
+    const double B = xs[4 * K];
+    double part = 0.0;
+    bool bad = false;
+    for (int i = lane; i < n; i += WAVE) {
+        const double4 ob = o[i];
+        double f = 0.0;
+        for (int c = 0; c < K; c++) {
+            const double a = ::exp(xs[4 * c]), tr = ::exp(xs[4 * c + 2]), tf = ::exp(xs[4 * c + 3]);
+            const double dt = ob.x - xs[4 * c + 1];
+            const double e_fall = ::exp(-dt / tf);
+            const double sig = 1.0 / (1.0 + ::exp(-dt / tr));
+            f += a * e_fall * sig;
+        }
+        f += B;
+        if (!__builtin_isfinite(f)) { bad = true; continue; }
+        const double tv = ob.z + se2;
+        if (ob.w != 0.0) part -= log_normal_cdf<double>((ob.y - f) / ::sqrt(tv));
+        else {
+            const double d = f - ob.y;
+            part += d * d / tv + ::log(tv);
+        }
+    }
+    if (wave_any(bad)) return 1e99;
+    return wave_sum(part) / (double)(n > 1 ? n : 1);
+}
+
 // Cost of the vector in sm.xs (all lanes, after a barrier).
 __device__ double job_cost(const CpuPsoArgs& a, const CpuPsoJob& J, const double4* o, int n, Smem& sm) {
+    if (J.kind == CPSO_SWARM_MULTIBAZIN) return multi_bazin_job_cost(J.model, sm.xs, o, n);
     return J.kind == CPSO_SWARM_JOINT ? joint_cost(a, J, o, sm) : band_cost_dispatch(J, o, n, sm);
 }
 
@@ -325,7 +362,8 @@ __device__ void swarm(const CpuPsoArgs& a, const CpuPsoJob& J, int jid, const do
             break;
         }
     }
-    if (own) a.out_x[(int64_t)jid * CPSO_MAX_DIM + lane] = gb;
+    // every lane writes its column: zeros beyond the search dimensions
+    a.out_x[(int64_t)jid * CPSO_MAX_DIM + lane] = own ? gb : 0.0;
     if (lane == 0) {
         a.out_cost[jid] = gcost;
         a.out_iters[jid] = it;
@@ -492,7 +530,7 @@ __device__ void polish(const CpuPsoArgs& a, const CpuPsoJob& J, int jid, const d
     const double xc = fmin(fmax(bx, lo), hi);
     const double fc = bcost(xc);
     const bool keep = fc < J.start_cost && __builtin_isfinite(fc);
-    if (own) a.out_x[r0 + lane] = keep ? xc : x0;
+    a.out_x[r0 + lane] = own ? (keep ? xc : x0) : 0.0;
     if (lane == 0) {
         a.out_cost[jid] = keep ? fc : J.start_cost;
         a.out_iters[jid] = it;

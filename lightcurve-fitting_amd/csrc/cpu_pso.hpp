@@ -17,11 +17,12 @@ enum CpsoKind : int32_t {
     CPSO_SWARM_BAND = 0,   // PsoCost of one band (src/parametric.rs:1303-1370)
     CPSO_SWARM_JOINT = 1,  // joint Villar cost of a band group (:3905-4000)
     CPSO_POLISH = 2,       // lbfgs_refine from x0 (:1453-1497)
+    CPSO_SWARM_MULTIBAZIN = 3,  // multi_bazin_cost of one band, K = model components (:983-1006)
 };
 
 struct CpuPsoJob {
     int32_t kind;
-    int32_t model;        // model id (CPSO_SWARM_BAND, CPSO_POLISH)
+    int32_t model;        // model id (CPSO_SWARM_BAND, CPSO_POLISH); components K (CPSO_SWARM_MULTIBAZIN)
     int32_t band;         // first band (batch index)
     int32_t n_bands;      // joint: consecutive bands, reference band first
     int32_t dim;          // search dimensions

@@ -1145,7 +1145,7 @@ int lcf_cpu_pso(lcf_ctx* c, const lcf_batch* b, const lcf_pso_job* jobs, int64_t
     for (int64_t k = 0; k < n_jobs; k++) {
         const lcf_pso_job& J = jobs[k];
         const int nbands = J.kind == LCF_CPSO_SWARM_JOINT ? J.n_bands : 1;
-        if (J.kind < 0 || J.kind > 2) return fail(c, LCF_E_INVALID, "job %lld: unknown kind %d", (long long)k, J.kind);
+        if (J.kind < 0 || J.kind > 3) return fail(c, LCF_E_INVALID, "job %lld: unknown kind %d", (long long)k, J.kind);
         if (J.dim <= 0 || J.dim > lcf::CPSO_MAX_DIM)
             return fail(c, LCF_E_INVALID, "job %lld: dim %d outside [1, 64]", (long long)k, J.dim);
         if (J.band < 0 || nbands < 1 || (int64_t)J.band + nbands > nb)
@@ -1153,6 +1153,10 @@ int lcf_cpu_pso(lcf_ctx* c, const lcf_batch* b, const lcf_pso_job* jobs, int64_t
         if (J.kind == LCF_CPSO_SWARM_JOINT) {
             if (nbands > lcf::CPSO_MAX_BANDS || J.dim != 7 * nbands)
                 return fail(c, LCF_E_INVALID, "job %lld: a joint fit needs dim = 7 bands, bands <= 9", (long long)k);
+        } else if (J.kind == LCF_CPSO_SWARM_MULTIBAZIN) {
+            if (J.model < 1 || J.model > 4 || J.dim != 4 * J.model + 2)
+                return fail(c, LCF_E_INVALID, "job %lld: MultiBazin needs K in [1, 4] and dim = 4 K + 2",
+                            (long long)k);
         } else {
             if (J.model < 0 || J.model >= LCF_N_MODELS)
                 return fail(c, LCF_E_INVALID, "job %lld: unknown model id %d", (long long)k, J.model);

@@ -301,9 +301,12 @@ struct SviJob {
             return;
         }
         if (n <= a.tile) stage<R>(lds, g, n);
+        // the noise stream's job word: the job's index, or 0 for every job
+        // (the CPU fit seeds a fresh SmallRng(42) per band, src/parametric.rs:2239)
+        const uint32_t rng_job = (a.flags & LCF_SVI_SHARED_STREAM_FLAG) ? 0u : (uint32_t)job;
         R gmu[NP], gls[NP];
         if (a.grad_only) {
-            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, (uint32_t)job, (uint32_t)a.step0, S, G,
+            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)a.step0, S, G,
                             a.seed, (a.flags & LCF_SVI_EXACT_TAIL_FLAG) != 0, gmu, gls);
             double om = 0.0, ol = 0.0;
 #pragma unroll
@@ -328,7 +331,7 @@ struct SviJob {
         double ema = -1e300, best = -1e300;
         int stall = 0, step = 0;
         for (; step < a.n_steps;) {
-            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, (uint32_t)job, (uint32_t)step, S, G, a.seed,
+            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)step, S, G, a.seed,
                             exact_tail, gmu, gls);
             final_elbo = e;
             step++;

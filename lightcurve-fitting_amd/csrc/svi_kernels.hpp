@@ -10,6 +10,7 @@ constexpr uint32_t LCF_SVI_CPU_STOPPING_FLAG = 1u;  // min 200 / stall 50 (svi_f
 constexpr uint32_t LCF_SVI_INFLATE_FLAG = 2u;       // add ln 4 to the returned log sigma
 constexpr uint32_t LCF_SVI_FIXED_STEPS_FLAG = 8u;   // no early stop: all n_steps steps
 constexpr uint32_t LCF_SVI_EXACT_TAIL_FLAG = 16u;   // upper-limit adjoint = d log_normal_cdf / dz
+constexpr uint32_t LCF_SVI_SHARED_STREAM_FLAG = 32u; // every job draws the noise of job 0
 
 struct SviArgs {
     const void* obs;           // packed {t, y, var, flag} records (float4 / double4)

@@ -26,7 +26,7 @@ PSO_STALL_FROM_FIRST_EVAL = 1
 FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2 = 1, 2, 4
 FINALIZE_DEFAULT = FINALIZE_T0 | FINALIZE_ALL | FINALIZE_MAG_CHI2
 FINALIZE_SVI_CHI2 = 8
-CPSO_SWARM_BAND, CPSO_SWARM_JOINT, CPSO_POLISH = 0, 1, 2
+CPSO_SWARM_BAND, CPSO_SWARM_JOINT, CPSO_POLISH, CPSO_SWARM_MULTIBAZIN = 0, 1, 2, 3
 # struct lcf_pso_job (include/lcf_hip.h)
 PSO_JOB_DTYPE = np.dtype([("kind", "<i4"), ("model", "<i4"), ("band", "<i4"), ("n_bands", "<i4"), ("dim", "<i4"),
                           ("n_particles", "<i4"), ("max_iters", "<i4"), ("stall_iters", "<i4"),

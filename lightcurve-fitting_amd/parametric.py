@@ -23,7 +23,7 @@ from typing import Mapping, Sequence
 
 import numpy as np
 
-from .gpu import (ALL_GPU_MODELS, FINALIZE_DEFAULT, MAX_PARAMS, GpuBatchData, GpuContext, GpuModelName,
+from .gpu import (ALL_GPU_MODELS, FINALIZE_DEFAULT, FP64, MAX_PARAMS, GpuBatchData, GpuContext, GpuModelName,
                   MultiBazinResult, svi_prior_for_model)
 from .nonparametric import BandData
 
@@ -38,7 +38,7 @@ SviModelName = GpuModelName
 SVI_LR, SVI_N_STEPS, SVI_N_SAMPLES = 0.01, 1000, 4
 # the feature extractor's batched SVI (src/bin/extract_features.rs:697)
 PIPELINE_SVI_STEPS, PIPELINE_SVI_SAMPLES = 150, 2
-_SVI_CPU_STOPPING, _SVI_INFLATE_SIGMA, _SVI_FD_GRADIENT = 1, 2, 4
+_SVI_CPU_STOPPING, _SVI_INFLATE_SIGMA, _SVI_FD_GRADIENT, _SVI_SHARED_STREAM = 1, 2, 4, 32
 
 
 class UncertaintyMethod(enum.Enum):
@@ -245,10 +245,20 @@ def _svi_on_device(ctx: GpuContext, datas, models, pso, n_steps: int, n_samples:
                    cpu_semantics: bool):
     """Device SVI for (band, model) jobs; returns [(mu, log_sigma, elbo)].
     With ``cpu_semantics`` the CPU fit's stopping rule and ln 4 inflation
-    apply, and MetzgerKN (no analytic gradient) is differentiated
-    numerically as the CPU fit does."""
+    apply, every job draws the same noise stream (svi_fit seeds a fresh
+    SmallRng(42) per band), and MetzgerKN (no analytic gradient) is
+    differentiated numerically as the CPU fit does."""
     mids = np.asarray(models, dtype=np.int32)
-    flags = (_SVI_CPU_STOPPING | _SVI_INFLATE_SIGMA) if cpu_semantics else 0
+    flags = (_SVI_CPU_STOPPING | _SVI_INFLATE_SIGMA | _SVI_SHARED_STREAM) if cpu_semantics else 0
+    if cpu_semantics and ctx.precision != FP64:
+        # the CPU fit is FP64 throughout: run its SVI leg in FP64 whatever
+        # the context's swarm precision
+        prev = ctx.precision
+        ctx.set_precision(FP64)
+        try:
+            return _svi_on_device(ctx, datas, models, pso, n_steps, n_samples, lr, cpu_semantics)
+        finally:
+            ctx.set_precision(prev)
     res = {}
     for fd in (False, True):
         sel = np.nonzero((mids == int(GpuModelName.MetzgerKN)) == fd)[0]
@@ -416,7 +426,8 @@ __all__ = ["ALL_GPU_MODELS", "BandFitData", "GpuPsoBandResult", "ParametricBandR
 # search, posteriors, finalize and MultiBazin as batched device calls; the
 # host builds the jobs and assembles the records.
 # ---------------------------------------------------------------------------
-from .gpu import CPSO_POLISH, CPSO_SWARM_BAND, CPSO_SWARM_JOINT, FINALIZE_SVI_CHI2, PSO_JOB_DTYPE, _PSO_BOUNDS
+from .gpu import (CPSO_POLISH, CPSO_SWARM_BAND, CPSO_SWARM_JOINT, CPSO_SWARM_MULTIBAZIN, FINALIZE_SVI_CHI2,
+                  PSO_JOB_DTYPE, _PSO_BOUNDS)
 
 _T0_IDX = (2, 3, 2, 1, 1, 1, 1, 3)
 _LN10 = 2.30258509299404568402
@@ -467,6 +478,10 @@ def _clamp(v, lo, hi):
 
 def _job(kind, model, band, dim, lo, hi, *, n_bands=1, particles=30, iters=60, stall=12, prior=1, fixed_t0=0,
          t0=0.0, seeded=0, x0=None, seed=42, stream=0, start_cost=0.0):
+    """One lcf_pso_job record.  Every job of the CPU path keeps stream 0: the
+    reference seeds a fresh SmallRng(seed) for each pso_minimize call, so
+    its swarms draw the same numbers for every band, and a band's fit does
+    not depend on which batch it is in."""
     j = np.zeros((), dtype=PSO_JOB_DTYPE)
     j["kind"], j["model"], j["band"], j["n_bands"], j["dim"] = kind, model, band, n_bands, dim
     j["n_particles"], j["max_iters"], j["stall_iters"] = particles, iters, stall
@@ -507,8 +522,7 @@ def fit_single_models(ctx: GpuContext, batch: GpuBatchData, datas: Sequence[Band
         np_ = len(lo)
         for r, sd in enumerate(_SEEDS):
             x0 = seeds[r]
-            jobs.append(_job(CPSO_SWARM_BAND, m, b, np_, lo, hi, seeded=int(x0 is not None), x0=x0, seed=sd,
-                             stream=b))
+            jobs.append(_job(CPSO_SWARM_BAND, m, b, np_, lo, hi, seeded=int(x0 is not None), x0=x0, seed=sd))
         meta.append((b, m, lo, hi))
     if not jobs:
         return {}
@@ -524,7 +538,7 @@ def fit_single_models(ctx: GpuContext, batch: GpuBatchData, datas: Sequence[Band
                 bc, bp = float(c), list(r["x"][3 * k + rr, :np_])
         best[(b, m)] = (bp, bc)
         if bp and m != int(GpuModelName.MetzgerKN):
-            pol.append(_job(CPSO_POLISH, m, b, np_, lo, hi, iters=50, x0=bp, start_cost=bc, stream=b))
+            pol.append(_job(CPSO_POLISH, m, b, np_, lo, hi, iters=50, x0=bp, start_cost=bc))
             pol_meta.append((b, m, np_))
     if pol:
         rp = ctx.cpu_pso(batch, np.array(pol))
@@ -587,7 +601,7 @@ def profile_t0_search_batch(ctx: GpuContext, batch: GpuBatchData, datas: Sequenc
             lo, hi = pso_bounds_cpu(m)
             t0i = _T0_IDX[m]
             del lo[t0i], hi[t0i]
-            jobs.append(_job(CPSO_SWARM_BAND, m, b, len(lo), lo, hi, prior=0, fixed_t0=1, t0=t0, seed=42, stream=b))
+            jobs.append(_job(CPSO_SWARM_BAND, m, b, len(lo), lo, hi, prior=0, fixed_t0=1, t0=t0, seed=42))
             meta.append((b, t0, t0i, len(lo)))
         r = ctx.cpu_pso(batch, np.array(jobs)) if jobs else None
         res = []
@@ -629,25 +643,115 @@ def _posteriors(ctx, datas, models, params, method):
     return [(list(params[k]), list(r["log_sigma"][k, :len(params[k])]), float(r["elbo"][k])) for k in range(len(datas))]
 
 
-def _multi_bazin_first(ctx, datas_first):
-    """MultiBazin on each source's first band (the device's greedy fit; the
-    reference's CPU fit_multi_bazin returns nothing below 5 points)."""
+# fit_multi_bazin (src/parametric.rs:1008-1157)
+MB_MAX_K = 4
+
+
+def _mb_bounds(k: int, t_min: float, t_max: float) -> tuple[list, list]:
+    """multi_bazin_bounds (src/parametric.rs:966-981)."""
+    lo, hi = [], []
+    for _ in range(k):
+        lo += [-3.0, t_min - 30.0, -2.0, -2.0]
+        hi += [3.0, t_max + 30.0, 5.0, 6.0]
+    return lo + [-0.3, -5.0], hi + [0.3, 0.0]
+
+
+def _mb_eval(p, k: int, t: float) -> float:
+    """multi_bazin_eval (src/parametric.rs:931-940), the reference's
+    expression in binary64 (math.exp: the C library's exponential)."""
+    s = 0.0
+    for c in range(k):
+        a, tr, tf = math.exp(p[4 * c]), math.exp(p[4 * c + 2]), math.exp(p[4 * c + 3])
+        dt = t - p[4 * c + 1]
+        s += a * math.exp(-dt / tf) * (1.0 / (1.0 + math.exp(-dt / tr)))
+    return s + p[4 * k]
+
+
+def _mb_seed(data: BandFitData, prev, k: int, lo, hi) -> list:
+    """fit_multi_bazin's K-component start (:1045-1082): the K - 1 components,
+    a component at the largest detection residual (first index on ties;
+    observation 0 and ln 1e-10 when there is no detection), B and sigma_x of
+    the K - 1 fit, clamped to the box."""
+    nc = 4 * (k - 1)
+    peak_idx, peak_val = 0, -math.inf
+    for i in range(data.times.size):
+        if data.is_upper[i]:
+            continue
+        r = float(data.flux[i]) - _mb_eval(prev, k - 1, float(data.times[i]))
+        if r > peak_val:
+            peak_val, peak_idx = r, i
+    init = list(prev[:nc]) + [math.log(max(peak_val, 1e-10)), float(data.times[peak_idx]), 1.0, 1.0,
+                              prev[nc], prev[nc + 1]]
+    return [_clamp(init[i], lo[i], hi[i]) for i in range(len(init))]
+
+
+def fit_multi_bazin_batch(ctx: GpuContext, datas: Sequence[BandFitData | None]) -> list:
+    """fit_multi_bazin (src/parametric.rs:1008-1157) for a list of bands
+    (None entries skipped): for K = 1..4 the swarms run as device jobs
+    (30 particles x 60 iterations, stall 12; seed 42, then 137 from the
+    residual-seeded start for K >= 2, then 271; the best in that order),
+    BIC = 2 cost n + (4 K + 2) ln n, stop once a K's BIC exceeds K - 1's by
+    more than 2.  Bands of fewer than 5 points give None, as does a fit
+    without parameters."""
     from .gpu import BatchSource
 
-    idx = [k for k, d in enumerate(datas_first) if d is not None and d.times.size >= 5]
-    out = [None] * len(datas_first)
+    idx = [k for k, d in enumerate(datas) if d is not None and d.times.size >= 5]
+    out = [None] * len(datas)
     if not idx:
         return out
-    srcs = [BatchSource(datas_first[k].times, datas_first[k].flux, datas_first[k].obs_var,
-                        datas_first[k].is_upper.astype(bool), datas_first[k].upper_flux) for k in idx]
-    b = GpuBatchData(ctx, srcs)
+    sub = [datas[k] for k in idx]
+    batch = GpuBatchData(ctx, csr=_fit_csr(sub))
+    st = [{"best_k": 1, "params": [], "cost": math.inf, "bic": math.inf, "pk_cost": [], "pk_bic": [],
+           "prev": None, "done": False} for _ in sub]
     try:
-        mb = ctx.batch_pso_multi_bazin(b, None, 30, 60, 12, 42)
+        for k in range(1, MB_MAX_K + 1):
+            live = [b for b in range(len(sub)) if not st[b]["done"]]
+            if not live:
+                break
+            jobs, meta = [], []
+            for b in live:
+                d = sub[b]
+                lo, hi = _mb_bounds(k, float(np.min(d.times)), float(np.max(d.times)))
+                init = _mb_seed(d, st[b]["prev"], k, lo, hi) if k > 1 and st[b]["prev"] is not None else None
+                kw = dict(particles=30, iters=60, stall=12, prior=0)
+                r = [_job(CPSO_SWARM_MULTIBAZIN, k, b, 4 * k + 2, lo, hi, seed=42, **kw)]
+                if init is not None:
+                    r.append(_job(CPSO_SWARM_MULTIBAZIN, k, b, 4 * k + 2, lo, hi, seed=137, seeded=1, x0=init, **kw))
+                r.append(_job(CPSO_SWARM_MULTIBAZIN, k, b, 4 * k + 2, lo, hi, seed=271, **kw))
+                meta.append((b, len(jobs), len(r)))
+                jobs += r
+            res = ctx.cpu_pso(batch, np.array(jobs))
+            for b, j0, nr in meta:
+                np_ = 4 * k + 2
+                p1, c1 = list(res["x"][j0, :np_]), float(res["cost"][j0])
+                for j in range(j0 + 1, j0 + nr):
+                    if res["cost"][j] < c1:
+                        p1, c1 = list(res["x"][j, :np_]), float(res["cost"][j])
+                s = st[b]
+                n = float(sub[b].times.size)
+                bic = 2.0 * c1 * n + np_ * math.log(n)
+                s["pk_cost"].append(c1)
+                s["pk_bic"].append(bic)
+                if bic < s["bic"]:
+                    s["bic"], s["cost"], s["best_k"], s["params"] = bic, c1, k, p1
+                s["prev"] = p1
+                if k > 1 and bic > s["pk_bic"][k - 2] + 2.0:
+                    s["pk_cost"] += [math.nan] * (MB_MAX_K - k)
+                    s["pk_bic"] += [math.nan] * (MB_MAX_K - k)
+                    s["done"] = True
     finally:
-        b.close()
-    for k, r in zip(idx, mb):
-        out[k] = r if r.params else None
+        batch.close()
+    for b, k in enumerate(idx):
+        s = st[b]
+        if s["params"]:
+            out[k] = MultiBazinResult(s["best_k"], s["params"], s["cost"], s["bic"], s["pk_cost"], s["pk_bic"])
     return out
+
+
+def _multi_bazin_first(ctx, datas_first):
+    """MultiBazin on each source's first band: the CPU path's
+    fit_multi_bazin."""
+    return fit_multi_bazin_batch(ctx, datas_first)
 
 
 def fit_parametric_model_batch(ctx: GpuContext, sources: Sequence[Mapping[str, BandData]],
@@ -734,7 +838,7 @@ def fit_parametric_multiband_batch(ctx: GpuContext, sources: Sequence[Mapping[st
             x0 += [_clamp(_REL_MEAN[j], lo[7 * b + j], hi[7 * b + j]) for j in range(7)]
         for r, sd in enumerate(_SEEDS):
             jobs.append(_job(CPSO_SWARM_JOINT, 1, first[s], 7 * B, lo, hi, n_bands=B, particles=200, iters=1500,
-                             stall=60, seeded=int(r == 1), x0=x0, seed=sd, stream=first[s]))
+                             stall=60, seeded=int(r == 1), x0=x0, seed=sd))
         meta.append((s, B))
     try:
         r = ctx.cpu_pso(batch, np.array(jobs))

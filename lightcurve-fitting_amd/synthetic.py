@@ -241,6 +241,33 @@ def bazin_mag_batch(n_sources: int, seed: int = 0, n_bands: int = 3, n_min: int 
             "t_lo": np.zeros(nb), "t_hi": tmax[band_src]}
 
 
+def parametric_sources(n_sources: int = 100, seed: int = 0, n_per_band: int = 30, bands=("g", "r", "i")) -> list:
+    """The plain parametric workload as the reference's fit_parametric takes
+    it: ``n_sources`` Bazin sources (A ~ U(100, 1000) uJy scaled per band by
+    U(0.7, 1.3), t0 ~ U(20, 80), tau_rise ~ U(1, 10), tau_fall ~ U(10, 60),
+    B ~ N(0, 5); noise sigma = 0.03 A + 2 uJy) with ``n_per_band`` points per
+    band, times uniform on [0, 100] d, sorted, given as magnitudes (23.9 -
+    2.5 log10 max(F, 1), error 1.0857 sigma / max(F, 1)) and turned into flux
+    bands by build_flux_bands (src/common.rs:79-111).  Returns one
+    {band name: BandData} map per source, in g, r, i order."""
+    from .common import build_flux_bands
+
+    c = bazin_mag_batch(n_sources, seed=seed, n_bands=len(bands), n_min=n_per_band, n_max=n_per_band,
+                        t_span=100.0)
+    off = c["offsets"]
+    out = []
+    for s in range(n_sources):
+        ts, ms, es, bs = [], [], [], []
+        for k, name in enumerate(bands):
+            b = s * len(bands) + k
+            ts += list(c["times"][off[b]:off[b + 1]])
+            ms += list(c["mags"][off[b]:off[b + 1]])
+            es += list(c["mag_errs"][off[b]:off[b + 1]])
+            bs += [name] * int(off[b + 1] - off[b])
+        out.append(build_flux_bands(ts, ms, es, bs))
+    return out
+
+
 GRI_WAVELENGTH = (4770.0, 6231.0, 7625.0)  # effective wavelengths of g, r, i (src/gp2d.rs:84-90), Angstrom
 
 

@@ -127,8 +127,37 @@ Obs band_obs(const JobCtx& c, int b) {
     return Obs{c.t + a, c.y + a, c.v + a, c.up + a, (int)(c.off[b + 1] - a)};
 }
 
+// multi_bazin_cost (src/parametric.rs:983-1006): K components as written,
+// the device's strided partial sums and butterfly.
+double multi_bazin_job_cost(int K, const double* xs, const Obs& o) {
+    const double se = std::exp(xs[4 * K + 1]);
+    const double se2 = se * se;
+    double part[64] = {0};
+    for (int i = 0; i < o.n; i++) {
+        double f = 0.0;
+        for (int c = 0; c < K; c++) {
+            const double a = std::exp(xs[4 * c]), tr = std::exp(xs[4 * c + 2]), tf = std::exp(xs[4 * c + 3]);
+            const double dt = o.t[i] - xs[4 * c + 1];
+            const double e_fall = std::exp(-dt / tf);
+            const double sig = 1.0 / (1.0 + std::exp(-dt / tr));
+            f += a * e_fall * sig;
+        }
+        f += xs[4 * K];
+        if (!std::isfinite(f)) return 1e99;
+        const double tv = o.v[i] + se2;
+        double& s = part[i & 63];
+        if (o.up[i]) s -= log_normal_cdf((o.y[i] - f) / std::sqrt(tv));
+        else {
+            const double d = f - o.y[i];
+            s += d * d / tv + std::log(tv);
+        }
+    }
+    return bfly(part) / (double)std::max(o.n, 1);
+}
+
 double job_cost(const JobCtx& c, const double* xs) {
     const orc_pso_job& J = *c.J;
+    if (J.kind == ORC_CPSO_SWARM_MULTIBAZIN) return multi_bazin_job_cost(J.model, xs, band_obs(c, J.band));
     if (J.kind == ORC_CPSO_SWARM_JOINT) {
         const int B = J.n_bands;
         const int n_tot = (int)(c.off[J.band + B] - c.off[J.band]);
@@ -387,8 +416,12 @@ extern "C" int orc_cpu_pso(const int64_t* off, const double* t, const double* fl
     for (int64_t i = 0; i < n_obs; i++) y[(size_t)i] = is_upper[i] ? upper_flux[i] : flux[i];
     for (int64_t k = 0; k < n_jobs; k++) {
         const orc_pso_job& J = jobs[k];
-        if (J.dim <= 0 || J.dim > 64 || J.kind < 0 || J.kind > 2) return -1;
-        if (J.kind != ORC_CPSO_SWARM_JOINT && orc::n_params(J.model) < 0) return -1;
+        if (J.dim <= 0 || J.dim > 64 || J.kind < 0 || J.kind > 3) return -1;
+        if (J.kind == ORC_CPSO_SWARM_MULTIBAZIN) {
+            if (J.model < 1 || J.model > 4 || J.dim != 4 * J.model + 2) return -1;
+        } else if (J.kind != ORC_CPSO_SWARM_JOINT && orc::n_params(J.model) < 0) {
+            return -1;
+        }
     }
     orc::parallel_for(n_jobs, n_threads, [&](int64_t k) {
         orc::JobCtx c{&jobs[k], off, t, y.data(), obs_var, is_upper};

@@ -121,6 +121,7 @@ int orc_batch_multi_bazin(int64_t n_src, const int64_t* offsets, const double* t
 #define ORC_SVI_INFLATE_SIGMA 2u /* flag: add ln 4 to log sigma (src/parametric.rs:2441-2446) */
 #define ORC_SVI_FIXED_STEPS 8u   /* flag: no early stop, all n_steps steps */
 #define ORC_SVI_EXACT_TAIL 16u   /* flag: upper-limit adjoint = exact derivative of log_normal_cdf */
+#define ORC_SVI_SHARED_STREAM 32u /* flag: every job draws job 0's Philox noise */
 
 /* Philox normals of (job, step, sample): eps[np]. */
 int orc_svi_normals(uint64_t seed, uint32_t job, uint32_t step, uint32_t sample, int np, double* eps);
@@ -288,6 +289,7 @@ int orc_batch_gp2d_train(int64_t n_src, const int64_t* off, const double* t, con
 #define ORC_CPSO_SWARM_BAND 0
 #define ORC_CPSO_SWARM_JOINT 1
 #define ORC_CPSO_POLISH 2
+#define ORC_CPSO_SWARM_MULTIBAZIN 3  /* model = K components, dim = 4 K + 2 */
 typedef struct orc_pso_job {
     int32_t kind, model, band, n_bands, dim, n_particles, max_iters, stall_iters, use_prior, fixed_t0, seeded;
     uint32_t stream;
@@ -298,6 +300,25 @@ typedef struct orc_pso_job {
 int orc_cpu_pso(const int64_t* off, const double* t, const double* flux, const double* obs_var,
                 const int32_t* is_upper, const double* upper_flux, int64_t n_obs, const orc_pso_job* jobs,
                 int64_t n_jobs, double* out_x, double* out_cost, int32_t* out_iters, int n_threads);
+
+/* ---- fit_parametric (src/parametric.rs:3563-4131) -------------------------
+ * The CPU parametric pipeline for n_src sources: source s has bands
+ * src_off[s] .. src_off[s+1]-1 of the band CSR band_off over (times, flux
+ * values, flux errors), in the order that stands for the band map's
+ * iteration order.  method 0 = Svi, 1 = Laplace; forced_model >= 0 runs
+ * fit_parametric_model with that model (no joint route).  One record of
+ * ORC_FP_RECORD doubles per input band (NaN where unset):
+ *   [0] rank of the band's result in its source's list (-1: none), [1] model,
+ *   [2..8] pso params, [9] pso chi2, [10..16] posterior mu, [17..23] log
+ *   sigma, [24] elbo, [25] n_obs, [26] magnitude chi2, [27..34] per-model
+ *   chi2 by model id (NaN = None), [35..42] key present (0/1), [43] route
+ *   (0 per band, 1 joint), [44] MultiBazin present, [45] best K, [46..63]
+ *   params, [64] cost, [65] BIC, [66..69] per-K cost, [70..73] per-K BIC,
+ *   [74..129] per-model params [8][7] (fit_all_models). */
+#define ORC_FP_RECORD 130
+int orc_fit_parametric(int64_t n_src, const int64_t* src_off, const int64_t* band_off, const double* t,
+                       const double* flux, const double* err, int fit_all_models, int method,
+                       int forced_model, double* out, int n_threads);
 
 #ifdef __cplusplus
 }

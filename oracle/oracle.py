@@ -52,7 +52,7 @@ SIGNATURES = {
 
 SVI_RNG_PHILOX, SVI_RNG_REFERENCE = 0, 1
 SVI_CPU_STOPPING, SVI_INFLATE_SIGMA = 1, 2
-SVI_FIXED_STEPS, SVI_EXACT_TAIL = 8, 16
+SVI_FIXED_STEPS, SVI_EXACT_TAIL, SVI_SHARED_STREAM = 8, 16, 32
 
 _LIB = None
 
@@ -655,7 +655,7 @@ def batch_fit_gp_2d(csr: dict, qt, qw, n_threads: int | None = None) -> dict:
     return {"hyp": hyp, "train_rms": rms, "n_train": ntr, "pred": pred, "std": sd}
 
 
-CPSO_SWARM_BAND, CPSO_SWARM_JOINT, CPSO_POLISH = 0, 1, 2
+CPSO_SWARM_BAND, CPSO_SWARM_JOINT, CPSO_POLISH, CPSO_SWARM_MULTIBAZIN = 0, 1, 2, 3
 PSO_JOB_DTYPE = np.dtype([("kind", "<i4"), ("model", "<i4"), ("band", "<i4"), ("n_bands", "<i4"), ("dim", "<i4"),
                           ("n_particles", "<i4"), ("max_iters", "<i4"), ("stall_iters", "<i4"),
                           ("use_prior", "<i4"), ("fixed_t0", "<i4"), ("seeded", "<i4"), ("stream", "<u4"),
@@ -681,3 +681,65 @@ def cpu_pso(csr: dict, jobs, n_threads: int | None = None) -> dict:
                        n_threads or default_threads())
     assert rc == 0
     return {"x": x, "cost": cost, "iters": it}
+
+
+FP_RECORD = 130
+
+
+def fit_parametric(sources, fit_all_models: bool = False, method: str = "Svi", forced_model: int = -1,
+                   n_threads: int | None = None) -> list:
+    """The CPU parametric pipeline (fit_parametric / fit_parametric_model)
+    per source.  ``sources``: list of lists of (times, flux values, flux
+    errors) bands, in map order.  Returns per source the result records in
+    the reference's order, each a dict with the ParametricBandResult fields
+    (``band`` = index of the input band) plus ``route`` (0 per band, 1
+    joint)."""
+    src_off = [0]
+    band_off = [0]
+    ts, fs, es = [], [], []
+    for bands in sources:
+        for t, f, e in bands:
+            ts.append(f64(t))
+            fs.append(f64(f))
+            es.append(f64(e))
+            band_off.append(band_off[-1] + len(t))
+        src_off.append(len(band_off) - 1)
+    cat = lambda v: np.ascontiguousarray(np.concatenate(v)) if v else np.zeros(1)
+    t, f, e = cat(ts), cat(fs), cat(es)
+    src_off = np.asarray(src_off, dtype=np.int64)
+    band_off = np.asarray(band_off, dtype=np.int64)
+    nb = int(band_off.size - 1)
+    out = np.zeros((max(nb, 1), FP_RECORD))
+    L = lib()
+    L.orc_fit_parametric.restype = _i32
+    L.orc_fit_parametric.argtypes = [_i64, _pi64, _pi64, _pd, _pd, _pd, _i32, _i32, _i32, _pd, _i32]
+    rc = L.orc_fit_parametric(len(sources), _p(src_off, C.c_int64), _p(band_off, C.c_int64), _p(t, C.c_double),
+                              _p(f, C.c_double), _p(e, C.c_double), int(fit_all_models),
+                              0 if method == "Svi" else 1, int(forced_model), _p(out, C.c_double),
+                              n_threads or default_threads())
+    assert rc == 0
+    res = []
+    for s in range(len(sources)):
+        recs = []
+        for b in range(src_off[s], src_off[s + 1]):
+            r = out[b]
+            if r[0] < 0:
+                continue
+            m = int(r[1])
+            np_ = n_params(m)
+            nan_none = lambda v: None if np.isnan(v) else float(v)
+            rec = {"band": int(b - src_off[s]), "rank": int(r[0]), "model": m, "pso_params": list(r[2:2 + np_]),
+                   "pso_chi2": nan_none(r[9]), "svi_mu": list(r[10:10 + np_]), "svi_log_sigma": list(r[17:17 + np_]),
+                   "svi_elbo": nan_none(r[24]), "n_obs": int(r[25]), "mag_chi2": nan_none(r[26]),
+                   "per_model_chi2": {k: nan_none(r[27 + k]) for k in range(8) if r[35 + k] > 0.5},
+                   "route": int(r[43]), "multi_bazin": None,
+                   "per_model_params": ({k: list(r[74 + 7 * k:74 + 7 * k + n_params(k)]) for k in range(8)
+                                         if r[35 + k] > 0.5} if fit_all_models else {})}
+            if r[44] > 0.5:
+                k = int(r[45])
+                rec["multi_bazin"] = {"best_k": k, "params": list(r[46:46 + 4 * k + 2]), "cost": float(r[64]),
+                                      "bic": float(r[65]), "per_k_cost": list(r[66:70]), "per_k_bic": list(r[70:74])}
+            recs.append(rec)
+        recs.sort(key=lambda x: x["rank"])
+        res.append(recs)
+    return res

@@ -180,7 +180,8 @@ static SviOut svi_job(int m, int64_t job, const double* pso, const double* pc, c
             if (rng_mode == ORC_SVI_RNG_REFERENCE)
                 for (int j = 0; j < np; j++) epsv[s * np + j] = xs.normal();
             else
-                philox_normals(seed, (uint32_t)job, (uint32_t)step, (uint32_t)s, np, &epsv[s * np]);
+                philox_normals(seed, (flags & ORC_SVI_SHARED_STREAM) ? 0u : (uint32_t)job, (uint32_t)step,
+                               (uint32_t)s, np, &epsv[s * np]);
         }
         final_elbo = elbo_grad_step(m, np, se, mu, ls, pc, pw, epsv.data(), S, t, f, v, up, uf, n,
                                     grad_mode, gmu, gls);

@@ -1,6 +1,7 @@
 """GPU parity of the CPU-path swarm engine (needs an MI355X) against the
 oracle through lcf_cpu_pso: per-band swarms of all eight models (plain,
-seeded, t0 held fixed), joint multi-band swarms of 1-3 bands, and L-BFGS
+seeded, t0 held fixed), joint multi-band swarms of 1-3 bands, MultiBazin
+swarms of K = 1..4 components, and L-BFGS
 polish jobs -- swarm best positions and costs identical to 1e-12 relative
 (the same Philox stream, the same summation order; only the device
 exponential and logarithm round differently), polish end points close and
@@ -53,6 +54,16 @@ def test_engine_matches_oracle(pkg, orc, ctx):
         jobs.append(job(orc, kind=orc.CPSO_SWARM_JOINT, model=1, band=B, n_bands=B, dim=7 * B, n_particles=40,
                         max_iters=30, stall_iters=10, seeded=1, lo=lo, hi=hi, x0=0.5 * (lo + hi), seed=137,
                         stream=B))
+    for k in (1, 2, 3, 4):  # fit_multi_bazin's swarms
+        t = csr["times"][csr["offsets"][4]:csr["offsets"][5]]
+        lo, hi = [], []
+        for _ in range(k):
+            lo += [-3.0, t.min() - 30.0, -2.0, -2.0]
+            hi += [3.0, t.max() + 30.0, 5.0, 6.0]
+        lo, hi = np.array(lo + [-0.3, -5.0]), np.array(hi + [0.3, 0.0])
+        jobs.append(job(orc, kind=orc.CPSO_SWARM_MULTIBAZIN, model=k, band=4, dim=4 * k + 2, n_particles=30,
+                        max_iters=60, stall_iters=12, seeded=int(k > 1), lo=lo, hi=hi, x0=0.5 * (lo + hi),
+                        seed=137))
     jobs = np.array(jobs)
     o = orc.cpu_pso(csr, jobs)
     # polish from the swarm results of the closed-form models

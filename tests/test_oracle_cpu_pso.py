@@ -12,6 +12,7 @@
   * recovery: a noise-free Bazin band fitted to within its noise floor;
   * the polish never raises the cost and lowers it from a perturbed start.
 """
+import math
 import numpy as np
 import pytest
 
@@ -120,3 +121,67 @@ def test_polish(orc):
     r = orc.cpu_pso(csr, np.array([j]))
     assert r["cost"][0] < c0 - 1e-3
     assert evaluate(orc, csr, 0, r["x"][0, :6]) == r["cost"][0]
+
+
+def _exp(x):
+    """IEEE exp: +inf on overflow (math.exp raises instead)."""
+    return math.exp(x) if x < 709.0 else (math.exp(x) if x <= 709.78 else math.inf)
+
+
+def multi_bazin_cost_py(p, k, t, f, v, up, uf):
+    """multi_bazin_cost (src/parametric.rs:983-1006) transcribed in plain
+    Python: K Bazin components plus the baseline, the censored term for
+    upper limits, 1e99 on a non-finite prediction, / max(n, 1)."""
+    se2 = math.exp(p[4 * k + 1]) ** 2
+    nll = 0.0
+    for i in range(len(t)):
+        pred = 0.0
+        for c in range(k):
+            a, tr, tf = math.exp(p[4 * c]), math.exp(p[4 * c + 2]), math.exp(p[4 * c + 3])
+            dt = t[i] - p[4 * c + 1]
+            pred += a * _exp(-dt / tf) * (1.0 / (1.0 + _exp(-dt / tr)))
+        pred += p[4 * k]
+        if not math.isfinite(pred):
+            return 1e99
+        tv = v[i] + se2
+        if up[i]:
+            z = (uf[i] - pred) / math.sqrt(tv)
+            # log_normal_cdf, A&S 7.1.26 erfc (src/parametric.rs:882-901)
+            if z > 8.0:
+                lz = 0.0
+            elif z < -30.0:
+                lz = -0.5 * z * z - 0.5 * math.log(2 * math.pi) - math.log(-z)
+            else:
+                zz = -z / math.sqrt(2.0)
+                tt = 1.0 / (1.0 + 0.3275911 * abs(zz))
+                poly = tt * (0.254829592 + tt * (-0.284496736 + tt * (1.421413741 + tt * (-1.453152027 + tt * 1.061405429))))
+                ec = poly * math.exp(-zz * zz)
+                lz = math.log(max(0.5 * ec if zz >= 0.0 else 1.0 - 0.5 * ec, 1e-300))
+            nll -= lz
+        else:
+            d = pred - f[i]
+            nll += d * d / tv + math.log(tv)
+    return nll / max(len(t), 1)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_multi_bazin_job_cost_brute_force(orc, pkg, k):
+    """The MultiBazin swarm job (fit_multi_bazin's swarms): the cost it
+    reports for its returned position equals the plain-Python
+    multi_bazin_cost there, and the swarm respects its box."""
+    rng = np.random.default_rng(40 + k)
+    c = pkg.synthetic.ragged_selection_batch(3, seed=60 + k, n_min=20, n_max=60)
+    csr = pkg.synthetic.subset(c, np.arange(1))
+    t = csr["times"]
+    lo, hi = [], []
+    for _ in range(k):
+        lo += [-3.0, t.min() - 30.0, -2.0, -2.0]
+        hi += [3.0, t.max() + 30.0, 5.0, 6.0]
+    lo, hi = np.array(lo + [-0.3, -5.0]), np.array(hi + [0.3, 0.0])
+    j = job(orc, kind=orc.CPSO_SWARM_MULTIBAZIN, model=k, band=0, dim=4 * k + 2, n_particles=20, max_iters=15,
+            stall_iters=15, lo=lo, hi=hi, seed=42)
+    o = orc.cpu_pso(csr, np.array([j]))
+    x = o["x"][0, :4 * k + 2]
+    assert np.all(x >= lo) and np.all(x <= hi)
+    want = multi_bazin_cost_py(list(x), k, *(csr[q] for q in OBS))
+    assert o["cost"][0] == pytest.approx(want, rel=1e-12, abs=1e-300)
