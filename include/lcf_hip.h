@@ -119,7 +119,10 @@ int lcf_batch_all_models(lcf_ctx* ctx, const lcf_batch* data, int n_particles,
  * stream per (source, particle) seeded by a Philox block, so a source's fit
  * does not depend on the batch it is in (the reference's sequential
  * SmallRng stream is not reproduced).  The FP32 cost loop's
- * censored-likelihood table is built once per device by lcf_ctx_create. */
+ * censored-likelihood table is built once per device by lcf_ctx_create.
+ * With FP32 precision the swarms run in FP32 and the cost returned for each
+ * model (and compared by the model choice) is the reference's FP64 cost at
+ * the returned parameters, from one FP64 evaluation per band and model. */
 int lcf_batch_fit_models(lcf_ctx* ctx, const lcf_batch* data, const int32_t* models,
                          int n_models, int use_gate, double gate, int n_particles,
                          int max_iters, int stall_iters, uint64_t seed, uint32_t flags,

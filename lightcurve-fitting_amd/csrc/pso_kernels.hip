@@ -9,6 +9,7 @@ const void* pso_kernel_f32(int model, int ppl, bool persistent);
 const void* pso_kernel_f64(int model, int ppl, bool persistent);
 const void* mb_kernel_f32(int ppl);
 const void* mb_kernel_f64(int ppl);
+const void* cost64_kernel(int model);
 
 static const void* pick_kernel(int model, bool fp64, int ppl, bool persistent) {
     return fp64 ? pso_kernel_f64(model, ppl, persistent) : pso_kernel_f32(model, ppl, persistent);
@@ -47,6 +48,14 @@ hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool pers
     }
     void* args[] = {const_cast<PsoArgs*>(&a)};
     return hipLaunchKernel(k, dim3(grid), dim3(64), args, pso_lds_bytes(fp64, a.tile), st);
+}
+
+hipError_t launch_cost64(int model, const PsoArgs& a, hipStream_t st) {
+    const void* k = cost64_kernel(model);
+    if (!k) return hipSuccess;  // no FP32 cost-loop form: nothing to re-cost
+    if (a.n_bands <= 0 || !a.obs64) return hipErrorInvalidValue;
+    void* args[] = {const_cast<PsoArgs*>(&a)};
+    return hipLaunchKernel(k, dim3(a.n_bands), dim3(64), args, 0, st);
 }
 
 hipError_t pso_occupancy(int model, bool fp64, int ppl, int tile, int* blocks_per_cu) {

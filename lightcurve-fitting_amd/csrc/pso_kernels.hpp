@@ -8,6 +8,7 @@ namespace lcf {
 
 struct PsoArgs {
     const void* obs;            // packed records: float4 (FP32) or double4 (FP64) {t, y, var, flag}
+    const double4* obs64;       // the FP64 records (the FP32 swarms' final FP64 cost)
     const int64_t* offsets;     // [n_bands + 1] CSR offsets into obs
     const int32_t* order;       // [n_tasks] band indices, decreasing size
     int n_tasks;
@@ -56,6 +57,9 @@ size_t pso_lds_bytes(bool fp64, int tile);
 hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool persistent, int grid,
                       hipStream_t st);
 hipError_t pso_occupancy(int model, bool fp64, int ppl, int tile, int* blocks_per_cu);
+// FP64 re-costing of the best positions of an FP32 swarm launch (slot a.slot,
+// every band of the batch; models 0-6, others a no-op).
+hipError_t launch_cost64(int model, const PsoArgs& a, hipStream_t st);
 // Builds the FP32 cost loop's censored-likelihood table on the current device
 // the first time it is called there (one launch, then a wait on `st`); later
 // calls only check a flag.

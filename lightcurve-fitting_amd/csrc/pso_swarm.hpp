@@ -351,6 +351,52 @@ struct Swarm {
     }
 };
 
+// The reference's FP64 cost (the FP64 swarm's arithmetic: reference model
+// form, 1e99 for a non-finite prediction, NLL / n + prior / n^2) of one
+// wave-uniform parameter vector, lanes over the band's FP64 records.
+template <int MODEL>
+__device__ double cost64_at(const double* p, const double4* __restrict__ g64, int n) {
+    constexpr int NP = ModelTraits<MODEL>::NP;
+    const int lane = lane_id();
+    Model<MODEL, double> mdl;
+    mdl.setup(p);
+    const double se = ::exp(p[NP - 1]);
+    const double se2 = se * se;
+    double part = 0.0;
+    bool bad = false;
+    for (int i = lane; i < n; i += WAVE) {
+        const double4 o = g64[i];
+        double f;
+        const bool good = mdl.pred(o.x, f) && __builtin_isfinite(f);
+        if (!good) {
+            bad = true;
+            continue;
+        }
+        const double tv = o.z + se2;
+        if (o.w != 0.0) {
+            part -= log_normal_cdf<double>((o.y - f) / ::sqrt(tv));
+        } else {
+            const double d = f - o.y;
+            part += d * d / tv + ::log(tv);
+        }
+    }
+    bad = __ballot(bad) != 0ull;
+    const double sum = wave_sum(part);
+    const double nn = (double)(n > 0 ? n : 1);
+    double pri = 0.0;
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        const double w = c_pw[MODEL][j];
+        if (w > 0.0) {
+            const double z = (p[j] - c_pc[MODEL][j]) / w;
+            pri += 0.5 * z * z;
+        }
+    }
+    double c = (bad || n <= 0) ? 1e99 : sum / nn;
+    c = c + pri / (nn * nn);
+    return __builtin_isfinite(c) ? c : 1e99;
+}
+
 // Fits model MODEL on one band and writes its slot of the per-model outputs.
 // With a gate, the band is skipped unless the gate model's cost >= gate
 // (batch_model_select, src/gpu_cuda.rs:584-619).
@@ -396,6 +442,30 @@ __device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
         a.model_cost[slot] = cst;
         a.model_iters[slot] = iters;
     }
+}
+
+// After an FP32 swarm launch: the returned best position of every fitted
+// band of slot `slot` re-costed in FP64 (cost64_at), so that the cost a fit
+// returns -- and the model selection that compares costs -- is the
+// reference's at the returned parameters, also where the reference's cost
+// jumps (an FP64 exponential overflowing at a phase of 709.78 d, a threshold
+// that FP32 times cannot place to better than 1e-4 d).  One wavefront per
+// band; gated bands (no iterations) and empty bands keep their sentinel.
+// A separate launch, so the FP64 registers do not lower the swarm kernel's
+// occupancy.
+template <int MODEL>
+__global__ __launch_bounds__(64) void pso_cost64_kernel(PsoArgs a) {
+    const int band = (int)blockIdx.x;
+    if (band >= a.n_bands) return;
+    const int64_t slot = (int64_t)a.slot * a.n_bands + band;
+    const int n = (int)(a.offsets[band + 1] - a.offsets[band]);
+    if (n <= 0 || a.model_iters[slot] <= 0) return;
+    constexpr int NP = ModelTraits<MODEL>::NP;
+    double p[NP];
+#pragma unroll
+    for (int d = 0; d < NP; d++) p[d] = a.model_params[slot * 7 + d];
+    const double c = cost64_at<MODEL>(p, a.obs64 + a.offsets[band], n);
+    if (lane_id() == 0) a.model_cost[slot] = c;
 }
 
 // Persistent form: each single-wave workgroup pulls band indices from a
@@ -616,6 +686,19 @@ static const void* kernel_ptr_m(int model, bool persistent) {
         case 5: return kernel_ptr<5, R, PPL>(persistent);
         case 6: return kernel_ptr<6, R, PPL>(persistent);
         case 7: return kernel_ptr<7, R, PPL>(persistent);
+        default: return nullptr;
+    }
+}
+
+static const void* pick_cost64(int model) {
+    switch (model) {
+        case 0: return (const void*)pso_cost64_kernel<0>;
+        case 1: return (const void*)pso_cost64_kernel<1>;
+        case 2: return (const void*)pso_cost64_kernel<2>;
+        case 3: return (const void*)pso_cost64_kernel<3>;
+        case 4: return (const void*)pso_cost64_kernel<4>;
+        case 5: return (const void*)pso_cost64_kernel<5>;
+        case 6: return (const void*)pso_cost64_kernel<6>;
         default: return nullptr;
     }
 }

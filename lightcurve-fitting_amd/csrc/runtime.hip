@@ -174,6 +174,7 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
 
     lcf::PsoArgs a{};
     a.obs = fp64 ? (const void*)b->d_obs64 : (const void*)b->d_obs32;
+    a.obs64 = b->d_obs64;
     a.offsets = b->d_off;
     a.order = b->d_order;
     a.n_bands = (int)ns;
@@ -210,6 +211,7 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
             else
                 HIPCHK(c, hipMemsetAsync(d.queue, 0, sizeof(int), c->stream));
             HIPCHK(c, lcf::launch_pso(model, a, fp64, ppl, true, grid, c->stream));
+            if (!fp64) HIPCHK(c, lcf::launch_cost64(model, a, c->stream));
         } else {
             // One synchronised launch per power-of-two size class, in
             // decreasing size (the band order is already size-sorted).
@@ -232,6 +234,7 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
                 HIPCHK(c, hipStreamSynchronize(c->stream));
                 i = j;
             }
+            if (!fp64) HIPCHK(c, lcf::launch_cost64(model, a, c->stream));
         }
     }
     HIPCHK(c, hipEventRecord(c->ev_slot[n_models], c->stream));

@@ -15,8 +15,10 @@ are re-evaluated by the oracle itself.  Measured on MI355X (3932 bands):
     12.0 % of bands and better on 10.5 %: a symmetric spread (FP32 finds
     optima as good as FP64's), which a polish of both end points by L-BFGS-B
     does not remove (different basins, not unconverged minima);
-  * the cost FP32 reports for its returned parameters equals the oracle's
-    FP64 cost there to 1.1e-6 relative.
+  * the cost FP32 reports for its returned parameters is the reference's
+    FP64 cost there (the swarm re-costs its best position in FP64; before
+    that, its FP32 cost agreed to 1.1e-6, except where the reference's cost
+    jumps at a phase of 709.78 d, which FP32 times cannot place).
 The selection rule is the reference's: strict '<' over models in order
 (src/gpu_cuda.rs:595-616).
 """
@@ -84,4 +86,5 @@ def test_fp32_reported_cost_is_the_fp64_cost_at_its_parameters(runs, orc):
                 assert got >= 1e99
                 continue
             worst = max(worst, abs(got - want) / max(abs(want), 1.0))
-    assert worst <= 1e-5, worst
+    # the FP32 swarms re-cost their best positions in FP64
+    assert worst <= 1e-12, worst

@@ -172,3 +172,21 @@ def test_tables(orc):
     c, w = orc.prior_params(6)
     assert list(c) == [0.0, 0.0, 1.0, 0.8, 2.0, 0.0] and list(w) == [2.0, 2.0, 2.0, 1.0, 1.0, 2.0]
     assert [orc.n_params(m) for m in range(8)] == [6, 7, 7, 5, 5, 5, 6, 5]
+
+
+@pytest.mark.parametrize("m", [0, 1, 2, 3, 4])
+def test_eval_batch_against_numpy_transcription(orc, pkg, m):
+    """The oracle's closed-form models against the package's independent
+    numpy transcription of the model formulas (synthetic.model_flux, written
+    from the formulas, not from the oracle), on random parameters of the
+    synthetic workloads' boxes and times before, across and after the peak."""
+    lo, hi = (np.asarray(x) for x in pkg.synthetic._BOXES[m])
+    rng = np.random.default_rng(100 + m)
+    t = np.linspace(-20.0, 200.0, 221)
+    for _ in range(25):
+        p = lo + rng.uniform(size=lo.size) * (hi - lo)
+        want = pkg.synthetic.model_flux(m, p, t)
+        got = orc.eval_batch(m, p, t)
+        fin = np.isfinite(want)
+        assert np.array_equal(fin, np.isfinite(got))
+        assert np.allclose(got[fin], want[fin], rtol=1e-11, atol=1e-14), np.max(np.abs(got - want))
