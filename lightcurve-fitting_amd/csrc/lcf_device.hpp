@@ -178,6 +178,82 @@ __device__ __forceinline__ R wave_sum(R v) {
     return v;
 }
 
+// Cross-lane moves without LDS.  swap32: lanes 32..63 of a trade places with
+// lanes 0..31 of b (v_permlane32_swap_b32); swap16: the odd 16-lane rows of
+// a trade places with the even rows of b (v_permlane16_swap_b32); dpp<CTRL>:
+// a DPP move within 16-lane rows.  All lanes must be active.
+__device__ __forceinline__ void swap32(float& a, float& b) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap16(float& a, float& b) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap32(double& a, double& b) {
+    uint64_t x = (uint64_t)__double_as_longlong(a), y = (uint64_t)__double_as_longlong(b);
+    auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)x, (uint32_t)y, false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(x >> 32), (uint32_t)(y >> 32), false, false);
+    a = __longlong_as_double((long long)(((uint64_t)hi[0] << 32) | lo[0]));
+    b = __longlong_as_double((long long)(((uint64_t)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ void swap16(double& a, double& b) {
+    uint64_t x = (uint64_t)__double_as_longlong(a), y = (uint64_t)__double_as_longlong(b);
+    auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)x, (uint32_t)y, false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(x >> 32), (uint32_t)(y >> 32), false, false);
+    a = __longlong_as_double((long long)(((uint64_t)hi[0] << 32) | lo[0]));
+    b = __longlong_as_double((long long)(((uint64_t)hi[1] << 32) | lo[1]));
+}
+constexpr int DPP_QP_1032 = 0xB1;        // quad_perm [1,0,3,2]: lane ^ 1
+constexpr int DPP_QP_2301 = 0x4E;        // quad_perm [2,3,0,1]: lane ^ 2
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;  // 7 - lane within 8-lane halves
+constexpr int DPP_ROW_ROR8 = 0x128;      // rotate by 8 within 16-lane rows: lane ^ 8
+template <int CTRL> __device__ __forceinline__ float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL> __device__ __forceinline__ double dpp(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, true);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sixteen wave sums at once (a reduce-scatter): returns in lane l the sum
+// over all 64 lanes of v[l >> 2].  Each stage halves the values a lane
+// carries, pairing lanes that differ in one bit (bit 5, 4, 3, then a mirror
+// partner that differs in bit 2) and ends with a full sum over the quad:
+// 8 + 4 + 2 + 1 + 2 cross-lane moves where sixteen butterflies take 96.
+template <typename R>
+__device__ __forceinline__ R wave_sum16(const R (&v)[16]) {
+    const int lane = lane_id();
+    R w8[8], w4[4], w2[2];
+#pragma unroll
+    for (int i = 0; i < 8; i++) {  // lanes 0..31 keep values 0..7, lanes 32..63 values 8..15
+        R a = v[i], b = v[8 + i];
+        swap32(a, b);
+        w8[i] = a + b;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {  // even rows keep the lower four, odd rows the upper
+        R a = w8[i], b = w8[4 + i];
+        swap16(a, b);
+        w4[i] = a + b;
+    }
+    const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        R keep = b3 ? w4[2 + i] : w4[i], send = b3 ? w4[i] : w4[2 + i];
+        w2[i] = keep + dpp<DPP_ROW_ROR8>(send);
+    }
+    R keep = b2 ? w2[1] : w2[0], send = b2 ? w2[0] : w2[1];
+    R w = keep + dpp<DPP_ROW_HALF_MIRROR>(send);
+    w += dpp<DPP_QP_2301>(w);
+    w += dpp<DPP_QP_1032>(w);
+    return w;
+}
+
 // Next task of a persistent single-wave workgroup from a device work queue.
 // Every lane takes part in the atomic (lane 0 adds 1, the others 0) and the
 // value lane 0 received is read back with readlane, so the fetch has no

@@ -527,4 +527,40 @@ __device__ __forceinline__ R dlog_normal_cdf(R x) {
     return -S2 * dE / (R(1) - E);
 }
 
+// log_normal_cdf(x) and its derivative dlog_normal_cdf(x) from one
+// evaluation of the shared terms (t, Q, e^{-z^2}).  FP64 returns exactly the
+// two functions above; FP32 forms the quotients with v_rcp_f32.
+template <typename R>
+__device__ __forceinline__ R log_normal_cdf_d(R x, R& d) {
+    d = dlog_normal_cdf<R>(x);
+    return log_normal_cdf<R>(x);
+}
+
+template <>
+__device__ __forceinline__ float log_normal_cdf_d<float>(float x, float& d) {
+    using M = Math<float>;
+    const float S2 = 0.707106781f, p = 0.3275911f;
+    if (x > 8.0f) { d = 0.0f; return 0.0f; }
+    if (x < -30.0f) {
+        d = -x - M::rcp(x);
+        return -0.5f * x * x - 0.918938533f - M::log(-x);
+    }
+    const float z = -x * S2, az = fabsf(z);
+    const float t = M::rcp(1.0f + p * az);
+    const float q = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f +
+                    t * (-1.453152027f + t * 1.061405429f))));
+    const float dq = 0.254829592f + t * (2.0f * -0.284496736f + t * (3.0f * 1.421413741f +
+                     t * (4.0f * -1.453152027f + t * (5.0f * 1.061405429f))));
+    const float dt = -p * t * t;
+    if (z >= 0.0f) {
+        d = -S2 * (dq * dt * M::rcp(q) - 2.0f * az);
+        return M::log(0.5f * q) - z * z;
+    }
+    const float e = M::exp(-z * z);
+    const float E = 0.5f * q * e;
+    const float ome = 1.0f - E;
+    d = -S2 * (0.5f * e * (dq * dt - 2.0f * az * q)) * M::rcp(ome);
+    return M::log(ome);
+}
+
 }  // namespace lcf

@@ -10,9 +10,13 @@
 // band's LDS copy, accumulating the log-likelihood (Kahan) and its adjoint
 // dLL/dtheta.  The adjoint is formed per observation from dLL/df and the
 // model's parameter gradient, recomputed with the forward value (nothing per
-// observation is stored).  One wave reduction per step combines
-// sum LL, sum dF/dtheta and sum dF/dtheta * sigma * eps, which are exactly the
-// reparameterisation gradients w.r.t. mu and log sigma.
+// observation is stored).  One reduce-scatter per step (permlane swaps and
+// DPP, no LDS) forms sum LL, sum dF/dtheta and sum dF/dtheta * sigma * eps,
+// which are exactly the reparameterisation gradients w.r.t. mu and log
+// sigma, and leaves each total in the lanes that own that component's Adam
+// state; the group lanes of a sample draw its Philox pairs once between
+// them.  Per step the wave thus spends its time on observations, not on
+// replicated per-job bookkeeping.
 //
 // Semantics follow the reference's GPU SVI (cuda/svi.cu:48-330): PSO point as
 // the initial mean, log sigma = -1, Gaussian prior with the given centres and
@@ -96,27 +100,75 @@ __device__ __forceinline__ void sincos_rev(double u, double& s, double& c) {
     ::sincos(6.28318530717958647692 * u, &s, &c);
 }
 
+// Box-Muller of one Philox block: the pair of normals from (x, y).
+template <typename R>
+__device__ __forceinline__ void box_muller(U4 r, R& e0, R& e1) {
+    R u1 = (R((r.x >> 9)) + R(0.5)) * R(1.0 / 8388608.0);
+    R u2 = R(r.y >> 8) * R(1.0 / 16777216.0);
+    R rad = Math<R>::sqrt(R(-2) * Math<R>::log(u1));
+    R s, c;
+    sincos_rev(u2, s, c);
+    e0 = rad * c;
+    e1 = rad * s;
+}
+
 // Philox normals of (job, step, sample): eps[2q], eps[2q+1] from pair q.
 template <typename R, int NP>
 __device__ __forceinline__ void draw_normals(uint64_t seed, uint32_t job, uint32_t step, uint32_t sample,
                                              R (&eps)[NP]) {
 #pragma unroll
     for (int q = 0; q < (NP + 1) / 2; q++) {
-        U4 r = philox4x32_10(job, step, sample, (uint32_t)q, (uint32_t)seed, (uint32_t)(seed >> 32));
-        R u1 = (R((r.x >> 9)) + R(0.5)) * R(1.0 / 8388608.0);
-        R u2 = R(r.y >> 8) * R(1.0 / 16777216.0);
-        R rad = Math<R>::sqrt(R(-2) * Math<R>::log(u1));
-        R s, c;
-        sincos_rev(u2, s, c);
-        eps[2 * q] = rad * c;
-        if (2 * q + 1 < NP) eps[2 * q + 1] = rad * s;
+        R e0, e1;
+        box_muller(philox4x32_10(job, step, sample, (uint32_t)q, (uint32_t)seed, (uint32_t)(seed >> 32)), e0, e1);
+        eps[2 * q] = e0;
+        if (2 * q + 1 < NP) eps[2 * q + 1] = e1;
     }
 }
 
+// The same normals with each pair drawn once: lane (s, g) of the G >= pairs
+// group lanes s G .. s G + G - 1 draws pair g and the group exchanges them.
+template <typename R, int NP>
+__device__ __forceinline__ void draw_normals_split(uint64_t seed, uint32_t job, uint32_t step, int s, int grp,
+                                                   int G, R (&eps)[NP]) {
+    R e0, e1;
+    box_muller(philox4x32_10(job, step, (uint32_t)s, (uint32_t)grp, (uint32_t)seed, (uint32_t)(seed >> 32)), e0, e1);
+    const int base = s * G;
+#pragma unroll
+    for (int q = 0; q < (NP + 1) / 2; q++) {
+        eps[2 * q] = __shfl(e0, base + q, WAVE);
+        if (2 * q + 1 < NP) eps[2 * q + 1] = __shfl(e1, base + q, WAVE);
+    }
+}
+
+// Copies cnt observations into LDS with the measured ones first and the
+// upper limits after them, each kind in its original order.  One pass over
+// a tile hands G consecutive observations to the wave (one per group, all
+// samples), so after this partition only the pass that straddles the two
+// kinds takes both branches of the likelihood.
 template <typename R>
 __device__ __forceinline__ void stage(typename ObsR<R>::T* lds, const typename ObsR<R>::T* __restrict__ g, int cnt) {
+    using O = typename ObsR<R>::T;
+    const int lane = lane_id();
     __syncthreads();
-    for (int i = lane_id(); i < cnt; i += WAVE) lds[i] = g[i];
+    int n_meas = 0;
+    for (int b = 0; b < cnt; b += WAVE) {
+        const int i = b + lane;
+        n_meas += __popcll(__ballot(i < cnt && g[i].w == R(0)));
+    }
+    int km = 0, ku = n_meas;
+    for (int b = 0; b < cnt; b += WAVE) {
+        const int i = b + lane;
+        const bool valid = i < cnt;
+        const O v = g[valid ? i : cnt - 1];  // cnt >= 1
+        const bool meas = valid && v.w == R(0), up = valid && !meas;
+        const unsigned long long bm = __ballot(meas), bu = __ballot(up);
+        const int rm = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0u));
+        const int ru = __builtin_amdgcn_mbcnt_hi((uint32_t)(bu >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bu, 0u));
+        if (meas) lds[km + rm] = v;
+        if (up) lds[ku + ru] = v;
+        km += __popcll(bm);
+        ku += __popcll(bu);
+    }
     __syncthreads();
 }
 
@@ -185,59 +237,78 @@ struct SviJob {
             for (int i = grp; i < cnt; i += G) {
                 O o = lds[i];
                 R f, g[NP];
-                if (!ev(o.x, f, g) || !M::finite(f)) continue;
+                const bool ok = ev(o.x, f, g) && M::finite(f);
                 R tv = o.z + se2;
+                R term, wgt, dse;  // LL term, dLL/df, dLL/d ln sigma_x
                 if (o.w != R(0)) {
-                    R st = M::sqrt(tv);
-                    R z = (o.y - f) / st;
-                    R lz = log_normal_cdf<R>(z);
-                    if (F64) ll.s += lz; else ll.add(lz);
-                    R ratio, dpred;
-                    if (exact_tail) {  // the derivative of the lz summed above
-                        ratio = dlog_normal_cdf<R>(z);
-                        dpred = -ratio / st;
+                    const R r = o.y - f;
+                    R st, rst, z, ratio;
+                    if (F64) {
+                        st = M::sqrt(tv);
+                        z = r / st;
+                    } else {
+                        rst = M::rsqrt(tv);
+                        z = r * rst;
+                    }
+                    if (exact_tail) {  // the term and its own derivative
+                        term = log_normal_cdf_d<R>(z, ratio);
+                        wgt = F64 ? -ratio / st : -ratio * rst;
                     } else if (F64) {  // cuda/svi.cu:200-204 as written
+                        term = log_normal_cdf<R>(z);
                         R phi, cdf;
                         phi_cdf(z, phi, cdf);
                         ratio = phi / cdf;
-                        dpred = -phi / (cdf * st);
+                        wgt = -phi / (cdf * st);
                     } else {
+                        term = log_normal_cdf<R>(z);
                         ratio = mills_ratio<R>(z);
-                        dpred = -ratio / st;
+                        wgt = -ratio * rst;
                     }
-#pragma unroll
-                    for (int j = 0; j < NP - 1; j++)
-                        if (M::finite(g[j])) dll[j] += dpred * g[j];
-                    dll[SE] += ratio * (-(o.y - f) * se2 / (st * tv));
+                    dse = F64 ? ratio * (-r * se2 / (st * tv)) : -ratio * z * se2 * (rst * rst);
                 } else {
                     R inv = M::rcp(tv);
                     R r = o.y - f, r2 = r * r;
-                    R lt = R(-0.5) * (r2 * inv + M::log(R(6.28318530717958647692) * tv));
-                    if (F64) ll.s += lt; else ll.add(lt);
-                    R ri = r * inv;
-#pragma unroll
-                    for (int j = 0; j < NP - 1; j++)
-                        if (M::finite(g[j])) dll[j] += ri * g[j];
-                    dll[SE] += (r2 * inv * inv - inv) * se2;
+                    term = R(-0.5) * (r2 * inv + M::log(R(6.28318530717958647692) * tv));
+                    wgt = r * inv;
+                    dse = (r2 * inv * inv - inv) * se2;
                 }
+                // An observation whose model value is not finite is skipped,
+                // and so are non-finite gradient components.  One test of the
+                // components' sum clears the common case; the rare one zeroes
+                // what is skipped, so the accumulators have one update site.
+                R gs = g[0];
+#pragma unroll
+                for (int j = 1; j < NP - 1; j++) gs += g[j];
+                if (!ok || !M::finite(gs)) {
+#pragma unroll
+                    for (int j = 0; j < NP - 1; j++) g[j] = ok && M::finite(g[j]) ? g[j] : R(0);
+                    if (!ok) term = wgt = dse = R(0);
+                }
+                if (F64) ll.s += term; else ll.add(term);
+#pragma unroll
+                for (int j = 0; j < NP - 1; j++) dll[j] += wgt * g[j];
+                dll[SE] += dse;
             }
         }
         return ll.s;
     }
 
-    // Monte-Carlo ELBO and its gradient at (mu, ls) for step `step`:
-    // returns the ELBO, fills gmu / gls (wave-uniform).
-    __device__ __forceinline__ R elbo_grad(const R (&mu)[NP], const R (&ls)[NP], const R (&pc)[NP],
-                                           const R (&pw2)[NP], const O* lds, const O* __restrict__ gobs,
-                                           int n, int tile, uint32_t job, uint32_t step, int S, int G,
-                                           uint64_t seed, bool exact_tail, R (&gmu)[NP], R (&gls)[NP]) {
+    // Monte-Carlo ELBO of step `step` at the wave-uniform (mu, sigma =
+    // e^ls): returns the ELBO and, in lane l, the wave total of value l >> 2
+    // of {sum LL + log prior, sum dL/dtheta_j (j < NP), sum dL/dtheta_j
+    // sigma_j eps_j (j < NP)}, the unnormalised reparameterisation gradients
+    // w.r.t. mu_j and ls_j.
+    __device__ __forceinline__ R elbo_grad(const R (&mu)[NP], const R (&sig)[NP], const R (&ls)[NP],
+                                           const R (&pc)[NP], const R (&pw2)[NP], const O* lds,
+                                           const O* __restrict__ gobs, int n, int tile, uint32_t job,
+                                           uint32_t step, int S, int G, uint64_t seed, bool exact_tail, R& red) {
+        constexpr bool F64 = sizeof(R) == 8;
         const int lane = lane_id();
         const int s = lane / G, grp = lane - s * G;
         const bool active = s < S;
-        R sig[NP], eps[NP], th[NP], dll[NP];
-#pragma unroll
-        for (int j = 0; j < NP; j++) sig[j] = M::exp(ls[j]);
-        draw_normals<R, NP>(seed, job, step, (uint32_t)s, eps);
+        R eps[NP], th[NP], dll[NP];
+        if (G >= (NP + 1) / 2) draw_normals_split<R, NP>(seed, job, step, s, grp, G, eps);
+        else draw_normals<R, NP>(seed, job, step, (uint32_t)s, eps);
 #pragma unroll
         for (int j = 0; j < NP; j++) th[j] = mu[j] + sig[j] * eps[j];
         R L = loglik(th, lds, gobs, n, tile, grp, G, exact_tail, dll);
@@ -246,30 +317,34 @@ struct SviJob {
 #pragma unroll
             for (int j = 0; j < NP; j++) {
                 R d = th[j] - pc[j];
-                lp += R(-0.5) * d * d / pw2[j];
-                dll[j] -= d / pw2[j];
+                if (F64) {  // as the reference forms it (cuda/svi.cu:236-240)
+                    lp += R(-0.5) * d * d / pw2[j];
+                    dll[j] -= d / pw2[j];
+                } else {  // pw2 holds 1 / width^2
+                    R q = d * pw2[j];
+                    lp += R(-0.5) * d * q;
+                    dll[j] -= q;
+                }
             }
             L += lp;
         }
-        if (!active) {
-            L = R(0);
-#pragma unroll
-            for (int j = 0; j < NP; j++) dll[j] = R(0);
-        }
-        L = wave_sum(L);
-        const R rS = R(S);
+        R v[16];
+        v[0] = active ? L : R(0);
 #pragma unroll
         for (int j = 0; j < NP; j++) {
-            R b = dll[j] * sig[j] * eps[j];
-            gmu[j] = wave_sum(dll[j]) / rS;
-            gls[j] = wave_sum(b) / rS;
+            v[1 + j] = active ? dll[j] : R(0);
+            v[1 + NP + j] = active ? dll[j] * sig[j] * eps[j] : R(0);
         }
+#pragma unroll
+        for (int k = 1 + 2 * NP; k < 16; k++) v[k] = R(0);
+        red = wave_sum16(v);
+        const R tot = bcast(red, 0);
         R ent = R(0);
 #pragma unroll
         for (int j = 0; j < NP; j++) ent += ls[j];
         // 0.5 np ln(2 pi e), as the reference forms it (cuda/svi.cu:264)
         ent += R(0.5 * NP) * (R)::log(2.0 * 3.14159265358979323846 * 2.71828182845904523536);
-        return L / rS + ent;
+        return tot / R(S) + ent;
     }
 
     __device__ void run(const SviArgs& a, O* lds, int job) {
@@ -280,15 +355,6 @@ struct SviJob {
         int sp = 1;
         while (sp < S) sp <<= 1;
         const int G = WAVE / sp;
-        R mu[NP], ls[NP], pc[NP], pw2[NP];
-#pragma unroll
-        for (int j = 0; j < NP; j++) {
-            mu[j] = (R)a.mu_in[(int64_t)job * 7 + j];
-            ls[j] = a.ls_in ? (R)a.ls_in[(int64_t)job * 7 + j] : R(-1);
-            pc[j] = (R)a.prior_c[(int64_t)job * 7 + j];
-            R w = (R)a.prior_w[(int64_t)job * 7 + j];
-            pw2[j] = w * w;
-        }
         if (n <= 0) {
             if (lane < 7) {
                 a.out_mu[(int64_t)job * 7 + lane] = 0.0;
@@ -300,29 +366,61 @@ struct SviJob {
             }
             return;
         }
+        // Lane roles: the reduction leaves value v in lanes 4v .. 4v + 3, so
+        // lanes 4 (1 + j) carry mu_j and its Adam moments and lanes
+        // 4 (1 + NP + j) carry ls_j and its moments; the wave reads the
+        // current (mu, ls) from them with readlane once per step.
+        const int vi = lane >> 2;
+        const bool own_mu = vi >= 1 && vi <= NP, own_ls = vi > NP && vi <= 2 * NP;
+        const bool writer = (lane & 3) == 0 && (own_mu || own_ls);
+        const int pj = own_mu ? vi - 1 : (own_ls ? vi - 1 - NP : 0);
+        const int64_t row = (int64_t)job * 7;
+        R x = R(0);
+        if (own_mu) x = (R)a.mu_in[row + pj];
+        if (own_ls) x = a.ls_in ? (R)a.ls_in[row + pj] : R(-1);
+        R pc[NP], pw2[NP];
+#pragma unroll
+        for (int j = 0; j < NP; j++) {
+            pc[j] = bcast((R)a.prior_c[row + j], 0);
+            R w = (R)a.prior_w[row + j];
+            pw2[j] = bcast(sizeof(R) == 8 ? w * w : R(1) / (w * w), 0);
+        }
+        auto current = [&](R (&mu)[NP], R (&sig)[NP], R (&ls)[NP]) {
+            const R sx = own_ls ? M::exp(x) : x;
+#pragma unroll
+            for (int j = 0; j < NP; j++) {
+                mu[j] = bcast(x, 4 * (1 + j));
+                ls[j] = bcast(x, 4 * (1 + NP + j));
+                sig[j] = bcast(sx, 4 * (1 + NP + j));
+            }
+        };
+        auto zero_unused = [&]() {  // padding components of the 7-wide rows
+            if (lane >= NP && lane < 7) {
+                a.out_mu[row + lane] = 0.0;
+                a.out_ls[row + lane] = 0.0;
+            }
+        };
         if (n <= a.tile) stage<R>(lds, g, n);
         // the noise stream's job word: the job's index, or 0 for every job
         // (the CPU fit seeds a fresh SmallRng(42) per band, src/parametric.rs:2239)
         const uint32_t rng_job = (a.flags & LCF_SVI_SHARED_STREAM_FLAG) ? 0u : (uint32_t)job;
-        R gmu[NP], gls[NP];
+        const bool exact_tail = (a.flags & LCF_SVI_EXACT_TAIL_FLAG) != 0;
+        const R rS = R(S);
+        R red;
         if (a.grad_only) {
-            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)a.step0, S, G,
-                            a.seed, (a.flags & LCF_SVI_EXACT_TAIL_FLAG) != 0, gmu, gls);
-            double om = 0.0, ol = 0.0;
-#pragma unroll
-            for (int j = 0; j < NP; j++)  // + 1: derivative of the entropy term
-                if (lane == j) { om = (double)gmu[j]; ol = (double)gls[j] + 1.0; }
-            if (lane < 7) {
-                a.out_mu[(int64_t)job * 7 + lane] = om;
-                a.out_ls[(int64_t)job * 7 + lane] = ol;
+            R mu[NP], sig[NP], ls[NP];
+            current(mu, sig, ls);
+            R e = elbo_grad(mu, sig, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)a.step0, S, G,
+                            a.seed, exact_tail, red);
+            if (writer) {  // + 1 on ls: derivative of the entropy term
+                if (own_mu) a.out_mu[row + pj] = (double)(red / rS);
+                else a.out_ls[row + pj] = (double)(red / rS) + 1.0;
             }
+            zero_unused();
             if (lane == 0) a.out_elbo[job] = (double)e;
             return;
         }
-        R am[NP], av[NP], bm[NP], bv[NP];
-#pragma unroll
-        for (int j = 0; j < NP; j++) am[j] = av[j] = bm[j] = bv[j] = R(0);
-        const bool exact_tail = (a.flags & LCF_SVI_EXACT_TAIL_FLAG) != 0;
+        R m1 = R(0), m2 = R(0);
         const int min_it = (a.flags & LCF_SVI_CPU_STOPPING_FLAG) ? 200 : 50;
         const int stall_it = (a.flags & LCF_SVI_CPU_STOPPING_FLAG) ? 50 : 20;
         const R lr = (R)a.lr;
@@ -331,11 +429,14 @@ struct SviJob {
         double ema = -1e300, best = -1e300;
         int stall = 0, step = 0;
         for (; step < a.n_steps;) {
-            R e = elbo_grad(mu, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)step, S, G, a.seed,
-                            exact_tail, gmu, gls);
+            R mu[NP], sig[NP], ls[NP];
+            current(mu, sig, ls);
+            R e = elbo_grad(mu, sig, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)step, S, G, a.seed,
+                            exact_tail, red);
             final_elbo = e;
             step++;
-            // Adam (cuda/svi.cu:266-291), non-finite components skipped
+            // Adam (cuda/svi.cu:266-291) on the owning lanes, non-finite
+            // components skipped, ls clamped to [-6, 2]
             b1t *= R(0.9);
             b2t *= R(0.999);
             R bc1 = R(1) - b1t, bc2 = R(1) - b2t;
@@ -343,22 +444,13 @@ struct SviJob {
                 bc1 = (R)(1.0 - ::pow(0.9, (double)step));
                 bc2 = (R)(1.0 - ::pow(0.999, (double)step));
             }
-#pragma unroll
-            for (int j = 0; j < NP; j++) {
-                R gm = -gmu[j];
-                if (M::finite(gm)) {
-                    am[j] = R(0.9) * am[j] + R(0.1) * gm;
-                    av[j] = R(0.999) * av[j] + R(0.001) * gm * gm;
-                    mu[j] -= lr * (am[j] / bc1) / (M::sqrt(av[j] / bc2) + R(1e-8));
-                }
-                R gl = -(gls[j] + R(1));
-                if (M::finite(gl)) {
-                    bm[j] = R(0.9) * bm[j] + R(0.1) * gl;
-                    bv[j] = R(0.999) * bv[j] + R(0.001) * gl * gl;
-                    ls[j] -= lr * (bm[j] / bc1) / (M::sqrt(bv[j] / bc2) + R(1e-8));
-                }
-                ls[j] = ls[j] < R(-6) ? R(-6) : (ls[j] > R(2) ? R(2) : ls[j]);
+            R gg = -(red / rS + (own_ls ? R(1) : R(0)));
+            if (M::finite(gg)) {
+                m1 = R(0.9) * m1 + R(0.1) * gg;
+                m2 = R(0.999) * m2 + R(0.001) * gg * gg;
+                x -= lr * (m1 / bc1) / (M::sqrt(m2 / bc2) + R(1e-8));
             }
+            if (own_ls) x = x < R(-6) ? R(-6) : (x > R(2) ? R(2) : x);
             // EMA early stop (cuda/svi.cu:297-312), unless all steps are asked for
             if (!(a.flags & LCF_SVI_FIXED_STEPS_FLAG) && step - 1 >= min_it) {
                 double fe = (double)final_elbo;
@@ -368,14 +460,11 @@ struct SviJob {
             }
         }
         const double infl = (a.flags & LCF_SVI_INFLATE_FLAG) ? 1.3862943611198906 : 0.0;
-        double om = 0.0, ol = 0.0;
-#pragma unroll
-        for (int j = 0; j < NP; j++)
-            if (lane == j) { om = (double)mu[j]; ol = (double)ls[j] + infl; }
-        if (lane < 7) {
-            a.out_mu[(int64_t)job * 7 + lane] = om;
-            a.out_ls[(int64_t)job * 7 + lane] = ol;
+        if (writer) {
+            if (own_mu) a.out_mu[row + pj] = (double)x;
+            else a.out_ls[row + pj] = (double)x + infl;
         }
+        zero_unused();
         if (lane == 0) {
             a.out_elbo[job] = step > 0 ? (double)final_elbo : -1e300;
             if (a.out_steps) a.out_steps[job] = step;
