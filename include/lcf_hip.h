@@ -6,7 +6,8 @@
  *
  * Conventions
  *   - Every call returns 0 on success or a negative LCF_E* code; the message
- *     of the last failure on a context is lcf_last_error(ctx).  This replaces
+ *     of the last failure on a context is lcf_last_error(ctx) (of the calling
+ *     thread's last context-free call: lcf_last_error(NULL)).  This replaces
  *     the reference's `Err(format!("CUDA error {code}: {msg}"))`
  *     (src/gpu_cuda.rs:170-186).
  *   - Plain host pointers and sizes only.  Inputs are read, outputs written,
@@ -71,11 +72,22 @@ int lcf_ctx_synchronize(lcf_ctx* ctx);
 
 /* ---- resident source data: GpuBatchData (src/gpu_cuda.rs:236-292) ------ */
 /* offsets[n_sources+1] index the concatenated per-observation arrays
- * (BatchSource fields, src/gpu_types.rs:122-130); is_upper is 0/1. */
+ * (BatchSource fields, src/gpu_types.rs:122-130); is_upper is 0/1.  A batch
+ * belongs to a device, not to a context: every context on the device that
+ * holds it may use it. */
 int lcf_batch_create(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets,
                      const double* times, const double* flux, const double* obs_var,
                      const int32_t* is_upper, const double* upper_flux,
                      lcf_batch** out);
+/* GpuBatchData::new(sources) (src/gpu_cuda.rs:251-285) with the reference's
+ * signature: no context; the arrays go to the calling thread's current device
+ * (hipSetDevice, or lcf_ctx_create, which makes its device current), as the
+ * reference's cudaMalloc uploads do.  Any context on that device may use the
+ * batch.  On failure the message is lcf_last_error(NULL), the calling
+ * thread's last context-free error. */
+int lcf_batch_create_current(int64_t n_sources, const int64_t* offsets, const double* times,
+                             const double* flux, const double* obs_var, const int32_t* is_upper,
+                             const double* upper_flux, lcf_batch** out);
 void lcf_batch_destroy(lcf_batch* batch);
 int64_t lcf_batch_n_sources(const lcf_batch* batch);
 /* GpuBatchData::n_obs_per_source: observation count, 1 when out of range. */

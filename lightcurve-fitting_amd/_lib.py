@@ -28,6 +28,7 @@ SIGNATURES = {
     "lcf_ctx_num_cus": (_i32, [_vp]),
     "lcf_ctx_synchronize": (_i32, [_vp]),
     "lcf_batch_create": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _ppv]),
+    "lcf_batch_create_current": (_i32, [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _ppv]),
     "lcf_batch_destroy": (None, [_vp]),
     "lcf_batch_n_sources": (_i64, [_vp]),
     "lcf_batch_n_obs": (_i64, [_vp, _i64]),

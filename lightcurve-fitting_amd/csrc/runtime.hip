@@ -37,7 +37,7 @@ struct lcf_ctx {
 };
 
 struct lcf_batch {
-    lcf_ctx* ctx = nullptr;
+    int device = -1;              // the device holding the arrays (any context on it may use them)
     int64_t n_sources = 0;
     int64_t n_obs = 0;
     std::vector<int64_t> h_off;
@@ -52,6 +52,9 @@ struct lcf_batch {
 
 namespace {
 
+// Failure text of the calling thread's last call made without a context.
+thread_local std::string t_err;
+
 int fail(lcf_ctx* c, int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
@@ -59,6 +62,7 @@ int fail(lcf_ctx* c, int code, const char* fmt, ...) {
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     if (c) c->err = buf;
+    else t_err = buf;
     return code;
 }
 
@@ -143,7 +147,7 @@ struct FitBufs {
 int check_fit_args(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models, int P, int max_iters,
                    int stall_iters) {
     if (!c) return LCF_E_INVALID;
-    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!b || b->device != c->device) return fail(c, LCF_E_INVALID, "batch is null or on another device");
     if (!models || n_models <= 0 || n_models > LCF_N_MODELS)
         return fail(c, LCF_E_INVALID, "n_models must be in [1, %d]", LCF_N_MODELS);
     for (int i = 0; i < n_models; i++)
@@ -318,7 +322,7 @@ void lcf_ctx_destroy(lcf_ctx* c) {
     delete c;
 }
 
-const char* lcf_last_error(const lcf_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* lcf_last_error(const lcf_ctx* c) { return c ? c->err.c_str() : t_err.c_str(); }
 
 int lcf_ctx_set_precision(lcf_ctx* c, int p) {
     if (!c) return LCF_E_INVALID;
@@ -351,21 +355,31 @@ double lcf_last_model_ms(const lcf_ctx* c, int slot) {
     return (c && slot >= 0 && slot < LCF_N_MODELS) ? c->slot_ms[slot] : 0.0;
 }
 
-int lcf_batch_create(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t,
-                     const double* f, const double* v, const int32_t* up, const double* uf,
-                     lcf_batch** out) {
-    if (!c || !out) return LCF_E_INVALID;
-    *out = nullptr;
+namespace {
+
+// Shared body of lcf_batch_create / lcf_batch_create_current: validates the
+// CSR arrays, uploads them on `st` (device `device` current) and packs the
+// observation records.  Failures are reported on c (the thread's
+// context-free error when c is null).
+int check_csr(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t, const double* f, const double* v,
+              const int32_t* up, const double* uf) {
     if (n_src < 0 || n_src > INT32_MAX - 1 || !off)
         return fail(c, LCF_E_INVALID, "invalid source count %lld", (long long)n_src);
     if (off[0] != 0) return fail(c, LCF_E_INVALID, "offsets[0] must be 0");
     for (int64_t s = 0; s < n_src; s++)
         if (off[s + 1] < off[s]) return fail(c, LCF_E_INVALID, "offsets not non-decreasing at %lld", (long long)s);
+    if (off[n_src] > 0 && (!t || !f || !v || !up || !uf)) return fail(c, LCF_E_INVALID, "null observation array");
+    return LCF_OK;
+}
+
+int batch_create(lcf_ctx* c, int device, hipStream_t st, int64_t n_src, const int64_t* off, const double* t,
+                 const double* f, const double* v, const int32_t* up, const double* uf, lcf_batch** out) {
+    if (!out) return fail(c, LCF_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (int rc = check_csr(c, n_src, off, t, f, v, up, uf)) return rc;
     const int64_t n = off[n_src];
-    if (n > 0 && (!t || !f || !v || !up || !uf)) return fail(c, LCF_E_INVALID, "null observation array");
-    HIPCHK(c, hipSetDevice(c->device));
     lcf_batch* b = new lcf_batch();
-    b->ctx = c;
+    b->device = device;
     b->n_sources = n_src;
     b->n_obs = n;
     b->h_off.assign(off, off + n_src + 1);
@@ -391,9 +405,9 @@ int lcf_batch_create(lcf_ctx* c, int64_t n_src, const int64_t* off, const double
             rc = fail(c, LCF_E_NOMEM, "device allocation failed for %lld observations", (long long)n);
             break;
         }
-        if (hipMemcpyAsync(b->d_off, off, sizeof(int64_t) * (n_src + 1), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        if (hipMemcpyAsync(b->d_off, off, sizeof(int64_t) * (n_src + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
             (n_src > 0 && hipMemcpyAsync(b->d_order, b->h_order.data(), sizeof(int32_t) * n_src,
-                                         hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+                                         hipMemcpyHostToDevice, st) != hipSuccess)) {
             rc = fail(c, LCF_E_HIP, "upload failed");
             break;
         }
@@ -405,23 +419,23 @@ int lcf_batch_create(lcf_ctx* c, int64_t n_src, const int64_t* off, const double
                 break;
             }
             hipError_t e = hipSuccess;
-            e = e ? e : hipMemcpyAsync(dt.p, t, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
-            e = e ? e : hipMemcpyAsync(df.p, f, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
-            e = e ? e : hipMemcpyAsync(dv.p, v, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
-            e = e ? e : hipMemcpyAsync(duf.p, uf, sizeof(double) * n, hipMemcpyHostToDevice, c->stream);
-            e = e ? e : hipMemcpyAsync(dup.p, up, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream);
+            e = e ? e : hipMemcpyAsync(dt.p, t, sizeof(double) * n, hipMemcpyHostToDevice, st);
+            e = e ? e : hipMemcpyAsync(df.p, f, sizeof(double) * n, hipMemcpyHostToDevice, st);
+            e = e ? e : hipMemcpyAsync(dv.p, v, sizeof(double) * n, hipMemcpyHostToDevice, st);
+            e = e ? e : hipMemcpyAsync(duf.p, uf, sizeof(double) * n, hipMemcpyHostToDevice, st);
+            e = e ? e : hipMemcpyAsync(dup.p, up, sizeof(int32_t) * n, hipMemcpyHostToDevice, st);
             if (e == hipSuccess) {
                 int64_t blocks = (n + 255) / 256;
-                hipLaunchKernelGGL(pack_obs, dim3((unsigned)blocks), dim3(256), 0, c->stream, dt.p, df.p,
+                hipLaunchKernelGGL(pack_obs, dim3((unsigned)blocks), dim3(256), 0, st, dt.p, df.p,
                                    dv.p, dup.p, duf.p, n, b->d_obs32, b->d_obs64);
                 e = hipGetLastError();
             }
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
             if (e != hipSuccess) {
                 rc = fail(c, LCF_E_HIP, "HIP error %d: %s (batch upload)", (int)e, hipGetErrorString(e));
                 break;
             }
-        } else if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        } else if (hipStreamSynchronize(st) != hipSuccess) {
             rc = fail(c, LCF_E_HIP, "upload sync failed");
             break;
         }
@@ -431,9 +445,35 @@ int lcf_batch_create(lcf_ctx* c, int64_t n_src, const int64_t* off, const double
     return LCF_OK;
 }
 
+}  // namespace
+
+int lcf_batch_create(lcf_ctx* c, int64_t n_src, const int64_t* off, const double* t, const double* f,
+                     const double* v, const int32_t* up, const double* uf, lcf_batch** out) {
+    if (!c || !out) return LCF_E_INVALID;
+    HIPCHK(c, hipSetDevice(c->device));
+    return batch_create(c, c->device, c->stream, n_src, off, t, f, v, up, uf, out);
+}
+
+int lcf_batch_create_current(int64_t n_src, const int64_t* off, const double* t, const double* f,
+                             const double* v, const int32_t* up, const double* uf, lcf_batch** out) {
+    if (!out) return fail(nullptr, LCF_E_INVALID, "null output handle");
+    *out = nullptr;
+    if (int rc = check_csr(nullptr, n_src, off, t, f, v, up, uf)) return rc;
+    int device = -1;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return fail(nullptr, LCF_E_HIP, "HIP error %d: %s (no current device)", (int)e,
+                                     hipGetErrorString(e));
+    hipStream_t st = nullptr;
+    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(nullptr, LCF_E_HIP, "HIP error %d: %s (stream)", (int)e, hipGetErrorString(e));
+    const int rc = batch_create(nullptr, device, st, n_src, off, t, f, v, up, uf, out);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
 void lcf_batch_destroy(lcf_batch* b) {
     if (!b) return;
-    if (b->ctx) (void)hipSetDevice(b->ctx->device);
+    if (b->device >= 0) (void)hipSetDevice(b->device);
     if (b->d_off) (void)hipFree(b->d_off);
     if (b->d_obs32) (void)hipFree(b->d_obs32);
     if (b->d_obs64) (void)hipFree(b->d_obs64);
@@ -525,7 +565,7 @@ int lcf_batch_multi_bazin(lcf_ctx* c, const lcf_batch* b, int P, int max_iters, 
                           uint64_t seed, int32_t* out_best_k, double* out_params, double* out_cost,
                           double* out_bic, double* out_per_k_cost, double* out_per_k_bic) {
     if (!c) return LCF_E_INVALID;
-    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!b || b->device != c->device) return fail(c, LCF_E_INVALID, "batch is null or on another device");
     if (P <= 0 || P > 4 * 64) return fail(c, LCF_E_INVALID, "n_particles must be in [1, 256], got %d", P);
     if (max_iters <= 0) return fail(c, LCF_E_INVALID, "max_iters must be positive");
     if (stall_iters < 0) return fail(c, LCF_E_INVALID, "stall_iters must be non-negative");
@@ -621,7 +661,7 @@ int svi_run(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const doub
             uint64_t seed, uint32_t flags, int grad_only, int step0, double* out_mu, double* out_ls,
             double* out_elbo, int32_t* out_steps) {
     if (!c) return LCF_E_INVALID;
-    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!b || b->device != c->device) return fail(c, LCF_E_INVALID, "batch is null or on another device");
     const int64_t nj = b->n_sources;
     if (nj == 0) return LCF_OK;
     if (!model_ids || !mu || !pc || !pw || !out_mu || !out_ls || !out_elbo)
@@ -1065,7 +1105,7 @@ int lcf_batch_gp2d_fit(lcf_ctx* c, int64_t n_src, const int64_t* off, const doub
 int lcf_batch_laplace(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const double* map_params,
                       int mode, double* out_log_sigma, double* out_elbo, double* out_hessian) {
     if (!c) return LCF_E_INVALID;
-    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!b || b->device != c->device) return fail(c, LCF_E_INVALID, "batch is null or on another device");
     const int64_t nj = b->n_sources;
     if (nj == 0) return LCF_OK;
     if (!model_ids || !map_params || !out_log_sigma || !out_elbo) return fail(c, LCF_E_INVALID, "null pointer");
@@ -1136,7 +1176,7 @@ int lcf_batch_laplace(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, 
 int lcf_cpu_pso(lcf_ctx* c, const lcf_batch* b, const lcf_pso_job* jobs, int64_t n_jobs, double* out_x,
                 double* out_cost, int32_t* out_iters) {
     if (!c) return LCF_E_INVALID;
-    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!b || b->device != c->device) return fail(c, LCF_E_INVALID, "batch is null or on another device");
     if (n_jobs < 0 || n_jobs > INT32_MAX) return fail(c, LCF_E_INVALID, "invalid job count");
     if (n_jobs == 0) return LCF_OK;
     if (!jobs || !out_x || !out_cost || !out_iters) return fail(c, LCF_E_INVALID, "null pointer");
@@ -1260,7 +1300,7 @@ int lcf_batch_finalize(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids,
                        const double* mag_obs, const double* peak_flux, int flags, double* out_mu,
                        double* out_log_sigma, double* out_mag_chi2) {
     if (!c) return LCF_E_INVALID;
-    if (!b || b->ctx != c) return fail(c, LCF_E_INVALID, "batch is null or belongs to another context");
+    if (!b || b->device != c->device) return fail(c, LCF_E_INVALID, "batch is null or on another device");
     const int64_t nj = b->n_sources, no = b->n_obs;
     if (nj == 0) return LCF_OK;
     if (!model_ids || !pso_params || !mu || !log_sigma || !flux || !flux_err || !mag_obs || !peak_flux ||

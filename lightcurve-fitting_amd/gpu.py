@@ -821,12 +821,18 @@ def svi_prior_for_model(model: int, pso_params: Sequence[float]) -> tuple[list, 
 
 
 class GpuBatchData:
-    """Resident CSR observation data (``GpuBatchData``, src/gpu_cuda.rs:236-292)."""
+    """Resident CSR observation data (``GpuBatchData``, src/gpu_cuda.rs:236-292).
 
-    def __init__(self, ctx: GpuContext, sources: Sequence[BatchSource] | None = None, *,
+    ``GpuBatchData(ctx, sources)`` uploads to ``ctx``'s device;
+    ``GpuBatchData(None, sources)`` and ``GpuBatchData.new(sources)`` (the
+    reference's signature) upload to the calling thread's current HIP device
+    through ``lcf_batch_create_current``.  Either way any context on that
+    device may use the batch."""
+
+    def __init__(self, ctx: GpuContext | None = None, sources: Sequence[BatchSource] | None = None, *,
                  csr: dict | None = None):
         self._ctx = ctx
-        self._lib = ctx._lib
+        self._lib = ctx._lib if ctx is not None else _lib.load()
         if csr is None:
             csr = sources_to_csr(sources or [])
         off = np.ascontiguousarray(np.asarray(csr["offsets"], dtype=np.int64))
@@ -840,19 +846,22 @@ class GpuBatchData:
             if a.size != n:
                 raise GpuError(f"{name} has {a.size} entries, offsets say {n}")
         h = C.c_void_p()
-        rc = self._lib.lcf_batch_create(ctx._h, off.size - 1, _lib.ptr(off, C.c_int64),
-                                        _lib.ptr(t, C.c_double), _lib.ptr(f, C.c_double),
-                                        _lib.ptr(v, C.c_double), _lib.ptr(up, C.c_int32),
-                                        _lib.ptr(uf, C.c_double), C.byref(h))
-        ctx._check(rc)
+        arrays = (off.size - 1, _lib.ptr(off, C.c_int64), _lib.ptr(t, C.c_double), _lib.ptr(f, C.c_double),
+                  _lib.ptr(v, C.c_double), _lib.ptr(up, C.c_int32), _lib.ptr(uf, C.c_double), C.byref(h))
+        if ctx is not None:
+            ctx._check(self._lib.lcf_batch_create(ctx._h, *arrays))
+        else:
+            rc = self._lib.lcf_batch_create_current(*arrays)
+            if rc != 0:
+                raise GpuError((self._lib.lcf_last_error(None) or b"").decode() or f"status {rc}")
         self._h = h
         self.n_sources = off.size - 1
         self.n_obs = n
         self._n_obs = np.diff(off)
 
     @classmethod
-    def new(cls, ctx: GpuContext, sources: Sequence[BatchSource]) -> "GpuBatchData":
-        return cls(ctx, sources)
+    def new(cls, sources: Sequence[BatchSource]) -> "GpuBatchData":
+        return cls(None, sources)
 
     def n_obs_per_source(self, source_idx: int) -> int:
         return int(self._lib.lcf_batch_n_obs(self._h, int(source_idx)))

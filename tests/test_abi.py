@@ -43,10 +43,27 @@ def test_null_and_bad_arguments_rejected_without_device(pkg):
     assert lib.lcf_ctx_set_precision(None, 0) == -1
     assert lib.lcf_batch_n_obs(None, 0) == 1          # reference: 1 when out of range
     assert lib.lcf_batch_n_sources(None) == 0
-    assert lib.lcf_last_error(None) == b"null context"
     h = C.c_void_p()
     # device index beyond the device count (there is none here)
     assert lib.lcf_ctx_create(10_000, C.byref(h)) == -1 and not h.value
+    # the context-free batch upload checks its CSR arrays before it looks
+    # for a device, and reports through lcf_last_error(NULL)
+    off = (C.c_int64 * 3)(0, 2, 1)
+    assert lib.lcf_batch_create_current(2, off, None, None, None, None, None, C.byref(h)) == -1
+    assert lib.lcf_last_error(None) == b"offsets not non-decreasing at 1" and not h.value
+    assert lib.lcf_batch_create_current(-1, off, None, None, None, None, None, C.byref(h)) == -1
+    assert b"invalid source count" in lib.lcf_last_error(None)
+
+
+def test_c_driver_compiles_against_the_header(tmp_path):
+    """tests/c_driver/lcf_driver.c, the C caller that makes the crate module's
+    calls, is plain C99 against include/lcf_hip.h (gcc, warnings as errors)."""
+    import subprocess
+
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", f"-I{ROOT / 'include'}", "-c",
+                        str(ROOT / "tests" / "c_driver" / "lcf_driver.c"), "-o", str(tmp_path / "d.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
 
 
 def test_model_metadata_mirrors_reference(pkg):
