@@ -7,6 +7,8 @@
 
 namespace lcf {
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 constexpr int GP_THREADS = 256;
 constexpr int GP_WAVES = GP_THREADS / WAVE;
 constexpr int GP_ROWS = 4;  // rows per lane in the wave-level solves (m <= 256)
@@ -71,65 +73,57 @@ constexpr int CHOL_NB = 16;  // block-column width of the blocked Cholesky
 
 // In-place Cholesky of the packed m x m matrix in L over the workgroup,
 // left-looking by block columns of CHOL_NB: (A) the panel below and including
-// the diagonal block loses sum_{k < J0} l_ik l_jk, register-tiled 4 x 4 per
-// thread; (B) wavefront 0 factors the diagonal block with its rows in
-// registers (lane r holds row J0 + r; column j is scaled, then the columns to
-// its right lose l_ij l_cj); (C) every row below solves against the diagonal
-// block.  Each element is reduced term by term in increasing k, as the
-// row-major Cholesky-Crout of src/sparse_gp.rs:27-49 does, so the factor
-// rounds identically when built without multiply-add contraction.
+// the diagonal block loses sum_{k < J0} l_ik l_jk, as 16 x 16 tile products
+// on the FP64 matrix cores; (B) wavefront 0 factors the diagonal block with
+// its rows in registers (lane r holds row J0 + r; column j is scaled, then
+// the columns to its right lose l_ij l_cj); (C) every row below solves
+// against the diagonal block.  Within a block the elements are reduced term
+// by term in increasing k, as the row-major Cholesky-Crout of
+// src/sparse_gp.rs:27-49; the panel sums of (A) round as tile products.
 // rows = m + 1 appends row m (its first m entries, e.g. y - ybar): it is
 // reduced like any row below the factor and ends as L^-1 times itself, the
-// forward substitution of solve_l (src/sparse_gp.rs:52-58) term for term.
+// forward substitution of solve_l (src/sparse_gp.rs:52-58).
 // Three barriers per block column; returns false (uniformly, after a
 // barrier) on a pivot <= 0.  The matrix must be complete and visible.
-__device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* flag) {
+// rinv (LDS, CHOL_NB doubles) carries the reciprocals of the block's pivots
+// from (B) to (C), whose rows multiply by them instead of dividing (within an
+// ulp per element of the reference's division; the division sat on each
+// row's dependent chain).
+__device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* flag, double* rinv) {
     const int tid = threadIdx.x;
     const int lane = lane_id();
     for (int J0 = 0; J0 < m; J0 += CHOL_NB) {
         const int jb = m - J0 < CHOL_NB ? m - J0 : CHOL_NB;
-        // (A) panel update, k < J0
+        // (A) panel update, k < J0, on the FP64 matrix cores: wave w takes
+        // the 16-row tiles w, w + 4, ... of rows J0 .. rows - 1 against the
+        // block's 16 columns; D[r][c] = sum_k L[i_r][k] L[J0 + c][k] over
+        // four-deep k steps (two accumulators, alternate steps), subtracted
+        // once at the end (the reference subtracts term by term: the sum
+        // rounds differently, within the factor's backward error).
+        // v_mfma_f64_16x16x4f64 lanes: A lane l -> A[l % 16][l / 16],
+        // B lane l -> B[l / 16][l % 16], D lane l element e -> D[4 e + l / 16][l % 16].
         if (J0 > 0) {
-            const int nrt = (rows - J0 + 3) >> 2;
-            for (int t = tid; t < nrt * (CHOL_NB / 4); t += GP_THREADS) {
-                const int i0 = J0 + 4 * (t >> 2), j0 = J0 + 4 * (t & 3);
-                if (j0 >= J0 + jb || j0 > i0 + 3) continue;
-                double acc[4][4];
-                int ri[4], rj[4];
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int i = i0 + r < rows ? i0 + r : rows - 1;
-                    const int j = j0 + r < m ? j0 + r : m - 1;
-                    ri[r] = rs(i);
-                    rj[r] = rs(j);
+            const int ntile = (rows - J0 + 15) >> 4;
+            const int kk = lane >> 4, c16 = lane & 15;
+            const int jb_row = J0 + c16;
+            const bool b_ok = c16 < jb;
+            const int rb = rs(b_ok ? jb_row : J0) + kk;
+            for (int T = tid >> 6; T < ntile; T += GP_WAVES) {
+                const int i_a = J0 + 16 * T + c16;
+                const bool a_ok = i_a < rows;
+                const int ra = rs(a_ok ? i_a : J0) + kk;
+                v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+                for (int k0 = 0; k0 < J0; k0 += 8) {  // J0 is a multiple of CHOL_NB = 16
+                    const double a0 = a_ok ? L[ra + k0] : 0.0, b0 = b_ok ? L[rb + k0] : 0.0;
+                    const double a1 = a_ok ? L[ra + k0 + 4] : 0.0, b1 = b_ok ? L[rb + k0 + 4] : 0.0;
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; r++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int i = i0 + r, j = j0 + c;
-                        acc[r][c] = (i < rows && j <= i && j < m) ? L[ri[r] + j] : 0.0;
-                    }
-#pragma unroll 2
-                for (int k = 0; k < J0; k++) {
-                    double a[4], b[4];
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        a[r] = L[ri[r] + k];
-                        b[r] = L[rj[r] + k];
-                    }
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-#pragma unroll
-                        for (int c = 0; c < 4; c++) acc[r][c] -= a[r] * b[c];
+                for (int e = 0; e < 4; e++) {
+                    const int i = J0 + 16 * T + 4 * e + kk, j = J0 + c16;
+                    if (i < rows && c16 < jb && j <= i) L[rs(i) + j] -= acc0[e] + acc1[e];
                 }
-#pragma unroll
-                for (int r = 0; r < 4; r++)
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int i = i0 + r, j = j0 + c;
-                        if (i < rows && j <= i && j < J0 + jb) L[ri[r] + j] = acc[r][c];
-                    }
             }
             __syncthreads();
         }
@@ -156,10 +150,15 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
                     }
                 }
             }
-            if (own)
+            if (own) {
 #pragma unroll
                 for (int c = 0; c < CHOL_NB; c++)
                     if (c <= lane) L[rr + c] = v[c];
+                double d = v[0];
+#pragma unroll
+                for (int c = 1; c < CHOL_NB; c++) d = c == lane ? v[c] : d;
+                rinv[lane] = 1.0 / d;
+            }
             if (tid == 0) *flag = bad;
         }
         __syncthreads();
@@ -180,7 +179,7 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
                     double acc = v[j];
 #pragma unroll
                     for (int k = 0; k < j; k++) acc -= v[k] * L[rj0 + k];
-                    v[j] = acc / L[rj0 + j];
+                    v[j] = acc * rinv[j];
                 }
             }
 #pragma unroll
@@ -194,11 +193,25 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
 
 // One wavefront: b <- L^-1 b (rows lane + 64 r in registers); returns the
 // squared norm of the solution (wave-uniform order j = 0..m-1).
+// The pivots' reciprocals are formed up front, one per lane and row, so no
+// division sits on the dependent chain of the substitution (each step
+// multiplies: within an ulp of the reference's division).
+__device__ __forceinline__ void wave_rdiag(const double* L, int m, double (&rd)[GP_ROWS]) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int r = 0; r < GP_ROWS; r++) {
+        const int i = lane + WAVE * r;
+        rd[r] = i < m ? 1.0 / L[tri(i, i)] : 0.0;
+    }
+}
+
 __device__ __forceinline__ double wave_solve_l(const double* L, int m, double (&b)[GP_ROWS]) {
     const int lane = lane_id();
     double nrm = 0.0;
+    double rd[GP_ROWS];
+    wave_rdiag(L, m, rd);
     for (int j = 0; j < m; j++) {
-        double zj = row_of(b, j) / L[tri(j, j)];
+        double zj = row_of(b, j) * row_of(rd, j);
         nrm += zj * zj;
 #pragma unroll
         for (int r = 0; r < GP_ROWS; r++) {
@@ -213,8 +226,10 @@ __device__ __forceinline__ double wave_solve_l(const double* L, int m, double (&
 // One wavefront: b <- L^-T b.
 __device__ __forceinline__ void wave_solve_lt(const double* L, int m, double (&b)[GP_ROWS]) {
     const int lane = lane_id();
+    double rd[GP_ROWS];
+    wave_rdiag(L, m, rd);
     for (int j = m - 1; j >= 0; j--) {
-        double aj = row_of(b, j) / L[tri(j, j)];
+        double aj = row_of(b, j) * row_of(rd, j);
 #pragma unroll
         for (int r = 0; r < GP_ROWS; r++) {
             int i = lane + WAVE * r;

@@ -37,7 +37,7 @@ constexpr double OFF_EFOLD = 1.0857362047581294;    // 2.5 log10(e)
 constexpr double OFF_HALF = 0.7525749891599529;     // 2.5 log10(2)
 constexpr int FITC_TRI = NP_MAX_IND * (NP_MAX_IND + 1) / 2;
 constexpr int FITC_DOUBLES = 2 * FITC_TRI + NP_CHUNK * NP_MAX_IND + 2 * NP_CHUNK + NP_MAX_IND;
-constexpr int MISC_DOUBLES = 48;
+constexpr int MISC_DOUBLES = 64;
 
 __host__ __device__ constexpr int tri_n(int c) { return c * (c + 1) / 2; }
 
@@ -62,6 +62,7 @@ enum {
     M_SCAL = 8,     // scalar broadcast slots
     M_Q = 16,       // [9] predictive queries
     M_INT = 32,     // int view starts here (16 doubles = 32 ints)
+    M_RINV = 48,    // [CHOL_NB] pivot reciprocals of the blocked Cholesky
 };
 
 struct Lds {
@@ -183,9 +184,10 @@ struct Pred {
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) q[u] = 0.0;
+        double rd[GP_ROWS];  // pivot reciprocals of this lane's rows, formed once
+        wave_rdiag(F, m, rd);
         for (int j = 0; j < m; j++) {
-            // the reciprocal depends on j only, so it is off the chain through b
-            const double rljj = 1.0 / F[tri(j, j)];
+            const double rljj = row_of(rd, j);
             double z[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -213,7 +215,7 @@ struct Pred {
 // reading x again.
 template <typename D>
 __device__ __forceinline__ bool dense_factor(double* L, const D& d, int m, double amp, double it, double ybar,
-                                             int* flag) {
+                                             int* flag, double* rinv) {
     const int ne = rs(m);
     for (int e = threadIdx.x; e < ne; e += GP_THREADS) {
         int i, j;
@@ -225,7 +227,7 @@ __device__ __forceinline__ bool dense_factor(double* L, const D& d, int m, doubl
     __syncthreads();
     for (int j = threadIdx.x; j < m; j += GP_THREADS) L[ne + j] = d.yy(j) - ybar;
     __syncthreads();
-    return chol_blocked(L, m, m + 1, flag);
+    return chol_blocked(L, m, m + 1, flag, rinv);
 }
 
 // Wavefront 0: alpha = L^-T z with z = L^-1 (y - ybar) from row m, written
@@ -275,6 +277,7 @@ struct Fitc {
     double *lmm, *lb, *kc, *li, *lv, *w;  // in r0
     double *z, *az;                        // in the weight slot
     int* flag;
+    double* rinv;                          // pivot reciprocals (misc)
 };
 
 __device__ __forceinline__ Fitc fitc_carve(const Lds& s) {
@@ -288,6 +291,7 @@ __device__ __forceinline__ Fitc fitc_carve(const Lds& s) {
     f.z = s.al;
     f.az = s.al + NP_MAX_IND;
     f.flag = s.imisc + 20;
+    f.rinv = s.misc + M_RINV;
     return f;
 }
 
@@ -307,7 +311,7 @@ __device__ __forceinline__ bool fitc_fit(const Fitc& f, const Det& d, int nd, in
     }
     if (tid < m) f.w[tid] = 0.0;
     __syncthreads();
-    if (!chol_blocked(f.lmm, m, m, f.flag)) return false;
+    if (!chol_blocked(f.lmm, m, m, f.flag, f.rinv)) return false;
     for (int j0 = 0; j0 < nd; j0 += NP_CHUNK) {
         const int cn = nd - j0 < NP_CHUNK ? nd - j0 : NP_CHUNK;
         for (int e = tid; e < cn * m; e += GP_THREADS) {
@@ -359,7 +363,7 @@ __device__ __forceinline__ bool fitc_fit(const Fitc& f, const Det& d, int nd, in
         }
         __syncthreads();
     }
-    if (!chol_blocked(f.lb, m, m, f.flag)) return false;
+    if (!chol_blocked(f.lb, m, m, f.flag, f.rinv)) return false;
     if (tid < WAVE) {
         double b[GP_ROWS];
 #pragma unroll
@@ -738,7 +742,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
         const double it = 0.5 / (ls * ls);
         double score;
         if (!sparse) {
-            const bool ok = dense_factor(s.r0, dls, nd, amp, it, ybar, flag);
+            const bool ok = dense_factor(s.r0, dls, nd, amp, it, ybar, flag, s.misc + M_RINV);
             if (ok && tid < WAVE) dense_alpha(s.r0, nd, s.al);
             restage(s.xs, dls, nd);
             if (!ok) continue;
@@ -768,7 +772,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     P.ybar = ybar;
     P.d = dls;
     if (!sparse) {
-        dense_factor(s.r0, dls, nd, famp, fit, ybar, flag);
+        dense_factor(s.r0, dls, nd, famp, fit, ybar, flag, s.misc + M_RINV);
         if (tid < WAVE) dense_alpha(s.r0, nd, st);
         restage(s.xs, dls, nd);
         for (int i = tid; i < nd; i += GP_THREADS) st[a.state_cap + i] = dx[i];
@@ -796,7 +800,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
             const double amp = a.amps[c / a.n_ls];
             const double ls = fmax(dur / a.lsf[c % a.n_ls], 0.1);
             const double it = 0.5 / (ls * ls);
-            const bool ok = dense_factor(s.r0, dsub, ms, amp, it, ys, flag);
+            const bool ok = dense_factor(s.r0, dsub, ms, amp, it, ys, flag, s.misc + M_RINV);
             if (ok && tid < WAVE) dense_alpha(s.r0, ms, st);
             restage(s.xs, dsub, ms);
             if (!ok) continue;

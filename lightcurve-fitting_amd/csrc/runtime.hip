@@ -1627,9 +1627,22 @@ int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, con
     if (!out_status || !out_features || !out_pred || !out_std || !out_dpred || !out_state)
         return fail(c, LCF_E_INVALID, "null output");
     HIPCHK(c, hipSetDevice(c->device));
-    // Size classes by raw band size: a band of at most dense_max observations
-    // never needs FITC and goes to the smallest dense class that holds it;
-    // larger bands go to the class with the FITC work area.
+    // Size classes by detection count, which sets the dense training set and
+    // so the LDS a band needs.  A detection has SNR = (2.5 / ln 10) / err >= 3
+    // (mag2flux cancels); the host counts err <= (2.5 / ln 10) / (3 (1 - 1e-9)),
+    // an upper bound on the device's count whatever the rounding of the
+    // flux, so a band is never routed to a class too small for it.  A band of
+    // at most dense_max detections goes to the smallest dense class that
+    // holds them; larger ones to the class with the FITC work area.
+    std::vector<int32_t> nd_up(n_bands);
+    {
+        const double err_max = 1.0857362047581294 / (3.0 * (1.0 - 1e-9));
+        for (int64_t b = 0; b < n_bands; b++) {
+            int32_t k = 0;
+            for (int64_t i = off[b]; i < off[b + 1]; i++) k += err[i] > 0.0 && err[i] <= err_max;
+            nd_up[b] = k;
+        }
+    }
     struct Cls { int cap, fitc; std::vector<int32_t> bands; };
     std::vector<Cls> cls;
     for (int cp : {64, 128})
@@ -1637,7 +1650,7 @@ int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, con
     cls.push_back({state_cap, 0, {}});
     cls.push_back({state_cap, 1, {}});
     for (int64_t b = 0; b < n_bands; b++) {
-        const int64_t nb = off[b + 1] - off[b];
+        const int64_t nb = nd_up[b];
         size_t k = cls.size() - 1;
         if (nb <= dense_max)
             for (size_t q = 0; q + 1 < cls.size(); q++)
@@ -1645,9 +1658,8 @@ int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, con
         cls[k].bands.push_back((int32_t)b);
     }
     for (auto& q : cls)
-        std::stable_sort(q.bands.begin(), q.bands.end(), [&](int32_t a, int32_t b) {
-            return off[a + 1] - off[a] > off[b + 1] - off[b];
-        });
+        std::stable_sort(q.bands.begin(), q.bands.end(),
+                         [&](int32_t a, int32_t b) { return nd_up[a] > nd_up[b]; });
     std::vector<int32_t> order;
     for (auto& q : cls) order.insert(order.end(), q.bands.begin(), q.bands.end());
     const int stride = 2 * state_cap + 4;
