@@ -191,8 +191,6 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
     return true;
 }
 
-// One wavefront: b <- L^-1 b (rows lane + 64 r in registers); returns the
-// squared norm of the solution (wave-uniform order j = 0..m-1).
 // The pivots' reciprocals are formed up front, one per lane and row, so no
 // division sits on the dependent chain of the substitution (each step
 // multiplies: within an ulp of the reference's division).
@@ -205,38 +203,70 @@ __device__ __forceinline__ void wave_rdiag(const double* L, int m, double (&rd)[
     }
 }
 
+// The substitutions below run j in four segments of 64 (register R = j / 64
+// holds row j), so the row's register is chosen at compile time and the
+// registers of rows that are already final are left alone.
+
+// b <- L^-1 b over rows [64 R, min(64 R + 64, m)); adds z_j^2 to nrm.
+template <int R>
+__device__ __forceinline__ void solve_l_seg(const double* L, int m, double (&b)[GP_ROWS], const double (&rd)[GP_ROWS],
+                                            const int (&rsi)[GP_ROWS], double& nrm) {
+    const int lane = lane_id();
+    const int jhi = m < 64 * R + 64 ? m : 64 * R + 64;
+    for (int j = 64 * R; j < jhi; j++) {
+        const int src = j & 63;
+        const double zj = bcast(b[R], src) * bcast(rd[R], src);
+        nrm += zj * zj;
+        if (lane == src) b[R] = zj;
+        else if (lane > src && lane + 64 * R < m) b[R] -= L[rsi[R] + j] * zj;
+#pragma unroll
+        for (int r = R + 1; r < GP_ROWS; r++)
+            if (lane + 64 * r < m) b[r] -= L[rsi[r] + j] * zj;
+    }
+}
+
+// One wavefront: b <- L^-1 b (rows lane + 64 r in registers); returns the
+// squared norm of the solution (wave-uniform order j = 0..m-1).
 __device__ __forceinline__ double wave_solve_l(const double* L, int m, double (&b)[GP_ROWS]) {
     const int lane = lane_id();
-    double nrm = 0.0;
     double rd[GP_ROWS];
+    int rsi[GP_ROWS];
     wave_rdiag(L, m, rd);
-    for (int j = 0; j < m; j++) {
-        double zj = row_of(b, j) * row_of(rd, j);
-        nrm += zj * zj;
 #pragma unroll
-        for (int r = 0; r < GP_ROWS; r++) {
-            int i = lane + WAVE * r;
-            if (i == j) b[r] = zj;
-            else if (i > j && i < m) b[r] -= L[tri(i, j)] * zj;
-        }
-    }
+    for (int r = 0; r < GP_ROWS; r++) rsi[r] = rs(lane + 64 * r);
+    double nrm = 0.0;
+    solve_l_seg<0>(L, m, b, rd, rsi, nrm);
+    if (m > 64) solve_l_seg<1>(L, m, b, rd, rsi, nrm);
+    if (m > 128) solve_l_seg<2>(L, m, b, rd, rsi, nrm);
+    if (m > 192) solve_l_seg<3>(L, m, b, rd, rsi, nrm);
     return nrm;
+}
+
+// b <- L^-T b over rows j in [64 R, min(64 R + 64, m)), descending.
+template <int R>
+__device__ __forceinline__ void solve_lt_seg(const double* L, int m, double (&b)[GP_ROWS],
+                                             const double (&rd)[GP_ROWS]) {
+    const int lane = lane_id();
+    const int jhi = m < 64 * R + 64 ? m : 64 * R + 64;
+    for (int j = jhi - 1; j >= 64 * R; j--) {
+        const int src = j & 63;
+        const double aj = bcast(b[R], src) * bcast(rd[R], src);
+        const int rj = rs(j);
+        if (lane == src) b[R] = aj;
+        else if (lane < src) b[R] -= L[rj + lane + 64 * R] * aj;
+#pragma unroll
+        for (int r = 0; r < R; r++) b[r] -= L[rj + lane + 64 * r] * aj;
+    }
 }
 
 // One wavefront: b <- L^-T b.
 __device__ __forceinline__ void wave_solve_lt(const double* L, int m, double (&b)[GP_ROWS]) {
-    const int lane = lane_id();
     double rd[GP_ROWS];
     wave_rdiag(L, m, rd);
-    for (int j = m - 1; j >= 0; j--) {
-        double aj = row_of(b, j) * row_of(rd, j);
-#pragma unroll
-        for (int r = 0; r < GP_ROWS; r++) {
-            int i = lane + WAVE * r;
-            if (i == j) b[r] = aj;
-            else if (i < j) b[r] -= L[tri(j, i)] * aj;
-        }
-    }
+    if (m > 192) solve_lt_seg<3>(L, m, b, rd);
+    if (m > 128) solve_lt_seg<2>(L, m, b, rd);
+    if (m > 64) solve_lt_seg<1>(L, m, b, rd);
+    solve_lt_seg<0>(L, m, b, rd);
 }
 
 // Workgroup reductions through red[GP_WAVES] in LDS; every thread gets the

@@ -185,24 +185,45 @@ struct Pred {
 #pragma unroll
         for (int u = 0; u < 4; u++) q[u] = 0.0;
         double rd[GP_ROWS];  // pivot reciprocals of this lane's rows, formed once
+        int rsi[GP_ROWS];
         wave_rdiag(F, m, rd);
-        for (int j = 0; j < m; j++) {
-            const double rljj = row_of(rd, j);
+#pragma unroll
+        for (int r = 0; r < GP_ROWS; r++) rsi[r] = rs(lane + WAVE * r);
+        sqnorm4_seg<0>(F, b, rd, rsi, q);
+        if (m > 64) sqnorm4_seg<1>(F, b, rd, rsi, q);
+        if (m > 128) sqnorm4_seg<2>(F, b, rd, rsi, q);
+        if (m > 192) sqnorm4_seg<3>(F, b, rd, rsi, q);
+    }
+    // Forward substitution of the four right-hand sides over the rows j of
+    // register R (j in [64 R, 64 R + 64)); the multiply-adds are fused (an
+    // ulp from the reference's separate rounding, far inside the std's
+    // tolerance).
+    template <int R>
+    __device__ __forceinline__ void sqnorm4_seg(const double* F, double (&b)[4][GP_ROWS], const double (&rd)[GP_ROWS],
+                                                const int (&rsi)[GP_ROWS], double (&q)[4]) const {
+        const int lane = lane_id();
+        const int jhi = m < 64 * R + 64 ? m : 64 * R + 64;
+        for (int j = 64 * R; j < jhi; j++) {
+            const int src = j & 63;
+            const double rljj = bcast(rd[R], src);
             double z[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                z[u] = row_of(b[u], j) * rljj;
-                q[u] += z[u] * z[u];
+                z[u] = bcast(b[u][R], src) * rljj;
+                q[u] = __builtin_fma(z[u], z[u], q[u]);
+            }
+            if (lane > src && lane + 64 * R < m) {
+                const double lij = F[rsi[R] + j];
+#pragma unroll
+                for (int u = 0; u < 4; u++) b[u][R] = __builtin_fma(-lij, z[u], b[u][R]);
             }
 #pragma unroll
-            for (int r = 0; r < GP_ROWS; r++) {
-                const int i = lane + WAVE * r;
-                if (i > j && i < m) {
-                    const double lij = F[tri(i, j)];
+            for (int r = R + 1; r < GP_ROWS; r++)
+                if (lane + 64 * r < m) {
+                    const double lij = F[rsi[r] + j];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) b[u][r] -= lij * z[u];
+                    for (int u = 0; u < 4; u++) b[u][r] = __builtin_fma(-lij, z[u], b[u][r]);
                 }
-            }
         }
     }
 };
