@@ -78,8 +78,11 @@ constexpr int CHOL_NB = 16;  // block-column width of the blocked Cholesky
 // its rows in registers (lane r holds row J0 + r; column j is scaled, then
 // the columns to its right lose l_ij l_cj); (C) every row below solves
 // against the diagonal block.  Within a block the elements are reduced term
-// by term in increasing k, as the row-major Cholesky-Crout of
-// src/sparse_gp.rs:27-49; the panel sums of (A) round as tile products.
+// by term in increasing k (fused multiply-adds), as the row-major
+// Cholesky-Crout of src/sparse_gp.rs:27-49 orders them; the panel sums of
+// (A) round as tile products, and the column scalings multiply by the
+// pivot's reciprocal: the factor agrees with the reference's to within its
+// backward error, not bit for bit.
 // rows = m + 1 appends row m (its first m entries, e.g. y - ybar): it is
 // reduced like any row below the factor and ends as L^-1 times itself, the
 // forward substitution of solve_l (src/sparse_gp.rs:52-58).
@@ -135,18 +138,23 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
 #pragma unroll
             for (int c = 0; c < CHOL_NB; c++) v[c] = (own && c <= lane) ? L[rr + c] : 0.0;
             int bad = 0;
+            double rl = 0.0;  // lane j: 1 / l_jj
 #pragma unroll
             for (int j = 0; j < CHOL_NB; j++) {
                 if (j < jb) {
                     const double piv = bcast(v[j], j);
                     bad |= piv <= 0.0;
+                    // l_jj = sqrt(piv) and the column scaled by its
+                    // reciprocal (within an ulp of dividing by l_jj)
                     const double ljj = ::sqrt(piv);
-                    v[j] = lane == j ? ljj : (lane > j ? v[j] / ljj : v[j]);
+                    const double r = 1.0 / ljj;
+                    if (lane == j) rl = r;
+                    v[j] = lane == j ? ljj : (lane > j ? v[j] * r : v[j]);
                     // columns c > j of the rows below c lose l_ij l_cj
 #pragma unroll
                     for (int c = j + 1; c < CHOL_NB; c++) {
                         const double lcj = bcast(v[j], c);
-                        if (lane >= c) v[c] -= v[j] * lcj;
+                        if (lane >= c) v[c] = __builtin_fma(-v[j], lcj, v[c]);
                     }
                 }
             }
@@ -154,10 +162,7 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
 #pragma unroll
                 for (int c = 0; c < CHOL_NB; c++)
                     if (c <= lane) L[rr + c] = v[c];
-                double d = v[0];
-#pragma unroll
-                for (int c = 1; c < CHOL_NB; c++) d = c == lane ? v[c] : d;
-                rinv[lane] = 1.0 / d;
+                rinv[lane] = rl;
             }
             if (tid == 0) *flag = bad;
         }
@@ -178,7 +183,7 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
                     const int rj0 = rs(J0 + j) + J0;
                     double acc = v[j];
 #pragma unroll
-                    for (int k = 0; k < j; k++) acc -= v[k] * L[rj0 + k];
+                    for (int k = 0; k < j; k++) acc = __builtin_fma(-v[k], L[rj0 + k], acc);
                     v[j] = acc * rinv[j];
                 }
             }
