@@ -138,24 +138,24 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
 #pragma unroll
             for (int c = 0; c < CHOL_NB; c++) v[c] = (own && c <= lane) ? L[rr + c] : 0.0;
             int bad = 0;
+            // Entries above a lane's diagonal (v[c], c > lane) and the rows of
+            // lanes >= jb are never read or stored, so they are updated
+            // unconditionally (no selects on the chain).
             double rl = 0.0;  // lane j: 1 / l_jj
 #pragma unroll
             for (int j = 0; j < CHOL_NB; j++) {
                 if (j < jb) {
                     const double piv = bcast(v[j], j);
                     bad |= piv <= 0.0;
-                    // l_jj = sqrt(piv) and the column scaled by its
-                    // reciprocal (within an ulp of dividing by l_jj)
-                    const double ljj = ::sqrt(piv);
-                    const double r = 1.0 / ljj;
+                    // r = 1 / sqrt(piv): v_rsq_f64 and one Newton step (a few
+                    // ulp); l_jj = piv r and the column scaled by r
+                    double r = __builtin_amdgcn_rsq(piv);
+                    r = r * __builtin_fma(-0.5 * piv * r, r, 1.5);
                     if (lane == j) rl = r;
-                    v[j] = lane == j ? ljj : (lane > j ? v[j] * r : v[j]);
+                    v[j] = lane == j ? piv * r : v[j] * r;
                     // columns c > j of the rows below c lose l_ij l_cj
 #pragma unroll
-                    for (int c = j + 1; c < CHOL_NB; c++) {
-                        const double lcj = bcast(v[j], c);
-                        if (lane >= c) v[c] = __builtin_fma(-v[j], lcj, v[c]);
-                    }
+                    for (int c = j + 1; c < CHOL_NB; c++) v[c] = __builtin_fma(-v[j], bcast(v[j], c), v[c]);
                 }
             }
             if (own) {

@@ -270,10 +270,13 @@ __device__ __forceinline__ void dense_alpha(const double* L, int m, double* alph
     }
 }
 
-// All threads: the training times back into LDS, then a barrier.
+// All threads: a barrier, the training times back into LDS where the
+// augmented row m of the factor (packed at [rs(m), rs(m) + m) of r0, which
+// xs follows) reached them, then a barrier.
 template <typename D>
-__device__ __forceinline__ void restage(double* xs, const D& d, int m) {
+__device__ __forceinline__ void restage(const double* r0, double* xs, const D& d, int m) {
     __syncthreads();
+    if (rs(m) + m <= (int)(xs - r0)) return;
     for (int i = threadIdx.x; i < m; i += GP_THREADS) xs[i] = d.xg(i);
     __syncthreads();
 }
@@ -765,7 +768,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
         if (!sparse) {
             const bool ok = dense_factor(s.r0, dls, nd, amp, it, ybar, flag, s.misc + M_RINV);
             if (ok && tid < WAVE) dense_alpha(s.r0, nd, s.al);
-            restage(s.xs, dls, nd);
+            restage(s.r0, s.xs, dls, nd);
             if (!ok) continue;
             score = dense_train_rms(dls, nd, amp, it, ybar, s.al, red);
         } else {
@@ -795,7 +798,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     if (!sparse) {
         dense_factor(s.r0, dls, nd, famp, fit, ybar, flag, s.misc + M_RINV);
         if (tid < WAVE) dense_alpha(s.r0, nd, st);
-        restage(s.xs, dls, nd);
+        restage(s.r0, s.xs, dls, nd);
         for (int i = tid; i < nd; i += GP_THREADS) st[a.state_cap + i] = dx[i];
         if (tid == 0) {
             st[2 * a.state_cap] = famp;
@@ -823,7 +826,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
             const double it = 0.5 / (ls * ls);
             const bool ok = dense_factor(s.r0, dsub, ms, amp, it, ys, flag, s.misc + M_RINV);
             if (ok && tid < WAVE) dense_alpha(s.r0, ms, st);
-            restage(s.xs, dsub, ms);
+            restage(s.r0, s.xs, dsub, ms);
             if (!ok) continue;
             for (int i = tid; i < ms; i += GP_THREADS) st[a.state_cap + i] = dsub.x(i);
             if (tid == 0) {
