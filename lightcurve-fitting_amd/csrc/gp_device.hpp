@@ -11,15 +11,18 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int GP_THREADS = 256;
 constexpr int GP_WAVES = GP_THREADS / WAVE;
-constexpr int GP_ROWS = 4;  // rows per lane in the wave-level solves (m <= 256)
+constexpr int GP_ROWS = 4;  // rows per lane in the wave-level solves (m <= 256); callers may use fewer
 
 // Packed lower triangle: element (i, j), j <= i.
 __device__ __forceinline__ int tri(int i, int j) { return (i * (i + 1) >> 1) + j; }
 
-// Row k's value from rows held as (lane + 64 r): wave-uniform k.
-__device__ __forceinline__ double row_of(const double (&v)[GP_ROWS], int k) {
+// Row k's value from rows held as (lane + 64 r), r < ROWS: wave-uniform k.
+template <int ROWS>
+__device__ __forceinline__ double row_of(const double (&v)[ROWS], int k) {
     const int r = k >> 6;
-    double c = r == 0 ? v[0] : (r == 1 ? v[1] : (r == 2 ? v[2] : v[3]));
+    double c = v[0];
+#pragma unroll
+    for (int q = 1; q < ROWS; q++) c = r == q ? v[q] : c;
     return bcast(c, k & (WAVE - 1));
 }
 
@@ -199,10 +202,11 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
 // The pivots' reciprocals are formed up front, one per lane and row, so no
 // division sits on the dependent chain of the substitution (each step
 // multiplies: within an ulp of the reference's division).
-__device__ __forceinline__ void wave_rdiag(const double* L, int m, double (&rd)[GP_ROWS]) {
+template <int ROWS>
+__device__ __forceinline__ void wave_rdiag(const double* L, int m, double (&rd)[ROWS]) {
     const int lane = lane_id();
 #pragma unroll
-    for (int r = 0; r < GP_ROWS; r++) {
+    for (int r = 0; r < ROWS; r++) {
         const int i = lane + WAVE * r;
         rd[r] = i < m ? 1.0 / L[tri(i, i)] : 0.0;
     }
@@ -213,9 +217,9 @@ __device__ __forceinline__ void wave_rdiag(const double* L, int m, double (&rd)[
 // registers of rows that are already final are left alone.
 
 // b <- L^-1 b over rows [64 R, min(64 R + 64, m)); adds z_j^2 to nrm.
-template <int R>
-__device__ __forceinline__ void solve_l_seg(const double* L, int m, double (&b)[GP_ROWS], const double (&rd)[GP_ROWS],
-                                            const int (&rsi)[GP_ROWS], double& nrm) {
+template <int R, int ROWS>
+__device__ __forceinline__ void solve_l_seg(const double* L, int m, double (&b)[ROWS], const double (&rd)[ROWS],
+                                            const int (&rsi)[ROWS], double& nrm) {
     const int lane = lane_id();
     const int jhi = m < 64 * R + 64 ? m : 64 * R + 64;
     for (int j = 64 * R; j < jhi; j++) {
@@ -225,32 +229,32 @@ __device__ __forceinline__ void solve_l_seg(const double* L, int m, double (&b)[
         if (lane == src) b[R] = zj;
         else if (lane > src && lane + 64 * R < m) b[R] -= L[rsi[R] + j] * zj;
 #pragma unroll
-        for (int r = R + 1; r < GP_ROWS; r++)
+        for (int r = R + 1; r < ROWS; r++)
             if (lane + 64 * r < m) b[r] -= L[rsi[r] + j] * zj;
     }
 }
 
-// One wavefront: b <- L^-1 b (rows lane + 64 r in registers); returns the
-// squared norm of the solution (wave-uniform order j = 0..m-1).
-__device__ __forceinline__ double wave_solve_l(const double* L, int m, double (&b)[GP_ROWS]) {
+// One wavefront: b <- L^-1 b (rows lane + 64 r in registers, m <= 64 ROWS);
+// returns the squared norm of the solution (wave-uniform order j = 0..m-1).
+template <int ROWS>
+__device__ __forceinline__ double wave_solve_l(const double* L, int m, double (&b)[ROWS]) {
     const int lane = lane_id();
-    double rd[GP_ROWS];
-    int rsi[GP_ROWS];
+    double rd[ROWS];
+    int rsi[ROWS];
     wave_rdiag(L, m, rd);
 #pragma unroll
-    for (int r = 0; r < GP_ROWS; r++) rsi[r] = rs(lane + 64 * r);
+    for (int r = 0; r < ROWS; r++) rsi[r] = rs(lane + 64 * r);
     double nrm = 0.0;
     solve_l_seg<0>(L, m, b, rd, rsi, nrm);
-    if (m > 64) solve_l_seg<1>(L, m, b, rd, rsi, nrm);
-    if (m > 128) solve_l_seg<2>(L, m, b, rd, rsi, nrm);
-    if (m > 192) solve_l_seg<3>(L, m, b, rd, rsi, nrm);
+    if constexpr (ROWS > 1) if (m > 64) solve_l_seg<1>(L, m, b, rd, rsi, nrm);
+    if constexpr (ROWS > 2) if (m > 128) solve_l_seg<2>(L, m, b, rd, rsi, nrm);
+    if constexpr (ROWS > 3) if (m > 192) solve_l_seg<3>(L, m, b, rd, rsi, nrm);
     return nrm;
 }
 
 // b <- L^-T b over rows j in [64 R, min(64 R + 64, m)), descending.
-template <int R>
-__device__ __forceinline__ void solve_lt_seg(const double* L, int m, double (&b)[GP_ROWS],
-                                             const double (&rd)[GP_ROWS]) {
+template <int R, int ROWS>
+__device__ __forceinline__ void solve_lt_seg(const double* L, int m, double (&b)[ROWS], const double (&rd)[ROWS]) {
     const int lane = lane_id();
     const int jhi = m < 64 * R + 64 ? m : 64 * R + 64;
     for (int j = jhi - 1; j >= 64 * R; j--) {
@@ -264,13 +268,14 @@ __device__ __forceinline__ void solve_lt_seg(const double* L, int m, double (&b)
     }
 }
 
-// One wavefront: b <- L^-T b.
-__device__ __forceinline__ void wave_solve_lt(const double* L, int m, double (&b)[GP_ROWS]) {
-    double rd[GP_ROWS];
+// One wavefront: b <- L^-T b (m <= 64 ROWS).
+template <int ROWS>
+__device__ __forceinline__ void wave_solve_lt(const double* L, int m, double (&b)[ROWS]) {
+    double rd[ROWS];
     wave_rdiag(L, m, rd);
-    if (m > 192) solve_lt_seg<3>(L, m, b, rd);
-    if (m > 128) solve_lt_seg<2>(L, m, b, rd);
-    if (m > 64) solve_lt_seg<1>(L, m, b, rd);
+    if constexpr (ROWS > 3) if (m > 192) solve_lt_seg<3>(L, m, b, rd);
+    if constexpr (ROWS > 2) if (m > 128) solve_lt_seg<2>(L, m, b, rd);
+    if constexpr (ROWS > 1) if (m > 64) solve_lt_seg<1>(L, m, b, rd);
     solve_lt_seg<0>(L, m, b, rd);
 }
 

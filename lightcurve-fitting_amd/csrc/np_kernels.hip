@@ -132,6 +132,7 @@ struct DetSub {
 // inducing points z (LDS).  Every pointer refers to LDS and none is null: a
 // null merged with an LDS address makes the compiler test the LDS aperture,
 // which it cannot encode here.
+template <int ROWS>
 struct Pred {
     int sparse, m;
     double amp, it, ybar;
@@ -174,9 +175,9 @@ struct Pred {
     }
     __device__ __forceinline__ void wave_sqnorm4(const double* F, const double (&tq)[4], double (&q)[4]) const {
         const int lane = lane_id();
-        double b[4][GP_ROWS];
+        double b[4][ROWS];
 #pragma unroll
-        for (int r = 0; r < GP_ROWS; r++) {
+        for (int r = 0; r < ROWS; r++) {
             const int i = lane + WAVE * r;
             const double xi = i < m ? xk(i) : 0.0;
 #pragma unroll
@@ -184,23 +185,23 @@ struct Pred {
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) q[u] = 0.0;
-        double rd[GP_ROWS];  // pivot reciprocals of this lane's rows, formed once
-        int rsi[GP_ROWS];
+        double rd[ROWS];  // pivot reciprocals of this lane's rows, formed once
+        int rsi[ROWS];
         wave_rdiag(F, m, rd);
 #pragma unroll
-        for (int r = 0; r < GP_ROWS; r++) rsi[r] = rs(lane + WAVE * r);
+        for (int r = 0; r < ROWS; r++) rsi[r] = rs(lane + WAVE * r);
         sqnorm4_seg<0>(F, b, rd, rsi, q);
-        if (m > 64) sqnorm4_seg<1>(F, b, rd, rsi, q);
-        if (m > 128) sqnorm4_seg<2>(F, b, rd, rsi, q);
-        if (m > 192) sqnorm4_seg<3>(F, b, rd, rsi, q);
+        if constexpr (ROWS > 1) if (m > 64) sqnorm4_seg<1>(F, b, rd, rsi, q);
+        if constexpr (ROWS > 2) if (m > 128) sqnorm4_seg<2>(F, b, rd, rsi, q);
+        if constexpr (ROWS > 3) if (m > 192) sqnorm4_seg<3>(F, b, rd, rsi, q);
     }
     // Forward substitution of the four right-hand sides over the rows j of
     // register R (j in [64 R, 64 R + 64)); the multiply-adds are fused (an
     // ulp from the reference's separate rounding, far inside the std's
     // tolerance).
     template <int R>
-    __device__ __forceinline__ void sqnorm4_seg(const double* F, double (&b)[4][GP_ROWS], const double (&rd)[GP_ROWS],
-                                                const int (&rsi)[GP_ROWS], double (&q)[4]) const {
+    __device__ __forceinline__ void sqnorm4_seg(const double* F, double (&b)[4][ROWS], const double (&rd)[ROWS],
+                                                const int (&rsi)[ROWS], double (&q)[4]) const {
         const int lane = lane_id();
         const int jhi = m < 64 * R + 64 ? m : 64 * R + 64;
         for (int j = 64 * R; j < jhi; j++) {
@@ -218,7 +219,7 @@ struct Pred {
                 for (int u = 0; u < 4; u++) b[u][R] = __builtin_fma(-lij, z[u], b[u][R]);
             }
 #pragma unroll
-            for (int r = R + 1; r < GP_ROWS; r++)
+            for (int r = R + 1; r < ROWS; r++)
                 if (lane + 64 * r < m) {
                     const double lij = F[rsi[r] + j];
 #pragma unroll
@@ -253,18 +254,19 @@ __device__ __forceinline__ bool dense_factor(double* L, const D& d, int m, doubl
 
 // Wavefront 0: alpha = L^-T z with z = L^-1 (y - ybar) from row m, written
 // to alpha after the solve (alpha may overlap the tail of L).
+template <int ROWS>
 __device__ __forceinline__ void dense_alpha(const double* L, int m, double* alpha) {
     const int lane = lane_id();
     const int rz = rs(m);
-    double b[GP_ROWS];
+    double b[ROWS];
 #pragma unroll
-    for (int r = 0; r < GP_ROWS; r++) {
+    for (int r = 0; r < ROWS; r++) {
         int i = lane + WAVE * r;
         b[r] = i < m ? L[rz + i] : 0.0;
     }
     wave_solve_lt(L, m, b);
 #pragma unroll
-    for (int r = 0; r < GP_ROWS; r++) {
+    for (int r = 0; r < ROWS; r++) {
         int i = lane + WAVE * r;
         if (i < m) alpha[i] = b[r];
     }
@@ -389,12 +391,8 @@ __device__ __forceinline__ bool fitc_fit(const Fitc& f, const Det& d, int nd, in
     }
     if (!chol_blocked(f.lb, m, m, f.flag, f.rinv)) return false;
     if (tid < WAVE) {
-        double b[GP_ROWS];
-#pragma unroll
-        for (int r = 0; r < GP_ROWS; r++) {
-            int i = tid + WAVE * r;
-            b[r] = i < m ? f.w[i] : 0.0;
-        }
+        double b[1];  // m <= NP_MAX_IND <= 64: one row per lane
+        b[0] = tid < m ? f.w[tid] : 0.0;
         wave_solve_l(f.lb, m, b);
         wave_solve_lt(f.lb, m, b);
         if (tid < m) f.az[tid] = b[0];
@@ -630,6 +628,7 @@ __device__ __forceinline__ double block_scan(double v, double* red, double* tota
     return before + v;
 }
 
+template <int ROWS>
 __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band) {
     const int tid = threadIdx.x;
     const int lane = lane_id(), wave = tid / WAVE;
@@ -655,6 +654,11 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     int* flag = s.imisc + 20;
 
     for (int k = tid; k < NP_NFEAT; k += GP_THREADS) feat[k] = k == 10 ? (double)n : NAN;
+    // Each feature goes out as soon as it is known (thread 0; the values are
+    // workgroup-uniform), so no register holds it through the later phases.
+    auto put = [&](int k, double v) {
+        if (tid == 0) feat[k] = finite_or_nan(v);
+    };
     for (int k = tid; k < ng; k += GP_THREADS) gpred[k] = gstd[k] = gdpred[k] = NAN;
     for (int k = tid; k < sstride; k += GP_THREADS) st[k] = 0.0;
     if (tid == 0) a.status[band] = 0;
@@ -767,7 +771,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
         double score;
         if (!sparse) {
             const bool ok = dense_factor(s.r0, dls, nd, amp, it, ybar, flag, s.misc + M_RINV);
-            if (ok && tid < WAVE) dense_alpha(s.r0, nd, s.al);
+            if (ok && tid < WAVE) dense_alpha<ROWS>(s.r0, nd, s.al);
             restage(s.r0, s.xs, dls, nd);
             if (!ok) continue;
             score = dense_train_rms(dls, nd, amp, it, ybar, s.al, red);
@@ -785,11 +789,13 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     const double famp = a.amps[best_c / a.n_ls];
     const double fls = fmax(dur / a.lsf[best_c % a.n_ls], 0.1);
     const double fit = 0.5 / (fls * fls);
+    put(45, famp);
+    put(46, fls);
 
     // refit the winner; a FITC band first fits its subsampled dense GP.  The
     // dense weights go to the state row first: the factor is still needed
     // for the standard deviations, and the weight slot overlaps its tail.
-    Pred P;
+    Pred<ROWS> P;
     P.sparse = sparse;
     P.amp = famp;
     P.it = fit;
@@ -797,7 +803,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     P.d = dls;
     if (!sparse) {
         dense_factor(s.r0, dls, nd, famp, fit, ybar, flag, s.misc + M_RINV);
-        if (tid < WAVE) dense_alpha(s.r0, nd, st);
+        if (tid < WAVE) dense_alpha<ROWS>(s.r0, nd, st);
         restage(s.r0, s.xs, dls, nd);
         for (int i = tid; i < nd; i += GP_THREADS) st[a.state_cap + i] = dx[i];
         if (tid == 0) {
@@ -825,7 +831,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
             const double ls = fmax(dur / a.lsf[c % a.n_ls], 0.1);
             const double it = 0.5 / (ls * ls);
             const bool ok = dense_factor(s.r0, dsub, ms, amp, it, ys, flag, s.misc + M_RINV);
-            if (ok && tid < WAVE) dense_alpha(s.r0, ms, st);
+            if (ok && tid < WAVE) dense_alpha<ROWS>(s.r0, ms, st);
             restage(s.r0, s.xs, dsub, ms);
             if (!ok) continue;
             for (int i = tid; i < ms; i += GP_THREADS) st[a.state_cap + i] = dsub.x(i);
@@ -912,6 +918,9 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     omin = block_min(omin, red);
     snr = block_max(snr, red);
     const double rms = ::sqrt(rss / (double)nd);
+    put(8, chi / (double)nd);
+    put(9, bv / (double)nd);
+    put(25, snr);
     {
         double scale = 1.0;
         if (scnt > 0.0) {
@@ -941,9 +950,12 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     }
     const double t0 = t_lo + (double)pk * dur / gstep_den;
     const double peak_mag = fmax(gpred[pk], omin - 2.0);
+    put(6, t0);
+    put(7, peak_mag);
     double t_last = -INFINITY;
     for (int i = tid; i < n; i += GP_THREADS) t_last = fmax(t_last, bt[i]);
     t_last = block_max(t_last, red);
+    put(21, t0 - t_last);
 
     // predictive queries: t_last - 2 .. t_last + 3, t0 + 30 / 60 / 90
     if (tid < 9) {
@@ -976,6 +988,17 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
         int pkf = block_argmin(bvv, bi, red, ired);
         if (pkf == 0x7fffffff) pkf = 0;
         curve_metrics(cv, pkf, red, ired, fine);
+        put(0, fine[4]);
+        put(1, fine[5]);
+        put(2, fine[6]);
+        put(3, fine[0]);
+        put(4, fine[1]);
+        put(5, fine[2]);
+        put(11, fine[3]);
+        put(12, fine[7]);
+        put(13, fine[8]);
+        put(14, fine[9]);
+        put(15, fine[10]);
     }
 
     // grid statistics (compute_predictive_features, fit_decay_power_law, ...)
@@ -1008,6 +1031,13 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     for (int k = tid; k < ng; k += GP_THREADS) var += (gpred[k] - gmean) * (gpred[k] - gmean);
     var = block_sum(var, red);
     const double sigma_f = ::sqrt(var / ngf);
+    put(23, sigma_f);
+    put(24, mx - mn);
+    put(26, dmx);
+    put(27, dmn);
+    put(28, gpred[ng - 1] / mn);
+    put(29, sv / ngf);
+    put(30, svx);
     double skew = NAN, kurt = NAN;
     if (ng > 3) {
         const double sdev = ::sqrt(var / ngf);
@@ -1037,6 +1067,9 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
         }
         n_infl = (int)block_sum((double)part, red);
     }
+    put(31, skew);
+    put(32, kurt);
+    put(33, (double)n_infl);
     // power law over t - t_peak >= 5 after the grid peak
     double pl_b = NAN, pl_chi = NAN;
     {
@@ -1085,6 +1118,8 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
             }
         }
     }
+    put(34, pl_b);
+    put(35, pl_chi);
     // post-peak monotonicity and local maxima
     double mono = NAN;
     {
@@ -1116,6 +1151,8 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
             n_max = ng == 0 ? 0 : 1;
         }
     }
+    put(42, mono);
+    put(43, (double)n_max);
 
     // variability features of the raw band (all observations)
     double vn = NAN, stk = NAN, exv = NAN, cus = NAN, pre_rms = NAN, pre_sig = NAN;
@@ -1228,6 +1265,12 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
         }
     }
 
+    put(39, vn);
+    put(40, pre_rms);
+    put(41, pre_sig);
+    put(47, stk);
+    put(48, exv);
+    put(49, cus);
     __syncthreads();  // predictive queries written
     if (tid == 0) {
         const double* yw = s.misc + M_Q;
@@ -1241,64 +1284,25 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
             }
         }
         const double f_m1 = yw[1], f_0 = yw[2], f_p1 = yw[3], f_p2 = yw[4], f_p3 = yw[5];
-        double f[NP_NFEAT];
-        f[0] = fine[4];
-        f[1] = fine[5];
-        f[2] = fine[6];
-        f[3] = fine[0];
-        f[4] = fine[1];
-        f[5] = fine[2];
-        f[6] = t0;
-        f[7] = peak_mag;
-        f[8] = chi / (double)nd;
-        f[9] = bv / (double)nd;
-        f[10] = (double)n;
-        f[11] = fine[3];
-        f[12] = fine[7];
-        f[13] = fine[8];
-        f[14] = fine[9];
-        f[15] = fine[10];
-        f[16] = nsl > 0 ? sl / (double)nsl : NAN;
-        f[17] = (f_p1 - f_0) / 1.0;
-        f[18] = (f_p1 - 2.0 * f_0 + f_m1) / (1.0 * 1.0);
-        f[19] = f_p1;
-        f[20] = f_p2;
-        f[21] = t0 - t_last;
-        f[22] = (f_p3 - f_p2) / 1.0;
-        f[23] = sigma_f;
-        f[24] = mx - mn;
-        f[25] = snr;
-        f[26] = dmx;
-        f[27] = dmn;
-        f[28] = gpred[ng - 1] / mn;
-        f[29] = sv / ngf;
-        f[30] = svx;
-        f[31] = skew;
-        f[32] = kurt;
-        f[33] = (double)n_infl;
-        f[34] = pl_b;
-        f[35] = pl_chi;
-        f[36] = yw[6];
-        f[37] = yw[7];
-        f[38] = yw[8];
-        f[39] = vn;
-        f[40] = pre_rms;
-        f[41] = pre_sig;
-        f[42] = mono;
-        f[43] = (double)n_max;
-        f[44] = (double)(n - nd);
-        f[45] = famp;
-        f[46] = fls;
-        f[47] = stk;
-        f[48] = exv;
-        f[49] = cus;
-        for (int k = 0; k < NP_NFEAT; k++) feat[k] = finite_or_nan(f[k]);
+        put(16, nsl > 0 ? sl / (double)nsl : NAN);
+        put(17, (f_p1 - f_0) / 1.0);
+        put(18, (f_p1 - 2.0 * f_0 + f_m1) / (1.0 * 1.0));
+        put(19, f_p1);
+        put(20, f_p2);
+        put(22, (f_p3 - f_p2) / 1.0);
+        put(36, yw[6]);
+        put(37, yw[7]);
+        put(38, yw[8]);
     }
 }
 
 }  // namespace
 
-__global__ __launch_bounds__(GP_THREADS, 2) void np_band_kernel(NpArgs a) {
+// Rows per lane of the wave-level solves: the class capacity sets it (a
+// 64-detection class keeps one row per lane, so its register arrays are a
+// quarter of the 200-detection class's and four workgroups fit a CU).
+template <int ROWS>
+__global__ __launch_bounds__(GP_THREADS, ROWS == 1 ? 4 : 2) void np_band_kernel(NpArgs a) {
     extern __shared__ double np_lds[];
     const Lds s = carve(np_lds, a.cap, a.fitc);
     for (;;) {
@@ -1308,7 +1312,7 @@ __global__ __launch_bounds__(GP_THREADS, 2) void np_band_kernel(NpArgs a) {
         const int t = s.imisc[16];
         __syncthreads();
         if (t >= a.n_tasks) break;
-        np_band(a, s, a.order[t]);
+        np_band<ROWS>(a, s, a.order[t]);
     }
 }
 
@@ -1316,15 +1320,21 @@ size_t np_lds_bytes(int cap, int fitc) {
     return sizeof(double) * ((size_t)r0_doubles(cap, fitc) + (size_t)cap + MISC_DOUBLES);
 }
 
+static const void* np_kernel_for(int cap) {
+    if (cap <= 64) return (const void*)np_band_kernel<1>;
+    if (cap <= 128) return (const void*)np_band_kernel<2>;
+    return (const void*)np_band_kernel<4>;
+}
+
 hipError_t launch_np(const NpArgs& a, int grid, hipStream_t st) {
     if (a.cap < 1 || a.cap > NP_CAP_MAX || a.n_ind < 1 || a.n_ind > NP_MAX_IND || a.n_grid < 2 || grid <= 0)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(np_band_kernel, dim3(grid), dim3(GP_THREADS), np_lds_bytes(a.cap, a.fitc), st, a);
-    return hipGetLastError();
+    void* args[] = {const_cast<NpArgs*>(&a)};
+    return hipLaunchKernel(np_kernel_for(a.cap), dim3(grid), dim3(GP_THREADS), args, np_lds_bytes(a.cap, a.fitc), st);
 }
 
 hipError_t np_occupancy(int cap, int fitc, int* blocks_per_cu) {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, (const void*)np_band_kernel, GP_THREADS,
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, np_kernel_for(cap), GP_THREADS,
                                                          np_lds_bytes(cap, fitc));
 }
 
