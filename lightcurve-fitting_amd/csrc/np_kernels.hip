@@ -1298,11 +1298,12 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
 
 }  // namespace
 
-// Rows per lane of the wave-level solves: the class capacity sets it (a
-// 64-detection class keeps one row per lane, so its register arrays are a
-// quarter of the 200-detection class's and four workgroups fit a CU).
-template <int ROWS>
-__global__ __launch_bounds__(GP_THREADS, ROWS == 1 ? 4 : 2) void np_band_kernel(NpArgs a) {
+// Rows per lane of the wave-level solves (ROWS) and workgroups per CU (WGS)
+// follow the class capacity: a 64-detection class keeps one row per lane, so
+// its register arrays are a quarter of the 200-detection class's and four
+// workgroups fit a CU; a 96-detection class fits three (LDS 38 KB each).
+template <int ROWS, int WGS>
+__global__ __launch_bounds__(GP_THREADS, WGS) void np_band_kernel(NpArgs a) {
     extern __shared__ double np_lds[];
     const Lds s = carve(np_lds, a.cap, a.fitc);
     for (;;) {
@@ -1321,9 +1322,10 @@ size_t np_lds_bytes(int cap, int fitc) {
 }
 
 static const void* np_kernel_for(int cap) {
-    if (cap <= 64) return (const void*)np_band_kernel<1>;
-    if (cap <= 128) return (const void*)np_band_kernel<2>;
-    return (const void*)np_band_kernel<4>;
+    if (cap <= 64) return (const void*)np_band_kernel<1, 4>;
+    if (cap <= 96) return (const void*)np_band_kernel<2, 3>;
+    if (cap <= 128) return (const void*)np_band_kernel<2, 2>;
+    return (const void*)np_band_kernel<4, 2>;
 }
 
 hipError_t launch_np(const NpArgs& a, int grid, hipStream_t st) {

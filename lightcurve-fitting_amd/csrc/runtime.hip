@@ -1645,7 +1645,7 @@ int lcf_batch_nonparametric(lcf_ctx* c, int64_t n_bands, const int64_t* off, con
     }
     struct Cls { int cap, fitc; std::vector<int32_t> bands; };
     std::vector<Cls> cls;
-    for (int cp : {64, 128})
+    for (int cp : {64, 96, 128})
         if (cp < state_cap) cls.push_back({cp, 0, {}});
     cls.push_back({state_cap, 0, {}});
     cls.push_back({state_cap, 1, {}});
