@@ -416,6 +416,7 @@ def main() -> int:
     ap.add_argument("--cpu-sources", type=int, default=192)
     ap.add_argument("--cpu-sources-1t", type=int, default=16)
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--accuracy-sources", type=int, default=2000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -522,6 +523,28 @@ def main() -> int:
     data_e2e.close()
     log(f"end to end with upload: {ms_e2e:.1f} ms (upload {ms_upload:.1f} ms)")
 
+    # autotuned launch shapes: measured per band-size bucket (untimed), then
+    # the same steps under the tuned schedule (identical results)
+    tuned = None
+    if not args.no_tune:
+        t_tune = time.perf_counter()
+        table = ctx.autotune_pso(data, MODELS, args.particles, 40, 40, seed=42)
+        ms_tune = (time.perf_counter() - t_tune) * 1e3
+        ctx.set_schedule(pkg.SCHED_TUNED)
+        step()
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        ctx.synchronize()
+        tdt = (time.perf_counter() - t2) / args.steps
+        ctx.set_schedule(pkg.SCHED_PERSISTENT)
+        tuned = {"value": round(args.sources / tdt, 2), "unit": "sources/s", "ms_per_step": round(tdt * 1e3, 3),
+                 "speedup_vs_default": round(ms_per_step / (tdt * 1e3), 4), "tuning_ms": round(ms_tune, 1),
+                 "schedule": "one persistent launch per band-size bucket with the measured LDS tile and "
+                             "waves per CU", "table": table}
+        log(f"autotuned schedule: {tdt * 1e3:.1f} ms per step (tuning {ms_tune:.0f} ms)")
+
     accuracy = None
     if not args.no_accuracy:
         accuracy = pso_accuracy(pkg, csr, first, args.accuracy_sources, args.particles, args.iters)
@@ -586,6 +609,7 @@ def main() -> int:
         "ms_upload": round(ms_upload, 3),
         "accuracy": accuracy,
         "plain": plain,
+        "autotuned": tuned,
         "per_model_ms": {MODEL_NAMES[m]: round(float(slot_ms[i]), 3) for i, m in enumerate(MODELS)},
         "svi": svi,
         "gp2d": gp2,

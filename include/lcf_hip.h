@@ -45,6 +45,9 @@ extern "C" {
 /* work distribution for batched fits */
 #define LCF_SCHED_PERSISTENT 0  /* one persistent launch, size-sorted device queue (default) */
 #define LCF_SCHED_PER_CLASS 1   /* one launch per power-of-two size class, synchronised */
+#define LCF_SCHED_TUNED 2       /* PSO fits: one persistent launch per size bucket with the context's
+                                   autotuned launch shape (lcf_ctx_autotune_pso); other fits as
+                                   LCF_SCHED_PERSISTENT */
 
 /* PSO flags */
 #define LCF_PSO_STALL_FROM_FIRST_EVAL 1u /* opt-in: stall baseline = first global best
@@ -69,6 +72,37 @@ int lcf_ctx_num_cus(const lcf_ctx* ctx);
  * cudaDeviceSynchronize, src/gpu_cuda.rs:353-354); this is for callers that
  * time or order work around them. */
 int lcf_ctx_synchronize(lcf_ctx* ctx);
+
+/* ---- autotuned launch shapes of the PSO swarms ------------------------- */
+/* The reference hard-codes its launch shapes (src/gpu_cuda.rs:1029-1031,
+ * 1365-1368).  Here the shape of the persistent swarm launches -- the LDS
+ * observation tile (records staged per wave; a tile below the band's size is
+ * re-staged every iteration) and the waves per CU -- is measured per
+ * band-size bucket: bucket k holds bands of at most 64 << k observations
+ * (k < LCF_TUNE_BUCKETS - 1; the last holds the rest).
+ * lcf_ctx_autotune_pso times, for every bucket present in `data` and every
+ * model, the candidate shapes on a sample of that bucket's bands with the
+ * given swarm settings (two launches each, the second timed) and keeps the
+ * fastest in the context, for the context's precision and the particles'
+ * lane layout.  Fits use the table under LCF_SCHED_TUNED; the shape changes
+ * no result (every band is fitted by one wavefront whatever the grid, and
+ * the tile only splits the staging of its observations).
+ * Per model it then times the tuned per-bucket launches against the single
+ * persistent launch of the default shape over all of `data` and records a
+ * whole-batch entry (bucket LCF_TUNE_BUCKETS, tile 0, waves 0) when the
+ * single launch is not slower; LCF_SCHED_TUNED then keeps that launch.
+ * Adjacent buckets of equal shape share one launch.
+ * The table is read and written as entries of LCF_TUNE_FIELDS int32:
+ * {precision, model, particles per lane (1, 2, 4), bucket, tile,
+ * waves per CU}; lcf_ctx_get_tuning returns the number of entries (writing
+ * at most max_entries), lcf_ctx_set_tuning replaces the table (n = 0
+ * clears it). */
+#define LCF_TUNE_BUCKETS 7
+#define LCF_TUNE_FIELDS 6
+int lcf_ctx_autotune_pso(lcf_ctx* ctx, const lcf_batch* data, const int32_t* models, int n_models,
+                         int n_particles, int max_iters, int stall_iters, uint64_t seed);
+int lcf_ctx_get_tuning(const lcf_ctx* ctx, int32_t* out, int max_entries);
+int lcf_ctx_set_tuning(lcf_ctx* ctx, const int32_t* entries, int n_entries);
 
 /* ---- resident source data: GpuBatchData (src/gpu_cuda.rs:236-292) ------ */
 /* offsets[n_sources+1] index the concatenated per-observation arrays

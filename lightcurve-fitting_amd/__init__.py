@@ -14,6 +14,9 @@ from .gpu import (  # noqa: F401
     PSO_STALL_FROM_FIRST_EVAL,
     SCHED_PER_CLASS,
     SCHED_PERSISTENT,
+    SCHED_TUNED,
+    TUNE_BUCKETS,
+    TUNE_FIELDS,
     BatchPsoResult,
     BatchSource,
     DenseGpState,

@@ -27,6 +27,9 @@ SIGNATURES = {
     "lcf_ctx_set_schedule": (_i32, [_vp, _i32]),
     "lcf_ctx_num_cus": (_i32, [_vp]),
     "lcf_ctx_synchronize": (_i32, [_vp]),
+    "lcf_ctx_autotune_pso": (_i32, [_vp, _vp, _pi32, _i32, _i32, _i32, _i32, _u64]),
+    "lcf_ctx_get_tuning": (_i32, [_vp, _pi32, _i32]),
+    "lcf_ctx_set_tuning": (_i32, [_vp, _pi32, _i32]),
     "lcf_batch_create": (_i32, [_vp, _i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _ppv]),
     "lcf_batch_create_current": (_i32, [_i64, _pi64, _pd, _pd, _pd, _pi32, _pd, _ppv]),
     "lcf_batch_destroy": (None, [_vp]),

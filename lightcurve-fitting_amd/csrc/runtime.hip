@@ -21,7 +21,13 @@
 #include <string>
 #include <vector>
 
+// One measured launch shape of the PSO swarm kernels for a band-size bucket.
+struct TuneEntry {
+    int32_t precision, model, ppl, bucket, tile, waves_per_cu;
+};
+
 struct lcf_ctx {
+    std::vector<TuneEntry> tune;  // autotuned launch shapes (lcf_ctx_autotune_pso)
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -159,6 +165,71 @@ int check_fit_args(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_
     return LCF_OK;
 }
 
+// Band-size buckets of the autotuner: n <= 64, 128, ..., 2048, larger.
+int tune_bucket(int64_t n) {
+    int k = 0;
+    for (int64_t lim = 64; k < LCF_TUNE_BUCKETS - 1 && n > lim; lim <<= 1) k++;
+    return k;
+}
+
+// Contiguous ranges [first, end) of the size-sorted band order, one per
+// bucket present (largest bands first).
+struct BucketRange { int bucket; int64_t first, end; int max_n; };
+std::vector<BucketRange> bucket_ranges(const lcf_batch* b) {
+    std::vector<BucketRange> r;
+    for (int64_t i = 0; i < b->n_sources; i++) {
+        const int32_t s = b->h_order[i];
+        const int64_t n = b->h_off[s + 1] - b->h_off[s];
+        const int k = tune_bucket(n);
+        if (r.empty() || r.back().bucket != k) r.push_back({k, i, i, (int)n});
+        r.back().end = i + 1;
+    }
+    return r;
+}
+
+const TuneEntry* find_tune(const lcf_ctx* c, int model, int ppl, int bucket) {
+    for (const auto& e : c->tune)
+        if (e.precision == c->precision && e.model == model && e.ppl == ppl && e.bucket == bucket) return &e;
+    return nullptr;
+}
+
+// The default launch shape: the whole band in LDS up to 16 KiB per wave, and
+// as many waves per CU as the kernel's occupancy allows.
+int default_tile(bool fp64, int max_n) {
+    const int rec = fp64 ? (int)sizeof(double4) : (int)sizeof(float4);
+    return std::max(1, std::min(16384 / rec, std::max(max_n, 1)));
+}
+
+// The launches of one model under LCF_SCHED_TUNED: a whole-batch entry
+// (bucket LCF_TUNE_BUCKETS) keeps the single persistent launch of the
+// default shape; otherwise one persistent launch per bucket with its
+// measured shape (the default where none was measured), adjacent buckets of
+// equal shape merged into one launch.
+struct TunedLaunch { int64_t first, end; int tile, waves_per_cu; };
+std::vector<TunedLaunch> tuned_launches(const lcf_ctx* c, const lcf_batch* b, int model, int ppl, bool fp64,
+                                        int whole_tile) {
+    std::vector<TunedLaunch> v;
+    auto occ = [&](int tile) {
+        int per_cu = 0;
+        if (lcf::pso_occupancy(model, fp64, ppl, tile, &per_cu) != hipSuccess) per_cu = 1;
+        return std::max(per_cu, 1);
+    };
+    if (find_tune(c, model, ppl, LCF_TUNE_BUCKETS)) {
+        v.push_back({0, b->n_sources, whole_tile, occ(whole_tile)});
+        return v;
+    }
+    for (const BucketRange& br : bucket_ranges(b)) {
+        const TuneEntry* te = find_tune(c, model, ppl, br.bucket);
+        const int tile = te ? te->tile : default_tile(fp64, br.max_n);
+        const int wpc = te ? std::min(occ(tile), std::max(te->waves_per_cu, 1)) : occ(tile);
+        if (!v.empty() && v.back().tile == tile && v.back().waves_per_cu == wpc && v.back().end == br.first)
+            v.back().end = br.end;
+        else
+            v.push_back({br.first, br.end, tile, wpc});
+    }
+    return v;
+}
+
 // Launches, times and downloads one lcf_batch_fit_models call into the
 // device buffers `d` (arguments already checked, batch non-empty).
 int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models, int use_gate,
@@ -203,7 +274,22 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
         a.slot = mi;
         HIPCHK(c, hipEventRecord(c->ev_slot[mi], c->stream));
         a.gate_cost = (use_gate && mi > 0) ? d.mc : nullptr;
-        if (c->schedule == LCF_SCHED_PERSISTENT) {
+        if (c->schedule == LCF_SCHED_TUNED) {
+            for (const TunedLaunch& tl : tuned_launches(c, b, model, ppl, fp64, tile)) {
+                lcf::PsoArgs ab = a;
+                ab.tile = tl.tile;
+                ab.order = b->d_order + tl.first;
+                ab.task_base = 0;
+                ab.n_tasks = (int)(tl.end - tl.first);
+                const int grid = (int)std::min<int64_t>((int64_t)tl.waves_per_cu * c->n_cu, ab.n_tasks);
+                if (d.fill_queue)
+                    HIPCHK(c, launch_zero_ints(d.queue, 1, c->stream));
+                else
+                    HIPCHK(c, hipMemsetAsync(d.queue, 0, sizeof(int), c->stream));
+                HIPCHK(c, lcf::launch_pso(model, ab, fp64, ppl, true, grid, c->stream));
+            }
+            if (!fp64) HIPCHK(c, lcf::launch_cost64(model, a, c->stream));
+        } else if (c->schedule == LCF_SCHED_PERSISTENT) {
             int per_cu = 0;
             HIPCHK(c, lcf::pso_occupancy(model, fp64, ppl, tile, &per_cu));
             per_cu = std::max(per_cu, 1);
@@ -333,7 +419,7 @@ int lcf_ctx_set_precision(lcf_ctx* c, int p) {
 
 int lcf_ctx_set_schedule(lcf_ctx* c, int s) {
     if (!c) return LCF_E_INVALID;
-    if (s != LCF_SCHED_PERSISTENT && s != LCF_SCHED_PER_CLASS)
+    if (s != LCF_SCHED_PERSISTENT && s != LCF_SCHED_PER_CLASS && s != LCF_SCHED_TUNED)
         return fail(c, LCF_E_INVALID, "unknown schedule %d", s);
     c->schedule = s;
     return LCF_OK;
@@ -531,6 +617,160 @@ int lcf_batch_fit_models(lcf_ctx* c, const lcf_batch* b, const int32_t* models, 
     FitBufs d{d_mp.p, d_mc.p, d_mi.p, d_models.p, d_bp.p, d_bc.p, d_bm.p, c->d_queue, false};
     return fit_models_core(c, b, models, n_models, use_gate, gate, P, max_iters, stall_iters, seed, flags, d,
                            model_params, model_cost, model_iters, best_model, best_params, best_cost);
+}
+
+static void set_tune(lcf_ctx* c, const TuneEntry& t) {
+    auto it = std::find_if(c->tune.begin(), c->tune.end(), [&](const TuneEntry& e) {
+        return e.precision == t.precision && e.model == t.model && e.ppl == t.ppl && e.bucket == t.bucket;
+    });
+    if (it != c->tune.end()) *it = t;
+    else c->tune.push_back(t);
+}
+
+int lcf_ctx_autotune_pso(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models, int P,
+                         int max_iters, int stall_iters, uint64_t seed) {
+    int rc = check_fit_args(c, b, models, n_models, P, max_iters, stall_iters);
+    if (rc != LCF_OK || b->n_sources == 0) return rc;
+    HIPCHK(c, hipSetDevice(c->device));
+    const bool fp64 = c->precision == LCF_FP64;
+    const int ppl = P <= 64 ? 1 : (P <= 128 ? 2 : 4);
+    const int64_t ns = b->n_sources;
+    DevBuf<double> d_mp, d_mc;
+    DevBuf<int32_t> d_mi;
+    HIPCHK(c, d_mp.alloc(ns * 7));
+    HIPCHK(c, d_mc.alloc(ns));
+    HIPCHK(c, d_mi.alloc(ns));
+    lcf::PsoArgs a{};
+    a.obs = fp64 ? (const void*)b->d_obs64 : (const void*)b->d_obs32;
+    a.obs64 = b->d_obs64;
+    a.offsets = b->d_off;
+    a.n_bands = (int)ns;
+    a.queue = c->d_queue;
+    a.n_particles = P;
+    a.max_iters = max_iters;
+    a.stall_iters = stall_iters;
+    a.seed = seed;
+    a.model_params = d_mp.p;
+    a.model_cost = d_mc.p;
+    a.model_iters = d_mi.p;
+    if (!fp64) HIPCHK(c, lcf::pso_f32_prepare(c->stream));
+    for (const BucketRange& br : bucket_ranges(b)) {
+        // candidate LDS tiles: the whole band, and halves / quarters of it
+        // for large bands (re-staged per iteration, more waves per CU)
+        const int full = default_tile(fp64, br.max_n);
+        std::vector<int> tiles{full};
+        for (int t = full / 2; br.max_n > 128 && t >= 64 && tiles.size() < 3; t /= 2) tiles.push_back(t);
+        for (int mi = 0; mi < n_models; mi++) {
+            const int model = models[mi];
+            TuneEntry best{c->precision, model, ppl, br.bucket, full, 0};
+            float best_ms = INFINITY;
+            for (int tile : tiles) {
+                int max_pc = 0;
+                HIPCHK(c, lcf::pso_occupancy(model, fp64, ppl, tile, &max_pc));
+                max_pc = std::max(max_pc, 1);
+                // the measured sample: enough bands of the bucket for several
+                // rounds of the largest grid
+                const int64_t cnt = std::min<int64_t>(br.end - br.first, (int64_t)4 * max_pc * c->n_cu);
+                std::vector<int> wpcs{max_pc};
+                for (int w : {(3 * max_pc) / 4, max_pc / 2})
+                    if (w >= 1 && std::find(wpcs.begin(), wpcs.end(), w) == wpcs.end()) wpcs.push_back(w);
+                for (int wpc : wpcs) {
+                    lcf::PsoArgs ab = a;
+                    ab.tile = tile;
+                    ab.order = b->d_order + br.first;
+                    ab.task_base = 0;
+                    ab.n_tasks = (int)cnt;
+                    const int grid = (int)std::min<int64_t>((int64_t)wpc * c->n_cu, cnt);
+                    float ms = INFINITY;
+                    for (int rep = 0; rep < 2; rep++) {  // the first launch warms caches and code
+                        HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+                        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+                        HIPCHK(c, lcf::launch_pso(model, ab, fp64, ppl, true, grid, c->stream));
+                        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+                        HIPCHK(c, hipEventSynchronize(c->ev1));
+                        float t = 0.f;
+                        HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+                        if (rep > 0) ms = std::min(ms, t);
+                    }
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        best.tile = tile;
+                        best.waves_per_cu = wpc;
+                    }
+                }
+            }
+            set_tune(c, best);
+        }
+    }
+    // Per model: the tuned per-bucket launches against the single persistent
+    // launch of the default shape, over the whole batch; the faster stays.
+    const int whole_tile = default_tile(fp64, b->max_n);
+    for (int mi = 0; mi < n_models; mi++) {
+        const int model = models[mi];
+        auto drop_whole = [&]() {
+            c->tune.erase(std::remove_if(c->tune.begin(), c->tune.end(), [&](const TuneEntry& e) {
+                              return e.precision == c->precision && e.model == model && e.ppl == ppl &&
+                                     e.bucket == LCF_TUNE_BUCKETS;
+                          }), c->tune.end());
+        };
+        float t_split = 0.f, t_whole = 0.f;
+        for (int variant = 0; variant < 2; variant++) {
+            drop_whole();
+            if (variant == 1) c->tune.push_back({c->precision, model, ppl, LCF_TUNE_BUCKETS, 0, 0});
+            float ms = INFINITY;
+            for (int rep = 0; rep < 2; rep++) {
+                HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+                for (const TunedLaunch& tl : tuned_launches(c, b, model, ppl, fp64, whole_tile)) {
+                    lcf::PsoArgs ab = a;
+                    ab.tile = tl.tile;
+                    ab.order = b->d_order + tl.first;
+                    ab.task_base = 0;
+                    ab.n_tasks = (int)(tl.end - tl.first);
+                    const int grid = (int)std::min<int64_t>((int64_t)tl.waves_per_cu * c->n_cu, ab.n_tasks);
+                    HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
+                    HIPCHK(c, lcf::launch_pso(model, ab, fp64, ppl, true, grid, c->stream));
+                }
+                HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+                HIPCHK(c, hipEventSynchronize(c->ev1));
+                float t = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+                if (rep > 0) ms = std::min(ms, t);
+            }
+            (variant == 0 ? t_split : t_whole) = ms;
+        }
+        if (!(t_whole <= t_split)) drop_whole();
+    }
+    return LCF_OK;
+}
+
+int lcf_ctx_get_tuning(const lcf_ctx* c, int32_t* out, int max_entries) {
+    if (!c) return LCF_E_INVALID;
+    const int n = (int)c->tune.size();
+    if (out)
+        for (int i = 0; i < n && i < max_entries; i++) {
+            const TuneEntry& e = c->tune[i];
+            const int32_t v[LCF_TUNE_FIELDS] = {e.precision, e.model, e.ppl, e.bucket, e.tile, e.waves_per_cu};
+            std::copy(v, v + LCF_TUNE_FIELDS, out + (int64_t)i * LCF_TUNE_FIELDS);
+        }
+    return n;
+}
+
+int lcf_ctx_set_tuning(lcf_ctx* c, const int32_t* in, int n_entries) {
+    if (!c) return LCF_E_INVALID;
+    if (n_entries < 0 || (n_entries > 0 && !in)) return fail(c, LCF_E_INVALID, "invalid tuning table");
+    std::vector<TuneEntry> t;
+    for (int i = 0; i < n_entries; i++) {
+        const int32_t* v = in + (int64_t)i * LCF_TUNE_FIELDS;
+        const bool whole = v[3] == LCF_TUNE_BUCKETS && v[4] == 0 && v[5] == 0;
+        const bool ok = (v[0] == LCF_FP32 || v[0] == LCF_FP64) && v[1] >= 0 && v[1] < LCF_N_MODELS &&
+                        (v[2] == 1 || v[2] == 2 || v[2] == 4) &&
+                        (whole || (v[3] >= 0 && v[3] < LCF_TUNE_BUCKETS && v[4] >= 1 &&
+                                   v[4] <= 16384 / (v[0] == LCF_FP64 ? 32 : 16) && v[5] >= 1));
+        if (!ok) return fail(c, LCF_E_INVALID, "tuning entry %d is out of range", i);
+        t.push_back({v[0], v[1], v[2], v[3], v[4], v[5]});
+    }
+    c->tune = t;
+    return LCF_OK;
 }
 
 int lcf_batch_fit_models_noalloc(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n_models,
@@ -731,7 +971,7 @@ int svi_run(lcf_ctx* c, const lcf_batch* b, const int32_t* model_ids, const doub
         const int64_t cnt = starts[m + 1] - starts[m];
         if (cnt == 0) continue;
         a.order = d_order.p + starts[m];
-        if (c->schedule == LCF_SCHED_PERSISTENT) {
+        if (c->schedule != LCF_SCHED_PER_CLASS) {
             int per_cu = 0;
             HIPCHK(c, lcf::svi_occupancy(m, fp64, fd, tile, &per_cu));
             per_cu = std::max(per_cu, 1);
@@ -1493,7 +1733,7 @@ int gp2d_hyper_run(lcf_ctx* c, int64_t ns, const int64_t* off, const double* t, 
     a.pred = n_query > 0 ? d_pred.p : nullptr;
     a.stdv = n_query > 0 ? d_std.p : nullptr;
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    if (c->schedule == LCF_SCHED_PERSISTENT) {
+    if (c->schedule != LCF_SCHED_PER_CLASS) {
         HIPCHK(c, hipMemsetAsync(c->d_queue, 0, sizeof(int), c->stream));
         HIPCHK(c, lcf::launch_gp2d_hyper(a, grid, c->stream));
     } else {

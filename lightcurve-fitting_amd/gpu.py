@@ -20,7 +20,8 @@ from . import _lib
 MAX_PARAMS = 7
 NP_NFEAT = 50  # NonparametricBandResult fields without the band name
 FP32, FP64 = 0, 1
-SCHED_PERSISTENT, SCHED_PER_CLASS = 0, 1
+SCHED_PERSISTENT, SCHED_PER_CLASS, SCHED_TUNED = 0, 1, 2
+TUNE_BUCKETS, TUNE_FIELDS = 7, 6
 NOALLOC_MAX_BANDS = 16384  # LCF_NOALLOC_MAX_BANDS
 PSO_STALL_FROM_FIRST_EVAL = 1
 FINALIZE_T0, FINALIZE_ALL, FINALIZE_MAG_CHI2 = 1, 2, 4
@@ -158,6 +159,37 @@ class GpuContext:
     def set_schedule(self, schedule: int) -> None:
         self._check(self._lib.lcf_ctx_set_schedule(self._h, int(schedule)))
         self.schedule = int(schedule)
+
+    # -- autotuned launch shapes (lcf_ctx_autotune_pso) ---------------------------
+    def autotune_pso(self, data: "GpuBatchData", models: Sequence[int], n_particles: int, max_iters: int,
+                     stall_iters: int, seed: int = 42) -> list[dict]:
+        """Measure the PSO swarms' launch shape (LDS tile, waves per CU) per
+        band-size bucket of ``data`` for ``models`` at this context's
+        precision; returns the resulting table (``tuning()``).  Fits use it
+        under ``SCHED_TUNED``."""
+        ms = np.ascontiguousarray(np.asarray([int(m) for m in models], dtype=np.int32))
+        self._check(self._lib.lcf_ctx_autotune_pso(self._h, data._h, _lib.ptr(ms, C.c_int32), ms.size,
+                                                   int(n_particles), int(max_iters), int(stall_iters),
+                                                   C.c_uint64(seed & 0xFFFFFFFFFFFFFFFF)))
+        return self.tuning()
+
+    def tuning(self) -> list[dict]:
+        """The context's tuning table, one dict per (precision, model,
+        particles per lane, bucket): its tile and waves per CU."""
+        n = self._lib.lcf_ctx_get_tuning(self._h, None, 0)
+        if n < 0:
+            self._check(n)
+        buf = np.zeros((max(n, 1), TUNE_FIELDS), dtype=np.int32)
+        self._lib.lcf_ctx_get_tuning(self._h, _lib.ptr(buf, C.c_int32), n)
+        keys = ("precision", "model", "ppl", "bucket", "tile", "waves_per_cu")
+        return [dict(zip(keys, map(int, row))) for row in buf[:n]]
+
+    def set_tuning(self, table: Sequence[dict]) -> None:
+        """Replace the tuning table (e.g. one saved as JSON from ``tuning()``)."""
+        keys = ("precision", "model", "ppl", "bucket", "tile", "waves_per_cu")
+        buf = np.ascontiguousarray(np.asarray([[int(e[k]) for k in keys] for e in table], dtype=np.int32)
+                                   .reshape(-1, TUNE_FIELDS))
+        self._check(self._lib.lcf_ctx_set_tuning(self._h, _lib.ptr(buf, C.c_int32), len(table)))
 
     @property
     def num_cus(self) -> int:
