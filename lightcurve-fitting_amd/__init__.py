@@ -36,6 +36,7 @@ from .gpu import (  # noqa: F401
     svi_prior_for_model,
 )
 from . import common, gp2d, nonparametric, parametric, synthetic, thermal  # noqa: F401
+from . import features, io  # noqa: F401
 from .nonparametric import BandData, NonparametricBandResult  # noqa: F401
 from .parametric import GpuPsoBandResult, ParametricBandResult, UncertaintyMethod  # noqa: F401
 from ._lib import LIB_PATH  # noqa: F401

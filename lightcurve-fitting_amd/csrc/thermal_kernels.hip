@@ -15,7 +15,9 @@
 // The swarm follows argmin 0.11's ParticleSwarm (inertia 1/(2 ln 2),
 // cognitive = social = 1/2 + ln 2, box [3, 6] x [-0.05, 0.01], 10 particles,
 // 30 iterations, two runs); random numbers are Philox4x32-10 on the counter
-// (source, 64 run + particle, iteration or 0xFFFFFFFF, 0x54480000).
+// (0, 64 run + particle, iteration or 0xFFFFFFFF, 0x54480000): every source
+// draws the same stream, as every fit_thermal call of the reference starts
+// its own generator, so a source's result does not depend on its batch.
 // Built without multiply-add contraction, like the reference's arithmetic.
 #include "thermal_kernels.hpp"
 #include "lcf_device.hpp"
@@ -267,7 +269,7 @@ __device__ __forceinline__ void thermal_source(const ThermalArgs& a, ThLds& s, i
     const int run = lane / TH_P, p = lane - run * TH_P;
     double x[2] = {0.0, 0.0}, v[2] = {0.0, 0.0}, pb[2] = {0.0, 0.0}, pc = INFINITY, c = INFINITY;
     if (act) {
-        U4 r = philox4x32_10((uint32_t)src, (uint32_t)(64 * run + p), 0xFFFFFFFFu, TH_TAG, k0, k1);
+        U4 r = philox4x32_10(0u, (uint32_t)(64 * run + p), 0xFFFFFFFFu, TH_TAG, k0, k1);
         const uint32_t ru[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
         for (int d = 0; d < 2; d++) {
@@ -298,7 +300,7 @@ __device__ __forceinline__ void thermal_source(const ThermalArgs& a, ThLds& s, i
     }
     for (int itn = 0; itn < TH_ITERS; itn++) {
         if (act) {
-            U4 r = philox4x32_10((uint32_t)src, (uint32_t)(64 * run + p), (uint32_t)itn, TH_TAG, k0, k1);
+            U4 r = philox4x32_10(0u, (uint32_t)(64 * run + p), (uint32_t)itn, TH_TAG, k0, k1);
             const uint32_t ru[4] = {r.x, r.y, r.z, r.w};
 #pragma unroll
             for (int d = 0; d < 2; d++) {

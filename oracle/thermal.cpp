@@ -25,8 +25,8 @@
 //     particle order); best of the two runs, errors = population std of the
 //     two runs' best positions; log T at the last colour time clamped to
 //     [2, 7].
-// Random numbers: Philox4x32-10 keyed by the seed, counter (source,
-// 64 run + particle, iteration or 0xFFFFFFFF, 0x54480000), in place of
+// Random numbers: Philox4x32-10 keyed by the seed, counter (0,
+// 64 run + particle, iteration or 0xFFFFFFFF, 0x54480000) for every source, in place of
 // argmin's entropy-seeded generator (not reproducible even by the reference:
 // parity of the swarm's path is unpinned).
 #include "lcf_oracle.h"
@@ -73,14 +73,14 @@ static double cost(const std::vector<Obs>& o, double lref, double lt0, double ra
 }
 
 // One particle-swarm run; returns the best position and cost.
-static void swarm(const std::vector<Obs>& o, double lref, uint32_t src, int run, const uint32_t key[2],
-                  double* best_p, double* best_c) {
+static void swarm(const std::vector<Obs>& o, double lref, int run, const uint32_t key[2], double* best_p,
+                  double* best_c) {
     const int P = 10, IT = 30;
     const double lo[2] = {3.0, -0.05}, hi[2] = {6.0, 0.01};
     const double w = 1.0 / (2.0 * std::log(2.0)), cc = 0.5 + std::log(2.0), cs = 0.5 + std::log(2.0);
     double x[P][2], v[P][2], pb[P][2], pc[P], c[P];
     for (int p = 0; p < P; p++) {
-        uint32_t ctr[4] = {src, (uint32_t)(64 * run + p), 0xFFFFFFFFu, 0x54480000u}, r[4];
+        uint32_t ctr[4] = {0u, (uint32_t)(64 * run + p), 0xFFFFFFFFu, 0x54480000u}, r[4];
         orc_philox4x32_10(ctr, key, r);
         for (int d = 0; d < 2; d++) {
             double rg = hi[d] - lo[d];
@@ -96,7 +96,7 @@ static void swarm(const std::vector<Obs>& o, double lref, uint32_t src, int run,
     double gp[2] = {x[g][0], x[g][1]}, gc = c[g];
     for (int it = 0; it < IT; it++) {
         for (int p = 0; p < P; p++) {
-            uint32_t ctr[4] = {src, (uint32_t)(64 * run + p), (uint32_t)it, 0x54480000u}, r[4];
+            uint32_t ctr[4] = {0u, (uint32_t)(64 * run + p), (uint32_t)it, 0x54480000u}, r[4];
             orc_philox4x32_10(ctr, key, r);
             for (int d = 0; d < 2; d++) {
                 double mom = v[p][d] * w;
@@ -275,7 +275,7 @@ int orc_batch_thermal(int64_t n_src, const int64_t* ref_off, const double* ref_t
         if (obs.size() < 3) return;
         double bp[2][2], bc[2];
         for (int run = 0; run < 2; run++)
-            orc::th::swarm(obs, lambda_ref[s], (uint32_t)s, run, key, bp[run], &bc[run]);
+            orc::th::swarm(obs, lambda_ref[s], run, key, bp[run], &bc[run]);
         const int b = bc[1] < bc[0] ? 1 : 0;
         double m0 = (bp[0][0] + bp[1][0]) / 2.0, m1 = (bp[0][1] + bp[1][1]) / 2.0;
         double v0 = ((bp[0][0] - m0) * (bp[0][0] - m0) + (bp[1][0] - m0) * (bp[1][0] - m0)) / 2.0;

@@ -139,8 +139,8 @@ def _replay_swarm(orc, obs, lref, src, run, seed):
     position clamped to the box), then evaluates all of them and updates the
     personal and global bests on strict improvement in particle order; the
     initial global best is the first particle of least cost.  Uniforms from
-    the oracle's Philox stream (counter (source, 64 run + particle,
-    iteration or 0xFFFFFFFF, 0x54480000))."""
+    the oracle's Philox stream (counter (0, 64 run + particle,
+    iteration or 0xFFFFFFFF, 0x54480000), the same for every source)."""
     u24 = lambda x: float(int(x) >> 8) / 16777216.0
     key = [seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF]
     lo, hi = (3.0, -0.05), (6.0, 0.01)
@@ -157,7 +157,7 @@ def _replay_swarm(orc, obs, lref, src, run, seed):
 
     x, v, pb, pc = [], [], [], []
     for p in range(10):
-        r = orc.philox([src, 64 * run + p, 0xFFFFFFFF, 0x54480000], key)
+        r = orc.philox([0, 64 * run + p, 0xFFFFFFFF, 0x54480000], key)
         xi = [lo[d] + u24(r[d]) * (hi[d] - lo[d]) for d in range(2)]
         vi = [-(hi[d] - lo[d]) + u24(r[2 + d]) * (2.0 * (hi[d] - lo[d])) for d in range(2)]
         x.append(xi), v.append(vi), pb.append(list(xi)), pc.append(cost(*xi))
@@ -165,7 +165,7 @@ def _replay_swarm(orc, obs, lref, src, run, seed):
     gp, gc = list(x[g]), pc[g]
     for it in range(30):
         for p in range(10):
-            r = orc.philox([src, 64 * run + p, it, 0x54480000], key)
+            r = orc.philox([0, 64 * run + p, it, 0x54480000], key)
             for d in range(2):
                 v[p][d] = v[p][d] * w + (pb[p][d] - x[p][d]) * u24(r[d]) * cc + (gp[d] - x[p][d]) * u24(r[2 + d]) * cc
                 x[p][d] = min(max(x[p][d] + v[p][d], lo[d]), hi[d])
