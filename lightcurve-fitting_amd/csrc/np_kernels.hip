@@ -7,14 +7,16 @@
 //      length scale from the median positive gap of the sorted detection
 //      times (rank sort, rank selection);
 //   2. every (amp, l) candidate: dense GP on all detections (packed Cholesky
-//      in LDS, right-looking over the workgroup; alpha by wave-level solves)
-//      scored by train RMS, or above the dense threshold the FITC
+//      in LDS, blocked over the workgroup; alpha by wave-level solves)
+//      scored by train RMS (from the weights: residual -D_i alpha_i), or
+//      above the dense threshold the FITC
 //      approximation (inducing-point factor in LDS; the rank-1 updates of B
 //      accumulated per chunk of 64 observations, each entry in observation
 //      order) scored by approx_nlml; for a FITC band also the subsampled
 //      dense GP kept for thermal reuse;
-//   3. predictive mean / derivative per query thread and predictive std per
-//      wavefront on the grid, the mean at every detection (residual RMS,
+//   3. predictive mean / derivative per query thread and predictive std
+//      (dense: 16 queries per wavefront, blocked forward substitution on the
+//      FP64 matrix cores) on the grid, the mean at every detection (residual RMS,
 //      chi2, baseline chi2), the std at (a 200-point subsample of) the
 //      detection times and the std rescale;
 //   4. the peak on the grid, the 1-day fine grid (<= 2000 points, LDS) and
@@ -173,6 +175,84 @@ struct Pred {
 #pragma unroll
         for (int u = 0; u < 4; u++) sd[u] = ::sqrt(fmax(amp - q[u] + qb[u], 1e-10));
     }
+    // Dense predictor, one wavefront: the predictive std of 16 queries --
+    // lane l serves query l & 15, whose time tq is -- in every lane of the
+    // query's column.  v = L^-1 k* by 16-row blocks I: the finished blocks'
+    // contribution sum_{J<I} L_IJ v_J as v_mfma_f64_16x16x4f64 tile products
+    // (A: the factor's tile, read from LDS; B: v_J, kept in registers in the
+    // product's own output layout, whose element e is the B operand of the
+    // k-step e), then the block's forward substitution, each solved row
+    // multiplied by its pivot's reciprocal and sent from its lane group to
+    // the other three.  var = amp - |v|^2, floored at 1e-10 (the reference's
+    // predict_with_std, src/sparse_gp.rs:131-158, solves column by column;
+    // the sums here round in tile order).  Layouts: A lane l -> A[l % 16][l / 16],
+    // B lane l -> B[l / 16][l % 16], D lane l element e -> D[4 e + l / 16][l % 16].
+    // NB: the block capacity (m <= 16 NB).
+    template <int NB>
+    __device__ __forceinline__ double dense_std16(double tq) const {
+        const int lane = lane_id();
+        const int g = lane >> 4, c = lane & 15;
+        const int nb = (m + 15) >> 4;
+        double v[NB][4];
+        double q = 0.0;
+#pragma unroll
+        for (int I = 0; I < NB; I++) {
+            if (I < nb) {
+                const int i0 = 16 * I;
+                double rhs[4];
+                int rr[4];  // start of this lane's rows at column i0 (row 0 for rows past m)
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int i = i0 + 4 * e + g;
+                    rhs[e] = i < m ? rbf(tq, d.x(i), amp, it) : 0.0;
+                    rr[e] = rs(i < m ? i : 0) + i0;
+                }
+                if (I > 0) {
+                    const int ia = i0 + c;
+                    const bool a_ok = ia < m;
+                    const int ra = rs(a_ok ? ia : 0) + g;
+                    v4d acc = {0.0, 0.0, 0.0, 0.0};
+                    // constant trip count, so that v[J] is indexed at compile time
+#pragma unroll
+                    for (int J = 0; J < NB - 1; J++) {
+                        if (J >= I) continue;
+#pragma unroll
+                        for (int s4 = 0; s4 < 4; s4++) {
+                            const double av = a_ok ? L[ra + 16 * J + 4 * s4] : 0.0;
+                            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, v[J][s4], acc, 0, 0, 0);
+                        }
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; e++) rhs[e] -= acc[e];
+                }
+                // lanes 0..15: the reciprocal of pivot i0 + lane (0 past m, so
+                // rows past m solve to 0)
+                const int jd = i0 + c;
+                const double rdl = jd < m ? 1.0 / L[rs(jd) + jd] : 0.0;
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int gr = r & 3, er = r >> 2;
+                    const double x = rhs[er] * bcast(rdl, r);  // row r, in lane group gr
+                    const double xb = __shfl(x, (gr << 4) | c, WAVE);
+                    if (g == gr) rhs[er] = x;
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        if (4 * e + 3 <= r) continue;  // rows already solved in every group
+                        const double lv = L[rr[e] + r];
+                        if (4 * e + g > r) rhs[e] = __builtin_fma(-lv, xb, rhs[e]);
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    v[I][e] = rhs[e];
+                    q = __builtin_fma(rhs[e], rhs[e], q);
+                }
+            }
+        }
+        q += __shfl_xor(q, 16, WAVE);
+        q += __shfl_xor(q, 32, WAVE);
+        return ::sqrt(fmax(amp - q, 1e-10));
+    }
     __device__ __forceinline__ void wave_sqnorm4(const double* F, const double (&tq)[4], double (&q)[4]) const {
         const int lane = lane_id();
         double b[4][ROWS];
@@ -283,15 +363,18 @@ __device__ __forceinline__ void restage(const double* r0, double* xs, const D& d
     __syncthreads();
 }
 
-__device__ __forceinline__ double dense_train_rms(const Det& d, int m, double amp, double it, double ybar, const double* alpha,
-                                  double* red) {
+// Train RMS (DenseGP::train_rms, src/sparse_gp.rs:169-175) from the weights
+// alone.  With K alpha = y - ybar and K = A + D (A the noise-free kernel
+// matrix at the training times, D = diag(max(nv, 1e-10))), the prediction at
+// training point i, (A alpha)_i + ybar, equals y_i - D_i alpha_i: the
+// residual is -D_i alpha_i.  m products instead of the m^2 kernel evaluations
+// of the explicit predictions, whose sum also cancels down to that residual
+// (relative rounding there grows with amp / D_i; here it does not).
+__device__ __forceinline__ double dense_train_rms(const Det& d, int m, const double* alpha, double* red) {
     double rss = 0.0;
     for (int i = threadIdx.x; i < m; i += GP_THREADS) {
-        const double xi = d.x(i);
-        double dot = 0.0;
-        for (int k = 0; k < m; k++) dot += rbf(xi, d.x(k), amp, it) * alpha[k];
-        double p = dot + ybar;
-        rss += (p - d.yy(i)) * (p - d.yy(i));
+        const double r = fmax(d.nv(i), 1e-10) * alpha[i];
+        rss += r * r;
     }
     rss = block_sum(rss, red);
     return ::sqrt(rss / (double)m);
@@ -774,7 +857,7 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
             if (ok && tid < WAVE) dense_alpha<ROWS>(s.r0, nd, s.al);
             restage(s.r0, s.xs, dls, nd);
             if (!ok) continue;
-            score = dense_train_rms(dls, nd, amp, it, ybar, s.al, red);
+            score = dense_train_rms(dls, nd, s.al, red);
         } else {
             if (!fitc_fit(fw, dgl, nd, m_ind, amp, it, ybar, tmin, tmax)) continue;
             score = fitc_nlml(fw, dgl, nd, m_ind, amp, it, ybar, red);
@@ -856,40 +939,69 @@ __device__ __forceinline__ void np_band(const NpArgs& a, const Lds& s, int band)
     // 3. predictive std (needs the factors) on the grid and at (a 200-point
     // subsample of) the detection times
     const double gstep_den = (double)(ng - 1);
-    for (int k0 = 4 * wave; k0 < ng; k0 += 4 * GP_WAVES) {
-        double tq[4], sd[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const int k = k0 + u < ng ? k0 + u : ng - 1;
-            tq[u] = t_lo + (double)k * dur / gstep_den;
-        }
-        P.wave_std4(tq, sd);
-#pragma unroll
-        for (int u = 0; u < 4; u++)
-            if (lane == u && k0 + u < ng) gstd[k0 + u] = sd[u];
-    }
+    constexpr int NB = ((ROWS * 64 < NP_CAP_MAX ? ROWS * 64 : NP_CAP_MAX) + 15) / 16;
     double ssum = 0.0, scnt = 0.0;
-    {
+    if (!sparse && nd <= 16 * NB) {
+        // the grid queries, then the detection subsample: 16 per wavefront
         const int ns = nd > 200 ? 200 : nd;
-        for (int k0 = 4 * wave; k0 < ns; k0 += 4 * GP_WAVES) {
+        const int nq = ng + ns;
+        for (int k0 = 16 * wave; k0 < nq; k0 += 16 * GP_WAVES) {
+            const int kq = k0 + (lane & 15);
+            const int kc = kq < nq ? kq : nq - 1;
+            double tq;
+            if (kc < ng) {
+                tq = t_lo + (double)kc * dur / gstep_den;
+            } else {
+                const int k = kc - ng;
+                tq = dx[nd > 200 ? (int)(((double)k + 0.5) * ((double)nd / 200.0)) : k];
+            }
+            const double sd = P.template dense_std16<NB>(tq);
+            if (lane < 16 && kq < nq) {
+                if (kq < ng) {
+                    gstd[kq] = sd;
+                } else if (__builtin_isfinite(sd)) {
+                    ssum += sd;
+                    scnt += 1.0;
+                }
+            }
+        }
+        ssum = block_sum(ssum, red);
+        scnt = block_sum(scnt, red);
+    } else {  // FITC: the wave-level solves against K_mm and B
+        for (int k0 = 4 * wave; k0 < ng; k0 += 4 * GP_WAVES) {
             double tq[4], sd[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int k = k0 + u < ns ? k0 + u : ns - 1;
-                const int i = nd > 200 ? (int)(((double)k + 0.5) * ((double)nd / 200.0)) : k;
-                tq[u] = dx[i];
+                const int k = k0 + u < ng ? k0 + u : ng - 1;
+                tq[u] = t_lo + (double)k * dur / gstep_den;
             }
             P.wave_std4(tq, sd);
 #pragma unroll
             for (int u = 0; u < 4; u++)
-                if (k0 + u < ns && __builtin_isfinite(sd[u])) {
-                    ssum += sd[u];
-                    scnt += 1.0;
-                }
+                if (lane == u && k0 + u < ng) gstd[k0 + u] = sd[u];
         }
-        // every lane of a wave holds the same partial: count it once per wave
-        ssum = block_sum(lane == 0 ? ssum : 0.0, red);
-        scnt = block_sum(lane == 0 ? scnt : 0.0, red);
+        {
+            const int ns = nd > 200 ? 200 : nd;
+            for (int k0 = 4 * wave; k0 < ns; k0 += 4 * GP_WAVES) {
+                double tq[4], sd[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int k = k0 + u < ns ? k0 + u : ns - 1;
+                    const int i = nd > 200 ? (int)(((double)k + 0.5) * ((double)nd / 200.0)) : k;
+                    tq[u] = dx[i];
+                }
+                P.wave_std4(tq, sd);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (k0 + u < ns && __builtin_isfinite(sd[u])) {
+                        ssum += sd[u];
+                        scnt += 1.0;
+                    }
+            }
+            // every lane of a wave holds the same partial: count it once per wave
+            ssum = block_sum(lane == 0 ? ssum : 0.0, red);
+            scnt = block_sum(lane == 0 ? scnt : 0.0, red);
+        }
     }
     if (!sparse) {  // dense weights into the slot, over the factor's tail
         for (int i = tid; i < nd; i += GP_THREADS) s.al[i] = st[i];

@@ -4,6 +4,8 @@
     Cholesky and solve) on the same kernel matrix (Rasmussen & Williams
     eqs. 2.25-2.26, 2.30);
   * the analytic derivative of the mean against central differences;
+  * train RMS equals the closed form sqrt(mean((D alpha)^2)) that the
+    device's band pipeline scores candidates by;
   * the reference's in-module acceptance: a DenseGP recovers sin(t/2)
     (src/sparse_gp.rs:334-360, tolerance 0.15);
   * batch selection: the chosen candidate is the first strict minimum of the
@@ -207,3 +209,32 @@ def test_batch_gp2d_reduces_to_1d(orc):
     assert np.array_equal(two["state"][:, 123:125], one["state"][:, 82:84])
     for k in ("pred", "std", "dpred", "pred_obs"):
         assert np.array_equal(two[k], one[k], equal_nan=True)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_train_rms_equals_weighted_noise_residual(orc, seed):
+    """DenseGP::train_rms (src/sparse_gp.rs:169-175) predicts at the training
+    times with the noise-free kernel, so with K alpha = y - ybar and
+    K = A + D (D = diag(max(nv, 1e-10))) each residual is -D_i alpha_i.  The
+    device's band pipeline scores candidates by that closed form; here the
+    oracle's explicit predictions are held to it, with alpha from numpy's
+    solve, across the nonparametric candidate amplitudes, short and long
+    length scales and magnitude-error noise down to the 1e-6 floor."""
+    rng = np.random.default_rng(100 + seed)
+    n = [8, 25, 60, 120, 200, 40][seed]
+    t = np.sort(rng.uniform(0, 150, n))
+    y = 19 + 1.5 * np.exp(-(t - 60) ** 2 / 400) + 0.05 * rng.normal(size=n)
+    e = rng.uniform(0.001 if seed == 5 else 0.01, 0.3, n)
+    nv = np.maximum(e * e, 1e-6)
+    for amp in (0.3, 1.0, 3.0, 10.0):
+        for ls in (150 / 48, 150 / 12, 150 / 3):
+            out = orc.dense_gp(t, y, nv, amp, ls, t[:1])
+            if out is None:
+                continue
+            tr = out[4]
+            K = amp * np.exp(-(t[:, None] - t[None, :]) ** 2 / (2 * ls * ls)) + np.diag(np.maximum(nv, 1e-10))
+            alpha = np.linalg.solve(K, y - y.mean())
+            closed = math.sqrt(np.mean((np.maximum(nv, 1e-10) * alpha) ** 2))
+            # the explicit sum cancels down to the residual; measured within
+            # 1.2e-11 relative (amp / D_i up to 1e7 at seed 5)
+            assert tr == pytest.approx(closed, rel=1e-9, abs=1e-12), (n, amp, ls)
