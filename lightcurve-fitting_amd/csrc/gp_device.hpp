@@ -80,16 +80,18 @@ constexpr int CHOL_NB = 16;  // block-column width of the blocked Cholesky
 // on the FP64 matrix cores; (B) wavefront 0 factors the diagonal block with
 // its rows in registers (lane r holds row J0 + r; column j is scaled, then
 // the columns to its right lose l_ij l_cj); (C) every row below solves
-// against the diagonal block.  Within a block the elements are reduced term
-// by term in increasing k (fused multiply-adds), as the row-major
-// Cholesky-Crout of src/sparse_gp.rs:27-49 orders them; the panel sums of
-// (A) round as tile products, and the column scalings multiply by the
-// pivot's reciprocal: the factor agrees with the reference's to within its
-// backward error, not bit for bit.
+// against the diagonal block.  (A) and (B) share one phase: wavefront 0
+// takes the diagonal tile's panel update and goes straight on to factor it,
+// while the other three update the rest of the panel.  Within a block the
+// elements are reduced term by term in increasing k (fused multiply-adds),
+// as the row-major Cholesky-Crout of src/sparse_gp.rs:27-49 orders them; the
+// panel sums of (A) round as tile products, and the column scalings multiply
+// by the pivot's reciprocal: the factor agrees with the reference's to
+// within its backward error, not bit for bit.
 // rows = m + 1 appends row m (its first m entries, e.g. y - ybar): it is
 // reduced like any row below the factor and ends as L^-1 times itself, the
 // forward substitution of solve_l (src/sparse_gp.rs:52-58).
-// Three barriers per block column; returns false (uniformly, after a
+// Two barriers per block column; returns false (uniformly, after a
 // barrier) on a pivot <= 0.  The matrix must be complete and visible.
 // rinv (LDS, CHOL_NB doubles) carries the reciprocals of the block's pivots
 // from (B) to (C), whose rows multiply by them instead of dividing (within an
@@ -100,9 +102,10 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
     const int lane = lane_id();
     for (int J0 = 0; J0 < m; J0 += CHOL_NB) {
         const int jb = m - J0 < CHOL_NB ? m - J0 : CHOL_NB;
-        // (A) panel update, k < J0, on the FP64 matrix cores: wave w takes
-        // the 16-row tiles w, w + 4, ... of rows J0 .. rows - 1 against the
-        // block's 16 columns; D[r][c] = sum_k L[i_r][k] L[J0 + c][k] over
+        // (A) panel update, k < J0, on the FP64 matrix cores: wave 0 takes
+        // the diagonal tile (rows J0 .. J0 + 15), waves 1-3 the 16-row tiles
+        // 1, 2, ... of rows J0 .. rows - 1 in turn, against the block's 16
+        // columns; D[r][c] = sum_k L[i_r][k] L[J0 + c][k] over
         // four-deep k steps (two accumulators, alternate steps), subtracted
         // once at the end (the reference subtracts term by term: the sum
         // rounds differently, within the factor's backward error).
@@ -114,16 +117,23 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
             const int jb_row = J0 + c16;
             const bool b_ok = c16 < jb;
             const int rb = rs(b_ok ? jb_row : J0) + kk;
-            for (int T = tid >> 6; T < ntile; T += GP_WAVES) {
+            const int wave = tid >> 6;
+            for (int T = wave; T < ntile; T += wave == 0 ? ntile : GP_WAVES - 1) {
                 const int i_a = J0 + 16 * T + c16;
                 const bool a_ok = i_a < rows;
                 const int ra = rs(a_ok ? i_a : J0) + kk;
                 v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+                // Rows past the matrix (a) and columns past the block (b)
+                // read row J0 instead: they only reach product rows /
+                // columns that are never written, so the loads need no
+                // select and are issued one k step ahead.
+                double a0 = L[ra], b0 = L[rb], a1 = L[ra + 4], b1 = L[rb + 4];
                 for (int k0 = 0; k0 < J0; k0 += 8) {  // J0 is a multiple of CHOL_NB = 16
-                    const double a0 = a_ok ? L[ra + k0] : 0.0, b0 = b_ok ? L[rb + k0] : 0.0;
-                    const double a1 = a_ok ? L[ra + k0 + 4] : 0.0, b1 = b_ok ? L[rb + k0 + 4] : 0.0;
+                    const int kn = k0 + 8 < J0 ? k0 + 8 : k0;
+                    const double na0 = L[ra + kn], nb0 = L[rb + kn], na1 = L[ra + kn + 4], nb1 = L[rb + kn + 4];
                     acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc1, 0, 0, 0);
+                    a0 = na0; b0 = nb0; a1 = na1; b1 = nb1;
                 }
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
@@ -131,9 +141,9 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
                     if (i < rows && c16 < jb && j <= i) L[rs(i) + j] -= acc0[e] + acc1[e];
                 }
             }
-            __syncthreads();
         }
-        // (B) diagonal block on wavefront 0: lane r holds row J0 + r
+        // (B) diagonal block on wavefront 0, straight after its own panel
+        // tile (its LDS writes are in order; no barrier): lane r holds row J0 + r
         if (tid < WAVE) {
             double v[CHOL_NB];
             const bool own = lane < jb;
@@ -252,19 +262,35 @@ __device__ __forceinline__ double wave_solve_l(const double* L, int m, double (&
     return nrm;
 }
 
-// b <- L^-T b over rows j in [64 R, min(64 R + 64, m)), descending.
+// b <- L^-T b over rows j in [64 R, min(64 R + 64, m)), descending.  Row
+// j's entries are loaded one step ahead, so the LDS latency stays off the
+// dependent chain (lane reads, multiply, subtract).  The loads of register R
+// run past row j's end in the lanes that do not use them (the next rows, or
+// the training times that follow the packed factor: inside the allocation).
 template <int R, int ROWS>
 __device__ __forceinline__ void solve_lt_seg(const double* L, int m, double (&b)[ROWS], const double (&rd)[ROWS]) {
     const int lane = lane_id();
     const int jhi = m < 64 * R + 64 ? m : 64 * R + 64;
+    if (jhi <= 64 * R) return;
+    double lc[R + 1];
+    {
+        const int rj = rs(jhi - 1);
+#pragma unroll
+        for (int r = 0; r <= R; r++) lc[r] = L[rj + lane + 64 * r];
+    }
     for (int j = jhi - 1; j >= 64 * R; j--) {
+        double ln[R + 1];
+        const int rn = rs(j > 64 * R ? j - 1 : j);
+#pragma unroll
+        for (int r = 0; r <= R; r++) ln[r] = L[rn + lane + 64 * r];
         const int src = j & 63;
         const double aj = bcast(b[R], src) * bcast(rd[R], src);
-        const int rj = rs(j);
-        if (lane == src) b[R] = aj;
-        else if (lane < src) b[R] -= L[rj + lane + 64 * R] * aj;
+        const double upd = b[R] - lc[R] * aj;  // selects, not branches, on the chain
+        b[R] = lane == src ? aj : (lane < src ? upd : b[R]);
 #pragma unroll
-        for (int r = 0; r < R; r++) b[r] -= L[rj + lane + 64 * r] * aj;
+        for (int r = 0; r < R; r++) b[r] -= lc[r] * aj;
+#pragma unroll
+        for (int r = 0; r <= R; r++) lc[r] = ln[r];
     }
 }
 
