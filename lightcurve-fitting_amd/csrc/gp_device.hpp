@@ -72,6 +72,29 @@ __device__ __forceinline__ void tri_ij(int e, int& i, int& j) {
     j = e - rs(r);
 }
 
+// DPP row broadcast (row_newbcast): every lane gets lane N of its 16-lane
+// row.  N folds to a constant in the callers' fully unrolled loops; unlike a
+// readlane it leaves the value in a vector register (no scalar round trip).
+template <int N>
+__device__ __forceinline__ double row_bcast_c(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), 0x150 + N, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x150 + N, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double row_bcast(double v, int n) {
+    switch (n) {
+        case 0: return row_bcast_c<0>(v);   case 1: return row_bcast_c<1>(v);
+        case 2: return row_bcast_c<2>(v);   case 3: return row_bcast_c<3>(v);
+        case 4: return row_bcast_c<4>(v);   case 5: return row_bcast_c<5>(v);
+        case 6: return row_bcast_c<6>(v);   case 7: return row_bcast_c<7>(v);
+        case 8: return row_bcast_c<8>(v);   case 9: return row_bcast_c<9>(v);
+        case 10: return row_bcast_c<10>(v); case 11: return row_bcast_c<11>(v);
+        case 12: return row_bcast_c<12>(v); case 13: return row_bcast_c<13>(v);
+        case 14: return row_bcast_c<14>(v); default: return row_bcast_c<15>(v);
+    }
+}
+
 constexpr int CHOL_NB = 16;  // block-column width of the blocked Cholesky
 
 // In-place Cholesky of the packed m x m matrix in L over the workgroup,
@@ -158,7 +181,7 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
 #pragma unroll
             for (int j = 0; j < CHOL_NB; j++) {
                 if (j < jb) {
-                    const double piv = bcast(v[j], j);
+                    const double piv = row_bcast(v[j], j);  // lanes 0..15 hold the block's rows
                     bad |= piv <= 0.0;
                     // r = 1 / sqrt(piv): v_rsq_f64 and one Newton step (a few
                     // ulp); l_jj = piv r and the column scaled by r
@@ -168,7 +191,8 @@ __device__ __forceinline__ bool chol_blocked(double* L, int m, int rows, int* fl
                     v[j] = lane == j ? piv * r : v[j] * r;
                     // columns c > j of the rows below c lose l_ij l_cj
 #pragma unroll
-                    for (int c = j + 1; c < CHOL_NB; c++) v[c] = __builtin_fma(-v[j], bcast(v[j], c), v[c]);
+                    for (int c = 1; c < CHOL_NB; c++)
+                        if (c > j) v[c] = __builtin_fma(-v[j], row_bcast(v[j], c), v[c]);
                 }
             }
             if (own) {
