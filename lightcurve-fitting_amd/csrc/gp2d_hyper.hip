@@ -140,21 +140,23 @@ __device__ __noinline__ void build_k(double* A, int ld, const Src& s, const doub
     const double a = ::exp(th[0]), lt = ::exp(th[1]), lw = ::exp(th[2]), sn = ::exp(th[3]);
     const double it = 1.0 / (2.0 * lt * lt), iw = 1.0 / (2.0 * lw * lw), sn2 = sn * sn;
     const int n = s.n, np = s.np;
-    for (int i = 0; i < np; i++) {
-        double* row = A + (size_t)i * ld;
+    // one element of the lower triangle per thread and step (rows of every
+    // length keep all threads busy)
+    const int ne = rs(np);
+    for (int e = threadIdx.x; e < ne; e += NT) {
+        int i, j;
+        tri_ij(e, i, j);
+        double k;
         if (i < n) {
-            const double ti = s.t[i], wi = s.w[i];
-            for (int j = threadIdx.x; j <= i; j += NT) {
-                const double dt = ti - s.t[j], dw = wi - s.w[j];
-                double k = a * ::exp(-(dt * dt * it + dw * dw * iw));
-                if (j == i) k += s.nv[i] + sn2;
-                row[j] = k;
-            }
+            const double dt = s.t[i] - s.t[j], dw = s.w[i] - s.w[j];
+            k = a * ::exp(-(dt * dt * it + dw * dw * iw));
+            if (j == i) k += s.nv[i] + sn2;
         } else if (i == n) {
-            for (int j = threadIdx.x; j <= i; j += NT) row[j] = j < n ? s.y[j] - ybar : 1.0;
+            k = j < n ? s.y[j] - ybar : 1.0;
         } else {
-            for (int j = threadIdx.x; j <= i; j += NT) row[j] = j == i ? 1.0 : 0.0;
+            k = j == i ? 1.0 : 0.0;
         }
+        A[(size_t)i * ld + j] = k;
     }
     __syncthreads();
 }
