@@ -232,9 +232,10 @@ def np_flops(csr: dict, n_grid: int, dense_max: int, amps, lsf) -> float:
     batch (dense bands), an FP64 exponential counted as 20: per hyper-parameter
     candidate (those with l >= the band's minimum length scale) the Cholesky
     m^3/3, the two triangular solves 2 m^2, the kernel matrix m(m+1)/2 exps
-    and the train-RMS predictions m^2 (exp + 2); for the winner the refit, the
-    std at n_grid + min(m, 200) points (m^2 + m exps each) and the means at
-    2 n_grid + m + fine-grid points (m (exp + 2) each)."""
+    and the train RMS from the weights (2 m: the residual at training point i
+    is -D_i alpha_i); for the winner the refit, the std at n_grid + min(m, 200)
+    points (m^2 + m exps each) and the means at 2 n_grid + m + fine-grid
+    points (m (exp + 2) each)."""
     EXP = 20.0
     off = csr["offsets"]
     tot = 0.0
@@ -252,7 +253,7 @@ def np_flops(csr: dict, n_grid: int, dense_max: int, amps, lsf) -> float:
         min_ls = max(2.0 * (g[g.size // 2] if g.size else 1.0), 0.1)
         dur = csr["t_hi"][b] - csr["t_lo"][b]
         nc = len(amps) * sum(1 for f in lsf if max(dur / f, 0.1) >= min_ls)
-        cand = m ** 3 / 3.0 + 2.0 * m * m + m * (m + 1) / 2.0 * EXP + m * m * (EXP + 2.0)
+        cand = m ** 3 / 3.0 + 2.0 * m * m + m * (m + 1) / 2.0 * EXP + 2.0 * m
         nf = min(int(max(dur, 1.0)) + 1, 2000)
         final = cand + (n_grid + min(m, 200)) * (m * m + m * EXP) + (2 * n_grid + m + nf) * m * (EXP + 2.0)
         tot += nc * cand + final
@@ -288,12 +289,14 @@ def np_leg(pkg, n_sources: int, n_grid: int, dense_max: int, reps: int, cpu_sour
            "dense_max": dense_max, "dtype": "f64", "value": round(n_sources / (best * 1e-3), 2),
            "unit": "sources/s", "ms": round(best, 3), "ms_with_transfers": round(best_call * 1e3, 3),
            "status_counts": np.bincount(r["status"], minlength=3).tolist(),
-           "roofline": {"bound": "mfma", "pipe": "fp64 matrix cores (Cholesky panel tiles) + fp64 vector ALU, both 78.6 TFLOP/s",
+           "roofline": {"bound": "mfma", "pipe": "fp64 matrix cores (Cholesky panel tiles, predictive-std "
+                                                  "substitution) + fp64 vector ALU, both 78.6 TFLOP/s",
                         "kernel": "np_band_kernel (all detection-count classes)",
                         "achieved": round(achieved, 4), "peak": 78.6, "unit": "TFLOP/s",
                         "frac": round(achieved / 78.6, 5), "flop_equivalents": flops,
-                        "note": "one wave per 16-row triangular solve chain: issue-latency bound "
-                                "(dependent FP64 FMAs, readlane broadcasts), not by FP64 throughput"}}
+                        "note": "bound by the latency of dependent chains (diagonal-block pivots, "
+                                "row and weight substitutions) with four waves per band, not by "
+                                "FP64 throughput"}}
     if do_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import oracle

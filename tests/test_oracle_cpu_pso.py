@@ -10,7 +10,8 @@
     cost of the returned position, a seeded run never ends above the seed's
     cost, determinism, the iteration cap and stall rule;
   * recovery: a noise-free Bazin band fitted to within its noise floor;
-  * the polish never raises the cost and lowers it from a perturbed start.
+  * the polish never raises the cost and lowers it from a perturbed start;
+    on the prior-free likelihood it ends where scipy's L-BFGS-B ends.
 """
 import math
 import numpy as np
@@ -185,3 +186,36 @@ def test_multi_bazin_job_cost_brute_force(orc, pkg, k):
     assert np.all(x >= lo) and np.all(x <= hi)
     want = multi_bazin_cost_py(list(x), k, *(csr[q] for q in OBS))
     assert o["cost"][0] == pytest.approx(want, rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("t0,amp,seed", [(15.0, 1.0, 1), (30.0, 2.0, 2), (8.0, 3.0, 3)])
+def test_polish_end_point_matches_scipy_lbfgsb(orc, t0, amp, seed):
+    """The polish (lbfgs_refine, src/parametric.rs:1453-1497, restated with a
+    backtracking Armijo line search in place of argmin's More-Thuente) on the
+    likelihood alone (its gradient, PsoCost::gradient, omits the prior, so
+    the prior-free cost is the consistent objective) ends where scipy's
+    L-BFGS-B ends on the same objective, gradient and box, from a start
+    inside the basin: the same minimum to 1e-9 in cost and 1e-4 in the
+    parameters."""
+    so = pytest.importorskip("scipy.optimize")
+    src = make_reference_test_source(t0, amp, 60, orc)
+    csr = single(src)
+    obs = [src[k] for k in OBS]
+    lo, hi = orc.bounds(0, cpu=True)
+    rng = np.random.default_rng(seed)
+    truth = np.array([math.log(amp), 0.05, t0, 0.0, math.log(15.0), -3.0])  # the fixture's parameters
+    p0 = np.clip(truth + rng.normal(0.0, 0.05, 6), lo + 1e-3, hi - 1e-3)
+    c0 = evaluate(orc, csr, 0, p0, prior=0)
+    j = job(orc, kind=orc.CPSO_POLISH, model=0, band=0, dim=6, max_iters=200, use_prior=0, lo=lo, hi=hi, x0=p0,
+            start_cost=c0)
+    r = orc.cpu_pso(csr, np.array([j]))
+    x_pol, f_pol = r["x"][0, :6], r["cost"][0]
+
+    def fg(p):
+        return evaluate(orc, csr, 0, p, prior=0), orc.pso_cost_grad(0, p, *obs)
+
+    ref = so.minimize(fg, p0, jac=True, method="L-BFGS-B", bounds=list(zip(lo, hi)),
+                      options={"ftol": 1e-15, "gtol": 1e-10, "maxiter": 1000})
+    assert f_pol < c0
+    assert f_pol == pytest.approx(ref.fun, rel=1e-9, abs=1e-12), (f_pol, ref.fun)
+    assert np.allclose(x_pol, ref.x, rtol=1e-4, atol=1e-4), (x_pol, ref.x)
