@@ -170,26 +170,41 @@ __device__ __noinline__ bool cholesky(double* A, int ld, const Src& s, Smem& sm)
         // (1) diagonal block on wave 0, left-looking in LDS: the block's lower
         // triangle column-major in sm.a (sm.a[c * T + r] = L[r][c]: a lane
         // reads its own row coalesced, the pivot row as a broadcast); lane r
-        // computes column j of row r as A[r][j] - sum_k<j L[r][k] L[j][k] in
-        // increasing k, the order of a right-looking sweep.
+        // computes column j of row r as A[r][j] - sum_k<j L[r][k] L[j][k],
+        // the sum in four interleaved partial sums (a quarter of the
+        // dependent chain), and scales it by the pivot's reciprocal square
+        // root (v_rsq_f64 and one Newton step: a few ulp).  The reciprocals
+        // go to sm.b[0 .. 63] for the panel.
         if (tid < WAVE) {
             const double* rowp = A + (size_t)(J0 + lane) * ld + J0;
             for (int c = 0; c < T; c++) sm.a[c * T + lane] = c <= lane ? rowp[c] : 0.0;
             int bad = 0;
+            double rl = 1.0;
             for (int j = 0; j < T; j++) {
                 if (J0 + j >= n) {
                     if (lane == j) sm.a[j * T + j] = 1.0;
                     continue;
                 }
-                double acc = sm.a[j * T + lane];
-                for (int k = 0; k < j; k++) acc = fma(-sm.a[k * T + lane], sm.a[k * T + j], acc);
+                double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+                int k = 0;
+                for (; k + 4 <= j; k += 4) {
+                    p0 = fma(sm.a[k * T + lane], sm.a[k * T + j], p0);
+                    p1 = fma(sm.a[(k + 1) * T + lane], sm.a[(k + 1) * T + j], p1);
+                    p2 = fma(sm.a[(k + 2) * T + lane], sm.a[(k + 2) * T + j], p2);
+                    p3 = fma(sm.a[(k + 3) * T + lane], sm.a[(k + 3) * T + j], p3);
+                }
+                for (; k < j; k++) p0 = fma(sm.a[k * T + lane], sm.a[k * T + j], p0);
+                const double acc = sm.a[j * T + lane] - ((p0 + p1) + (p2 + p3));
                 const double piv = bcast(acc, j);
                 bad |= !(piv > 0.0);
-                const double ljj = ::sqrt(piv);
-                if (lane >= j) sm.a[j * T + lane] = lane == j ? ljj : acc / ljj;
+                double r = __builtin_amdgcn_rsq(piv);
+                r = r * fma(-0.5 * piv * r, r, 1.5);
+                if (lane == j) rl = r;
+                if (lane >= j) sm.a[j * T + lane] = lane == j ? piv * r : acc * r;
             }
             double* wr = A + (size_t)(J0 + lane) * ld + J0;
             for (int c = 0; c <= lane; c++) wr[c] = sm.a[c * T + lane];
+            sm.b[lane] = rl;  // 1 for the identity pivots of the padding
             if (tid == 0) sm.flag[0] = bad;
         }
         __syncthreads();
@@ -217,7 +232,7 @@ __device__ __noinline__ bool cholesky(double* A, int ld, const Src& s, Smem& sm)
                     double acc = v[j];
 #pragma unroll
                     for (int k = 0; k < j; k++) acc = fma(-v[k], sm.a[(cb + k) * T + cb + j], acc);
-                    v[j] = acc / sm.a[(cb + j) * T + cb + j];
+                    v[j] = acc * sm.b[cb + j];  // the pivot's reciprocal
                 }
 #pragma unroll
                 for (int c = 0; c < 16; c++) rp[cb + c] = v[c];
