@@ -239,16 +239,19 @@ __device__ __noinline__ bool cholesky(double* A, int ld, const Src& s, Smem& sm)
             }
         }
         __syncthreads();
-        // (3) trailing update of the lower tiles with rows < n + 1
+        // (3) trailing update of the lower tiles with rows < n + 1: the
+        // panel rows of tile row I0 are staged once for the whole row of
+        // tiles, and the diagonal tile reads them as both operands.
         for (int I0 = J0 + T; I0 < rows; I0 += T) {
+            __syncthreads();
+            stage_rows(sm.a, A, ld, I0, J0, J0 + T);
             for (int C0 = J0 + T; C0 <= I0; C0 += T) {
                 Tile acc;
                 zero(acc);
                 __syncthreads();
-                stage_rows(sm.a, A, ld, I0, J0, J0 + T);
-                stage_rows(sm.b, A, ld, C0, J0, J0 + T);
+                if (C0 < I0) stage_rows(sm.b, A, ld, C0, J0, J0 + T);
                 __syncthreads();
-                mma64(acc, sm.a, sm.b);
+                mma64(acc, sm.a, C0 < I0 ? sm.b : sm.a);
 #pragma unroll
                 for (int rb = 0; rb < 2; rb++)
 #pragma unroll
