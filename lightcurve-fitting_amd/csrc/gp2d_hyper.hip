@@ -385,9 +385,9 @@ __device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& 
             for (int k0 = I0; k0 < n; k0 += T) {
                 __syncthreads();
                 stage_kmaj(sm.a, Xm, ld, I0, k0, n);
-                stage_kmaj(sm.b, Xm, ld, J0, k0, n);
+                if (J < I) stage_kmaj(sm.b, Xm, ld, J0, k0, n);  // a diagonal tile reads sm.a twice
                 __syncthreads();
-                mma64(acc, sm.a, sm.b);
+                mma64(acc, sm.a, J < I ? sm.b : sm.a);
             }
 #pragma unroll
             for (int rb = 0; rb < 2; rb++)
