@@ -52,12 +52,13 @@ MODEL_NAMES = {0: "Bazin", 1: "Villar", 2: "Tde", 3: "Arnett", 4: "Magnetar"}
 # They do not depend on how the kernel is compiled.
 FLOPEQ_ALGO = {0: 12 + 5 * 8, 1: 20 + 6 * 8, 2: 18 + 7 * 8, 3: 18 + 6 * 8, 4: 15 + 5 * 8}
 # Issued-instruction flop-equivalents of the shipped check-free cost loops of
-# pso_kernel<MODEL, fp32> (four observations per loop trip, /4): an FMA 2, a
-# transcendental 4 (two issue slots x 2), any other VALU instruction 1;
-# detections and upper limits (censored term) differ.  Reported beside the
-# algorithmic figure as the kernel's own issue rate.
-FLOPEQ_DET = {0: 34.5, 1: 53.5, 2: 49.5, 3: 49.5, 4: 41.5}
-FLOPEQ_UP = {0: 41.0, 1: 60.0, 2: 56.0, 3: 56.0, 4: 48.0}
+# pso_kernel<MODEL, fp32> (every VALU instruction of one loop trip, the
+# update of the particle's sum included, over the trip's four observations):
+# an FMA 2, a transcendental 4 (two issue slots x 2), any other VALU
+# instruction 1; detections and upper limits (censored term) differ.
+# Reported beside the algorithmic figure as the kernel's own issue rate.
+FLOPEQ_DET = {0: 33.0, 1: 47.0, 2: 48.0, 3: 48.0, 4: 40.0}
+FLOPEQ_UP = {0: 42.0, 1: 56.0, 2: 57.0, 3: 57.0, 4: 49.0}
 # SVI flop-equivalents per (sample, observation, step): the Villar forward
 # evaluation (20 ops + 6 transcendentals x 8) and its adjoint (about 45 ops).
 SVI_FLOPEQ_FWD, SVI_FLOPEQ_ADJ = 20 + 6 * 8, 45

@@ -25,11 +25,12 @@
 // softplus(ph) to a power and keep the series.
 //
 // Likelihood (cost_quad / cost_pair / cost_one): observations are consumed
-// four at a time from the detections-first LDS tile; ln v is taken as
-// log2(v_a v_b) ln 2 per pair (v >= e^-12 + var, so a product of two never
-// leaves the normal FP32 range for normalised variances below 1e19), and the
-// four terms enter one Kahan update.  Upper limits follow, four at a time,
-// with the censored term read from a table (neg_ln_phi).
+// four at a time from the detections-first LDS tile; a pair's terms share one
+// reciprocal and one logarithm, d_a^2/v_a + d_b^2/v_b = (d_a^2 v_b + d_b^2 v_a)
+// / (v_a v_b) and ln v_a + ln v_b = log2(v_a v_b) ln 2 (v >= e^-12 + var, so a
+// product of two never leaves the normal FP32 range for normalised variances
+// below 1e19), and the four terms enter one Kahan update.  Upper limits follow, four at a time, with the censored term read
+// from a table (neg_ln_phi_tab).
 #pragma once
 #include "lcf_device.hpp"
 #include "models_fit.hpp"
@@ -91,6 +92,11 @@ template <> struct Fast<0> {
 // Villar (src/parametric.rs:231-247): A s(ph/tr) [(1-w)(1-beta ph) +
 // w (1-beta gamma) e^{(gamma-ph)/tf}], w = s(10(ph-gamma)), with w e^{y}
 // merged into one exponential; non-finite when e^{(gamma-ph)/tf} overflows.
+// Both logistic weights share one reciprocal: with e_r = 2^{-ph/tr} and
+// e_z = 2^{-|z|}, s(ph/tr) w = 1 / ((1 + e_r)(1 + e_z)) times 1 (z >= 0) or
+// e_z (z < 0), and s(ph/tr)(1 - w) the same with the factors swapped.  The
+// product (1 + e_r)(1 + e_z) is formed as a plain product, so e_r = +inf
+// with e_z = 0 gives +inf (weight 0), never inf * 0.
 template <> struct Fast<1> {
     float a, abn, zc, t0, itr, itf, yc, aomb, p_inf;  // A folded into the bracket
     __device__ __forceinline__ void setup(const float* p) {
@@ -108,15 +114,15 @@ template <> struct Fast<1> {
     template <bool CHECK = true>
     __device__ __forceinline__ float pred(float tl) const {
         float ph = tl - t0;
-        float sr = rcp(1.0f + ex2(-ph * itr));
+        float er1 = 1.0f + ex2(-ph * itr);
         float z = __builtin_fmaf(ph, 10.0f, zc);
         float ez = ex2(-__builtin_fabsf(z));
-        float qz = rcp(1.0f + ez);
-        float omw = z >= 0.0f ? ez * qz : qz;
+        float q = rcp(er1 * (1.0f + ez));  // s(ph/tr) s(|z|)
+        float sel = z >= 0.0f ? ez : 1.0f;   // (1 - w) / s(|z|)
         float y = __builtin_fmaf(-ph, itf, yc);
-        float wexp = qz * ex2(y + __builtin_fminf(z, 0.0f));
-        float inner = __builtin_fmaf(omw, __builtin_fmaf(-abn, ph, a), aomb * wexp);
-        float f = sr * inner;
+        float e = ex2(y + __builtin_fminf(z, 0.0f));  // w e^{y} / s(|z|)
+        float inner = __builtin_fmaf(sel, __builtin_fmaf(-abn, ph, a), aomb * e);
+        float f = q * inner;
         return CHECK && ph < p_inf ? INFINITY : f;
     }
 };
@@ -294,11 +300,11 @@ __device__ __forceinline__ float cost_quad(const Fast<MODEL>& m, float se2, cons
     float v0 = r0.z + se2, v1 = r1.z + se2, v2 = r2.z + se2, v3 = r3.z + se2;
     float d0 = m.template pred<CHECK>(r0.x) - r0.y, d1 = m.template pred<CHECK>(r1.x) - r1.y;
     float d2 = m.template pred<CHECK>(r2.x) - r2.y, d3 = m.template pred<CHECK>(r3.x) - r3.y;
-    float q = d0 * (d0 * rcp(v0));
-    q = __builtin_fmaf(d1 * rcp(v1), d1, q);
-    q = __builtin_fmaf(d2 * rcp(v2), d2, q);
-    q = __builtin_fmaf(d3 * rcp(v3), d3, q);
-    float l = lg2(v0 * v1) + lg2(v2 * v3);
+    float v01 = v0 * v1, v23 = v2 * v3;
+    float n01 = __builtin_fmaf(d0 * d0, v1, (d1 * d1) * v0);
+    float n23 = __builtin_fmaf(d2 * d2, v3, (d3 * d3) * v2);
+    float q = __builtin_fmaf(n23, rcp(v23), n01 * rcp(v01));
+    float l = lg2(v01) + lg2(v23);
     return __builtin_fmaf(l, LN2, q);
 }
 template <bool CHECK, int MODEL>
@@ -306,8 +312,9 @@ __device__ __forceinline__ float cost_pair(const Fast<MODEL>& m, float se2, cons
                                            const float4& r1) {
     float v0 = r0.z + se2, v1 = r1.z + se2;
     float d0 = m.template pred<CHECK>(r0.x) - r0.y, d1 = m.template pred<CHECK>(r1.x) - r1.y;
-    float q = __builtin_fmaf(d1 * rcp(v1), d1, d0 * (d0 * rcp(v0)));
-    return __builtin_fmaf(lg2(v0 * v1), LN2, q);
+    float v01 = v0 * v1;
+    float q = __builtin_fmaf(d0 * d0, v1, (d1 * d1) * v0) * rcp(v01);
+    return __builtin_fmaf(lg2(v01), LN2, q);
 }
 template <bool CHECK, int MODEL>
 __device__ __forceinline__ float cost_one(const Fast<MODEL>& m, float se2, const float4& r) {
@@ -332,54 +339,60 @@ static __device__ float4 g_lnphi[LNPHI_N];
 
 // Censored term -ln Phi(x), x = (y_up - f) / sqrt(v), of an upper limit: the
 // table on [-30, 8] and the reference's asymptote x^2/2 + ln sqrt(2 pi) +
-// ln(-x) below -30.  The table index is clamped before the load, so any x
-// (NaN, +-inf) reads inside the table; a +inf prediction (x = -inf) gives
-// +inf.
+// ln(-x) below -30.  The loops work in the table coordinate u = 8 (x + 30):
+// upper-limit records are staged with var / 64 and the particle's sigma_x^2
+// enters as sigma_x^2 / 64 (both exact), so rsqrt(v / 64) = 8 / sqrt(v) and
+// u = (y_up - f) rsqrt(v / 64) + 240 is one fused multiply-add.  The table
+// index is clamped before the load, so any u (NaN, +-inf) reads inside the
+// table; a +inf prediction (u = -inf) gives +inf.
+constexpr float LNPHI_U0 = -LNPHI_X0 * LNPHI_SCALE;  // u at x = 0
 template <bool CHECK, int MODEL>
-__device__ __forceinline__ float upper_x(const Fast<MODEL>& m, float se2, const float4& r) {
-    return (r.y - m.template pred<CHECK>(r.x)) * __builtin_amdgcn_rsqf(r.z + se2);
+__device__ __forceinline__ float upper_u(const Fast<MODEL>& m, float se2u, const float4& r) {
+    return __builtin_fmaf(r.y - m.template pred<CHECK>(r.x), __builtin_amdgcn_rsqf(r.z + se2u), LNPHI_U0);
 }
 // Table part: interval index clamped to the table (v_med3), position in it
 // by v_fract; beyond x = 8 the last interval's cubic, < 1e-15 on all of
 // [0, 1], stands for the reference's exact 0.
-__device__ __forceinline__ float neg_ln_phi_tab(float x) {
-    float ug = __builtin_fmaf(x, LNPHI_SCALE, -LNPHI_X0 * LNPHI_SCALE);
+__device__ __forceinline__ float neg_ln_phi_tab(float ug) {
     unsigned i = (unsigned)__builtin_amdgcn_fmed3f(ug, 0.0f, (float)(LNPHI_N - 1));
     float u = __builtin_amdgcn_fractf(ug);
     float4 c = g_lnphi[i];
     return __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, c.w, c.z), c.y), c.x);
 }
-__device__ __forceinline__ float neg_ln_phi_asym(float x) {
+// The asymptote at x = u / 8 - 30 (u < 0).
+__device__ __forceinline__ float neg_ln_phi_asym(float ug) {
+    float x = __builtin_fmaf(ug, 1.0f / LNPHI_SCALE, LNPHI_X0);
     return __builtin_fmaf(LN2, lg2(-x), __builtin_fmaf(0.5f * x, x, 0.918938533204672742f));
 }
-__device__ __forceinline__ float neg_ln_phi(float x) {
-    return x < -30.0f ? neg_ln_phi_asym(x) : neg_ln_phi_tab(x);
+__device__ __forceinline__ float neg_ln_phi_u(float ug) {
+    return ug < 0.0f ? neg_ln_phi_asym(ug) : neg_ln_phi_tab(ug);
 }
 // Four upper limits: the asymptote is evaluated only when some lane has an
 // observation below x = -30 (a wave-uniform branch).
 template <bool CHECK, int MODEL>
-__device__ __forceinline__ float cost_upper_quad(const Fast<MODEL>& m, float se2, const float4& r0,
+__device__ __forceinline__ float cost_upper_quad(const Fast<MODEL>& m, float se2u, const float4& r0,
                                                  const float4& r1, const float4& r2, const float4& r3) {
-    float x0 = upper_x<CHECK>(m, se2, r0), x1 = upper_x<CHECK>(m, se2, r1);
-    float x2 = upper_x<CHECK>(m, se2, r2), x3 = upper_x<CHECK>(m, se2, r3);
-    float c0 = neg_ln_phi_tab(x0), c1 = neg_ln_phi_tab(x1);
-    float c2 = neg_ln_phi_tab(x2), c3 = neg_ln_phi_tab(x3);
-    float xm = __builtin_fminf(__builtin_fminf(x0, x1), __builtin_fminf(x2, x3));
-    if (__builtin_expect(__any(xm < -30.0f), 0)) {
-        c0 = x0 < -30.0f ? neg_ln_phi_asym(x0) : c0;
-        c1 = x1 < -30.0f ? neg_ln_phi_asym(x1) : c1;
-        c2 = x2 < -30.0f ? neg_ln_phi_asym(x2) : c2;
-        c3 = x3 < -30.0f ? neg_ln_phi_asym(x3) : c3;
+    float u0 = upper_u<CHECK>(m, se2u, r0), u1 = upper_u<CHECK>(m, se2u, r1);
+    float u2 = upper_u<CHECK>(m, se2u, r2), u3 = upper_u<CHECK>(m, se2u, r3);
+    float c0 = neg_ln_phi_tab(u0), c1 = neg_ln_phi_tab(u1);
+    float c2 = neg_ln_phi_tab(u2), c3 = neg_ln_phi_tab(u3);
+    float um = __builtin_fminf(__builtin_fminf(u0, u1), __builtin_fminf(u2, u3));
+    if (__builtin_expect(__any(um < 0.0f), 0)) {
+        c0 = u0 < 0.0f ? neg_ln_phi_asym(u0) : c0;
+        c1 = u1 < 0.0f ? neg_ln_phi_asym(u1) : c1;
+        c2 = u2 < 0.0f ? neg_ln_phi_asym(u2) : c2;
+        c3 = u3 < 0.0f ? neg_ln_phi_asym(u3) : c3;
     }
     return (c0 + c1) + (c2 + c3);
 }
 template <bool CHECK, int MODEL>
-__device__ __forceinline__ float cost_upper(const Fast<MODEL>& m, float se2, const float4& r) {
-    return neg_ln_phi(upper_x<CHECK>(m, se2, r));
+__device__ __forceinline__ float cost_upper(const Fast<MODEL>& m, float se2u, const float4& r) {
+    return neg_ln_phi_u(upper_u<CHECK>(m, se2u, r));
 }
 
 // Stage `cnt` packed records {t, y, var, flag} from global memory into LDS as
-// {t log2 e, y, var, flag}, detections first and upper limits after them,
+// {t log2 e, y, var, flag} (var / 64 for an upper limit, see upper_u),
+// detections first and upper limits after them,
 // each group in its original order.  Returns the number of detections and
 // the range [tmin, tmax] of t log2 e (wave-uniform).  Single-wave workgroups.
 __device__ __forceinline__ int stage_split(float4* lds, const float4* __restrict__ g, int cnt,
@@ -413,6 +426,7 @@ __device__ __forceinline__ int stage_split(float4* lds, const float4* __restrict
         bool det = valid && r.w == 0.0f, up = valid && r.w != 0.0f;
         unsigned long long bd = __ballot(det), bu = __ballot(up);
         r.x *= L2E;
+        if (up) r.z *= 1.0f / 64.0f;  // upper limits: var / 64 (the table coordinate's scale)
         int rd = __builtin_amdgcn_mbcnt_hi((uint32_t)(bd >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bd, 0u));
         int ru = __builtin_amdgcn_mbcnt_hi((uint32_t)(bu >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bu, 0u));
         if (det) lds[kd + rd] = r;

@@ -87,16 +87,19 @@ struct Swarm {
             for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_one<CHECK>(mdl[k], se2[k], r0));
             i++;
         }
+        float se2u[PPL];  // sigma_x^2 / 64 against the upper limits' var / 64
+#pragma unroll
+        for (int k = 0; k < PPL; k++) se2u[k] = se2[k] * (1.0f / 64.0f);
         for (; i + 4 <= cnt; i += 4) {
             const O r0 = lds[i], r1 = lds[i + 1], r2 = lds[i + 2], r3 = lds[i + 3];
 #pragma unroll
             for (int k = 0; k < PPL; k++)
-                acc[k].add(f32::cost_upper_quad<CHECK>(mdl[k], se2[k], r0, r1, r2, r3));
+                acc[k].add(f32::cost_upper_quad<CHECK>(mdl[k], se2u[k], r0, r1, r2, r3));
         }
         for (; i < cnt; i++) {
             const O r0 = lds[i];
 #pragma unroll
-            for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_upper<CHECK>(mdl[k], se2[k], r0));
+            for (int k = 0; k < PPL; k++) acc[k].add(f32::cost_upper<CHECK>(mdl[k], se2u[k], r0));
         }
     }
 
