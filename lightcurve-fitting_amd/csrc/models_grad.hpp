@@ -90,6 +90,83 @@ template <typename R> struct ModelGrad<1, R> : ModelStable<1, R> {
     }
 };
 
+// Villar in FP32 for the SVI walk (src/parametric.rs:231-247 and its
+// gradient), with one reciprocal for both logistic weights and the
+// exponentials as bare exp2 of pre-scaled arguments.  With x_r = ph/tr and
+// z = 10 (ph - gamma) in log2 units, e_r = 2^-|x_r|, e_z = 2^-|z|,
+// q = 1 / ((1 + e_r)(1 + e_z)) and N / M the numerators of a logistic and its
+// complement (N = 1, M = e at a non-negative argument, N = e, M = 1 below):
+//   s_r (1 - w) = N_r M_z q,  s_r w e^y = N_r q E_z,  E_z = 2^{y + min(z, 0)},
+//   s_r (1 - s_r) = N_r M_r q^2 (1 + e_z)^2,
+// so with A = a N_r q and P = M_z pl + c1 E_z (pl = 1 - beta ph,
+// c1 = 1 - beta gamma): f = A P, and every component of the gradient is A
+// times a short expression (derivation beside each line).  e_r, e_z <= 1, so
+// (1 + e_r)(1 + e_z) lies in [1, 4].  Not finite where the reference's FP64
+// e^{(gamma - ph)/tf} overflows (y log2 e > 1024): returns false, as the
+// generic form does.
+struct VillarGrad32 {
+    float a, beta, gamma, t0, itr, itr2, itf2, zc, yc, c1, c1itf, c2;
+    static constexpr float L2E = 1.44269504088896341f, K10 = 14.4269504088896341f, OVF2 = 1024.0f;
+    __device__ __forceinline__ void setup(const float* p) {
+        using M = Math<float>;
+        a = M::exp(p[0]); beta = p[1]; gamma = M::exp(p[2]); t0 = p[3];
+        itr = M::exp(-p[4]);
+        const float itf = M::exp(-p[5]);
+        itr2 = itr * L2E; itf2 = itf * L2E;
+        zc = -K10 * gamma; yc = gamma * itf2;
+        c1 = 1.0f - beta * gamma; c1itf = c1 * itf; c2 = c1itf - beta;
+    }
+    __device__ __forceinline__ bool pred(float t, float& f) const {
+        const float ph = t - t0;
+        const float er = __builtin_amdgcn_exp2f(-__builtin_fabsf(ph * itr2));
+        const float z = __builtin_fmaf(ph, K10, zc);
+        const float ez = __builtin_amdgcn_exp2f(-__builtin_fabsf(z));
+        const float y = __builtin_fmaf(-ph, itf2, yc);
+        const float ez_y = __builtin_amdgcn_exp2f(y + __builtin_fminf(z, 0.0f));
+        const float q = __builtin_amdgcn_rcpf((1.0f + er) * (1.0f + ez));
+        const float nr = ph >= 0.0f ? 1.0f : er, mz = z >= 0.0f ? ez : 1.0f;
+        f = (a * q) * nr * __builtin_fmaf(mz, __builtin_fmaf(-beta, ph, 1.0f), c1 * ez_y);
+        return !(y > OVF2);
+    }
+    __device__ __forceinline__ bool pred_grad(float t, float& f, float* g) const {
+        const float ph = t - t0;
+        const float er = __builtin_amdgcn_exp2f(-__builtin_fabsf(ph * itr2));
+        const float z = __builtin_fmaf(ph, K10, zc);
+        const float ez = __builtin_amdgcn_exp2f(-__builtin_fabsf(z));
+        const float y = __builtin_fmaf(-ph, itf2, yc);
+        const float ez_y = __builtin_amdgcn_exp2f(y + __builtin_fminf(z, 0.0f));
+        const float er1 = 1.0f + er, ez1 = 1.0f + ez;
+        const float q = __builtin_amdgcn_rcpf(er1 * ez1);
+        const bool pr = ph >= 0.0f, pz = z >= 0.0f;
+        const float nr = pr ? 1.0f : er, mr = pr ? er : 1.0f;
+        const float nz = pz ? 1.0f : ez, mz = pz ? ez : 1.0f;
+        const float pl = __builtin_fmaf(-beta, ph, 1.0f);
+        const float c1e = c1 * ez_y;
+        const float P = __builtin_fmaf(mz, pl, c1e);
+        const float A = (a * q) * nr;
+        f = A * P;
+        // a s_r kterm = A K: kterm = 10 (1-w) (w pl - c1 w e^y)
+        const float K = (10.0f * q) * er1 * (mz * __builtin_fmaf(nz, pl, -c1e));
+        // a s_r (1 - s_r) piece / tr = A B
+        const float B = itr * ((mr * ez1) * q * P);
+        g[0] = f;                                                             // d/d ln A
+        g[1] = -A * __builtin_fmaf(mz, ph, gamma * ez_y);                     // d/d beta
+        g[2] = (gamma * A) * __builtin_fmaf(c2, ez_y, K);                     // d/d ln gamma
+        g[3] = A * ((K - B) + __builtin_fmaf(beta, mz, c1itf * ez_y));        // d/d t0
+        g[4] = -ph * (A * B);                                                 // d/d ln tau_rise
+        g[5] = (c1itf * ez_y) * A * (ph - gamma);                             // d/d ln tau_fall
+        g[6] = 0.0f;
+        return !(y > OVF2);
+    }
+};
+template <> struct ModelGrad<1, float> : VillarGrad32 {};
+
+// The value-only form the numerical-gradient SVI evaluates at each perturbed
+// parameter vector: the kernels' model form, and for Villar in FP32 the
+// value of the form above (same operations as its gradient uses).
+template <int MODEL, typename R> struct ValueForm { using T = Model<MODEL, R>; };
+template <> struct ValueForm<1, float> { using T = VillarGrad32; };
+
 template <typename R> struct ModelGrad<2, R> : ModelStable<2, R> {
     using B = ModelStable<2, R>;
     __device__ __forceinline__ bool pred_grad(R t, R& f, R* g) const {

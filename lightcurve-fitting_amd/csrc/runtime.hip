@@ -379,8 +379,9 @@ int lcf_ctx_create(int device, lcf_ctx** out) {
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
     if (c->n_cu <= 0) c->n_cu = 256;
     // this device's copy of the fixed scratch array, resolved with the device
-    // current; then the FP32 cost loop's one-time table, so that no fit call
-    // builds it inside its timed region
+    // current; then the FP32 kernels' one-time tables (the PSO cost loop's
+    // censored term, the SVI upper-limit tail), so that no fit call builds
+    // them inside its timed region
     void* sp = nullptr;
     e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
     if (e != hipSuccess || !sp) {
@@ -388,7 +389,7 @@ int lcf_ctx_create(int device, lcf_ctx** out) {
         return LCF_E_HIP;
     }
     c->scratch = (char*)sp;
-    if (lcf::pso_f32_prepare(c->stream) != hipSuccess) {
+    if (lcf::pso_f32_prepare(c->stream) != hipSuccess || lcf::svi_f32_prepare(c->stream) != hipSuccess) {
         lcf_ctx_destroy(c);
         return LCF_E_HIP;
     }

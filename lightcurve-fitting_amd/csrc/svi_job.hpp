@@ -36,6 +36,21 @@ template <typename R> struct ObsR;
 template <> struct ObsR<float> { using T = float4; };
 template <> struct ObsR<double> { using T = double4; };
 
+// The FP32 upper-limit term of the exact-tail adjoint, log_normal_cdf(z) and
+// dlog_normal_cdf(z) (models_fit.hpp, the reference's A&S form and its own
+// derivative), tabulated on [-30, 8] in SVI_TAIL_N intervals of width 1/8:
+// interval i holds two float4 of cubic coefficients in u = 8 (z - z_i), one
+// per function, through the FP64 values at u = 0, 1/3, 2/3, 1 (built once per
+// device by svi_f32_prepare).  z = 0 is a node, so the |z| in the A&S form
+// never falls inside an interval.  Below -30 the reference's asymptote is
+// evaluated; above 8 the last interval's cubics (below 1e-14) stand for the
+// reference's exact 0.  One copy per translation unit: only svi_f32.hip
+// (LCF_SVI_F32_UNIT) fills and reads it.
+constexpr int SVI_TAIL_N = 304;
+constexpr float SVI_TAIL_Z0 = -30.0f;
+constexpr float SVI_TAIL_SCALE = 8.0f;
+static __device__ float4 g_svi_tail[2 * SVI_TAIL_N];
+
 // phi(z) / max(Phi(z), 1e-300) with the A&S 7.1.26 erfc.  FP64 evaluates the
 // reference's expression as written (cuda/svi.cu:200-204); FP32 reproduces
 // the value that expression takes in binary64, including its rounding of
@@ -140,13 +155,33 @@ __device__ __forceinline__ void draw_normals_split(uint64_t seed, uint32_t job, 
     }
 }
 
+// log_normal_cdf(z) and its derivative from the tail table (any z: the index
+// is clamped before the load, so NaN and +-inf read inside the table).
+__device__ __forceinline__ float svi_tail_tab(float z, float& d) {
+    const float ug = __builtin_fmaf(z, SVI_TAIL_SCALE, -SVI_TAIL_Z0 * SVI_TAIL_SCALE);
+    const unsigned i = (unsigned)__builtin_amdgcn_fmed3f(ug, 0.0f, (float)(SVI_TAIL_N - 1));
+    const float u = __builtin_amdgcn_fractf(ug);
+    const float4 a = g_svi_tail[2 * i], b = g_svi_tail[2 * i + 1];
+    d = __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, b.w, b.z), b.y), b.x);
+    return __builtin_fmaf(u, __builtin_fmaf(u, __builtin_fmaf(u, a.w, a.z), a.y), a.x);
+}
+// The reference's asymptote below z = -30: -z^2/2 - ln sqrt(2 pi) - ln(-z)
+// and its derivative -z - 1/z.
+__device__ __forceinline__ float svi_tail_asym(float z, float& d) {
+    d = -z - __builtin_amdgcn_rcpf(z);
+    return __builtin_fmaf(-0.5f * z, z, -0.918938533204672742f) -
+           0.693147180559945309f * __builtin_amdgcn_logf(-z);
+}
+
 // Copies cnt observations into LDS with the measured ones first and the
-// upper limits after them, each kind in its original order.  One pass over
-// a tile hands G consecutive observations to the wave (one per group, all
-// samples), so after this partition only the pass that straddles the two
-// kinds takes both branches of the likelihood.
+// upper limits after them, each kind in its original order, and returns the
+// number of measured ones.  One pass over a tile hands G consecutive
+// observations to the wave (one per group, all samples), so after this
+// partition only the pass that straddles the two kinds takes both branches
+// of the likelihood (FP64), and the FP32 walk runs the two kinds as two
+// loops.
 template <typename R>
-__device__ __forceinline__ void stage(typename ObsR<R>::T* lds, const typename ObsR<R>::T* __restrict__ g, int cnt) {
+__device__ __forceinline__ int stage(typename ObsR<R>::T* lds, const typename ObsR<R>::T* __restrict__ g, int cnt) {
     using O = typename ObsR<R>::T;
     const int lane = lane_id();
     __syncthreads();
@@ -170,6 +205,7 @@ __device__ __forceinline__ void stage(typename ObsR<R>::T* lds, const typename O
         ku += __popcll(bu);
     }
     __syncthreads();
+    return n_meas;
 }
 
 template <int MODEL, typename R, bool FD>
@@ -183,7 +219,7 @@ struct SviJob {
     // the reference's step h = max(1e-5, 1e-5 |theta_j|).
     struct Eval {
         ModelGrad<MODEL, R> base;
-        Model<MODEL, R> pert[FD ? NP - 1 : 1];
+        typename ValueForm<MODEL, R>::T pert[FD ? NP - 1 : 1];
         R h[NP];
         __device__ __forceinline__ void setup(const R (&th)[NP]) {
             base.setup(th);
@@ -293,6 +329,80 @@ struct SviJob {
         return ll.s;
     }
 
+    // FP32 walk of this lane's observations: the measured ones, then the
+    // upper limits (two loops over the staged tile, no branch on the kind),
+    // with no per-observation finiteness tests.  A non-finite model value or
+    // gradient makes the lane's sums non-finite; `fin` reports it and the
+    // caller then repeats the step's walk with loglik(), which skips such
+    // observations and components as the reference does -- so the result
+    // is loglik()'s whenever every term is finite, and loglik()'s exactly
+    // otherwise.  The exact-tail upper-limit term and its derivative come
+    // from the tail table (the asymptote only when some lane needs it).
+    __device__ __forceinline__ R loglik_fast(const R (&th)[NP], const O* lds, const O* __restrict__ gobs, int n,
+                                             int nd_staged, int tile, int grp, int G, bool exact_tail,
+                                             R (&dll)[NP], bool& fin) {
+#ifndef LCF_SVI_F32_UNIT
+        static_assert(sizeof(R) != 4, "FP32 SVI is instantiated in svi_f32.hip only (it builds the tail table)");
+#endif
+        Eval ev;
+        ev.setup(th);
+        const R se2 = M::exp(R(2) * th[SE]);
+        Kahan<R> ll;
+        ll.init();
+#pragma unroll
+        for (int j = 0; j < NP; j++) dll[j] = R(0);
+        for (int base = 0; base < n; base += tile) {
+            const int cnt = n - base < tile ? n - base : tile;
+            const int nd = n > tile ? stage<R>(const_cast<O*>(lds), gobs + base, cnt) : nd_staged;
+            for (int i = grp; i < nd; i += G) {
+                const O o = lds[i];
+                R f, g[NP];
+                const bool ok = ev(o.x, f, g);
+                f = ok ? f : R(INFINITY);
+                const R tv = o.z + se2, inv = M::rcp(tv);
+                const R r = o.y - f, ri = r * inv;
+                // -(r^2/tv + ln(2 pi tv)) / 2, dLL/df = r/tv, dLL/d ln sigma_x = (r^2/tv^2 - 1/tv) se2
+                const R term = __builtin_fmaf(R(-0.5) * r, ri,
+                                              __builtin_fmaf(__builtin_amdgcn_logf(tv), R(-0.346573590279972655),
+                                                             R(-0.918938533204672742)));
+                ll.add(term);
+#pragma unroll
+                for (int j = 0; j < NP - 1; j++) dll[j] = __builtin_fmaf(ri, g[j], dll[j]);
+                dll[SE] = __builtin_fmaf(__builtin_fmaf(ri, ri, -inv), se2, dll[SE]);
+            }
+            for (int i = nd + grp; i < cnt; i += G) {
+                const O o = lds[i];
+                R f, g[NP];
+                const bool ok = ev(o.x, f, g);
+                f = ok ? f : R(INFINITY);
+                const R rst = M::rsqrt(o.z + se2);
+                const R z = (o.y - f) * rst;
+                R term, ratio;
+                if (exact_tail) {
+                    term = svi_tail_tab(z, ratio);
+                    if (__builtin_expect(__any(z < SVI_TAIL_Z0), 0)) {
+                        R ra, ta = svi_tail_asym(z, ra);
+                        term = z < SVI_TAIL_Z0 ? ta : term;
+                        ratio = z < SVI_TAIL_Z0 ? ra : ratio;
+                    }
+                } else {
+                    term = log_normal_cdf<R>(z);
+                    ratio = mills_ratio<R>(z);
+                }
+                ll.add(term);
+                const R wgt = -ratio * rst;
+#pragma unroll
+                for (int j = 0; j < NP - 1; j++) dll[j] = __builtin_fmaf(wgt, g[j], dll[j]);
+                dll[SE] = __builtin_fmaf(wgt * z, se2 * rst, dll[SE]);
+            }
+        }
+        bool f_ok = M::finite(ll.s);
+#pragma unroll
+        for (int j = 0; j < NP; j++) f_ok = f_ok && M::finite(dll[j]);
+        fin = f_ok;
+        return ll.s;
+    }
+
     // Monte-Carlo ELBO of step `step` at the wave-uniform (mu, sigma =
     // e^ls): returns the ELBO and, in lane l, the wave total of value l >> 2
     // of {sum LL + log prior, sum dL/dtheta_j (j < NP), sum dL/dtheta_j
@@ -300,8 +410,9 @@ struct SviJob {
     // w.r.t. mu_j and ls_j.
     __device__ __forceinline__ R elbo_grad(const R (&mu)[NP], const R (&sig)[NP], const R (&ls)[NP],
                                            const R (&pc)[NP], const R (&pw2)[NP], const O* lds,
-                                           const O* __restrict__ gobs, int n, int tile, uint32_t job,
-                                           uint32_t step, int S, int G, uint64_t seed, bool exact_tail, R& red) {
+                                           const O* __restrict__ gobs, int n, int nd_staged, int tile,
+                                           uint32_t job, uint32_t step, int S, int G, uint64_t seed,
+                                           bool exact_tail, R& red) {
         constexpr bool F64 = sizeof(R) == 8;
         const int lane = lane_id();
         const int s = lane / G, grp = lane - s * G;
@@ -311,7 +422,14 @@ struct SviJob {
         else draw_normals<R, NP>(seed, job, step, (uint32_t)s, eps);
 #pragma unroll
         for (int j = 0; j < NP; j++) th[j] = mu[j] + sig[j] * eps[j];
-        R L = loglik(th, lds, gobs, n, tile, grp, G, exact_tail, dll);
+        R L;
+        if constexpr (F64) {
+            L = loglik(th, lds, gobs, n, tile, grp, G, exact_tail, dll);
+        } else {
+            bool fin;
+            L = loglik_fast(th, lds, gobs, n, nd_staged, tile, grp, G, exact_tail, dll, fin);
+            if (__builtin_expect(__any(!fin), 0)) L = loglik(th, lds, gobs, n, tile, grp, G, exact_tail, dll);
+        }
         if (grp == 0) {
             R lp = R(0);
 #pragma unroll
@@ -400,7 +518,7 @@ struct SviJob {
                 a.out_ls[row + lane] = 0.0;
             }
         };
-        if (n <= a.tile) stage<R>(lds, g, n);
+        const int nd_staged = n <= a.tile ? stage<R>(lds, g, n) : 0;
         // the noise stream's job word: the job's index, or 0 for every job
         // (the CPU fit seeds a fresh SmallRng(42) per band, src/parametric.rs:2239)
         const uint32_t rng_job = (a.flags & LCF_SVI_SHARED_STREAM_FLAG) ? 0u : (uint32_t)job;
@@ -410,8 +528,8 @@ struct SviJob {
         if (a.grad_only) {
             R mu[NP], sig[NP], ls[NP];
             current(mu, sig, ls);
-            R e = elbo_grad(mu, sig, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)a.step0, S, G,
-                            a.seed, exact_tail, red);
+            R e = elbo_grad(mu, sig, ls, pc, pw2, lds, g, n, nd_staged, a.tile, rng_job, (uint32_t)a.step0, S,
+                            G, a.seed, exact_tail, red);
             if (writer) {  // + 1 on ls: derivative of the entropy term
                 if (own_mu) a.out_mu[row + pj] = (double)(red / rS);
                 else a.out_ls[row + pj] = (double)(red / rS) + 1.0;
@@ -431,8 +549,8 @@ struct SviJob {
         for (; step < a.n_steps;) {
             R mu[NP], sig[NP], ls[NP];
             current(mu, sig, ls);
-            R e = elbo_grad(mu, sig, ls, pc, pw2, lds, g, n, a.tile, rng_job, (uint32_t)step, S, G, a.seed,
-                            exact_tail, red);
+            R e = elbo_grad(mu, sig, ls, pc, pw2, lds, g, n, nd_staged, a.tile, rng_job, (uint32_t)step, S, G,
+                            a.seed, exact_tail, red);
             final_elbo = e;
             step++;
             // Adam (cuda/svi.cu:266-291) on the owning lanes, non-finite
@@ -472,8 +590,14 @@ struct SviJob {
     }
 };
 
+// FP32 kernels are compiled for four waves per SIMD (at most 128 registers;
+// the spills this costs sit outside the observation walk), except MetzgerKN,
+// whose 200-point grid would spill inside it.
+template <int MODEL, typename R>
+constexpr int svi_waves_per_simd() { return sizeof(R) == 4 && MODEL != 7 ? 4 : 1; }
+
 template <int MODEL, typename R, bool FD, bool PERSISTENT>
-__global__ __launch_bounds__(64) void svi_kernel(SviArgs a) {
+__global__ __launch_bounds__(64, (svi_waves_per_simd<MODEL, R>())) void svi_kernel(SviArgs a) {
     extern __shared__ unsigned char lds_raw[];
     using O = typename ObsR<R>::T;
     O* lds = reinterpret_cast<O*>(lds_raw);

@@ -13,6 +13,10 @@ hipError_t launch_svi(int model, const SviArgs& a, bool fp64, bool fd, bool pers
                       hipStream_t st) {
     const void* k = fp64 ? svi_kernel_f64(model, fd, persistent) : svi_kernel_f32(model, fd, persistent);
     if (!k || grid <= 0) return hipErrorInvalidValue;
+    if (!fp64) {
+        hipError_t e = svi_f32_prepare(st);
+        if (e != hipSuccess) return e;
+    }
     void* args[] = {const_cast<SviArgs*>(&a)};
     return hipLaunchKernel(k, dim3(grid), dim3(64), args, svi_lds_bytes(fp64, a.tile), st);
 }

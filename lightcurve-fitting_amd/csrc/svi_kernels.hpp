@@ -37,6 +37,10 @@ struct SviArgs {
 };
 
 size_t svi_lds_bytes(bool fp64, int tile);
+// Builds the FP32 kernels' upper-limit tail table on the current device the
+// first time it is called there (one launch, then a wait on `st`); later
+// calls only check a flag.  launch_svi calls it before every FP32 launch.
+hipError_t svi_f32_prepare(hipStream_t st);
 hipError_t launch_svi(int model, const SviArgs& a, bool fp64, bool fd, bool persistent, int grid,
                       hipStream_t st);
 hipError_t svi_occupancy(int model, bool fp64, bool fd, int tile, int* blocks_per_cu);

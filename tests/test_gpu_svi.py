@@ -56,14 +56,15 @@ def jobs(pkg, orc):
     return jcsr, np.asarray(mids, dtype=np.int32), pso, pc, pw
 
 
-def oracle_grad(orc, jcsr, mids, mu, ls, pc, pw, S, seed, step):
+def oracle_grad(orc, jcsr, mids, mu, ls, pc, pw, S, seed, step, grad_mode=0):
     off = jcsr["offsets"]
     out = []
     for j, m in enumerate(mids):
         np_ = orc.n_params(m)
         eps = np.stack([orc.svi_normals(seed, j, step, s, np_) for s in range(S)])
         a, b = off[j], off[j + 1]
-        out.append(orc.svi_elbo_grad(m, mu[j], ls[j], pc[j], pw[j], eps, *(jcsr[k][a:b] for k in OBS)))
+        out.append(orc.svi_elbo_grad(m, mu[j], ls[j], pc[j], pw[j], eps, *(jcsr[k][a:b] for k in OBS),
+                                     grad_mode=grad_mode))
     return out
 
 
@@ -104,6 +105,46 @@ def test_elbo_grad_fp32_matches_oracle(ctx, orc, pkg, jobs):
         bad += not ok
     assert n >= len(ref) // 3, n
     assert bad <= n // 50, (bad, n)
+
+
+def test_elbo_grad_fp32_skips_non_finite_observations(ctx, orc, pkg):
+    """Villar bands whose earliest observations lie where the reference's FP64
+    e^{(gamma - ph)/tau_fall} overflows (ph < gamma - 709.78 tau_fall): the
+    reference skips those observations (cuda/svi.cu:176), so the FP32
+    kernel's walk sees a non-finite model value, repeats the step with the
+    skipping walk, and must land on the oracle's ELBO and gradient (1e-4 of
+    the gradient scale).  A band with upper limits in the skipped region and
+    one without any skipped observation ride along."""
+    rng = np.random.default_rng(17)
+    theta = np.array([0.0, 0.01, 2.0, 100.0, 1.0, -2.0, -3.0])
+    srcs = []
+    for k, t_lo in enumerate((0.0, 0.0, 40.0)):
+        t = np.sort(rng.uniform(t_lo, 150.0, 60))
+        f = orc.eval_batch(1, theta, t)
+        f = np.where(np.isfinite(f), f, 0.0)
+        up = (np.arange(t.size) % 4 == 0).astype(np.int32) if k == 1 else np.zeros(t.size, np.int32)
+        srcs.append({"times": t, "flux": f + rng.normal(0, 0.05, t.size), "obs_var": np.full(t.size, 0.05 ** 2),
+                     "is_upper": up, "upper_flux": np.where(up != 0, f + 0.3, 0.0)})
+    jcsr = stack_sources(srcs)
+    assert np.sum(jcsr["times"] < 11.0) > 0  # observations the reference skips
+    mids = np.ones(len(srcs), dtype=np.int32)
+    mu = np.tile(theta, (len(srcs), 1))
+    ls = np.full_like(mu, -2.5)
+    pc, pw = np.zeros_like(mu), np.zeros_like(mu)
+    for j in range(len(srcs)):
+        c, w = orc.svi_prior_for_model(1, mu[j])
+        pc[j, :c.size], pw[j, :w.size] = c, w
+    data = pkg.GpuBatchData(ctx, csr=jcsr)
+    # LCF_SVI_EXACT_TAIL (16) is the oracle's grad_mode 2 (the exact derivative of the tail term)
+    for flags, grad_mode in ((0, 0), (16, 2)):
+        e, gmu, gls = ctx.svi_elbo_grad(data, mids, mu, ls, pc, pw, 16, seed=3, step=1, flags=flags)
+        ref = oracle_grad(orc, jcsr, mids, mu, ls, pc, pw, 16, 3, 1, grad_mode=grad_mode)
+        for j, (eo, gmo, glo) in enumerate(ref):
+            scale = max(np.abs(gmo).max(), np.abs(glo).max(), 1.0)
+            assert np.isfinite(e[j]) and abs(e[j] - eo) <= 1e-4 * max(abs(eo), 1.0), (j, e[j], eo)
+            assert np.all(np.abs(gmu[j, :7] - gmo) <= 1e-4 * scale), j
+            assert np.all(np.abs(gls[j, :7] - glo) <= 1e-4 * scale), j
+    data.close()
 
 
 def smooth_jobs(orc, jcsr, mids, mu, ls, S, seed, step):
