@@ -462,7 +462,7 @@ struct SviJob {
         for (int j = 0; j < NP; j++) ent += ls[j];
         // 0.5 np ln(2 pi e), as the reference forms it (cuda/svi.cu:264)
         ent += R(0.5 * NP) * (R)::log(2.0 * 3.14159265358979323846 * 2.71828182845904523536);
-        return tot / R(S) + ent;
+        return (sizeof(R) == 8 ? tot / R(S) : tot * (R(1) / R(S))) + ent;
     }
 
     __device__ void run(const SviArgs& a, O* lds, int job) {
@@ -524,6 +524,7 @@ struct SviJob {
         const uint32_t rng_job = (a.flags & LCF_SVI_SHARED_STREAM_FLAG) ? 0u : (uint32_t)job;
         const bool exact_tail = (a.flags & LCF_SVI_EXACT_TAIL_FLAG) != 0;
         const R rS = R(S);
+        const R inv_s = R(1) / rS;
         R red;
         if (a.grad_only) {
             R mu[NP], sig[NP], ls[NP];
@@ -562,11 +563,21 @@ struct SviJob {
                 bc1 = (R)(1.0 - ::pow(0.9, (double)step));
                 bc2 = (R)(1.0 - ::pow(0.999, (double)step));
             }
-            R gg = -(red / rS + (own_ls ? R(1) : R(0)));
-            if (M::finite(gg)) {
-                m1 = R(0.9) * m1 + R(0.1) * gg;
-                m2 = R(0.999) * m2 + R(0.001) * gg * gg;
-                x -= lr * (m1 / bc1) / (M::sqrt(m2 / bc2) + R(1e-8));
+            if constexpr (sizeof(R) == 8) {  // the reference's divisions
+                R gg = -(red / rS + (own_ls ? R(1) : R(0)));
+                if (M::finite(gg)) {
+                    m1 = R(0.9) * m1 + R(0.1) * gg;
+                    m2 = R(0.999) * m2 + R(0.001) * gg * gg;
+                    x -= lr * (m1 / bc1) / (M::sqrt(m2 / bc2) + R(1e-8));
+                }
+            } else {  // FP32: hardware reciprocals in place of the divisions
+                R gg = -(red * inv_s + (own_ls ? R(1) : R(0)));
+                if (M::finite(gg)) {
+                    m1 = R(0.9) * m1 + R(0.1) * gg;
+                    m2 = R(0.999) * m2 + R(0.001) * gg * gg;
+                    const R mh = m1 * M::rcp(bc1), vh = m2 * M::rcp(bc2);
+                    x -= lr * mh * M::rcp(M::sqrt(vh) + R(1e-8));
+                }
             }
             if (own_ls) x = x < R(-6) ? R(-6) : (x > R(2) ? R(2) : x);
             // EMA early stop (cuda/svi.cu:297-312), unless all steps are asked for
