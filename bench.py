@@ -8,14 +8,17 @@ Kahan-compensated chi-square), and the best model per band is selected.  One
 "step" is one such pass over the whole batch, with the observations already
 resident in HBM.  The metric is sources/second.
 
+The five model families run as one persistent launch over every (band,
+model) task (mixed families on one size-sorted device queue).
+
 Also reported on the same JSON line:
-  * roofline: algorithmic FP32 flop-equivalents of the dominant (slowest)
-    model kernel divided by its HIP-event launch time, against MI355X's FP32
-    vector peak;
+  * roofline: algorithmic FP32 flop-equivalents of that launch divided by its
+    HIP-event launch time, against MI355X's FP32 vector peak;
+  * per_model: the same fit as one persistent launch per model, with each
+    model's launch time and the slowest launch's roofline;
   * cpu_baseline: the oracle (FP64 C++ restatement, all host threads up to 16)
     on a bounded sample of the same workload, in sources/second;
-  * plain: the same fit launched once per power-of-two size class instead of
-    one persistent size-sorted launch per model;
+  * plain: the same fit launched once per power-of-two size class and model;
   * svi: the SVI workload (10k Villar sources x 3 bands x 60 points, 16
     samples, up to 2000 Adam steps, FP32) fitted by the batched adjoint kernel
     and by the plain form (forward-difference gradients, one launch per size
@@ -63,14 +66,14 @@ FLOPEQ_UP = {0: 42.0, 1: 56.0, 2: 57.0, 3: 57.0, 4: 49.0}
 # evaluation (20 ops + 6 transcendentals x 8) and its adjoint (about 45 ops).
 SVI_FLOPEQ_FWD, SVI_FLOPEQ_ADJ = 20 + 6 * 8, 45
 FP32_PEAK_TFLOPS = 157.3
-# HBM traffic per launch of each model kernel on the default 100k-source
-# workload, from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB
-# per dispatch) over `bench.py --steps 1 --warmup 0 --no-cpu --no-plain
-# --no-svi --no-np --no-gp2d`, each scaled by what the same counter reads for
-# `bench.py --copy-probe 1024` (a 1 GiB device copy): FETCH_SIZE 524308 KiB
-# for 1048576 KiB read (x 2.0), WRITE_SIZE 1048576 KiB written (x 1.0).
-PSO_FETCH_KIB = {0: 236585.0, 1: 240138.0, 2: 236294.0, 3: 235933.0, 4: 230035.0}
-PSO_WRITE_KIB = {0: 37904.0, 1: 42918.0, 2: 37911.0, 3: 37906.0, 4: 35398.0}
+# HBM traffic of the mixed-family launch on the default 100k-source workload,
+# from two rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE, KiB per
+# dispatch) over `bench.py --steps 1 --warmup 0 --no-cpu --no-plain --no-svi
+# --no-np --no-gp2d --no-tune --no-accuracy`, each scaled by what the same
+# counter reads for `bench.py --copy-probe 1024` (a 1 GiB device copy):
+# FETCH_SIZE 524308 KiB for 1048576 KiB read (x 2.0), WRITE_SIZE 1048576 KiB
+# written (x 1.0).
+MIXED_FETCH_KIB, MIXED_WRITE_KIB = 969592.0, 272085.0
 FETCH_SCALE, WRITE_SCALE = 1048576.0 / 524308.0, 1.0
 
 
@@ -476,18 +479,19 @@ def main() -> int:
         log(f"warmup {w + 2}/{args.warmup} done")
     ctx.synchronize()
     t0 = time.perf_counter()
-    slot_ms = np.zeros(len(MODELS))
+    mixed_ms = 0.0
     for k in range(args.steps):
         step()
-        slot_ms += [ctx.last_model_ms(i) for i in range(len(MODELS))]
+        mixed_ms += ctx.last_model_ms(0)
         log(f"step {k + 1}/{args.steps}: {ctx.last_kernel_ms:.1f} ms kernels")
     ctx.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = dt * 1e3 / args.steps
     value = args.sources / (dt / args.steps)
-    slot_ms /= args.steps
+    mixed_ms /= args.steps
 
-    # roofline of the dominant model kernel
+    # roofline of the one launch over every (band, model) task: the models'
+    # algorithmic flop-equivalents over their executed iterations
     npts = np.diff(csr["offsets"]).astype(np.float64)
     nup = np.add.reduceat(csr["is_upper"].astype(np.float64), csr["offsets"][:-1]) if npts.size else npts
     nup = np.where(npts > 0, nup, 0.0)
@@ -495,28 +499,56 @@ def main() -> int:
     moe_up = (it64 * nup[None, :]).sum(axis=1) * args.particles
     moe = (it64 * npts[None, :]).sum(axis=1) * args.particles
     moe_det = moe - moe_up
-    dom = int(np.argmax(slot_ms))
-    m_dom = MODELS[dom]
-    flops = moe[dom] * FLOPEQ_ALGO[m_dom]
-    issued = moe_det[dom] * FLOPEQ_DET[m_dom] + moe_up[dom] * FLOPEQ_UP[m_dom]
+    fe_algo = np.array([FLOPEQ_ALGO[m] for m in MODELS], dtype=np.float64)
+    flops = float((moe * fe_algo).sum())
+    issued = float(sum(moe_det[i] * FLOPEQ_DET[m] + moe_up[i] * FLOPEQ_UP[m] for i, m in enumerate(MODELS)))
     # measured HBM bytes per launch (default workload only); the algorithmic
-    # bytes are one read of every packed record plus the outputs
+    # bytes are one read of every packed record per model plus the outputs
     traffic = None
-    if args.sources == 100_000 and args.particles == 64 and args.iters == 200:
-        traffic = round((PSO_FETCH_KIB[m_dom] * FETCH_SCALE + PSO_WRITE_KIB[m_dom] * WRITE_SCALE) * 1024.0)
-    algo_bytes = n_obs * 16 + n_bands * (7 * 8 + 8 + 4 + 4 + 8)
-    achieved = flops / (slot_ms[dom] * 1e-3) / 1e12
+    if args.sources == 100_000 and args.particles == 64 and args.iters == 200 and MIXED_FETCH_KIB:
+        traffic = round((MIXED_FETCH_KIB * FETCH_SCALE + MIXED_WRITE_KIB * WRITE_SCALE) * 1024.0)
+    algo_bytes = len(MODELS) * (n_obs * 16 + n_bands * (7 * 8 + 8 + 4 + 4 + 8))
+    achieved = flops / (mixed_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "pipe": "fp32 vector ALU + transcendental unit (no matrix ops)",
-            "kernel": f"pso_model_kernel<{MODEL_NAMES[MODELS[dom]]}, fp32>",
+            "kernel": "pso_mixed_kernel<fp32> (" + ", ".join(MODEL_NAMES[m] for m in MODELS) + ")",
             "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-            "launch_ms": round(float(slot_ms[dom]), 3),
+            "launch_ms": round(float(mixed_ms), 3),
             "algorithmic_bytes": int(algo_bytes),
-            "moe_per_launch": float(moe[dom]), "upper_limit_moe_fraction": round(float(moe_up[dom] / moe[dom]), 4),
-            "flopeq_per_moe": FLOPEQ_ALGO[m_dom],
-            "flopeq_convention": "model formula operation count, transcendental = 8",
-            "issued_flopeq_per_moe": {"detection": FLOPEQ_DET[m_dom], "upper_limit": FLOPEQ_UP[m_dom]},
-            "issued_tflops": round(issued / (slot_ms[dom] * 1e-3) / 1e12, 3)}
+            "moe_per_launch": float(moe.sum()), "upper_limit_moe_fraction": round(float(moe_up.sum() / moe.sum()), 4),
+            "flopeq_per_moe": round(flops / float(moe.sum()), 3),
+            "flopeq_convention": "model formula operation count, transcendental = 8 (per model: "
+                                 + ", ".join(f"{MODEL_NAMES[m]} {FLOPEQ_ALGO[m]}" for m in MODELS) + ")",
+            "issued_flopeq_per_moe": round(issued / float(moe.sum()), 3),
+            "issued_tflops": round(issued / (mixed_ms * 1e-3) / 1e12, 3)}
+
+    # the same fit as one persistent launch per model: per-model launch times
+    # and the roofline of the slowest model's launch
+    ctx.set_schedule(pkg.SCHED_PER_MODEL)
+    step()
+    ctx.synchronize()
+    t_pm = time.perf_counter()
+    slot_ms = np.zeros(len(MODELS))
+    for _ in range(args.steps):
+        step()
+        slot_ms += [ctx.last_model_ms(i) for i in range(len(MODELS))]
+    ctx.synchronize()
+    pm_dt = (time.perf_counter() - t_pm) / args.steps
+    slot_ms /= args.steps
+    ctx.set_schedule(pkg.SCHED_PERSISTENT)
+    dom = int(np.argmax(slot_ms))
+    m_dom = MODELS[dom]
+    ach_dom = moe[dom] * FLOPEQ_ALGO[m_dom] / (slot_ms[dom] * 1e-3) / 1e12
+    per_model = {"value": round(args.sources / pm_dt, 2), "unit": "sources/s", "ms_per_step": round(pm_dt * 1e3, 3),
+                 "schedule": "one persistent size-sorted launch per model",
+                 "per_model_ms": {MODEL_NAMES[m]: round(float(slot_ms[i]), 3) for i, m in enumerate(MODELS)},
+                 "slowest_kernel": {"kernel": f"pso_model_kernel<{MODEL_NAMES[m_dom]}, fp32>",
+                                    "launch_ms": round(float(slot_ms[dom]), 3), "achieved": round(ach_dom, 3),
+                                    "frac": round(ach_dom / FP32_PEAK_TFLOPS, 4),
+                                    "flopeq_per_moe": FLOPEQ_ALGO[m_dom],
+                                    "issued_flopeq_per_moe": {"detection": FLOPEQ_DET[m_dom],
+                                                              "upper_limit": FLOPEQ_UP[m_dom]}}}
+    log(f"per-model launches: {pm_dt * 1e3:.1f} ms per step")
 
     # end to end: upload + pack of the observations, the fit, the downloads
     ctx.synchronize()
@@ -607,7 +639,8 @@ def main() -> int:
                    "observations": n_obs, "points_per_band": "log-uniform[10,500]",
                    "bands_per_source": "U{1,2,3}", "models": [MODEL_NAMES[m] for m in MODELS],
                    "particles": args.particles, "iterations": args.iters,
-                   "schedule": "persistent size-sorted queue, one launch per model",
+                   "schedule": "one persistent launch over every (band, model) task: mixed model "
+                               "families on one size-sorted device queue",
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -616,7 +649,7 @@ def main() -> int:
         "accuracy": accuracy,
         "plain": plain,
         "autotuned": tuned,
-        "per_model_ms": {MODEL_NAMES[m]: round(float(slot_ms[i]), 3) for i, m in enumerate(MODELS)},
+        "per_model": per_model,
         "svi": svi,
         "gp2d": gp2,
         "nonparametric": npl,

@@ -43,11 +43,15 @@ extern "C" {
 #define LCF_FP64 1           /* FP64 throughout */
 
 /* work distribution for batched fits */
-#define LCF_SCHED_PERSISTENT 0  /* one persistent launch, size-sorted device queue (default) */
+#define LCF_SCHED_PERSISTENT 0  /* one persistent launch, size-sorted device queue (default); a PSO fit
+                                   of two or more FP32 models 0-6 (<= 64 particles, no gate) runs all of
+                                   them in that one launch, mixed model families on one queue */
 #define LCF_SCHED_PER_CLASS 1   /* one launch per power-of-two size class, synchronised */
 #define LCF_SCHED_TUNED 2       /* PSO fits: one persistent launch per size bucket with the context's
                                    autotuned launch shape (lcf_ctx_autotune_pso); other fits as
                                    LCF_SCHED_PERSISTENT */
+#define LCF_SCHED_PER_MODEL 3   /* PSO fits: one persistent launch per model (per-model launch
+                                   times); other fits as LCF_SCHED_PERSISTENT */
 
 /* PSO flags */
 #define LCF_PSO_STALL_FROM_FIRST_EVAL 1u /* opt-in: stall baseline = first global best
@@ -524,7 +528,9 @@ int lcf_gp2d_nlml_grad(lcf_ctx* ctx, int64_t n_sources, const int64_t* offsets, 
  * milliseconds between HIP events around the kernel launches (inputs already
  * resident, outputs still on the device). */
 double lcf_last_kernel_ms(const lcf_ctx* ctx);
-/* Milliseconds of the launch(es) of model slot `slot` in that call. */
+/* Milliseconds of the launch(es) of model slot `slot` in that call (when the
+ * models ran as one mixed-family launch: slot 0 holds that launch, the other
+ * slots 0). */
 double lcf_last_model_ms(const lcf_ctx* ctx, int slot);
 
 #ifdef __cplusplus

@@ -13,6 +13,7 @@ from .gpu import (  # noqa: F401
     NOALLOC_MAX_BANDS,
     PSO_STALL_FROM_FIRST_EVAL,
     SCHED_PER_CLASS,
+    SCHED_PER_MODEL,
     SCHED_PERSISTENT,
     SCHED_TUNED,
     TUNE_BUCKETS,

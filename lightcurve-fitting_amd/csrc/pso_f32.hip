@@ -10,6 +10,7 @@ namespace lcf {
 const void* pso_kernel_f32(int model, int ppl, bool persistent) { return pick_kernel_r<float>(model, ppl, persistent); }
 const void* mb_kernel_f32(int ppl) { return pick_mb_r<float>(ppl); }
 const void* cost64_kernel(int model) { return pick_cost64(model); }
+const void* pso_mixed_kernel_f32() { return (const void*)pso_mixed_kernel<float, 1>; }
 
 namespace {
 

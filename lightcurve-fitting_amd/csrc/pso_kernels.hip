@@ -10,6 +10,7 @@ const void* pso_kernel_f64(int model, int ppl, bool persistent);
 const void* mb_kernel_f32(int ppl);
 const void* mb_kernel_f64(int ppl);
 const void* cost64_kernel(int model);
+const void* pso_mixed_kernel_f32();
 
 static const void* pick_kernel(int model, bool fp64, int ppl, bool persistent) {
     return fp64 ? pso_kernel_f64(model, ppl, persistent) : pso_kernel_f32(model, ppl, persistent);
@@ -48,6 +49,21 @@ hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool pers
     }
     void* args[] = {const_cast<PsoArgs*>(&a)};
     return hipLaunchKernel(k, dim3(grid), dim3(64), args, pso_lds_bytes(fp64, a.tile), st);
+}
+
+hipError_t launch_pso_mixed(const PsoArgs& a, const PsoMix& mx, int grid, hipStream_t st) {
+    if (grid <= 0 || mx.n < 1 || mx.n > 8) return hipErrorInvalidValue;
+    for (int i = 0; i < mx.n; i++)
+        if (mx.model[i] < 0 || mx.model[i] > 6) return hipErrorInvalidValue;
+    hipError_t e = pso_f32_prepare(st);
+    if (e != hipSuccess) return e;
+    void* args[] = {const_cast<PsoArgs*>(&a), const_cast<PsoMix*>(&mx)};
+    return hipLaunchKernel(pso_mixed_kernel_f32(), dim3(grid), dim3(64), args, pso_lds_bytes(false, a.tile), st);
+}
+
+hipError_t pso_mixed_occupancy(int tile, int* blocks_per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, pso_mixed_kernel_f32(), 64,
+                                                        pso_lds_bytes(false, tile));
 }
 
 hipError_t launch_cost64(int model, const PsoArgs& a, hipStream_t st) {

@@ -27,6 +27,12 @@ struct PsoArgs {
     int32_t* model_iters;       // [n_slots][n_bands]
 };
 
+// Model families of one mixed launch (models 0-6, in output-slot order).
+struct PsoMix {
+    int n;
+    int model[8];
+};
+
 // Greedy MultiBazin fit (batch_pso_multi_bazin, src/gpu_cuda.rs:650-963).
 struct MbArgs {
     const void* obs;            // packed records of the swarm's precision
@@ -57,6 +63,10 @@ size_t pso_lds_bytes(bool fp64, int tile);
 hipError_t launch_pso(int model, const PsoArgs& a, bool fp64, int ppl, bool persistent, int grid,
                       hipStream_t st);
 hipError_t pso_occupancy(int model, bool fp64, int ppl, int tile, int* blocks_per_cu);
+// One persistent FP32 launch over every (band, model) task of mx (models 0-6,
+// one particle per lane; a.n_tasks = bands x mx.n, a.order the band order).
+hipError_t launch_pso_mixed(const PsoArgs& a, const PsoMix& mx, int grid, hipStream_t st);
+hipError_t pso_mixed_occupancy(int tile, int* blocks_per_cu);
 // FP64 re-costing of the best positions of an FP32 swarm launch (slot a.slot,
 // every band of the batch; models 0-6, others a no-op).
 hipError_t launch_cost64(int model, const PsoArgs& a, hipStream_t st);

@@ -404,9 +404,9 @@ __device__ double cost64_at(const double* p, const double4* __restrict__ g64, in
 // With a gate, the band is skipped unless the gate model's cost >= gate
 // (batch_model_select, src/gpu_cuda.rs:584-619).
 template <int MODEL, typename R, int PPL>
-__device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band) {
+__device__ void fit_band(const PsoArgs& a, typename ObsT<R>::T* lds, int band, int model_slot) {
     using O = typename ObsT<R>::T;
-    const int64_t slot = (int64_t)a.slot * a.n_bands + band;
+    const int64_t slot = (int64_t)model_slot * a.n_bands + band;
     const int lane = lane_id();
     if (a.gate_cost && !(a.gate_cost[band] >= a.gate)) {
         if (lane < 7) a.model_params[slot * 7 + lane] = 0.0;
@@ -494,12 +494,44 @@ __global__ __launch_bounds__(64, (pso_waves_per_simd<MODEL, R, PPL>())) void pso
         for (;;) {
             const int t = wave_next_task(a.queue);
             if (t >= a.n_tasks) break;
-            fit_band<MODEL, R, PPL>(a, lds, a.order[t]);
+            fit_band<MODEL, R, PPL>(a, lds, a.order[t], a.slot);
             wave_reconverge();
         }
     } else {
         int t = a.task_base + (int)blockIdx.x;
-        if (t < a.n_tasks) fit_band<MODEL, R, PPL>(a, lds, a.order[t]);
+        if (t < a.n_tasks) fit_band<MODEL, R, PPL>(a, lds, a.order[t], a.slot);
+    }
+}
+
+// One persistent launch over mixed model families: task t is (band
+// order[t / n], model slot t % n), so the queue hands out every model of the
+// longest band first and the bands in decreasing size, and a wave switches
+// model family from one task to the next.  FP32 swarms of models 0-6 with
+// one particle per lane (the instruction-lean cost loops, all within the
+// same 96 registers, so the switch costs no occupancy); no gate (a gated
+// model waits for its band's first model).
+template <typename R, int PPL>
+__global__ __launch_bounds__(64, (pso_waves_per_simd<0, R, PPL>())) void pso_mixed_kernel(PsoArgs a, PsoMix mx) {
+    static_assert(pso_waves_per_simd<0, R, PPL>() == pso_waves_per_simd<6, R, PPL>(),
+                  "the mixed launch shares one occupancy across models 0-6");
+    extern __shared__ unsigned char lds_raw[];
+    using O = typename ObsT<R>::T;
+    O* lds = reinterpret_cast<O*>(lds_raw);
+    for (;;) {
+        const int t = wave_next_task(a.queue);
+        if (t >= a.n_tasks) break;
+        const int k = t / mx.n, mi = t - k * mx.n;
+        const int band = a.order[k];
+        switch (mx.model[mi]) {
+            case 0: fit_band<0, R, PPL>(a, lds, band, mi); break;
+            case 1: fit_band<1, R, PPL>(a, lds, band, mi); break;
+            case 2: fit_band<2, R, PPL>(a, lds, band, mi); break;
+            case 3: fit_band<3, R, PPL>(a, lds, band, mi); break;
+            case 4: fit_band<4, R, PPL>(a, lds, band, mi); break;
+            case 5: fit_band<5, R, PPL>(a, lds, band, mi); break;
+            default: fit_band<6, R, PPL>(a, lds, band, mi); break;
+        }
+        wave_reconverge();
     }
 }
 

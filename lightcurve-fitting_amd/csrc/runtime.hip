@@ -269,7 +269,35 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
     // confirms it (no launch, no synchronisation inside the timed region)
     if (!fp64) HIPCHK(c, lcf::pso_f32_prepare(c->stream));
     HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-    for (int mi = 0; mi < n_models; mi++) {
+    bool mixed = c->schedule == LCF_SCHED_PERSISTENT && !fp64 && ppl == 1 && !use_gate && n_models >= 2 &&
+                 n_models <= 8;
+    for (int mi = 0; mi < n_models && mixed; mi++) mixed = models[mi] >= 0 && models[mi] <= 6;
+    if (mixed) {
+        // one launch over (band, model) tasks; slot 0's time is the launch's
+        // (the other slots' are 0), the FP64 re-costs follow it
+        lcf::PsoMix mx{};
+        mx.n = n_models;
+        for (int mi = 0; mi < n_models; mi++) mx.model[mi] = models[mi];
+        int per_cu = 0;
+        HIPCHK(c, lcf::pso_mixed_occupancy(tile, &per_cu));
+        per_cu = std::max(per_cu, 1);
+        a.task_base = 0;
+        a.n_tasks = (int)(ns * nm);
+        a.gate_cost = nullptr;
+        const int grid = (int)std::min<int64_t>((int64_t)per_cu * c->n_cu, ns * nm);
+        HIPCHK(c, hipEventRecord(c->ev_slot[0], c->stream));
+        if (d.fill_queue)
+            HIPCHK(c, launch_zero_ints(d.queue, 1, c->stream));
+        else
+            HIPCHK(c, hipMemsetAsync(d.queue, 0, sizeof(int), c->stream));
+        HIPCHK(c, lcf::launch_pso_mixed(a, mx, grid, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev_slot[1], c->stream));
+        for (int mi = 0; mi < n_models; mi++) {
+            a.slot = mi;
+            HIPCHK(c, lcf::launch_cost64(models[mi], a, c->stream));
+        }
+    }
+    for (int mi = 0; mi < n_models && !mixed; mi++) {
         const int model = models[mi];
         a.slot = mi;
         HIPCHK(c, hipEventRecord(c->ev_slot[mi], c->stream));
@@ -289,7 +317,7 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
                 HIPCHK(c, lcf::launch_pso(model, ab, fp64, ppl, true, grid, c->stream));
             }
             if (!fp64) HIPCHK(c, lcf::launch_cost64(model, a, c->stream));
-        } else if (c->schedule == LCF_SCHED_PERSISTENT) {
+        } else if (c->schedule == LCF_SCHED_PERSISTENT || c->schedule == LCF_SCHED_PER_MODEL) {
             int per_cu = 0;
             HIPCHK(c, lcf::pso_occupancy(model, fp64, ppl, tile, &per_cu));
             per_cu = std::max(per_cu, 1);
@@ -327,7 +355,7 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
             if (!fp64) HIPCHK(c, lcf::launch_cost64(model, a, c->stream));
         }
     }
-    HIPCHK(c, hipEventRecord(c->ev_slot[n_models], c->stream));
+    if (!mixed) HIPCHK(c, hipEventRecord(c->ev_slot[n_models], c->stream));
     if (best_model || best_params || best_cost)
         HIPCHK(c, lcf::launch_select_best(d.mp, d.mc, d.models, n_models, (int)ns, d.bm, d.bp,
                                           d.bc, c->stream));
@@ -337,7 +365,7 @@ int fit_models_core(lcf_ctx* c, const lcf_batch* b, const int32_t* models, int n
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
     for (int mi = 0; mi < LCF_N_MODELS; mi++) c->slot_ms[mi] = 0.0;
-    for (int mi = 0; mi < n_models; mi++) {
+    for (int mi = 0; mi < (mixed ? 1 : n_models); mi++) {
         HIPCHK(c, hipEventElapsedTime(&ms, c->ev_slot[mi], c->ev_slot[mi + 1]));
         c->slot_ms[mi] = ms;
     }
@@ -420,7 +448,7 @@ int lcf_ctx_set_precision(lcf_ctx* c, int p) {
 
 int lcf_ctx_set_schedule(lcf_ctx* c, int s) {
     if (!c) return LCF_E_INVALID;
-    if (s != LCF_SCHED_PERSISTENT && s != LCF_SCHED_PER_CLASS && s != LCF_SCHED_TUNED)
+    if (s != LCF_SCHED_PERSISTENT && s != LCF_SCHED_PER_CLASS && s != LCF_SCHED_TUNED && s != LCF_SCHED_PER_MODEL)
         return fail(c, LCF_E_INVALID, "unknown schedule %d", s);
     c->schedule = s;
     return LCF_OK;

@@ -20,7 +20,7 @@ from . import _lib
 MAX_PARAMS = 7
 NP_NFEAT = 50  # NonparametricBandResult fields without the band name
 FP32, FP64 = 0, 1
-SCHED_PERSISTENT, SCHED_PER_CLASS, SCHED_TUNED = 0, 1, 2
+SCHED_PERSISTENT, SCHED_PER_CLASS, SCHED_TUNED, SCHED_PER_MODEL = 0, 1, 2, 3
 TUNE_BUCKETS, TUNE_FIELDS = 7, 6
 NOALLOC_MAX_BANDS = 16384  # LCF_NOALLOC_MAX_BANDS
 PSO_STALL_FROM_FIRST_EVAL = 1

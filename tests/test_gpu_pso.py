@@ -163,13 +163,29 @@ def test_fp32_selection_statistics(ctx, orc, pkg):
 
 
 def test_schedules_agree(pkg, ragged):
-    a = pkg.GpuContext(0, schedule=pkg.SCHED_PERSISTENT)
+    """Per-model persistent, per-size-class and mixed-family (the default)
+    launches give identical fits: a band's swarm depends only on its data,
+    index and seed."""
+    a = pkg.GpuContext(0, schedule=pkg.SCHED_PER_MODEL)
     b = pkg.GpuContext(0, schedule=pkg.SCHED_PER_CLASS)
-    da, db = pkg.GpuBatchData(a, csr=ragged), pkg.GpuBatchData(b, csr=ragged)
-    oa = a.fit_models(da, [0, 1, 3], 64, 20, 20, seed=5)
-    ob = b.fit_models(db, [0, 1, 3], 64, 20, 20, seed=5)
+    c = pkg.GpuContext(0, schedule=pkg.SCHED_PERSISTENT)
+    da, db, dc = (pkg.GpuBatchData(x, csr=ragged) for x in (a, b, c))
+    for models, P in (([0, 1, 3], 64), ([6, 2, 5, 4, 0, 1, 3], 48), ([1, 2], 128)):
+        oa = a.fit_models(da, models, P, 20, 20, seed=5)
+        ob = b.fit_models(db, models, P, 20, 20, seed=5)
+        oc = c.fit_models(dc, models, P, 20, 20, seed=5)
+        for k in ("model_params", "model_cost", "model_iters", "best_model"):
+            assert np.array_equal(oa[k], ob[k]), (models, k)
+            assert np.array_equal(oa[k], oc[k]), (models, k)
+    # a gated selection runs as per-model launches under the default schedule
+    ga = a.fit_models(da, [0, 1, 3], 64, 20, 20, seed=5, gate=1.5)
+    gc = c.fit_models(dc, [0, 1, 3], 64, 20, 20, seed=5, gate=1.5)
+    assert c.last_model_ms(1) > 0.0
     for k in ("model_params", "model_cost", "model_iters", "best_model"):
-        assert np.array_equal(oa[k], ob[k]), k
+        assert np.array_equal(ga[k], gc[k]), k
+    # the mixed launch times as one (slot 0)
+    c.fit_models(dc, [0, 1, 3], 64, 20, 20, seed=5)
+    assert c.last_model_ms(0) > 0.0 and c.last_model_ms(1) == 0.0
 
 
 def test_reference_acceptance_on_gpu(ctx, orc, pkg):
