@@ -220,6 +220,25 @@ template <int CTRL> __device__ __forceinline__ double dpp(double v) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// Lowest lane holding the wave minimum of v (no NaN; all lanes active):
+// a min reduction by permlane swaps and DPP moves, then a ballot of the
+// lanes equal to it.  Wave-uniform.
+__device__ __forceinline__ int wave_argmin_lane(float v) {
+    float a = v, b = v;
+    swap32(a, b);
+    float m = __builtin_fminf(a, b);  // lanes l and l ^ 32
+    a = m;
+    b = m;
+    swap16(a, b);
+    m = __builtin_fminf(a, b);  // and l ^ 16
+    m = __builtin_fminf(m, dpp<DPP_ROW_ROR8>(m));         // and l ^ 8
+    m = __builtin_fminf(m, dpp<DPP_ROW_HALF_MIRROR>(m));  // 7 - l within 8-lane halves
+    m = __builtin_fminf(m, dpp<DPP_QP_2301>(m));          // l ^ 2
+    m = __builtin_fminf(m, dpp<DPP_QP_1032>(m));          // l ^ 1: all 64 lanes
+    const unsigned long long eq = __ballot(v == m);
+    return eq ? (int)__builtin_ctzll(eq) : 0;
+}
+
 // Sixteen wave sums at once (a reduce-scatter): returns in lane l the sum
 // over all 64 lanes of v[l >> 2].  Each stage halves the values a lane
 // carries, pairing lanes that differ in one bit (bit 5, 4, 3, then a mirror

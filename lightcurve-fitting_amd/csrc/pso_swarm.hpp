@@ -56,7 +56,8 @@ struct Swarm {
 
     R pos[PPL][NP], vel[PPL][NP], pb[PPL][NP], pcost[PPL];
     R gb[NP], gcost;
-    R ipw[NP];  // FAST: 1 / population-prior width (0 where the width is 0)
+    R ipw[NP];      // FAST: sqrt(1/2) / population-prior width (0 where the width is 0)
+    float inv_nf;   // FAST: 1 / max(n, 1), set once per band
 
     // FAST cost: the tile holds {t log2 e, y, var, flag} with the `nd`
     // detections first (staged once by the caller when the band fits one
@@ -126,15 +127,15 @@ struct Swarm {
             if (__all(safe)) tile_fast<false>(acc, mdl, se2, lds, nd, cnt);
             else tile_fast<true>(acc, mdl, se2, lds, nd, cnt);
         }
-        const float inv_n = 1.0f / (float)(n > 0 ? n : 1);
+        const float inv_n = inv_nf;
 #pragma unroll
         for (int k = 0; k < PPL; k++) {
-            float pri = 0.0f;
+            float pri = 0.0f;  // sum of z^2 / 2, z = (p - centre) / width
             if constexpr (MODEL < MB_MODEL0) {  // MultiBazin has no population prior
 #pragma unroll
                 for (int j = 0; j < NP; j++) {
                     float z = (pos[k][j] - (float)c_pc[MODEL][j]) * ipw[j];
-                    pri = __builtin_fmaf(0.5f * z, z, pri);
+                    pri = __builtin_fmaf(z, z, pri);
                 }
             }
             float c = n > 0 ? acc[k].s * inv_n : M::big();
@@ -250,9 +251,10 @@ struct Swarm {
 #pragma unroll
             for (int j = 0; j < NP; j++) {
                 const double w = c_pw[MODEL][j];
-                ipw[j] = w > 0.0 ? (R)(1.0 / w) : R(0);
+                ipw[j] = w > 0.0 ? (R)(0.70710678118654752440 / w) : R(0);
             }
         }
+        if constexpr (FAST) inv_nf = 1.0f / (float)(n > 0 ? n : 1);
         const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
         constexpr int NPAIR = (NP + 1) / 2;
         Xoshiro128 rng[PPL];
@@ -291,6 +293,7 @@ struct Swarm {
         gcost = R(INFINITY);
 
         const R w_max = R(0.9), w_min = R(0.4), c1 = R(1.5), c2 = R(1.5);
+        const R c1s = c1 * R(1.0 / 16777216.0), c2s = c2 * R(1.0 / 16777216.0);
         const R inv_max = R(1) / R(max_iters);
         R prev = R(INFINITY);
         int stall = 0;
@@ -314,7 +317,15 @@ struct Swarm {
                     if (cost[k] < bc || (cost[k] == bc && p < bi)) { bc = cost[k]; bi = p; }
                 }
             }
-            int win = wave_argmin<R>(bc, bi);
+            int win;
+            if constexpr (FAST && PPL == 1) {
+                // costs are never NaN here (mapped to the sentinel) and the
+                // particle index is the lane: the lowest lane holding the
+                // minimum is the lexicographic (cost, index) minimum
+                win = wave_argmin_lane(bc);
+            } else {
+                win = wave_argmin<R>(bc, bi);
+            }
             R wcost = bcast(bc, win & (WAVE - 1));
             if (wcost < gcost) {
                 gcost = wcost;
@@ -332,15 +343,18 @@ struct Swarm {
             for (int k = 0; k < PPL; k++) {
 #pragma unroll
                 for (int d = 0; d < NP; d++) {
-                    R r1 = u24<R>(rng[k].next());
-                    R r2 = u24<R>(rng[k].next());
+                    // c1 u24 and c2 u24 as one product each: (x >> 8) (1.5 2^-24)
+                    // is c1 (x >> 8) 2^-24 exactly (a power-of-two scale)
+                    R r1 = R(rng[k].next() >> 8) * c1s;
+                    R r2 = R(rng[k].next() >> 8) * c2s;
                     R x = pos[k][d];
-                    R v = w * vel[k][d] + c1 * r1 * (pb[k][d] - x) + c2 * r2 * (gb[d] - x);
-                    v = v < -vmax[d] ? -vmax[d] : (v > vmax[d] ? vmax[d] : v);
+                    R v = w * vel[k][d] + r1 * (pb[k][d] - x) + r2 * (gb[d] - x);
+                    // clamp to +-vmax; walls absorb (position on the wall, velocity 0)
+                    v = fmin(fmax(v, -vmax[d]), vmax[d]);
                     R nx = x + v;
-                    if (nx <= lo[d]) { pos[k][d] = lo[d]; vel[k][d] = R(0); }
-                    else if (nx >= hi[d]) { pos[k][d] = hi[d]; vel[k][d] = R(0); }
-                    else { pos[k][d] = nx; vel[k][d] = v; }
+                    const bool inside = nx > lo[d] && nx < hi[d];
+                    pos[k][d] = fmin(fmax(nx, lo[d]), hi[d]);
+                    vel[k][d] = inside ? v : R(0);
                 }
             }
             // stall rule (src/gpu_cuda.rs:537-552); prev starts at +inf as
