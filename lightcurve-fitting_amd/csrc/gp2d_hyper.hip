@@ -51,9 +51,15 @@ enum {
     S_GN = 24, S_S = 28, S_Y = 52, S_RHO = 76, S_HYP = 82, S_END = 96
 };
 
+// Second operand slab: a 32-deep half of a 64 x 64 tile operand (2048
+// doubles) or one packed lower-triangular 64 x 64 block (2080 doubles).  At
+// 52 KB of LDS in all, three workgroups share a CU.
+constexpr int SB = T * (T + 1) / 2;
+constexpr int KH = T / 2;  // depth of a half slab
+
 struct Smem {
     double a[T * T];
-    double b[T * T];
+    double b[SB];
     double red[NT];
     double st[S_END];
     int flag[8];  // 0 pivot failure, 1 stop, 2 accepted, 3 nm, 4 head, 5 step too small,
@@ -69,22 +75,25 @@ struct Src {
     int np;  // padded size: rows [n + 1, np) are identity
 };
 
-// s[kk * 64 + r] = M[(r0 + r) * ld + k0 + kk], zero for k0 + kk >= kend (rows
-// of a row-major matrix become the slab's columns).
+// s[kk * 64 + r] = M[(r0 + r) * ld + k0 + kk] for kk < KK, zero for
+// k0 + kk >= kend (rows of a row-major matrix become the slab's columns).
+template <int KK = T>
 __device__ __forceinline__ void stage_rows(double* s, const double* M, int ld, int r0, int k0, int kend) {
     const int rs_ = threadIdx.x >> 3, ks = threadIdx.x & 7;
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < KK / 4; q++) {
         const int r = rs_ + 32 * (q & 1), kk = ks + 8 * (q >> 1);
         const int k = k0 + kk;
         s[kk * T + r] = k < kend ? M[(size_t)(r0 + r) * ld + k] : 0.0;
     }
 }
 
-// s[kk * 64 + c] = M[(k0 + kk) * ld + c0 + c], zero for k0 + kk >= kend.
+// s[kk * 64 + c] = M[(k0 + kk) * ld + c0 + c] for kk < KK, zero for
+// k0 + kk >= kend.
+template <int KK = T>
 __device__ __forceinline__ void stage_kmaj(double* s, const double* M, int ld, int c0, int k0, int kend) {
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
+    for (int q = 0; q < KK / 4; q++) {
         const int idx = threadIdx.x + NT * q;
         const int kk = idx >> 6, c = idx & 63;
         const int k = k0 + kk;
@@ -120,12 +129,15 @@ __device__ __forceinline__ int tile_col(int cb) {
     return 32 * ((threadIdx.x >> 6) & 1) + 16 * cb + (threadIdx.x & 15);
 }
 
+// KK-deep product (KK = 64: a whole tile; 32: one half of the k range, the
+// halves accumulated in increasing k as one 64-deep product would).
+template <int KK = T>
 __device__ __forceinline__ void mma64(Tile& t, const double* sa, const double* sb) {
     const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
     const double* pa = sa + (l >> 4) * T + 32 * (w >> 1) + (l & 15);
     const double* pb = sb + (l >> 4) * T + 32 * (w & 1) + (l & 15);
 #pragma unroll 4
-    for (int k0 = 0; k0 < T; k0 += 4) {
+    for (int k0 = 0; k0 < KK; k0 += 4) {
         const double a0 = pa[k0 * T], a1 = pa[k0 * T + 16];
         const double b0 = pb[k0 * T], b1 = pb[k0 * T + 16];
         t.d[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, t.d[0][0], 0, 0, 0);
@@ -248,10 +260,17 @@ __device__ __noinline__ bool cholesky(double* A, int ld, const Src& s, Smem& sm)
             for (int C0 = J0 + T; C0 <= I0; C0 += T) {
                 Tile acc;
                 zero(acc);
-                __syncthreads();
-                if (C0 < I0) stage_rows(sm.b, A, ld, C0, J0, J0 + T);
-                __syncthreads();
-                mma64(acc, sm.a, C0 < I0 ? sm.b : sm.a);
+                if (C0 < I0) {  // the other tile row's panel in two 32-deep halves
+                    for (int h = 0; h < T; h += KH) {
+                        __syncthreads();
+                        stage_rows<KH>(sm.b, A, ld, C0, J0 + h, J0 + T);
+                        __syncthreads();
+                        mma64<KH>(acc, sm.a + h * T, sm.b);
+                    }
+                } else {
+                    __syncthreads();
+                    mma64(acc, sm.a, sm.a);
+                }
 #pragma unroll
                 for (int rb = 0; rb < 2; rb++)
 #pragma unroll
@@ -290,14 +309,17 @@ __device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& 
     // the augmented row leaves the factor: [[L, 0], [0, 1]]
     for (int j = tid; j < n; j += NT) A[(size_t)n * ld + j] = 0.0;
     __syncthreads();
-    // diagonal blocks of X = L^-1, two at a time (sm.a, sm.b): wave w of the
-    // pair's block, lane c solves L_II x = e_c
+    // diagonal blocks of X = L^-1, two at a time (sm.a, sm.b, each the
+    // block's lower triangle packed by rows: L[r][c] at r (r + 1) / 2 + c):
+    // wave w of the pair's block, lane c solves L_II x = e_c
     for (int I = 0; I < nb; I += 2) {
         __syncthreads();
         for (int e = tid; e < T * T; e += NT) {
             const int r = e >> 6, c = e & 63;
-            sm.a[e] = c <= r ? A[(size_t)(I * T + r) * ld + I * T + c] : 0.0;
-            if (I + 1 < nb) sm.b[e] = c <= r ? A[(size_t)((I + 1) * T + r) * ld + (I + 1) * T + c] : 0.0;
+            if (c > r) continue;
+            const int pk = (r * (r + 1) >> 1) + c;
+            sm.a[pk] = A[(size_t)(I * T + r) * ld + I * T + c];
+            if (I + 1 < nb) sm.b[pk] = A[(size_t)((I + 1) * T + r) * ld + (I + 1) * T + c];
         }
         __syncthreads();
         const int w = tid >> 6;
@@ -314,14 +336,15 @@ __device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& 
                 for (int q = 0; q < rb; q++) {
                     const double xq = xc[(size_t)q * ld];
 #pragma unroll
-                    for (int j = 0; j < 16; j++) x[j] = fma(-L[(rb + j) * T + q], xq, x[j]);
+                    for (int j = 0; j < 16; j++) x[j] = fma(-L[((rb + j) * (rb + j + 1) >> 1) + q], xq, x[j]);
                 }
 #pragma unroll
                 for (int j = 0; j < 16; j++) {
+                    const double* Lr = L + ((rb + j) * (rb + j + 1) >> 1);
                     double acc = x[j];
 #pragma unroll
-                    for (int q = 0; q < j; q++) acc = fma(-L[(rb + j) * T + rb + q], x[q], acc);
-                    x[j] = acc / L[(rb + j) * T + rb + j];
+                    for (int q = 0; q < j; q++) acc = fma(-Lr[rb + q], x[q], acc);
+                    x[j] = acc / Lr[rb + j];
                 }
 #pragma unroll
                 for (int j = 0; j < 16; j++) xc[(size_t)(rb + j) * ld] = x[j];
@@ -339,22 +362,30 @@ __device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& 
             for (int k0 = J0; k0 < I0; k0 += T) {
                 __syncthreads();
                 stage_rows(sm.a, A, ld, I0, k0, I0);
-                stage_kmaj(sm.b, Xm, ld, J0, k0, I0);
-                __syncthreads();
-                mma64(acc, sm.a, sm.b);
+                for (int h = 0; h < T; h += KH) {
+                    if (h) __syncthreads();
+                    stage_kmaj<KH>(sm.b, Xm, ld, J0, k0 + h, I0);
+                    __syncthreads();
+                    mma64<KH>(acc, sm.a + h * T, sm.b);
+                }
             }
+            // X_IJ = -X_II S: S (the sum above) becomes the second operand in
+            // sm.a, X_II's rows the first, in two 32-deep halves
             __syncthreads();
 #pragma unroll
             for (int rb = 0; rb < 2; rb++)
 #pragma unroll
                 for (int cb = 0; cb < 2; cb++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) sm.b[tile_row(rb, i) * T + tile_col(cb)] = acc.d[rb][cb][i];
-            stage_rows(sm.a, Xm, ld, I0, I0, I0 + T);
-            __syncthreads();
+                    for (int i = 0; i < 4; i++) sm.a[tile_row(rb, i) * T + tile_col(cb)] = acc.d[rb][cb][i];
             Tile x2;
             zero(x2);
-            mma64(x2, sm.a, sm.b);
+            for (int h = 0; h < T; h += KH) {
+                if (h) __syncthreads();
+                stage_rows<KH>(sm.b, Xm, ld, I0, I0 + h, I0 + T);
+                __syncthreads();
+                mma64<KH>(x2, sm.b, sm.a + h * T);
+            }
 #pragma unroll
             for (int rb = 0; rb < 2; rb++)
 #pragma unroll
@@ -385,9 +416,17 @@ __device__ __noinline__ void gradient(double* A, double* Xm, int ld, const Src& 
             for (int k0 = I0; k0 < n; k0 += T) {
                 __syncthreads();
                 stage_kmaj(sm.a, Xm, ld, I0, k0, n);
-                if (J < I) stage_kmaj(sm.b, Xm, ld, J0, k0, n);  // a diagonal tile reads sm.a twice
-                __syncthreads();
-                mma64(acc, sm.a, J < I ? sm.b : sm.a);
+                if (J < I) {  // the second operand in two 32-deep halves
+                    for (int h = 0; h < T; h += KH) {
+                        if (h) __syncthreads();
+                        stage_kmaj<KH>(sm.b, Xm, ld, J0, k0 + h, n);
+                        __syncthreads();
+                        mma64<KH>(acc, sm.a + h * T, sm.b);
+                    }
+                } else {  // a diagonal tile reads sm.a twice
+                    __syncthreads();
+                    mma64(acc, sm.a, sm.a);
+                }
             }
 #pragma unroll
             for (int rb = 0; rb < 2; rb++)
@@ -787,7 +826,7 @@ __device__ void fit_source(const Gp2dHyperArgs& a, int src, double* slot, Smem& 
 
 }  // namespace
 
-__global__ __launch_bounds__(NT, 2) void gp2d_hyper_kernel(Gp2dHyperArgs a) {
+__global__ __launch_bounds__(NT, 3) void gp2d_hyper_kernel(Gp2dHyperArgs a) {
     __shared__ Smem sm;
     __shared__ int ticket;
     double* slot = a.work + (size_t)blockIdx.x * gp2d_hyper_slot_doubles(a.ld);
