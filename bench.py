@@ -73,7 +73,7 @@ FP32_PEAK_TFLOPS = 157.3
 # counter reads for `bench.py --copy-probe 1024` (a 1 GiB device copy):
 # FETCH_SIZE 524308 KiB for 1048576 KiB read (x 2.0), WRITE_SIZE 1048576 KiB
 # written (x 1.0).
-MIXED_FETCH_KIB, MIXED_WRITE_KIB = 969592.0, 272085.0
+MIXED_FETCH_KIB, MIXED_WRITE_KIB = 917148.0, 187227.0
 FETCH_SCALE, WRITE_SCALE = 1048576.0 / 524308.0, 1.0
 
 
