@@ -1,8 +1,8 @@
 // gp2d_hyper.hip -- 2D (time x log10 wavelength) GP hyper-parameter fit, FP64.
 //
 // One 256-thread workgroup per source (persistent, sources taken from a
-// size-sorted queue; two workgroups per CU: the large phases are separate
-// functions so that each fits 256 registers), the source's matrices in a
+// size-sorted queue; three workgroups per CU: the large phases are separate
+// functions so that each fits 168 registers), the source's matrices in a
 // global slot of ld x ld doubles (ld a multiple of 64, L2 / MALL resident),
 // all O(n^3) work as 64 x 64 tile products staged through LDS and run on the
 // FP64 matrix cores (v_mfma_f64_16x16x4f64, a 32 x 32 quadrant per wave):
